@@ -1,0 +1,148 @@
+"""Benchmark: Scattering2D J=3 L=8 on 256x256 float32 images, batch 64.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+
+One step = one forward of the fused HIP pipeline over the batch (inputs
+resident in HBM; plan creation, filter upload and workspace allocation happen
+before the timed region).  Prints one JSON line:
+  value            images/s over the K timed steps
+  roofline         dominant kernel: algorithmic bytes per launch (stage-boundary
+                   byte model) / its mean launch time from HIP events recorded
+                   around every launch during the timed steps, vs 8 TB/s
+  pipeline_roofline the same model over the whole forward
+  cpu_baseline     the CPU oracle (op-for-op restatement of the reference's torch
+                   CPU path) on a bounded sample of the same workload
+Only one GPU is used; --gpus N > 1 exits with a message.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+METRIC = 'images/s Scattering2D J=3 L=8 256² fp32 (%HBM roofline); recon time to 1e-3'
+J, L, SIDE, BATCH, SEED = 3, 8, 256, 64, 2
+
+
+def cpu_baseline(budget_s=15.0, max_images=256):
+    from oracle import scat2d
+    from kymatio_amd import workloads
+    cores = os.cpu_count() or 1
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    torch.set_num_threads(cores)
+    sc = scat2d.Scattering2DTorchCPU(J, (SIDE, SIDE))
+    x = torch.from_numpy(workloads.grating_images(16, SIDE, seed=SEED))
+    n, t_total = 0, 0.0
+    with torch.no_grad():
+        sc(x[:1])  # warm-up
+        while t_total < budget_s and n < max_images:
+            t0 = time.perf_counter()
+            sc(x)
+            t_total += time.perf_counter() - t0
+            n += x.shape[0]
+    model = 'unknown'
+    try:
+        for line in open('/proc/cpuinfo'):
+            if line.startswith('model name'):
+                model = line.split(':', 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {'value': n / t_total, 'unit': 'images/s', 'cores': cores, 'kind': 'port',
+            'sample': '%d images of the same 256x256 J=3 workload (batches of 16), torch CPU FP32 '
+                      'restatement of the reference torch backend, %.1f s, %s' % (n, t_total, model)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=50)
+    ap.add_argument('--warmup', type=int, default=5)
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--chunk', type=int, default=0, help='images per launch chunk (0 = default)')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py measures one GPU only (the scattering path shards over the batch with no '
+              'exchange; see DESIGN.md). Run with --gpus 1.', file=sys.stderr)
+        sys.exit(2)
+
+    from kymatio_amd import workloads
+    from kymatio_amd.torch import Scattering2D
+
+    dev = torch.device('cuda:0')
+    x = torch.from_numpy(workloads.grating_images(BATCH, SIDE, seed=SEED)).to(dev)
+    sc = Scattering2D(J, (SIDE, SIDE), L=L).to(dev)
+    if args.chunk:
+        sc.images_per_chunk = args.chunk
+    with torch.no_grad():
+        for _ in range(max(1, args.warmup)):
+            out = sc(x)
+        torch.cuda.synchronize()
+        plan = sc._plan(dev, torch.float32)
+        launches = plan.launches()
+        # timed region: K forwards, HIP events around every launch
+        plan.profile(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            out = sc(x)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        ms, cnt = plan.profile_read()
+        plan.profile(False)
+    assert out.shape == (BATCH, 217, 32, 32)
+    value = BATCH * args.steps / elapsed
+
+    # per launch: model bytes per launch = model bytes per image * images of the launch
+    chunk = sc._chunk(plan, BATCH)
+    per_kernel = []
+    for i, l in enumerate(launches):
+        if cnt[i] == 0:
+            continue
+        mean_ms = ms[i] / cnt[i]
+        imgs = BATCH * args.steps / cnt[i]  # images per launch
+        bytes_per_launch = l['model_bytes_per_image'] * imgs
+        per_kernel.append({'launch': i, 'kind': l['kind'], 'res': l['res'], 'nodes': l['nodes'],
+                           'total_ms': ms[i], 'launches': cnt[i], 'mean_us': 1e3 * mean_ms,
+                           'model_bytes_per_launch': bytes_per_launch,
+                           'gbps': bytes_per_launch / (mean_ms * 1e-3) / 1e9})
+    dom = max(per_kernel, key=lambda r: r['total_ms'])
+    achieved = dom['model_bytes_per_launch'] / (dom['mean_us'] * 1e-6)
+    total_model = sum(l['model_bytes_per_image'] for l in launches) * BATCH
+    gpu_ms = sum(r['total_ms'] for r in per_kernel) / args.steps
+    res = {
+        'metric': METRIC, 'value': value, 'unit': 'images/s', 'n_gpus': 1, 'steps': args.steps,
+        'warmup': args.warmup, 'ms_per_step': 1e3 * elapsed / args.steps, 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic',
+        'config': {'workload': 'Scattering2D J=3 L=8 max_order=2 on 256x256 float32 images: N(0,1) noise '
+                               'plus 3 random gratings; output (64, 217, 32, 32)',
+                   'batch': BATCH, 'J': J, 'L': L, 'shape': [SIDE, SIDE], 'images_per_chunk': chunk,
+                   'seed': SEED},
+        'roofline': {'bound': 'hbm', 'achieved': achieved / 1e9, 'peak': HBM_PEAK / 1e9, 'unit': 'GB/s',
+                     'frac': achieved / HBM_PEAK, 'traffic': None,
+                     'kernel': '%s (res %d, %d nodes/image)' % (dom['kind'], dom['res'], dom['nodes']),
+                     'mean_launch_us': dom['mean_us'], 'model_bytes_per_launch': dom['model_bytes_per_launch']},
+        'pipeline_roofline': {'model_bytes_per_image': total_model / BATCH,
+                              'achieved_GBps': total_model / (elapsed / args.steps) / 1e9,
+                              'frac': total_model / (elapsed / args.steps) / HBM_PEAK,
+                              'gpu_busy_ms_per_step': gpu_ms},
+        'kernels': [{k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()}
+                    for r in sorted(per_kernel, key=lambda r: -r['total_ms'])[:8]],
+    }
+    if not args.no_cpu_baseline:
+        res['cpu_baseline'] = cpu_baseline()
+    print(json.dumps(res))
+
+
+if __name__ == '__main__':
+    main()

@@ -1,0 +1,89 @@
+/* kymatio_hip.h -- C ABI of the MI355X scattering library (libkymatio_hip.so).
+ *
+ * Plain pointers and sizes only.  Device pointers are HIP device addresses
+ * (e.g. torch.Tensor.data_ptr() of a tensor on the current device); `stream`
+ * is a hipStream_t (torch.cuda.current_stream().cuda_stream).  Every call is
+ * stream-ordered and never synchronises the host.  All functions return a
+ * KymStatus code; kym_last_error() gives the message of the last failure on
+ * the calling thread.  Status codes map to the reference's exception types:
+ *   1 -> TypeError, 2 -> RuntimeError, 3 -> ValueError, 4 HIP failure,
+ *   5 unsupported configuration.
+ *
+ * Ownership: the caller owns inputs, outputs, filters and workspace (torch
+ * tensors); a plan owns its constant tables (twiddles, digit-reversal maps,
+ * node tree) on the device that was current when it was created.  A plan is
+ * immutable after creation, so calls on different streams may share it.
+ */
+#ifndef KYMATIO_HIP_H
+#define KYMATIO_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KYM_ABI_VERSION 1
+
+/* dtype codes */
+#define KYM_DTYPE_F32 0
+#define KYM_DTYPE_F64 1
+
+const char* kym_last_error(void);
+int kym_version(void);
+
+/* ---------------------------------------------------------------- 2D ----
+ * Replaces the core call scattering2d(x, pad, unpad, backend, J, L, phi, psi,
+ * max_order, out_type) made by ScatteringTorch2D.scattering
+ * (kymatio/scattering2d/frontend/torch_frontend.py:98-99; core in
+ * kymatio/scattering2d/core/scattering2d.py:1-88) with one fused pipeline:
+ * reflect pad, fft2, filter multiply + periodisation, ifft2, modulus, fft2,
+ * low-pass, ifft2, unpad, written directly into the stacked output.
+ *
+ * kym_scat2d_create: plan for input shape (M, N), scale J, L angles,
+ *   max_order 1|2, pre_pad 0|1 (kymatio/scattering2d/frontend/
+ *   torch_frontend.py:9-21; padding from base_frontend.py:19-32).
+ *   lds_limit caps the LDS bytes a workgroup may use (0 = device maximum);
+ *   smaller values force the multi-pass route and exist for testing.
+ *   Fails with status 2 and the reference message 'The smallest dimension
+ *   should be larger than 2^J.' (base_frontend.py:22-23).
+ */
+int kym_scat2d_create(int M, int N, int J, int L, int max_order, int pre_pad, int dtype,
+                      long long lds_limit, void** plan);
+int kym_scat2d_destroy(void* plan);
+
+/* info[11] = {C, out_h, out_w, M_padded, N_padded, n_filters, filter_elems,
+ *             workspace_bytes_per_image, n_launches, pad_top, pad_left} */
+int kym_scat2d_info(void* plan, long long* info);
+
+/* Packed filter layout: filter i (the reference's buffer tensor{i}: phi
+ * levels first, then each psi's levels in (j, theta) order,
+ * kymatio/scattering2d/frontend/torch_frontend.py:28-43) occupies elements
+ * [offs[i], offs[i+1]) of one contiguous real array of the plan's dtype. */
+int kym_scat2d_filter_offsets(void* plan, long long* offs);
+
+/* x: (batch, M, N) real (or padded size with pre_pad); filt: packed filters;
+ * out: (batch, C, out_h, out_w); ws: workspace of ws_bytes (any multiple of
+ * workspace_bytes_per_image >= 1 image; the batch is processed in chunks of
+ * ws_bytes / workspace_bytes_per_image images). */
+int kym_scat2d_forward(void* plan, const void* x, const void* filt, void* out, long long batch,
+                       void* ws, long long ws_bytes, void* stream);
+
+/* Launch list introspection: rec[7] = {kind, nodes, tasks_per_node, threads,
+ * lds_bytes, model_bytes_per_image, resolution}.  model_bytes is the
+ * algorithmic HBM traffic the launch accounts for per image under the
+ * stage-boundary byte model (each stage-crossing tensor written and read
+ * once); it feeds the roofline figure of the benchmark.  No counterpart in
+ * the reference (measurement support). */
+int kym_scat2d_launch_info(void* plan, int index, long long* rec);
+
+/* Per-launch HIP-event timing of subsequent kym_scat2d_forward calls on this
+ * plan (enable 1/0; enabling clears earlier records).  profile_read waits
+ * for the recorded events and returns, per launch index i < n, the summed
+ * milliseconds and the number of launches.  Not thread-safe; measurement
+ * support with no counterpart in the reference. */
+int kym_scat2d_profile(void* plan, int enable);
+int kym_scat2d_profile_read(void* plan, double* ms, long long* count, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,250 @@
+// C ABI of the scattering HIP library (declarations: include/kymatio_hip.h).
+#include <cstdio>
+#include <memory>
+#include <mutex>
+#include <string>
+
+#include "../../include/kymatio_hip.h"
+#include "kernels2d.h"
+#include "scat2d_plan.h"
+
+static thread_local std::string g_last_error;
+
+void kym_set_error(const std::string& msg) { g_last_error = msg; }
+
+static int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define KYM_HIP_TRY(expr)                                                                    \
+    do {                                                                                     \
+        hipError_t _e = (expr);                                                              \
+        if (_e != hipSuccess)                                                                \
+            return fail(KYM_ERR_HIP, std::string(#expr " failed: ") + hipGetErrorString(_e)); \
+    } while (0)
+
+extern "C" const char* kym_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" int kym_version(void) { return KYM_ABI_VERSION; }
+
+static size_t device_lds_limit() {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 65536;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeSharedMemPerBlockOptin, dev) == hipSuccess && v > 0)
+        return (size_t)v;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+        return (size_t)v;
+    return 65536;
+}
+
+// ----------------------------------------------------------------- 2D plans
+// Optional per-launch timing with HIP events (bench / profiling only).
+struct LaunchProfile {
+    bool enabled = false;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> recs;  // launch index, events
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    hipEvent_t get() {
+        if (used == pool.size()) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            pool.push_back(e);
+        }
+        return pool[used++];
+    }
+    void reset() { recs.clear(); used = 0; }
+    ~LaunchProfile() {
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+struct KymScat2D {
+    int dtype = KYM_F32;
+    int device = 0;
+    void* dblob = nullptr;
+    std::unique_ptr<Scat2DPlan<float>> pf;
+    std::unique_ptr<Scat2DPlan<double>> pd;
+    LaunchProfile prof;
+};
+
+template <class T>
+static int upload_and_prepare(Scat2DPlan<T>& p, void** dblob) {
+    const size_t bytes = p.blob.bytes.size();
+    KYM_HIP_TRY(hipMalloc(dblob, bytes));
+    p.blob.relocate((uintptr_t)*dblob);
+    KYM_HIP_TRY(hipMemcpy(*dblob, p.blob.bytes.data(), bytes, hipMemcpyHostToDevice));
+    for (const Launch2D& l : p.launches) {
+        Scat2DKernel<T> k = pick_scat2d<T>(l.kind, l.threads, p.pclass);
+        if (l.lds_bytes > 65536)
+            KYM_HIP_TRY(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            (int)l.lds_bytes));
+    }
+    return KYM_OK;
+}
+
+extern "C" int kym_scat2d_create(int M, int N, int J, int L, int max_order, int pre_pad, int dtype,
+                                 long long lds_limit, void** out) {
+    if (!out) return fail(KYM_ERR_VALUE, "null plan pointer");
+    *out = nullptr;
+    if (dtype != KYM_F32 && dtype != KYM_F64) return fail(KYM_ERR_TYPE, "dtype must be float32 or float64");
+    auto h = std::make_unique<KymScat2D>();
+    h->dtype = dtype;
+    KYM_HIP_TRY(hipGetDevice(&h->device));
+    size_t lim = device_lds_limit();
+    if (lds_limit > 0 && (size_t)lds_limit < lim) lim = (size_t)lds_limit;
+    KymError e = {KYM_OK, ""};
+    int rc;
+    if (dtype == KYM_F32) {
+        h->pf = std::make_unique<Scat2DPlan<float>>();
+        if (!h->pf->build(M, N, J, L, max_order, pre_pad, lim, &e)) return fail(e.code, e.msg);
+        rc = upload_and_prepare(*h->pf, &h->dblob);
+    } else {
+        h->pd = std::make_unique<Scat2DPlan<double>>();
+        if (!h->pd->build(M, N, J, L, max_order, pre_pad, lim, &e)) return fail(e.code, e.msg);
+        rc = upload_and_prepare(*h->pd, &h->dblob);
+    }
+    if (rc != KYM_OK) {
+        if (h->dblob) (void)hipFree(h->dblob);
+        return rc;
+    }
+    *out = h.release();
+    return KYM_OK;
+}
+
+extern "C" int kym_scat2d_destroy(void* plan) {
+    if (!plan) return KYM_OK;
+    KymScat2D* h = static_cast<KymScat2D*>(plan);
+    if (h->dblob) {
+        int cur = 0;
+        (void)hipGetDevice(&cur);
+        (void)hipSetDevice(h->device);
+        (void)hipFree(h->dblob);
+        (void)hipSetDevice(cur);
+    }
+    delete h;
+    return KYM_OK;
+}
+
+template <class T>
+static void fill_info(const Scat2DPlan<T>& p, long long* info) {
+    info[0] = p.C;
+    info[1] = p.oh;
+    info[2] = p.ow;
+    info[3] = p.Mp;
+    info[4] = p.Np;
+    info[5] = (long long)p.filt_off.size() - 1;
+    info[6] = p.filt_off.back();
+    info[7] = p.ws_img * (long long)sizeof(cpx<T>);
+    info[8] = (long long)p.launches.size();
+    info[9] = p.top;
+    info[10] = p.left;
+}
+
+extern "C" int kym_scat2d_info(void* plan, long long* info) {
+    if (!plan || !info) return fail(KYM_ERR_VALUE, "null argument");
+    KymScat2D* h = static_cast<KymScat2D*>(plan);
+    if (h->pf) fill_info(*h->pf, info);
+    else fill_info(*h->pd, info);
+    return KYM_OK;
+}
+
+extern "C" int kym_scat2d_filter_offsets(void* plan, long long* offs) {
+    if (!plan || !offs) return fail(KYM_ERR_VALUE, "null argument");
+    KymScat2D* h = static_cast<KymScat2D*>(plan);
+    const std::vector<long long>& v = h->pf ? h->pf->filt_off : h->pd->filt_off;
+    for (size_t i = 0; i < v.size(); ++i) offs[i] = v[i];
+    return KYM_OK;
+}
+
+template <class T>
+static int forward2d(const Scat2DPlan<T>& p, const void* x, const void* filt, void* out, long long batch,
+                     void* ws, long long ws_bytes, hipStream_t stream, uintptr_t base, LaunchProfile* prof) {
+    const long long per = p.ws_img * (long long)sizeof(cpx<T>);
+    long long chunk = per > 0 ? ws_bytes / per : batch;
+    if (chunk < 1) return fail(KYM_ERR_VALUE, "workspace too small for one image");
+    if (chunk > batch) chunk = batch;
+    for (long long i0 = 0; i0 < batch; i0 += chunk) {
+        const int nimg = (int)std::min(chunk, batch - i0);
+        for (const Launch2D& l : p.launches) {
+            Scat2DArgs<T> a = p.args(l, base);
+            a.x = static_cast<const T*>(x);
+            a.filt = static_cast<const T*>(filt);
+            a.ws = static_cast<cpx<T>*>(ws);
+            a.out = static_cast<T*>(out);
+            a.img0 = (int)i0;
+            const long long ntask = (long long)nimg * l.ngroup * l.nblk;
+            if (ntask > (1ll << 30)) return fail(KYM_ERR_VALUE, "too many tasks in one launch");
+            a.ntask = (int)ntask;
+            const unsigned grid = (unsigned)((ntask + 7) / 8 * 8);
+            Scat2DKernel<T> k = pick_scat2d<T>(l.kind, l.threads, p.pclass);
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (prof && prof->enabled) {
+                e0 = prof->get();
+                e1 = prof->get();
+                if (e0) KYM_HIP_TRY(hipEventRecord(e0, stream));
+            }
+            hipLaunchKernelGGL(k, dim3(grid), dim3(l.threads), l.lds_bytes, stream, a);
+            KYM_HIP_TRY(hipGetLastError());
+            if (e0 && e1) {
+                KYM_HIP_TRY(hipEventRecord(e1, stream));
+                prof->recs.push_back({(int)(&l - p.launches.data()), {e0, e1}});
+            }
+        }
+    }
+    return KYM_OK;
+}
+
+extern "C" int kym_scat2d_forward(void* plan, const void* x, const void* filt, void* out, long long batch,
+                                  void* ws, long long ws_bytes, void* stream) {
+    if (!plan) return fail(KYM_ERR_VALUE, "null plan");
+    if (batch < 0) return fail(KYM_ERR_VALUE, "negative batch");
+    if (batch == 0) return KYM_OK;
+    if (!x || !filt || !out || !ws) return fail(KYM_ERR_VALUE, "null device pointer");
+    KymScat2D* h = static_cast<KymScat2D*>(plan);
+    int cur = 0;
+    KYM_HIP_TRY(hipGetDevice(&cur));
+    if (cur != h->device) return fail(KYM_ERR_TYPE, "plan was created for another device");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (h->pf) return forward2d(*h->pf, x, filt, out, batch, ws, ws_bytes, s, (uintptr_t)h->dblob, &h->prof);
+    return forward2d(*h->pd, x, filt, out, batch, ws, ws_bytes, s, (uintptr_t)h->dblob, &h->prof);
+}
+
+template <class T>
+static void launch_rec(const Scat2DPlan<T>& p, int i, long long* rec) {
+    const Launch2D& l = p.launches[i];
+    rec[0] = l.kind; rec[1] = l.ngroup; rec[2] = l.nblk; rec[3] = l.threads;
+    rec[4] = (long long)l.lds_bytes; rec[5] = l.model_bytes; rec[6] = l.res;
+}
+
+extern "C" int kym_scat2d_launch_info(void* plan, int i, long long* rec) {
+    if (!plan || !rec) return fail(KYM_ERR_VALUE, "null argument");
+    KymScat2D* h = static_cast<KymScat2D*>(plan);
+    const int n = h->pf ? (int)h->pf->launches.size() : (int)h->pd->launches.size();
+    if (i < 0 || i >= n) return fail(KYM_ERR_VALUE, "launch index out of range");
+    if (h->pf) launch_rec(*h->pf, i, rec);
+    else launch_rec(*h->pd, i, rec);
+    return KYM_OK;
+}
+
+extern "C" int kym_scat2d_profile(void* plan, int enable) {
+    if (!plan) return fail(KYM_ERR_VALUE, "null plan");
+    KymScat2D* h = static_cast<KymScat2D*>(plan);
+    h->prof.enabled = enable != 0;
+    h->prof.reset();
+    return KYM_OK;
+}
+
+extern "C" int kym_scat2d_profile_read(void* plan, double* ms, long long* count, int n) {
+    if (!plan || !ms || !count) return fail(KYM_ERR_VALUE, "null argument");
+    KymScat2D* h = static_cast<KymScat2D*>(plan);
+    for (int i = 0; i < n; ++i) { ms[i] = 0.0; count[i] = 0; }
+    for (auto& r : h->prof.recs) {
+        KYM_HIP_TRY(hipEventSynchronize(r.second.second));
+        float t = 0.f;
+        KYM_HIP_TRY(hipEventElapsedTime(&t, r.second.first, r.second.second));
+        if (r.first >= 0 && r.first < n) { ms[r.first] += t; count[r.first] += 1; }
+    }
+    h->prof.reset();
+    return KYM_OK;
+}

@@ -1,0 +1,58 @@
+// Shared types for the scattering HIP library (gfx950).
+//
+// Kernel bodies are written as workgroup functions of (tid, nthreads): every
+// parallel loop is `for (i = tid; i < n; i += nthreads)` followed by a
+// workgroup barrier, and every butterfly reads and writes only its own
+// elements.  The same body therefore gives the same result when a CPU test
+// harness runs it with one thread (tid = 0, nthreads = 1), which is how the
+// index arithmetic is checked without a GPU.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define KYM_HD __host__ __device__
+#define KYM_INL __forceinline__
+
+template <class T>
+struct alignas(2 * sizeof(T)) cpx {
+    T re, im;
+};
+
+template <class T> KYM_HD KYM_INL cpx<T> operator+(cpx<T> a, cpx<T> b) { return {a.re + b.re, a.im + b.im}; }
+template <class T> KYM_HD KYM_INL cpx<T> operator-(cpx<T> a, cpx<T> b) { return {a.re - b.re, a.im - b.im}; }
+template <class T> KYM_HD KYM_INL cpx<T> operator*(cpx<T> a, T s) { return {a.re * s, a.im * s}; }
+// a * b
+template <class T> KYM_HD KYM_INL cpx<T> cmul(cpx<T> a, cpx<T> b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+// a * conj(b)
+template <class T> KYM_HD KYM_INL cpx<T> cmulc(cpx<T> a, cpx<T> b) {
+    return {a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im};
+}
+// multiply by -i (forward transform sign) or +i (inverse)
+template <bool INV, class T> KYM_HD KYM_INL cpx<T> mul_mi(cpx<T> a) {
+    if (INV) return {-a.im, a.re};
+    return {a.im, -a.re};
+}
+
+// Workgroup barrier; a no-op when a body runs on the host with one thread.
+KYM_HD KYM_INL void wg_barrier() {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __syncthreads();
+#endif
+}
+
+template <class T> KYM_HD KYM_INL T kym_sqrt(T v);
+template <> KYM_HD KYM_INL float kym_sqrt<float>(float v) { return sqrtf(v); }
+template <> KYM_HD KYM_INL double kym_sqrt<double>(double v) { return sqrt(v); }
+
+// numpy-style 'reflect' index (edge sample not repeated), any pad width.
+KYM_HD KYM_INL int reflect_index(int i, int n) {
+    if (n == 1) return 0;
+    const int period = 2 * (n - 1);
+    i %= period;
+    if (i < 0) i += period;
+    return i < n ? i : period - i;
+}
+
+enum KymDtype { KYM_F32 = 0, KYM_F64 = 1 };

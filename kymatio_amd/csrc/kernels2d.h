@@ -1,0 +1,54 @@
+// __global__ wrappers around the 2D workgroup bodies.
+#pragma once
+#include "scat2d_body.h"
+
+// Workgroups are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8).
+// Give each XCD a contiguous range of tasks so that all consumers of one
+// image's parent spectrum run on one XCD and share its L2.  The grid is a
+// multiple of 8; tasks >= ntask exit before any barrier.
+__device__ KYM_INL int xcd_task(int bid, int nblocks) {
+    const int per = nblocks >> 3;
+    return (bid & 7) * per + (bid >> 3);
+}
+
+template <class T, int KIND, int NT, int MAXP>
+__global__ void __launch_bounds__(NT) k_scat2d(Scat2DArgs<T> a) {
+    extern __shared__ __attribute__((aligned(16))) char kym_smem2d[];
+    const int task = xcd_task(blockIdx.x, gridDim.x);
+    if (task >= a.ntask) return;
+    cpx<T>* lds = reinterpret_cast<cpx<T>*>(kym_smem2d);
+    dispatch_body2d<T, MAXP>(KIND, a, task, lds, threadIdx.x, NT);
+}
+
+template <class T>
+using Scat2DKernel = void (*)(Scat2DArgs<T>);
+
+template <class T, int KIND, int MAXP>
+Scat2DKernel<T> pick_scat2d_nt(int threads) {
+    if (threads >= 1024) return k_scat2d<T, KIND, 1024, MAXP>;
+    if (threads >= 512) return k_scat2d<T, KIND, 512, MAXP>;
+    return k_scat2d<T, KIND, 256, MAXP>;
+}
+
+template <class T, int MAXP>
+Scat2DKernel<T> pick_scat2d_p(int kind, int threads) {
+    switch (kind) {
+        case K2_LDS_NODE: return pick_scat2d_nt<T, K2_LDS_NODE, MAXP>(threads);
+        case K2_LEAF: return pick_scat2d_nt<T, K2_LEAF, MAXP>(threads);
+        case K2_ROOT_ROWS: return k_scat2d<T, K2_ROOT_ROWS, 256, MAXP>;
+        case K2_ICOL: return k_scat2d<T, K2_ICOL, 256, MAXP>;
+        case K2_ROWS_MOD: return k_scat2d<T, K2_ROWS_MOD, 256, MAXP>;
+        case K2_FWD_COLS: return k_scat2d<T, K2_FWD_COLS, 256, MAXP>;
+        case K2_LEAF_ICOL: return k_scat2d<T, K2_LEAF_ICOL, 256, MAXP>;
+        case K2_LEAF_ROWS: return k_scat2d<T, K2_LEAF_ROWS, 256, MAXP>;
+        default: return nullptr;
+    }
+}
+
+// pclass: 7, 17 or 31 (kym_prime_class of the plan's largest odd prime)
+template <class T>
+Scat2DKernel<T> pick_scat2d(int kind, int threads, int pclass) {
+    if (pclass <= 7) return pick_scat2d_p<T, 7>(kind, threads);
+    if (pclass <= 17) return pick_scat2d_p<T, 17>(kind, threads);
+    return pick_scat2d_p<T, 31>(kind, threads);
+}

@@ -1,0 +1,329 @@
+// Scattering2D: node tree, kernel arguments and workgroup bodies.
+//
+// The reference's path (kymatio/scattering2d/core/scattering2d.py:1-88) is a
+// tree of "nodes".  A node is a complex spectrum at resolution r
+// (H_r x W_r = P_M/2^r x P_N/2^r):
+//   root      U_0 = fft2(pad(x))                               r = 0
+//   order 1   U_1 = fft2(|ifft2(P_k(U_0 * psi[n1].levels[0]))|)  r = j1, k = 2^j1
+//   order 2   U_2 = fft2(|ifft2(P_k(U_1 * psi[n2].levels[j1]))|) r = j2, k = 2^(j2-j1)
+// where P_k is the Fourier periodisation (mean of the k x k aliases,
+// kymatio/scattering2d/backend/torch_backend.py:92-129).  Every node has one
+// low-pass leaf S = unpad(Re(ifft2(P_kl(U * phi.levels[r])))), kl = 2^(J-r),
+// written straight into output channel `leaf_ch` of (batch, C, H_J-2, W_J-2).
+//
+// Consumers pull: a child gathers its periodised input from the parent's
+// spectrum (kept in a workspace slot, usually resident in L2/MALL) while
+// loading, so the filter multiply and the periodisation are fused into the
+// load of the next transform.  Nodes that fit in LDS run as one workgroup
+// (gather, inverse FFT, modulus, forward FFT, leaf); bigger ones run as
+// column / row / column passes through their workspace slot.
+#pragma once
+#include "fft_core.h"
+
+#define KYM_MAX_RES 16
+
+struct Node2D {
+    int order;      // 0 root, 1, 2
+    int res;        // resolution index r
+    int parent;     // node index, -1 for the root
+    int filt;       // packed-filter index of the wavelet at the parent's resolution
+    int k;          // periodisation factor from the parent (1 = none)
+    int leaf_ch;    // output channel of this node's low-pass leaf
+    int leaf_filt;  // packed-filter index of phi.levels[res]
+    int leaf_k;     // 2^(J - res)
+    int leaf_mode;  // 0 in the node kernel, 1 separate small-leaf kernel, 2 big-leaf passes
+    int big;        // 1 = does not fit in LDS (column/row passes)
+    long long slot;       // offset (complex elements) of U in the per-image workspace, -1 none
+    long long leaf_slot;  // scratch for big leaves, -1 none
+};
+
+template <class T>
+struct Scat2DArgs {
+    const Node2D* nodes;
+    const FftLen<T>* lens;
+    const long long* filt_off;  // element offsets into `filt`
+    const int* group;           // node ids of this launch
+    int ngroup, nblk, blk;      // nodes per image, tasks per node, rows/cols per task
+    int lenH[KYM_MAX_RES], lenW[KYM_MAX_RES], H[KYM_MAX_RES], W[KYM_MAX_RES];
+    int J;
+    // input image(s): (batch, M, N); with pre_pad, M x N is the padded size
+    const T* x;
+    int M, N, top, left, pre_pad;
+    const T* filt;
+    // workspace: image i of the chunk starts at ws + i * ws_img
+    cpx<T>* ws;
+    long long ws_img;
+    // output (batch, C, oh, ow)
+    T* out;
+    int C, oh, ow;
+    int img0;   // first image of this chunk
+    int ntask;  // tasks in this launch
+};
+
+// ------------------------------------------------------------------ helpers
+template <class T>
+KYM_HD KYM_INL cpx<T> gather_periodized(const cpx<T>* U, const T* f, int Wp, int H, int W, int k, int y,
+                                        int x) {
+    cpx<T> acc = {T(0), T(0)};
+    for (int a = 0; a < k; ++a) {
+        const long long row = (long long)(y + a * H) * Wp + x;
+        for (int b = 0; b < k; ++b) {
+            const long long idx = row + b * W;
+            const T fv = f[idx];
+            const cpx<T> u = U[idx];
+            acc.re += u.re * fv;
+            acc.im += u.im * fv;
+        }
+    }
+    return acc;
+}
+
+template <class T>
+KYM_HD KYM_INL T input_pixel(const Scat2DArgs<T>& a, const T* xin, int y, int x) {
+    if (a.pre_pad) return xin[(long long)y * a.N + x];
+    return xin[(long long)reflect_index(y - a.top, a.M) * a.N + reflect_index(x - a.left, a.N)];
+}
+
+// Leaf: periodise U (H x W, natural, at `U`, stride W) times phi into `B`
+// (HJ x WJ), inverse FFT, real part, unpad, store into the output channel.
+template <class T, int MAXP>
+KYM_HD void leaf_body(const Scat2DArgs<T>& a, const cpx<T>* U, const T* phi, int H, int W, int kk, int ch,
+                      int img, cpx<T>* B, int tid, int nth) {
+    const int J = a.J, HJ = a.H[J], WJ = a.W[J];
+    const T inv_k2 = T(1) / T(kk * kk);
+    for (int i = tid; i < HJ * WJ; i += nth) {
+        const int y = i / WJ, x = i - (i / WJ) * WJ;
+        B[i] = gather_periodized(U, phi, W, HJ, WJ, kk, y, x) * inv_k2;
+    }
+    wg_barrier();
+    const FftLen<T>& fH = a.lens[a.lenH[J]];
+    const FftLen<T>& fW = a.lens[a.lenW[J]];
+    fft_dif<true, MAXP>(B, fW, Batch{HJ, 1, WJ, false}, tid, nth);
+    fft_dif<true, MAXP>(B, fH, Batch{WJ, WJ, 1, true}, tid, nth);
+    const T inv_n = T(1) / T(HJ * WJ);
+    T* o = a.out + ((long long)img * a.C + ch) * a.oh * a.ow;
+    for (int p = tid; p < HJ * WJ; p += nth) {
+        const int py = p / WJ, px = p - py * WJ;
+        const int y = fH.rev[py], x = fW.rev[px];
+        if (y >= 1 && y <= HJ - 2 && x >= 1 && x <= WJ - 2) o[(y - 1) * a.ow + (x - 1)] = B[p].re * inv_n;
+    }
+    wg_barrier();
+}
+
+// ------------------------------------------------------------------ bodies
+// A whole node in LDS.  task -> (image of chunk, node of group).
+template <class T, int MAXP>
+KYM_HD void body_lds_node(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
+    const int il = task / a.ngroup, gi = task - il * a.ngroup;
+    const int img = a.img0 + il;
+    const Node2D& nd = a.nodes[a.group[gi]];
+    const int r = nd.res, H = a.H[r], W = a.W[r];
+    const FftLen<T>& fH = a.lens[a.lenH[r]];
+    const FftLen<T>& fW = a.lens[a.lenW[r]];
+    cpx<T>* wsi = a.ws + il * a.ws_img;
+    if (nd.order == 0) {
+        const T* xin = a.x + (long long)img * a.M * a.N;
+        for (int i = tid; i < H * W; i += nth) {
+            const int py = i / W, px = i - py * W;
+            lds[i] = {input_pixel(a, xin, fH.rev[py], fW.rev[px]), T(0)};
+        }
+        wg_barrier();
+        fft_dit<false, MAXP>(lds, fW, Batch{H, 1, W, false}, tid, nth);
+        fft_dit<false, MAXP>(lds, fH, Batch{W, W, 1, true}, tid, nth);
+    } else {
+        const Node2D& pn = a.nodes[nd.parent];
+        const cpx<T>* U = wsi + pn.slot;
+        const T* f = a.filt + a.filt_off[nd.filt];
+        const int Wp = a.W[pn.res];
+        const T inv_k2 = T(1) / T(nd.k * nd.k);
+        for (int i = tid; i < H * W; i += nth) {
+            const int y = i / W, x = i - (i / W) * W;
+            lds[i] = gather_periodized(U, f, Wp, H, W, nd.k, y, x) * inv_k2;
+        }
+        wg_barrier();
+        fft_dif<true, MAXP>(lds, fW, Batch{H, 1, W, false}, tid, nth);
+        fft_dif<true, MAXP>(lds, fH, Batch{W, W, 1, true}, tid, nth);
+        const T inv_n = T(1) / T(H * W);
+        for (int i = tid; i < H * W; i += nth) {
+            const cpx<T> v = lds[i];
+            lds[i] = {kym_sqrt(v.re * v.re + v.im * v.im) * inv_n, T(0)};
+        }
+        wg_barrier();
+        fft_dit<false, MAXP>(lds, fW, Batch{H, 1, W, false}, tid, nth);
+        fft_dit<false, MAXP>(lds, fH, Batch{W, W, 1, true}, tid, nth);
+    }
+    if (nd.slot >= 0) {
+        cpx<T>* dst = wsi + nd.slot;
+        for (int i = tid; i < H * W; i += nth) dst[i] = lds[i];
+    }
+    if (nd.leaf_mode == 0) {
+        const T* phi = a.filt + a.filt_off[nd.leaf_filt];
+        leaf_body<T, MAXP>(a, lds, phi, H, W, nd.leaf_k, nd.leaf_ch, img, lds + H * W, tid, nth);
+    }
+}
+
+// Leaf from a spectrum kept in the workspace.  task -> (image, node).
+template <class T, int MAXP>
+KYM_HD void body_leaf(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
+    const int il = task / a.ngroup, gi = task - il * a.ngroup;
+    const Node2D& nd = a.nodes[a.group[gi]];
+    const int r = nd.res;
+    const cpx<T>* U = a.ws + il * a.ws_img + nd.slot;
+    const T* phi = a.filt + a.filt_off[nd.leaf_filt];
+    leaf_body<T, MAXP>(a, U, phi, a.H[r], a.W[r], nd.leaf_k, nd.leaf_ch, a.img0 + il, lds, tid, nth);
+}
+
+// Root row pass: rows of the padded image (row storage positions
+// digit-reversed), forward DIT along x.  task -> (image, row block).
+template <class T, int MAXP>
+KYM_HD void body_root_rows(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
+    const int il = task / a.nblk, rb = task - il * a.nblk;
+    const int img = a.img0 + il;
+    const Node2D& nd = a.nodes[0];
+    const int H = a.H[0], W = a.W[0];
+    const FftLen<T>& fH = a.lens[a.lenH[0]];
+    const FftLen<T>& fW = a.lens[a.lenW[0]];
+    const int r0 = rb * a.blk, nr = (H - r0 < a.blk) ? H - r0 : a.blk;
+    const T* xin = a.x + (long long)img * a.M * a.N;
+    for (int i = tid; i < nr * W; i += nth) {
+        const int rr = i / W, px = i - rr * W;
+        lds[i] = {input_pixel(a, xin, fH.rev[r0 + rr], fW.rev[px]), T(0)};
+    }
+    wg_barrier();
+    fft_dit<false, MAXP>(lds, fW, Batch{nr, 1, W, false}, tid, nth);
+    cpx<T>* dst = a.ws + il * a.ws_img + nd.slot + (long long)r0 * W;
+    for (int i = tid; i < nr * W; i += nth) dst[i] = lds[i];
+}
+
+// Gather + inverse column DIF.  Shared by big nodes (filter psi, slot) and
+// big leaves (filter phi, leaf_slot).  task -> (image, node, column block).
+template <class T, int MAXP>
+KYM_HD void icol_common(const Scat2DArgs<T>& a, const cpx<T>* U, const T* f, int Wp, int H, int W, int k,
+                        const FftLen<T>& fH, cpx<T>* dst, int c0, int nc, cpx<T>* lds, int tid, int nth) {
+    const T inv_k2 = T(1) / T(k * k);
+    for (int i = tid; i < H * nc; i += nth) {
+        const int y = i / nc, c = i - (i / nc) * nc;
+        lds[i] = gather_periodized(U, f, Wp, H, W, k, y, c0 + c) * inv_k2;
+    }
+    wg_barrier();
+    fft_dif<true, MAXP>(lds, fH, Batch{nc, nc, 1, true}, tid, nth);
+    for (int i = tid; i < H * nc; i += nth) {
+        const int p = i / nc, c = i - (i / nc) * nc;
+        dst[(long long)p * W + c0 + c] = lds[i];
+    }
+}
+
+template <class T, int MAXP>
+KYM_HD void body_icol(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
+    const int per = a.ngroup * a.nblk;
+    const int il = task / per, rem = task - il * per;
+    const int gi = rem / a.nblk, cb = rem - gi * a.nblk;
+    const Node2D& nd = a.nodes[a.group[gi]];
+    const Node2D& pn = a.nodes[nd.parent];
+    const int r = nd.res, H = a.H[r], W = a.W[r];
+    const int c0 = cb * a.blk, nc = (W - c0 < a.blk) ? W - c0 : a.blk;
+    cpx<T>* wsi = a.ws + il * a.ws_img;
+    icol_common<T, MAXP>(a, wsi + pn.slot, a.filt + a.filt_off[nd.filt], a.W[pn.res], H, W, nd.k, a.lens[a.lenH[r]],
+                wsi + nd.slot, c0, nc, lds, tid, nth);
+}
+
+template <class T, int MAXP>
+KYM_HD void body_leaf_icol(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
+    const int per = a.ngroup * a.nblk;
+    const int il = task / per, rem = task - il * per;
+    const int gi = rem / a.nblk, cb = rem - gi * a.nblk;
+    const Node2D& nd = a.nodes[a.group[gi]];
+    const int J = a.J, HJ = a.H[J], WJ = a.W[J];
+    const int c0 = cb * a.blk, nc = (WJ - c0 < a.blk) ? WJ - c0 : a.blk;
+    cpx<T>* wsi = a.ws + il * a.ws_img;
+    icol_common<T, MAXP>(a, wsi + nd.slot, a.filt + a.filt_off[nd.leaf_filt], a.W[nd.res], HJ, WJ, nd.leaf_k,
+                a.lens[a.lenH[J]], wsi + nd.leaf_slot, c0, nc, lds, tid, nth);
+}
+
+// Rows: inverse DIF along x, modulus, forward DIT along x (in place).
+template <class T, int MAXP>
+KYM_HD void body_rows_mod(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
+    const int per = a.ngroup * a.nblk;
+    const int il = task / per, rem = task - il * per;
+    const int gi = rem / a.nblk, rb = rem - gi * a.nblk;
+    const Node2D& nd = a.nodes[a.group[gi]];
+    const int r = nd.res, H = a.H[r], W = a.W[r];
+    const FftLen<T>& fW = a.lens[a.lenW[r]];
+    const int r0 = rb * a.blk, nr = (H - r0 < a.blk) ? H - r0 : a.blk;
+    cpx<T>* buf = a.ws + il * a.ws_img + nd.slot + (long long)r0 * W;
+    for (int i = tid; i < nr * W; i += nth) lds[i] = buf[i];
+    wg_barrier();
+    fft_dif<true, MAXP>(lds, fW, Batch{nr, 1, W, false}, tid, nth);
+    const T inv_n = T(1) / T(H * W);
+    for (int i = tid; i < nr * W; i += nth) {
+        const cpx<T> v = lds[i];
+        lds[i] = {kym_sqrt(v.re * v.re + v.im * v.im) * inv_n, T(0)};
+    }
+    wg_barrier();
+    fft_dit<false, MAXP>(lds, fW, Batch{nr, 1, W, false}, tid, nth);
+    for (int i = tid; i < nr * W; i += nth) buf[i] = lds[i];
+}
+
+// Rows of a big leaf: inverse DIF along x, real part, unpad, store.
+template <class T, int MAXP>
+KYM_HD void body_leaf_rows(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
+    const int per = a.ngroup * a.nblk;
+    const int il = task / per, rem = task - il * per;
+    const int gi = rem / a.nblk, rb = rem - gi * a.nblk;
+    const Node2D& nd = a.nodes[a.group[gi]];
+    const int J = a.J, HJ = a.H[J], WJ = a.W[J];
+    const FftLen<T>& fH = a.lens[a.lenH[J]];
+    const FftLen<T>& fW = a.lens[a.lenW[J]];
+    const int r0 = rb * a.blk, nr = (HJ - r0 < a.blk) ? HJ - r0 : a.blk;
+    const cpx<T>* buf = a.ws + il * a.ws_img + nd.leaf_slot + (long long)r0 * WJ;
+    for (int i = tid; i < nr * WJ; i += nth) lds[i] = buf[i];
+    wg_barrier();
+    fft_dif<true, MAXP>(lds, fW, Batch{nr, 1, WJ, false}, tid, nth);
+    const T inv_n = T(1) / T(HJ * WJ);
+    T* o = a.out + ((long long)(a.img0 + il) * a.C + nd.leaf_ch) * a.oh * a.ow;
+    for (int i = tid; i < nr * WJ; i += nth) {
+        const int rr = i / WJ, px = i - rr * WJ;
+        const int y = fH.rev[r0 + rr], x = fW.rev[px];
+        if (y >= 1 && y <= HJ - 2 && x >= 1 && x <= WJ - 2) o[(y - 1) * a.ow + (x - 1)] = lds[i].re * inv_n;
+    }
+}
+
+// Forward DIT along y over column blocks (rows stored digit-reversed) -> natural.
+template <class T, int MAXP>
+KYM_HD void body_fwd_cols(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
+    const int per = a.ngroup * a.nblk;
+    const int il = task / per, rem = task - il * per;
+    const int gi = rem / a.nblk, cb = rem - gi * a.nblk;
+    const Node2D& nd = a.nodes[a.group[gi]];
+    const int r = nd.res, H = a.H[r], W = a.W[r];
+    const FftLen<T>& fH = a.lens[a.lenH[r]];
+    const int c0 = cb * a.blk, nc = (W - c0 < a.blk) ? W - c0 : a.blk;
+    cpx<T>* buf = a.ws + il * a.ws_img + nd.slot + c0;
+    for (int i = tid; i < H * nc; i += nth) {
+        const int y = i / nc, c = i - y * nc;
+        lds[i] = buf[(long long)y * W + c];
+    }
+    wg_barrier();
+    fft_dit<false, MAXP>(lds, fH, Batch{nc, nc, 1, true}, tid, nth);
+    for (int i = tid; i < H * nc; i += nth) {
+        const int y = i / nc, c = i - y * nc;
+        buf[(long long)y * W + c] = lds[i];
+    }
+}
+
+enum Kind2D { K2_LDS_NODE = 0, K2_LEAF, K2_ROOT_ROWS, K2_ICOL, K2_ROWS_MOD, K2_FWD_COLS, K2_LEAF_ICOL, K2_LEAF_ROWS };
+
+template <class T, int MAXP>
+KYM_HD KYM_INL void dispatch_body2d(int kind, const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
+    switch (kind) {
+        case K2_LDS_NODE: body_lds_node<T, MAXP>(a, task, lds, tid, nth); break;
+        case K2_LEAF: body_leaf<T, MAXP>(a, task, lds, tid, nth); break;
+        case K2_ROOT_ROWS: body_root_rows<T, MAXP>(a, task, lds, tid, nth); break;
+        case K2_ICOL: body_icol<T, MAXP>(a, task, lds, tid, nth); break;
+        case K2_ROWS_MOD: body_rows_mod<T, MAXP>(a, task, lds, tid, nth); break;
+        case K2_FWD_COLS: body_fwd_cols<T, MAXP>(a, task, lds, tid, nth); break;
+        case K2_LEAF_ICOL: body_leaf_icol<T, MAXP>(a, task, lds, tid, nth); break;
+        case K2_LEAF_ROWS: body_leaf_rows<T, MAXP>(a, task, lds, tid, nth); break;
+        default: break;
+    }
+}

@@ -1,0 +1,270 @@
+// Scattering2D plan: node tree, packed-filter layout, workspace layout and
+// the launch list, built once on the host (kymatio/scattering2d/frontend/
+// base_frontend.py:19-36 for the padding, filter_bank.py:5-53 for the
+// filter order, core/scattering2d.py:1-88 for the path order).
+#pragma once
+#include <algorithm>
+
+#include "plan_common.h"
+#include "scat2d_body.h"
+
+struct Launch2D {
+    int kind;
+    size_t group_off;  // byte offset of the node-id list in the blob
+    int ngroup, nblk, blk;
+    int threads;
+    size_t lds_bytes;
+    int res;                // resolution of the launch's nodes
+    long long model_bytes;  // algorithmic HBM bytes per image (stage-boundary model)
+};
+
+template <class T>
+struct Scat2DPlan {
+    // configuration
+    int M = 0, N = 0, J = 0, L = 0, max_order = 2, pre_pad = 0;
+    int Mp = 0, Np = 0, top = 0, left = 0;
+    int H[KYM_MAX_RES] = {0}, W[KYM_MAX_RES] = {0};
+    int C = 0, oh = 0, ow = 0;
+    size_t lds_limit = 0;
+    // filters (reference buffer order: phi levels, then psi (j, theta) levels)
+    std::vector<long long> filt_off;  // nfilt + 1 entries
+    std::vector<int> filt_h, filt_w;
+    // tree
+    std::vector<Node2D> nodes;
+    long long ws_img = 0;  // complex elements of workspace per image
+    std::vector<Launch2D> launches;
+    LenTable<T> lens;
+    int lenH[KYM_MAX_RES] = {0}, lenW[KYM_MAX_RES] = {0};
+    // constants blob (relocated to its device or host home)
+    Blob blob;
+    size_t off_nodes = 0, off_lens = 0, off_filt = 0;
+    size_t max_lds = 0;
+    int max_threads = 256;
+    int pclass = 7;  // prime class of the kernels (largest odd radix family)
+
+    int phi_index(int r) const { return r; }
+    int psi_index(int n1, int level) const { return psi_base[n1] + level; }
+    std::vector<int> psi_base, psi_j, psi_nlev;
+
+    static int ilog2(int v) {
+        int r = 0;
+        while ((1 << r) < v) ++r;
+        return r;
+    }
+
+    bool build(int M_, int N_, int J_, int L_, int max_order_, int pre_pad_, size_t lds_limit_, KymError* e) {
+        M = M_; N = N_; J = J_; L = L_; max_order = max_order_; pre_pad = pre_pad_;
+        lds_limit = lds_limit_;
+        if (J < 0 || L < 1 || M < 1 || N < 1) { *e = {KYM_ERR_VALUE, "invalid J, L or shape"}; return false; }
+        if (J + 1 > KYM_MAX_RES) { *e = {KYM_ERR_UNSUPPORTED, "J too large"}; return false; }
+        if ((1 << J) > M || (1 << J) > N) {
+            *e = {KYM_ERR_RUNTIME, "The smallest dimension should be larger than 2^J."};
+            return false;
+        }
+        if (max_order != 1 && max_order != 2) { *e = {KYM_ERR_VALUE, "max_order must be 1 or 2"}; return false; }
+        // kymatio/scattering2d/utils.py:4-24 and base_frontend.py:19-32
+        Mp = ((M + (1 << J)) / (1 << J) + 1) * (1 << J);
+        Np = ((N + (1 << J)) / (1 << J) + 1) * (1 << J);
+        top = (Mp - M) / 2;
+        left = (Np - N) / 2;
+        for (int r = 0; r <= J; ++r) {
+            H[r] = Mp >> r;
+            W[r] = Np >> r;
+            lenH[r] = lens.add(H[r]);
+            lenW[r] = lens.add(W[r]);
+        }
+        oh = H[J] - 2;
+        ow = W[J] - 2;
+        {
+            int mp = 1;
+            for (int r = 0; r <= J; ++r) mp = std::max(mp, std::max(kym_max_odd_prime(H[r]), kym_max_odd_prime(W[r])));
+            pclass = kym_prime_class(mp);
+        }
+        // filters
+        filt_off.clear(); filt_h.clear(); filt_w.clear();
+        long long off = 0;
+        for (int r = 0; r < J; ++r) {
+            filt_off.push_back(off); filt_h.push_back(H[r]); filt_w.push_back(W[r]);
+            off += (long long)H[r] * W[r];
+        }
+        psi_base.clear(); psi_j.clear(); psi_nlev.clear();
+        for (int j = 0; j < J; ++j)
+            for (int th = 0; th < L; ++th) {
+                const int nlev = std::min(j + 1, std::max(J - 1, 1));
+                psi_base.push_back((int)filt_off.size());
+                psi_j.push_back(j);
+                psi_nlev.push_back(nlev);
+                for (int lv = 0; lv < nlev; ++lv) {
+                    filt_off.push_back(off); filt_h.push_back(H[lv]); filt_w.push_back(W[lv]);
+                    off += (long long)H[lv] * W[lv];
+                }
+            }
+        filt_off.push_back(off);
+        // nodes (path order of kymatio/scattering2d/core/scattering2d.py:25-78)
+        nodes.clear();
+        const int npsi = J * L;
+        Node2D root = {};
+        root.order = 0; root.res = 0; root.parent = -1; root.filt = -1; root.k = 1;
+        root.leaf_ch = 0; root.leaf_filt = phi_index(0); root.leaf_k = 1 << J;
+        nodes.push_back(root);
+        for (int n1 = 0; n1 < npsi; ++n1) {
+            Node2D nd = {};
+            const int j1 = psi_j[n1];
+            nd.order = 1; nd.res = j1; nd.parent = 0; nd.filt = psi_index(n1, 0); nd.k = 1 << j1;
+            nd.leaf_ch = 1 + n1; nd.leaf_filt = phi_index(j1); nd.leaf_k = 1 << (J - j1);
+            nodes.push_back(nd);
+        }
+        int ch = 1 + npsi;
+        if (max_order >= 2)
+            for (int n1 = 0; n1 < npsi; ++n1)
+                for (int n2 = 0; n2 < npsi; ++n2) {
+                    const int j1 = psi_j[n1], j2 = psi_j[n2];
+                    if (j2 <= j1) continue;
+                    Node2D nd = {};
+                    nd.order = 2; nd.res = j2; nd.parent = 1 + n1; nd.filt = psi_index(n2, j1);
+                    nd.k = 1 << (j2 - j1);
+                    nd.leaf_ch = ch++; nd.leaf_filt = phi_index(j2); nd.leaf_k = 1 << (J - j2);
+                    nodes.push_back(nd);
+                }
+        C = ch;
+        // placement: LDS-resident or column/row passes; leaf in-kernel or not
+        const size_t cs = sizeof(cpx<T>);
+        const size_t leaf_elems = (size_t)H[J] * W[J];
+        std::vector<char> has_children(nodes.size(), 0);
+        for (auto& nd : nodes)
+            if (nd.parent >= 0) has_children[nd.parent] = 1;
+        ws_img = 0;
+        for (size_t i = 0; i < nodes.size(); ++i) {
+            Node2D& nd = nodes[i];
+            const size_t elems = (size_t)H[nd.res] * W[nd.res];
+            if ((elems + leaf_elems) * cs <= lds_limit) { nd.big = 0; nd.leaf_mode = 0; }
+            else if (elems * cs <= lds_limit) { nd.big = 0; nd.leaf_mode = 1; }
+            else { nd.big = 1; nd.leaf_mode = (leaf_elems * cs <= lds_limit) ? 1 : 2; }
+            const bool need_slot = has_children[i] || nd.big || nd.leaf_mode != 0;
+            nd.slot = need_slot ? ws_img : -1;
+            if (need_slot) ws_img += (long long)elems;
+            nd.leaf_slot = -1;
+            if (nd.leaf_mode == 2) { nd.leaf_slot = ws_img; ws_img += (long long)leaf_elems; }
+            // keep slots 256-byte aligned
+            const long long align = 256 / (long long)cs;
+            ws_img = (ws_img + align - 1) / align * align;
+        }
+        // blob: nodes, lens, filter offsets, group lists
+        blob = Blob();
+        std::string err;
+        if (!lens.emit(blob, &off_lens, &err)) { *e = {KYM_ERR_UNSUPPORTED, err}; return false; }
+        off_nodes = blob.add(sizeof(Node2D) * nodes.size());
+        std::memcpy(blob.at<char>(off_nodes), nodes.data(), sizeof(Node2D) * nodes.size());
+        off_filt = blob.add(sizeof(long long) * filt_off.size());
+        std::memcpy(blob.at<char>(off_filt), filt_off.data(), sizeof(long long) * filt_off.size());
+        // launches
+        launches.clear();
+        max_lds = 0;
+        max_threads = 256;
+        auto add_group = [&](const std::vector<int>& ids) {
+            size_t o = blob.add(sizeof(int) * ids.size());
+            std::memcpy(blob.at<char>(o), ids.data(), sizeof(int) * ids.size());
+            return o;
+        };
+        auto node_threads = [&](size_t elems) { return elems >= 8192 ? 1024 : elems >= 2048 ? 512 : 256; };
+        // Stage-boundary byte model per node (per image): every tensor that
+        // crosses a stage boundary is written once and read once; complex 8 B,
+        // real 4 B.  Root: 4 M N input + 16 P_M P_N (U_0 spectrum); other
+        // nodes: 40 H_r W_r (periodised input, modulus, re-FFT'd spectrum,
+        // each written and read); every leaf: 16 H_J W_J + 4 (H_J-2)(W_J-2).
+        const long long leaf_model = 16ll * H[J] * W[J] + 4ll * (H[J] - 2) * (W[J] - 2);
+        auto node_model = [&](int id) {
+            const Node2D& nd = nodes[id];
+            if (nd.order == 0) return 4ll * M * N + 16ll * H[0] * W[0];
+            return 40ll * H[nd.res] * W[nd.res];
+        };
+        auto push = [&](int kind, const std::vector<int>& ids, int nblk, int blk, int threads, size_t lds) {
+            Launch2D l;
+            l.kind = kind; l.group_off = add_group(ids); l.ngroup = (int)ids.size();
+            l.nblk = nblk; l.blk = blk; l.threads = threads; l.lds_bytes = lds;
+            l.res = nodes[ids[0]].res;
+            long long mb = 0;
+            for (int id : ids) {
+                const Node2D& nd = nodes[id];
+                const bool leaf_here = (kind == K2_LDS_NODE && nd.leaf_mode == 0) || kind == K2_LEAF ||
+                                       kind == K2_LEAF_ROWS;
+                if (kind == K2_LDS_NODE) mb += node_model(id);
+                if (kind == K2_ROOT_ROWS || kind == K2_FWD_COLS) mb += node_model(id) / (nd.order == 0 ? 2 : 3);
+                if (kind == K2_ICOL || kind == K2_ROWS_MOD) mb += node_model(id) / 3;
+                if (leaf_here) mb += leaf_model;
+            }
+            l.model_bytes = mb;
+            launches.push_back(l);
+            max_lds = std::max(max_lds, lds);
+            max_threads = std::max(max_threads, threads);
+        };
+        const size_t pass_budget = std::min(lds_limit, (size_t)64 * 1024);
+        auto col_blk = [&](int Hh) {
+            int cb = (int)(pass_budget / (cs * Hh));
+            return std::max(1, std::min(16, cb));
+        };
+        auto row_blk = [&](int Ww) {
+            int rb = (int)(pass_budget / (cs * Ww));
+            return std::max(1, std::min(16, rb));
+        };
+        auto emit_leaves = [&](const std::vector<int>& ids) {
+            std::vector<int> small, big;
+            for (int id : ids) {
+                if (nodes[id].leaf_mode == 1) small.push_back(id);
+                if (nodes[id].leaf_mode == 2) big.push_back(id);
+            }
+            if (!small.empty()) push(K2_LEAF, small, 1, 0, node_threads(leaf_elems), leaf_elems * cs);
+            if (!big.empty()) {
+                const int cb = col_blk(H[J]), rb = row_blk(W[J]);
+                push(K2_LEAF_ICOL, big, (W[J] + cb - 1) / cb, cb, 256, (size_t)H[J] * cb * cs);
+                push(K2_LEAF_ROWS, big, (H[J] + rb - 1) / rb, rb, 256, (size_t)rb * W[J] * cs);
+            }
+        };
+        auto emit_nodes = [&](const std::vector<int>& ids, bool is_root) {
+            if (ids.empty()) return;
+            const Node2D& n0 = nodes[ids[0]];
+            const int r = n0.res;
+            const size_t elems = (size_t)H[r] * W[r];
+            if (!n0.big) {
+                const size_t lds = (elems + (n0.leaf_mode == 0 ? leaf_elems : 0)) * cs;
+                push(K2_LDS_NODE, ids, 1, 0, node_threads(elems), lds);
+            } else {
+                const int cb = col_blk(H[r]), rb = row_blk(W[r]);
+                if (is_root) push(K2_ROOT_ROWS, ids, (H[r] + rb - 1) / rb, rb, 256, (size_t)rb * W[r] * cs);
+                else {
+                    push(K2_ICOL, ids, (W[r] + cb - 1) / cb, cb, 256, (size_t)H[r] * cb * cs);
+                    push(K2_ROWS_MOD, ids, (H[r] + rb - 1) / rb, rb, 256, (size_t)rb * W[r] * cs);
+                }
+                push(K2_FWD_COLS, ids, (W[r] + cb - 1) / cb, cb, 256, (size_t)H[r] * cb * cs);
+            }
+            emit_leaves(ids);
+        };
+        emit_nodes({0}, true);
+        for (int order = 1; order <= max_order; ++order)
+            for (int r = 0; r < J; ++r)
+                for (int big = 0; big <= 1; ++big) {
+                    std::vector<int> ids;
+                    for (size_t i = 0; i < nodes.size(); ++i)
+                        if (nodes[i].order == order && nodes[i].res == r && nodes[i].big == big) ids.push_back((int)i);
+                    emit_nodes(ids, false);
+                }
+        return true;
+    }
+
+    // Kernel arguments for one launch; `base` is the relocated blob address.
+    Scat2DArgs<T> args(const Launch2D& l, uintptr_t base) const {
+        Scat2DArgs<T> a = {};
+        a.nodes = reinterpret_cast<const Node2D*>(base + off_nodes);
+        a.lens = reinterpret_cast<const FftLen<T>*>(base + off_lens);
+        a.filt_off = reinterpret_cast<const long long*>(base + off_filt);
+        a.group = reinterpret_cast<const int*>(base + l.group_off);
+        a.ngroup = l.ngroup; a.nblk = l.nblk; a.blk = l.blk;
+        for (int r = 0; r <= J; ++r) { a.lenH[r] = lenH[r]; a.lenW[r] = lenW[r]; a.H[r] = H[r]; a.W[r] = W[r]; }
+        a.J = J;
+        a.M = pre_pad ? Mp : M; a.N = pre_pad ? Np : N;
+        a.top = top; a.left = left; a.pre_pad = pre_pad;
+        a.ws_img = ws_img;
+        a.C = C; a.oh = oh; a.ow = ow;
+        return a;
+    }
+};

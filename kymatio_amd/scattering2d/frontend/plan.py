@@ -1,0 +1,62 @@
+"""Python handle of a Scattering2D plan (kym_scat2d_* in include/kymatio_hip.h)."""
+import ctypes
+
+import torch
+
+from ... import _lib
+
+
+class Scat2DPlanHandle:
+    def __init__(self, M, N, J, L, max_order, pre_pad, dtype, lds_limit=0):
+        lib = _lib.lib()
+        h = ctypes.c_void_p()
+        _lib.check(lib.kym_scat2d_create(M, N, J, L, max_order, pre_pad, _lib.DTYPE_CODE[dtype],
+                                         int(lds_limit), ctypes.byref(h)))
+        self._h = h
+        self._lib = lib
+        info = (ctypes.c_longlong * 11)()
+        _lib.check(lib.kym_scat2d_info(h, info))
+        (self.C, self.oh, self.ow, self.Mp, self.Np, self.n_filters, self.filter_elems,
+         self.ws_bytes_per_image, self.n_launches, self.top, self.left) = list(info)
+        offs = (ctypes.c_longlong * (self.n_filters + 1))()
+        _lib.check(lib.kym_scat2d_filter_offsets(h, offs))
+        self.filter_offsets = list(offs)
+        self.dtype = dtype
+
+    KIND_NAMES = ['lds_node', 'leaf', 'root_rows', 'icol', 'rows_mod', 'fwd_cols', 'leaf_icol', 'leaf_rows']
+
+    def launches(self):
+        """Per-launch records: kind, nodes, tasks per node, threads, LDS bytes,
+        model bytes per image, resolution."""
+        out = []
+        for i in range(self.n_launches):
+            rec = (ctypes.c_longlong * 7)()
+            _lib.check(self._lib.kym_scat2d_launch_info(self._h, i, rec))
+            r = list(rec)
+            out.append({'kind': self.KIND_NAMES[r[0]], 'nodes': r[1], 'tasks_per_node': r[2], 'threads': r[3],
+                        'lds_bytes': r[4], 'model_bytes_per_image': r[5], 'res': r[6]})
+        return out
+
+    def profile(self, enable):
+        _lib.check(self._lib.kym_scat2d_profile(self._h, int(bool(enable))))
+
+    def profile_read(self):
+        n = self.n_launches
+        ms = (ctypes.c_double * n)()
+        cnt = (ctypes.c_longlong * n)()
+        _lib.check(self._lib.kym_scat2d_profile_read(self._h, ms, cnt, n))
+        return list(ms), list(cnt)
+
+    def forward(self, x, filt, out, batch, ws):
+        _lib.check(self._lib.kym_scat2d_forward(
+            self._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(filt.data_ptr()),
+            ctypes.c_void_p(out.data_ptr()), int(batch), ctypes.c_void_p(ws.data_ptr()), ws.numel(),
+            _lib.stream_of(x.device)))
+
+    def __del__(self):
+        try:
+            if self._h:
+                self._lib.kym_scat2d_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass

@@ -1,0 +1,84 @@
+"""Shared test helpers: golden fixtures, packed filters, error metrics."""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, 'tests', 'golden')
+
+# Parity bar for FP32 (stated in the workload definition): per output channel,
+# max |ours - ref| <= 1e-5 * max |ref| over that channel (relative to the
+# channel's scale; element-wise relative error is ill-posed for coefficients
+# near zero).
+RTOL_CHANNEL = 1e-5
+
+
+def golden(name):
+    return np.load(os.path.join(GOLDEN, name))
+
+
+def channel_rel_err(ours, ref):
+    """max over (batch, channel) of max|ours-ref| / max|ref| of that channel."""
+    ours = np.asarray(ours, np.float64)
+    ref = np.asarray(ref, np.float64)
+    c_axis = ref.ndim - 3 if ref.ndim >= 3 else 0
+    lead = int(np.prod(ref.shape[:c_axis + 1]))
+    o = ours.reshape(lead, -1)
+    r = ref.reshape(lead, -1)
+    scale = np.abs(r).max(axis=1)
+    scale[scale == 0] = 1.0
+    return float((np.abs(o - r).max(axis=1) / scale).max())
+
+
+def pack_filters_2d(fb, dtype=np.float32):
+    arrs = list(fb['phi']['levels']) + [lev for p in fb['psi'] for lev in p['levels']]
+    return np.concatenate([np.asarray(a).reshape(-1) for a in arrs]).astype(dtype)
+
+
+def golden_2d_cases():
+    G = golden('golden_scat2d.npz')
+    tags = ['cfg1', 'cfg1o1', 'cfg4', 'odd31', 'odd33', 'rect37x32', 'sq32J5', 'rect40x24', 'prepad', 'L6', 'cfg2']
+    out = []
+    for tag in tags:
+        kw = {}
+        for k in ('pre_pad', 'L', 'max_order'):
+            if tag + '_' + k in G.files:
+                kw[k] = int(G[tag + '_' + k])
+        out.append((tag, G[tag + '_x'], int(G[tag + '_J']), tuple(int(v) for v in G[tag + '_shape']), kw,
+                    G[tag + '_Sx']))
+    return out
+
+
+class Emu:
+    """ctypes binding of libkymatio_emu.so (host-only harness running the GPU
+    workgroup bodies single-threaded)."""
+
+    def __init__(self):
+        path = os.path.join(ROOT, 'kymatio_amd', 'libkymatio_emu.so')
+        self.lib = ctypes.CDLL(path)
+        f = self.lib.kym_emu_scat2d_forward
+        f.argtypes = [ctypes.c_int] * 7 + [ctypes.c_longlong, ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.c_void_p, ctypes.c_longlong, ctypes.c_longlong,
+                                           ctypes.c_void_p]
+        f.restype = ctypes.c_int
+        self.lib.kym_emu_last_error.restype = ctypes.c_char_p
+
+    def scat2d(self, x, J, shape, filters_packed, L=8, max_order=2, pre_pad=0, lds_limit=160 * 1024,
+               chunk=0):
+        dt = filters_packed.dtype
+        code = 0 if dt == np.float32 else 1
+        M, N = shape
+        B = x.shape[0]
+        info = (ctypes.c_longlong * 8)()
+        rc = self.lib.kym_emu_scat2d_forward(M, N, J, L, max_order, pre_pad, code, lds_limit, None, None, None,
+                                             B, 0, info)
+        if rc != 0:
+            raise RuntimeError(self.lib.kym_emu_last_error().decode())
+        out = np.zeros((B, info[0], info[1], info[2]), dt)
+        xx = np.ascontiguousarray(x, dt)
+        rc = self.lib.kym_emu_scat2d_forward(M, N, J, L, max_order, pre_pad, code, lds_limit, xx.ctypes.data,
+                                             filters_packed.ctypes.data, out.ctypes.data, B, chunk, info)
+        if rc != 0:
+            raise RuntimeError(self.lib.kym_emu_last_error().decode())
+        return out, {'launches': info[3], 'big_nodes': info[4]}
