@@ -31,7 +31,7 @@ METRIC = 'images/s Scattering2D J=3 L=8 256² fp32 (%HBM roofline); recon time t
 J, L, SIDE, BATCH, SEED = 3, 8, 256, 64, 2
 
 
-def cpu_baseline(budget_s=15.0, max_images=256):
+def cpu_baseline(budget_s=15.0, max_images=2048):
     from oracle import scat2d
     from kymatio_amd import workloads
     cores = os.cpu_count() or 1
@@ -70,7 +70,18 @@ def main():
     ap.add_argument('--warmup', type=int, default=5)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--chunk', type=int, default=0, help='images per launch chunk (0 = default)')
+    ap.add_argument('--copy-calibration', type=int, default=0, metavar='BYTES',
+                    help='only run a plain device-to-device copy of BYTES (calibrates HBM counters)')
     args = ap.parse_args()
+    if args.copy_calibration:
+        n = args.copy_calibration
+        src = torch.ones(n, dtype=torch.uint8, device='cuda')
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({'copy_bytes': n}))
+        return
     if args.gpus != 1:
         print('bench.py measures one GPU only (the scattering path shards over the batch with no '
               'exchange; see DESIGN.md). Run with --gpus 1.', file=sys.stderr)

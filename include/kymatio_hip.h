@@ -83,6 +83,42 @@ int kym_scat2d_launch_info(void* plan, int index, long long* rec);
 int kym_scat2d_profile(void* plan, int enable);
 int kym_scat2d_profile_read(void* plan, double* ms, long long* count, int n);
 
+/* ---------------------------------------------------------------- 1D ----
+ * Replaces the generator call scattering1d(U_0, backend, filters,
+ * log2_stride, average_local) and the per-path unpad / stack of
+ * ScatteringBase1D.scattering (kymatio/scattering1d/frontend/
+ * base_frontend.py:106-155; core in kymatio/scattering1d/core/
+ * scattering1d.py:2-107) with one fused pipeline.
+ *
+ * kym_scat1d_create: the frontend's setup (base_frontend.py:37-98) passed
+ * in: input length N, padded length N_pad (power of two), pad_left,
+ * log2_stride, average_mode (0 local 'T', 1 none 'T=0', 2 'global'),
+ * oversampling, max_order 1|2, the first/second order filters' dyadic
+ * subsampling j and level counts, phi's level count and the border indices
+ * ind_start/ind_end[0..n_ind).  Status 5 if N_pad does not fit one
+ * workgroup's LDS for the dtype.
+ */
+int kym_scat1d_create(int N, int N_pad, int pad_left, int log2_stride, int average_mode, int oversampling,
+                      int max_order, int n_psi1, const int* psi1_j, const int* psi1_nlev, int n_psi2,
+                      const int* psi2_j, const int* psi2_nlev, int phi_nlev, const int* ind_start,
+                      const int* ind_end, int n_ind, int dtype, long long lds_limit, void** plan);
+int kym_scat1d_destroy(void* plan);
+
+/* info[6] = {n_paths, out_row_elems, n_filters, filter_elems,
+ *            workspace_bytes_per_signal, n_launches} */
+int kym_scat1d_info(void* plan, long long* info);
+
+/* filt_offs[n_filters + 1]: packed filter layout in the reference's buffer
+ * order (phi levels, psi1 levels, psi2 levels; kymatio/scattering1d/frontend/
+ * torch_frontend.py:30-56).  path_len[n_paths]: output length of each path in
+ * output order (sorted by (order, n) as base_frontend.py:147-150). */
+int kym_scat1d_layout(void* plan, long long* filt_offs, long long* path_len);
+
+/* x: (batch, N) real; out: (batch, out_row) with path c at
+ * [sum(path_len[:c]), ...); ws: workspace (>= 1 signal). */
+int kym_scat1d_forward(void* plan, const void* x, const void* filt, void* out, long long batch, void* ws,
+                       long long ws_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

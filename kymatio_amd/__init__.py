@@ -5,7 +5,8 @@ and ``kymatio_amd.torch.Scattering1D`` keep the reference's constructor
 arguments, buffers, output layout and exceptions, and run hand-written HIP
 kernels for gfx950 through the C ABI in include/kymatio_hip.h.
 """
+from .scattering1d import ScatteringEntry1D as Scattering1D
 from .scattering2d import ScatteringEntry2D as Scattering2D
 
 __version__ = '0.1.0'
-__all__ = ['Scattering2D']
+__all__ = ['Scattering1D', 'Scattering2D']

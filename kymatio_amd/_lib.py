@@ -31,6 +31,12 @@ _SIGNATURES = {
     'kym_scat2d_launch_info': [_c_vp, _c_i, ctypes.POINTER(_c_ll)],
     'kym_scat2d_profile': [_c_vp, _c_i],
     'kym_scat2d_profile_read': [_c_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_c_ll), _c_i],
+    'kym_scat1d_create': [_c_i] * 8 + [ctypes.POINTER(_c_i)] * 2 + [_c_i] + [ctypes.POINTER(_c_i)] * 2
+                         + [_c_i] + [ctypes.POINTER(_c_i)] * 2 + [_c_i, _c_i, _c_ll, ctypes.POINTER(_c_vp)],
+    'kym_scat1d_destroy': [_c_vp],
+    'kym_scat1d_info': [_c_vp, ctypes.POINTER(_c_ll)],
+    'kym_scat1d_layout': [_c_vp, ctypes.POINTER(_c_ll), ctypes.POINTER(_c_ll)],
+    'kym_scat1d_forward': [_c_vp, _c_vp, _c_vp, _c_vp, _c_ll, _c_vp, _c_ll, _c_vp],
 }
 
 

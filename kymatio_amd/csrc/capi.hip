@@ -5,7 +5,9 @@
 #include <string>
 
 #include "../../include/kymatio_hip.h"
+#include "kernels1d.h"
 #include "kernels2d.h"
+#include "scat1d_plan.h"
 #include "scat2d_plan.h"
 
 static thread_local std::string g_last_error;
@@ -75,7 +77,7 @@ static int upload_and_prepare(Scat2DPlan<T>& p, void** dblob) {
     p.blob.relocate((uintptr_t)*dblob);
     KYM_HIP_TRY(hipMemcpy(*dblob, p.blob.bytes.data(), bytes, hipMemcpyHostToDevice));
     for (const Launch2D& l : p.launches) {
-        Scat2DKernel<T> k = pick_scat2d<T>(l.kind, l.threads, p.pclass);
+        Scat2DKernel<T> k = pick_scat2d<T>(l.kind, l.threads, p.pclass, l.spec);
         if (l.lds_bytes > 65536)
             KYM_HIP_TRY(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize,
                                             (int)l.lds_bytes));
@@ -177,7 +179,7 @@ static int forward2d(const Scat2DPlan<T>& p, const void* x, const void* filt, vo
             if (ntask > (1ll << 30)) return fail(KYM_ERR_VALUE, "too many tasks in one launch");
             a.ntask = (int)ntask;
             const unsigned grid = (unsigned)((ntask + 7) / 8 * 8);
-            Scat2DKernel<T> k = pick_scat2d<T>(l.kind, l.threads, p.pclass);
+            Scat2DKernel<T> k = pick_scat2d<T>(l.kind, l.threads, p.pclass, l.spec);
             hipEvent_t e0 = nullptr, e1 = nullptr;
             if (prof && prof->enabled) {
                 e0 = prof->get();
@@ -247,4 +249,158 @@ extern "C" int kym_scat2d_profile_read(void* plan, double* ms, long long* count,
     }
     h->prof.reset();
     return KYM_OK;
+}
+
+// ----------------------------------------------------------------- 1D plans
+struct KymScat1D {
+    int dtype = KYM_F32;
+    int device = 0;
+    void* dblob = nullptr;
+    std::unique_ptr<Scat1DPlan<float>> pf;
+    std::unique_ptr<Scat1DPlan<double>> pd;
+    LaunchProfile prof;
+};
+
+template <class T>
+static int upload1d(Scat1DPlan<T>& p, void** dblob) {
+    const size_t bytes = p.blob.bytes.size();
+    KYM_HIP_TRY(hipMalloc(dblob, bytes));
+    p.blob.relocate((uintptr_t)*dblob);
+    KYM_HIP_TRY(hipMemcpy(*dblob, p.blob.bytes.data(), bytes, hipMemcpyHostToDevice));
+    for (const Launch1D& l : p.launches)
+        if (l.lds_bytes > 65536)
+            KYM_HIP_TRY(hipFuncSetAttribute((const void*)pick_scat1d<T>(l.threads),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.lds_bytes));
+    return KYM_OK;
+}
+
+extern "C" int kym_scat1d_create(int N, int N_pad, int pad_left, int log2_stride, int average_mode,
+                                 int oversampling, int max_order, int n_psi1, const int* psi1_j,
+                                 const int* psi1_nlev, int n_psi2, const int* psi2_j, const int* psi2_nlev,
+                                 int phi_nlev, const int* ind_start, const int* ind_end, int n_ind, int dtype,
+                                 long long lds_limit, void** out) {
+    if (!out) return fail(KYM_ERR_VALUE, "null plan pointer");
+    *out = nullptr;
+    if (dtype != KYM_F32 && dtype != KYM_F64) return fail(KYM_ERR_TYPE, "dtype must be float32 or float64");
+    if (average_mode < 0 || average_mode > 2) return fail(KYM_ERR_VALUE, "average_mode must be 0, 1 or 2");
+    Scat1DDesc d = {N, N_pad, pad_left, log2_stride, average_mode, oversampling, max_order, n_psi1, n_psi2, phi_nlev,
+                    psi1_j, psi1_nlev, psi2_j, psi2_nlev, ind_start, ind_end, n_ind};
+    auto h = std::make_unique<KymScat1D>();
+    h->dtype = dtype;
+    KYM_HIP_TRY(hipGetDevice(&h->device));
+    size_t lim = device_lds_limit();
+    if (lds_limit > 0 && (size_t)lds_limit < lim) lim = (size_t)lds_limit;
+    KymError e = {KYM_OK, ""};
+    int rc;
+    if (dtype == KYM_F32) {
+        h->pf = std::make_unique<Scat1DPlan<float>>();
+        if (!h->pf->build(d, lim, &e)) return fail(e.code, e.msg);
+        rc = upload1d(*h->pf, &h->dblob);
+    } else {
+        h->pd = std::make_unique<Scat1DPlan<double>>();
+        if (!h->pd->build(d, lim, &e)) return fail(e.code, e.msg);
+        rc = upload1d(*h->pd, &h->dblob);
+    }
+    if (rc != KYM_OK) {
+        if (h->dblob) (void)hipFree(h->dblob);
+        return rc;
+    }
+    *out = h.release();
+    return KYM_OK;
+}
+
+extern "C" int kym_scat1d_destroy(void* plan) {
+    if (!plan) return KYM_OK;
+    KymScat1D* h = static_cast<KymScat1D*>(plan);
+    if (h->dblob) {
+        int cur = 0;
+        (void)hipGetDevice(&cur);
+        (void)hipSetDevice(h->device);
+        (void)hipFree(h->dblob);
+        (void)hipSetDevice(cur);
+    }
+    delete h;
+    return KYM_OK;
+}
+
+template <class T>
+static void fill_info1d(const Scat1DPlan<T>& p, long long* info) {
+    info[0] = p.C;
+    info[1] = p.out_row;
+    info[2] = (long long)p.filt_off.size() - 1;
+    info[3] = p.filt_off.back();
+    info[4] = p.ws_sig * (long long)sizeof(cpx<T>);
+    info[5] = (long long)p.launches.size();
+}
+
+extern "C" int kym_scat1d_info(void* plan, long long* info) {
+    if (!plan || !info) return fail(KYM_ERR_VALUE, "null argument");
+    KymScat1D* h = static_cast<KymScat1D*>(plan);
+    if (h->pf) fill_info1d(*h->pf, info);
+    else fill_info1d(*h->pd, info);
+    return KYM_OK;
+}
+
+extern "C" int kym_scat1d_layout(void* plan, long long* filt_offs, long long* path_len) {
+    if (!plan) return fail(KYM_ERR_VALUE, "null plan");
+    KymScat1D* h = static_cast<KymScat1D*>(plan);
+    const std::vector<long long>& fo = h->pf ? h->pf->filt_off : h->pd->filt_off;
+    const std::vector<int>& pl = h->pf ? h->pf->out_len : h->pd->out_len;
+    if (filt_offs)
+        for (size_t i = 0; i < fo.size(); ++i) filt_offs[i] = fo[i];
+    if (path_len)
+        for (size_t i = 0; i < pl.size(); ++i) path_len[i] = pl[i];
+    return KYM_OK;
+}
+
+template <class T>
+static int forward1d(const Scat1DPlan<T>& p, const void* x, const void* filt, void* out, long long batch, void* ws,
+                     long long ws_bytes, hipStream_t stream, uintptr_t base, LaunchProfile* prof) {
+    const long long per = p.ws_sig * (long long)sizeof(cpx<T>);
+    long long chunk = ws_bytes / per;
+    if (chunk < 1) return fail(KYM_ERR_VALUE, "workspace too small for one signal");
+    if (chunk > batch) chunk = batch;
+    for (long long i0 = 0; i0 < batch; i0 += chunk) {
+        const int nimg = (int)std::min(chunk, batch - i0);
+        for (const Launch1D& l : p.launches) {
+            Scat1DArgs<T> a = p.args(l, base);
+            a.x = static_cast<const T*>(x);
+            a.filt = static_cast<const T*>(filt);
+            a.ws = static_cast<cpx<T>*>(ws);
+            a.out = static_cast<T*>(out);
+            a.img0 = (int)i0;
+            const long long ntask = (long long)nimg * l.ngroup;
+            if (ntask > (1ll << 30)) return fail(KYM_ERR_VALUE, "too many tasks in one launch");
+            a.ntask = (int)ntask;
+            const unsigned grid = (unsigned)((ntask + 7) / 8 * 8);
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (prof && prof->enabled) {
+                e0 = prof->get();
+                e1 = prof->get();
+                if (e0) KYM_HIP_TRY(hipEventRecord(e0, stream));
+            }
+            hipLaunchKernelGGL(pick_scat1d<T>(l.threads), dim3(grid), dim3(l.threads), l.lds_bytes, stream, a);
+            KYM_HIP_TRY(hipGetLastError());
+            if (e0 && e1) {
+                KYM_HIP_TRY(hipEventRecord(e1, stream));
+                prof->recs.push_back({(int)(&l - p.launches.data()), {e0, e1}});
+            }
+        }
+    }
+    return KYM_OK;
+}
+
+extern "C" int kym_scat1d_forward(void* plan, const void* x, const void* filt, void* out, long long batch, void* ws,
+                                  long long ws_bytes, void* stream) {
+    if (!plan) return fail(KYM_ERR_VALUE, "null plan");
+    if (batch < 0) return fail(KYM_ERR_VALUE, "negative batch");
+    if (batch == 0) return KYM_OK;
+    if (!x || !filt || !out || !ws) return fail(KYM_ERR_VALUE, "null device pointer");
+    KymScat1D* h = static_cast<KymScat1D*>(plan);
+    int cur = 0;
+    KYM_HIP_TRY(hipGetDevice(&cur));
+    if (cur != h->device) return fail(KYM_ERR_TYPE, "plan was created for another device");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (h->pf) return forward1d(*h->pf, x, filt, out, batch, ws, ws_bytes, s, (uintptr_t)h->dblob, &h->prof);
+    return forward1d(*h->pd, x, filt, out, batch, ws, ws_bytes, s, (uintptr_t)h->dblob, &h->prof);
 }

@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 
+#include "scat1d_plan.h"
 #include "scat2d_plan.h"
 
 static thread_local std::string g_emu_error;
@@ -28,9 +29,11 @@ static int emu_forward2d(int M, int N, int J, int L, int max_order, int pre_pad,
     const uintptr_t base = (uintptr_t)p.blob.bytes.data();
     if (info) {
         info[0] = p.C; info[1] = p.oh; info[2] = p.ow; info[3] = (long long)p.launches.size();
-        int nbig = 0;
+        int nbig = 0, nspec = 0;
         for (auto& nd : p.nodes) nbig += nd.big;
+        for (auto& l : p.launches) nspec += l.spec >= 0;
         info[4] = nbig;
+        info[5] = nspec;
     }
     if (!x) return 0;
     if (chunk <= 0 || chunk > batch) chunk = batch;
@@ -46,7 +49,11 @@ static int emu_forward2d(int M, int N, int J, int L, int max_order, int pre_pad,
             a.out = static_cast<T*>(out);
             a.img0 = (int)i0;
             a.ntask = nimg * l.ngroup * l.nblk;
-            for (int t = 0; t < a.ntask; ++t) dispatch_body2d<T, 31>(l.kind, a, t, lds.data(), 0, 1);
+            for (int t = 0; t < a.ntask; ++t) {
+                if (l.spec >= 0 && l.kind == K2_LDS_NODE) dispatch_node_spec<T, true>(l.spec, a, t, lds.data(), 0);
+                else if (l.spec >= 0 && l.kind == K2_LEAF) dispatch_leaf_spec<T, true>(l.spec, a, t, lds.data(), 0);
+                else dispatch_body2d<T, 31>(l.kind, a, t, lds.data(), 0, 1);
+            }
         }
     }
     return 0;
@@ -58,4 +65,42 @@ extern "C" int kym_emu_scat2d_forward(int M, int N, int J, int L, int max_order,
     if (dtype == 0)
         return emu_forward2d<float>(M, N, J, L, max_order, pre_pad, lds_limit, x, filt, out, batch, chunk, info);
     return emu_forward2d<double>(M, N, J, L, max_order, pre_pad, lds_limit, x, filt, out, batch, chunk, info);
+}
+
+extern "C" int kym_emu_scat1d_forward(int N, int N_pad, int pad_left, int log2_stride, int average_mode,
+                                      int oversampling, int max_order, int n_psi1, const int* psi1_j,
+                                      const int* psi1_nlev, int n_psi2, const int* psi2_j, const int* psi2_nlev,
+                                      int phi_nlev, const int* ind_start, const int* ind_end, int n_ind, int dtype,
+                                      const void* x, const void* filt, void* out, long long batch,
+                                      long long* info) {
+    Scat1DDesc d = {N, N_pad, pad_left, log2_stride, average_mode, oversampling, max_order, n_psi1, n_psi2, phi_nlev,
+                    psi1_j, psi1_nlev, psi2_j, psi2_nlev, ind_start, ind_end, n_ind};
+    auto run = [&](auto tag) -> int {
+        using T = decltype(tag);
+        Scat1DPlan<T> p;
+        KymError e = {0, ""};
+        if (!p.build(d, 160 * 1024, &e)) {
+            g_emu_error = e.msg;
+            return e.code;
+        }
+        p.blob.relocate((uintptr_t)p.blob.bytes.data());
+        const uintptr_t base = (uintptr_t)p.blob.bytes.data();
+        if (info) { info[0] = p.C; info[1] = p.out_row; info[2] = (long long)p.filt_off.size() - 1; info[3] = p.filt_off.back(); }
+        if (!x) return 0;
+        std::vector<cpx<T>> ws((size_t)(p.ws_sig * batch) + 1);
+        std::vector<cpx<T>> lds(p.max_lds / sizeof(cpx<T>) + 1);
+        for (const Launch1D& l : p.launches) {
+            Scat1DArgs<T> a = p.args(l, base);
+            a.x = static_cast<const T*>(x);
+            a.filt = static_cast<const T*>(filt);
+            a.ws = ws.data();
+            a.out = static_cast<T*>(out);
+            a.img0 = 0;
+            a.ntask = (int)(batch * l.ngroup);
+            for (int t = 0; t < a.ntask; ++t) body_node1d<T, 31>(a, t, lds.data(), 0, 1);
+        }
+        return 0;
+    };
+    if (dtype == 0) return run(float(0));
+    return run(double(0));
 }

@@ -277,36 +277,52 @@ KYM_HD void fft_dif(cpx<T>* buf, const FftLen<T>& fl, const Batch& bt, int tid, 
     }
 }
 
-// Host: factor n into supported radices (DIT order).  Returns false if a
-// prime factor above 31 remains.
-inline bool kym_factor(int n, int* rad, int* nrad) {
+// Factor n into supported radices (DIT order): odd primes first (their
+// butterflies then need no twiddles), then powers of two as 16/8/4/2.
+// constexpr so compile-time specialisations use exactly the runtime order.
+struct RadList {
+    int n = 0;
+    int nrad = 0;
+    int rad[KYM_MAX_RADICES] = {0};
+    bool ok = false;
+};
+
+KYM_HD constexpr RadList kym_factor_list(int n) {
+    RadList rl;
+    rl.n = n;
     int k = 0;
-    static const int odd[] = {3, 5, 7, 11, 13, 17, 19, 23, 29, 31};
     int m = n;
-    for (int p : odd) {
+    const int odd[10] = {3, 5, 7, 11, 13, 17, 19, 23, 29, 31};
+    for (int i = 0; i < 10; ++i) {
+        const int p = odd[i];
         while (m % p == 0) {
-            if (k >= KYM_MAX_RADICES) return false;
-            rad[k++] = p;
+            if (k >= KYM_MAX_RADICES) return rl;
+            rl.rad[k++] = p;
             m /= p;
         }
     }
     int a = 0;
     while (m % 2 == 0) { m /= 2; ++a; }
-    if (m != 1) return false;
-    // powers of two: as many 16s as possible, then balance the tail into 8/4/2
+    if (m != 1) return rl;
     while (a > 0) {
-        int r;
-        if (a >= 8 || a == 4) r = 16;
-        else if (a == 7) r = 16;
-        else if (a == 6 || a == 3) r = 8;
-        else if (a == 5) r = 8;
-        else if (a == 2) r = 4;
-        else r = 2;
-        if (k >= KYM_MAX_RADICES) return false;
-        rad[k++] = r;
-        a -= (r == 16 ? 4 : r == 8 ? 3 : r == 4 ? 2 : 1);
+        int r = 2, d = 1;
+        if (a >= 8 || a == 4 || a == 7) { r = 16; d = 4; }
+        else if (a == 6 || a == 3 || a == 5) { r = 8; d = 3; }
+        else if (a == 2) { r = 4; d = 2; }
+        if (k >= KYM_MAX_RADICES) return rl;
+        rl.rad[k++] = r;
+        a -= d;
     }
-    if (n == 1) { rad[k++] = 1; }
-    *nrad = k;
+    if (n == 1) rl.rad[k++] = 1;
+    rl.nrad = k;
+    rl.ok = true;
+    return rl;
+}
+
+inline bool kym_factor(int n, int* rad, int* nrad) {
+    const RadList rl = kym_factor_list(n);
+    if (!rl.ok) return false;
+    for (int i = 0; i < rl.nrad; ++i) rad[i] = rl.rad[i];
+    *nrad = rl.nrad;
     return true;
 }

@@ -1,0 +1,88 @@
+// Compile-time specialised variants of the in-place LDS FFT passes: the
+// length, the radix sequence, the batch layout and the thread count are
+// template parameters, so every index division is by a constant and the
+// pass loops have known trip counts.  Same arithmetic, same radix order
+// (kym_factor_list) and same digit-reversal permutation as fft_core.h.
+#pragma once
+#include "fft_core.h"
+
+template <int R, bool INV, int N, int LP, int NTR, int ES, int TS, bool TFAST, int NT, class T>
+KYM_HD KYM_INL void dit_pass_s(cpx<T>* buf, const cpx<T>* tw, int tid) {
+    constexpr int NB = N / R, L = LP * R, TSTEP = N / L, TOTAL = NB * NTR, ST = LP * ES;
+    for (int f = tid; f < TOTAL; f += NT) {
+        int u, t;
+        if constexpr (TFAST) { t = f % NTR; u = f / NTR; }
+        else { u = f % NB; t = f / NB; }
+        const int b = u / LP, j = u - b * LP;
+        cpx<T>* p = buf + t * TS + (b * L + j) * ES;
+        cpx<T> v[R];
+#pragma unroll
+        for (int m = 0; m < R; ++m) v[m] = p[m * ST];
+        if constexpr (LP > 1) {
+            if (j != 0) {
+#pragma unroll
+                for (int m = 1; m < R; ++m) {
+                    const cpx<T> w = tw[j * m * TSTEP];
+                    v[m] = INV ? cmulc(v[m], w) : cmul(v[m], w);
+                }
+            }
+        }
+        dft_r<R, INV>(v);
+#pragma unroll
+        for (int m = 0; m < R; ++m) p[m * ST] = v[m];
+    }
+}
+
+template <int R, bool INV, int N, int L, int NTR, int ES, int TS, bool TFAST, int NT, class T>
+KYM_HD KYM_INL void dif_pass_s(cpx<T>* buf, const cpx<T>* tw, int tid) {
+    constexpr int NB = N / R, LS = L / R, TSTEP = N / L, TOTAL = NB * NTR, ST = LS * ES;
+    for (int f = tid; f < TOTAL; f += NT) {
+        int u, t;
+        if constexpr (TFAST) { t = f % NTR; u = f / NTR; }
+        else { u = f % NB; t = f / NB; }
+        const int b = u / LS, j = u - b * LS;
+        cpx<T>* p = buf + t * TS + (b * L + j) * ES;
+        cpx<T> v[R];
+#pragma unroll
+        for (int m = 0; m < R; ++m) v[m] = p[m * ST];
+        dft_r<R, INV>(v);
+        if constexpr (LS > 1) {
+            if (j != 0) {
+#pragma unroll
+                for (int m = 1; m < R; ++m) {
+                    const cpx<T> w = tw[j * m * TSTEP];
+                    v[m] = INV ? cmulc(v[m], w) : cmul(v[m], w);
+                }
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < R; ++m) p[m * ST] = v[m];
+    }
+}
+
+// Full DIT (digit-reversed in, natural out) over a batch, barrier after each pass.
+template <bool INV, int N, int NTR, int ES, int TS, bool TFAST, int NT, class T, int Q = 0, int LP = 1>
+KYM_HD KYM_INL void fft_dit_s(cpx<T>* buf, const cpx<T>* tw, int tid) {
+    constexpr RadList rl = kym_factor_list(N);
+    static_assert(rl.ok, "unsupported static FFT length");
+    if constexpr (Q < rl.nrad) {
+        constexpr int R = rl.rad[Q];
+        dit_pass_s<R, INV, N, LP, NTR, ES, TS, TFAST, NT, T>(buf, tw, tid);
+        wg_barrier();
+        fft_dit_s<INV, N, NTR, ES, TS, TFAST, NT, T, Q + 1, LP * R>(buf, tw, tid);
+    }
+}
+
+// Full DIF (natural in, digit-reversed out); passes in reverse radix order.
+template <bool INV, int N, int NTR, int ES, int TS, bool TFAST, int NT, class T, int Q = -1, int L = N>
+KYM_HD KYM_INL void fft_dif_s(cpx<T>* buf, const cpx<T>* tw, int tid) {
+    constexpr RadList rl = kym_factor_list(N);
+    static_assert(rl.ok, "unsupported static FFT length");
+    constexpr int q = (Q < 0) ? rl.nrad - 1 : Q;
+    if constexpr (q >= 0 && L > 1) {
+        constexpr int R = rl.rad[q];
+        dif_pass_s<R, INV, N, L, NTR, ES, TS, TFAST, NT, T>(buf, tw, tid);
+        wg_barrier();
+        if constexpr (q > 0) fft_dif_s<INV, N, NTR, ES, TS, TFAST, NT, T, q - 1, L / R>(buf, tw, tid);
+    }
+}

@@ -20,8 +20,48 @@ __global__ void __launch_bounds__(NT) k_scat2d(Scat2DArgs<T> a) {
     dispatch_body2d<T, MAXP>(KIND, a, task, lds, threadIdx.x, NT);
 }
 
+template <class T, int SPEC, int NT>
+__global__ void __launch_bounds__(NT) k_node2d_spec(Scat2DArgs<T> a) {
+    extern __shared__ __attribute__((aligned(16))) char kym_smem2d_ns[];
+    const int task = xcd_task(blockIdx.x, gridDim.x);
+    if (task >= a.ntask) return;
+    dispatch_node_spec<T, false>(SPEC, a, task, reinterpret_cast<cpx<T>*>(kym_smem2d_ns), threadIdx.x);
+}
+
+template <class T, int SPEC, int NT>
+__global__ void __launch_bounds__(NT) k_leaf2d_spec(Scat2DArgs<T> a) {
+    extern __shared__ __attribute__((aligned(16))) char kym_smem2d_ls[];
+    const int task = xcd_task(blockIdx.x, gridDim.x);
+    if (task >= a.ntask) return;
+    dispatch_leaf_spec<T, false>(SPEC, a, task, reinterpret_cast<cpx<T>*>(kym_smem2d_ls), threadIdx.x);
+}
+
 template <class T>
 using Scat2DKernel = void (*)(Scat2DArgs<T>);
+
+template <class T>
+Scat2DKernel<T> pick_node_spec(int id) {
+    if constexpr (sizeof(T) != 4) return nullptr;  // specialisations are FP32 only
+    else switch (id) {
+#define KYM_NS_K(sid, h, w, hj, wj, nt) \
+    case sid: return k_node2d_spec<T, sid, nt>;
+        KYM_NODE_SPECS(KYM_NS_K)
+#undef KYM_NS_K
+        default: return nullptr;
+    }
+}
+
+template <class T>
+Scat2DKernel<T> pick_leaf_spec(int id) {
+    if constexpr (sizeof(T) != 4) return nullptr;
+    else switch (id) {
+#define KYM_LS_K(sid, hj, wj, nt) \
+    case sid: return k_leaf2d_spec<T, sid, nt>;
+        KYM_LEAF_SPECS(KYM_LS_K)
+#undef KYM_LS_K
+        default: return nullptr;
+    }
+}
 
 template <class T, int KIND, int MAXP>
 Scat2DKernel<T> pick_scat2d_nt(int threads) {
@@ -45,9 +85,12 @@ Scat2DKernel<T> pick_scat2d_p(int kind, int threads) {
     }
 }
 
-// pclass: 7, 17 or 31 (kym_prime_class of the plan's largest odd prime)
+// pclass: 7, 17 or 31 (kym_prime_class of the plan's largest odd prime);
+// spec >= 0 selects a compile-time specialisation of the node / leaf kernel.
 template <class T>
-Scat2DKernel<T> pick_scat2d(int kind, int threads, int pclass) {
+Scat2DKernel<T> pick_scat2d(int kind, int threads, int pclass, int spec = -1) {
+    if (spec >= 0 && kind == K2_LDS_NODE) return pick_node_spec<T>(spec);
+    if (spec >= 0 && kind == K2_LEAF) return pick_leaf_spec<T>(spec);
     if (pclass <= 7) return pick_scat2d_p<T, 7>(kind, threads);
     if (pclass <= 17) return pick_scat2d_p<T, 17>(kind, threads);
     return pick_scat2d_p<T, 31>(kind, threads);

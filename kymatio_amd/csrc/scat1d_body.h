@@ -1,0 +1,148 @@
+// Scattering1D: node tree, kernel arguments and workgroup bodies.
+//
+// Same tree as the 2D path, on the generator of
+// kymatio/scattering1d/core/scattering1d.py:2-107:
+//   root     U_0 = fft(pad(x))                                  length N_pad
+//   order 1  U_1 = |ifft(P_2^k1(U_0 * psi1[n1].levels[0]))|        res k1
+//   order 2  U_2 = |ifft(P_2^k2(fft(U_1) * psi2[n2].levels[k1]))|  res k1 + k2
+// with k1 = min(j1, log2_stride) (j1 without local averaging) and
+// k2 = max(min(j2, log2_stride) - k1, 0).  Each node emits one output path:
+//   local average  S = Re(ifft(P_kl(fft(U) * phi.levels[res]))), kl = 2^max(log2_stride-res, 0),
+//                  unpadded to [ind_start[log2_stride], ind_end[log2_stride])
+//   no average     U itself, unpadded at [ind_start[r], ind_end[r]), r = max(j - oversampling, 0)
+//   global         the sum of U over time (the DC bin of fft(U))
+// written at out[signal * out_row + out_off + t].
+#pragma once
+#include "fft_static.h"
+
+#define KYM_MAX_RES1 24
+
+enum Avg1D { AVG1_LOCAL = 0, AVG1_NONE = 1, AVG1_GLOBAL = 2 };
+
+struct Node1D {
+    int order;       // 0 root, 1, 2
+    int res;         // resolution index (length N_pad >> res)
+    int parent;      // node index, -1 for the root
+    int filt;        // packed-filter index of the wavelet at the parent's resolution
+    int k;           // periodisation factor from the parent
+    int leaf_filt;   // phi.levels[res] (local average)
+    int leaf_k;      // low-pass periodisation factor (local average)
+    int out_off;     // offset of this path in a signal's output row
+    int unpad_res;   // border-index resolution of the unpad
+    int need_fwd;    // forward FFT of |U| needed (children or local leaf or global sum)
+    long long slot;  // spectrum slot in the per-signal workspace, -1 none
+};
+
+template <class T>
+struct Scat1DArgs {
+    const Node1D* nodes;
+    const FftLen<T>* lens;
+    const long long* filt_off;
+    const int* group;
+    int ngroup;
+    int len[KYM_MAX_RES1], n[KYM_MAX_RES1];
+    int ind_start[KYM_MAX_RES1], ind_end[KYM_MAX_RES1];
+    int mode, log2_stride;
+    const T* x;
+    int N, pad_left;
+    const T* filt;
+    cpx<T>* ws;
+    long long ws_sig;
+    T* out;
+    long long out_row;
+    int img0, ntask;
+};
+
+// Output of one path from the spectrum / modulus in LDS.
+template <class T, int MAXP>
+KYM_HD void leaf1d(const Scat1DArgs<T>& a, const Node1D& nd, const cpx<T>* U, int n, int img, cpx<T>* B, int tid,
+                   int nth) {
+    T* o = a.out + (long long)img * a.out_row + nd.out_off;
+    if (a.mode == AVG1_GLOBAL) {
+        if (tid == 0) o[0] = U[0].re;
+        return;
+    }
+    const int kk = nd.leaf_k, nl = n / kk;
+    const T* phi = a.filt + a.filt_off[nd.leaf_filt];
+    const T inv_k = T(1) / T(kk);
+    for (int i = tid; i < nl; i += nth) {
+        cpx<T> acc = {T(0), T(0)};
+        for (int q = 0; q < kk; ++q) {
+            const cpx<T> u = U[i + q * nl];
+            const T f = phi[i + q * nl];
+            acc.re += u.re * f;
+            acc.im += u.im * f;
+        }
+        B[i] = acc * inv_k;
+    }
+    wg_barrier();
+    // the low-passed path has length N_pad >> max(res, log2_stride) = nl
+    const int lr = nd.res > a.log2_stride ? nd.res : a.log2_stride;
+    const FftLen<T>& fB = a.lens[a.len[lr]];
+    fft_dif<true, MAXP>(B, fB, Batch{1, 1, nl, false}, tid, nth);
+    const int s = a.ind_start[a.log2_stride], e = a.ind_end[a.log2_stride];
+    const T inv_n = T(1) / T(nl);
+    for (int p = tid; p < nl; p += nth) {
+        const int t = fB.rev[p];
+        if (t >= s && t < e) o[t - s] = B[p].re * inv_n;
+    }
+    wg_barrier();
+}
+
+template <class T, int MAXP>
+KYM_HD void body_node1d(const Scat1DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
+    const int il = task / a.ngroup, gi = task - il * a.ngroup;
+    const int img = a.img0 + il;
+    const Node1D& nd = a.nodes[a.group[gi]];
+    const int r = nd.res, n = a.n[r];
+    const FftLen<T>& fl = a.lens[a.len[r]];
+    cpx<T>* wsi = a.ws + il * a.ws_sig;
+    T* o = a.out + (long long)img * a.out_row + nd.out_off;
+    if (nd.order == 0) {
+        const T* xin = a.x + (long long)img * a.N;
+        for (int p = tid; p < n; p += nth) lds[p] = {xin[reflect_index(fl.rev[p] - a.pad_left, a.N)], T(0)};
+        if (a.mode == AVG1_NONE) {
+            // S_0 = U_0 unpadded = the input itself
+            const int s = a.ind_start[0], e = a.ind_end[0];
+            for (int t = s + tid; t < e; t += nth) o[t - s] = xin[reflect_index(t - a.pad_left, a.N)];
+        }
+        wg_barrier();
+        fft_dit<false, MAXP>(lds, fl, Batch{1, 1, n, false}, tid, nth);
+    } else {
+        const Node1D& pn = a.nodes[nd.parent];
+        const cpx<T>* U = wsi + pn.slot;
+        const T* f = a.filt + a.filt_off[nd.filt];
+        const int k = nd.k;
+        const T inv_k = T(1) / T(k);
+        for (int i = tid; i < n; i += nth) {
+            cpx<T> acc = {T(0), T(0)};
+            for (int q = 0; q < k; ++q) {
+                const cpx<T> u = U[i + q * n];
+                const T fv = f[i + q * n];
+                acc.re += u.re * fv;
+                acc.im += u.im * fv;
+            }
+            lds[i] = acc * inv_k;
+        }
+        wg_barrier();
+        fft_dif<true, MAXP>(lds, fl, Batch{1, 1, n, false}, tid, nth);
+        const T inv_n = T(1) / T(n);
+        const int s = a.ind_start[nd.unpad_res], e = a.ind_end[nd.unpad_res];
+        for (int p = tid; p < n; p += nth) {
+            const cpx<T> v = lds[p];
+            const T m = kym_sqrt(v.re * v.re + v.im * v.im) * inv_n;
+            lds[p] = {m, T(0)};
+            if (a.mode == AVG1_NONE) {
+                const int t = fl.rev[p];
+                if (t >= s && t < e) o[t - s] = m;
+            }
+        }
+        wg_barrier();
+        if (nd.need_fwd) fft_dit<false, MAXP>(lds, fl, Batch{1, 1, n, false}, tid, nth);
+    }
+    if (nd.slot >= 0) {
+        cpx<T>* dst = wsi + nd.slot;
+        for (int i = tid; i < n; i += nth) dst[i] = lds[i];
+    }
+    if (a.mode != AVG1_NONE) leaf1d<T, MAXP>(a, nd, lds, n, img, lds + n, tid, nth);
+}

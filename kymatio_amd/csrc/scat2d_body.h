@@ -18,7 +18,7 @@
 // (gather, inverse FFT, modulus, forward FFT, leaf); bigger ones run as
 // column / row / column passes through their workspace slot.
 #pragma once
-#include "fft_core.h"
+#include "fft_static.h"
 
 #define KYM_MAX_RES 16
 
@@ -84,82 +84,156 @@ KYM_HD KYM_INL T input_pixel(const Scat2DArgs<T>& a, const T* xin, int y, int x)
     return xin[(long long)reflect_index(y - a.top, a.M) * a.N + reflect_index(x - a.left, a.N)];
 }
 
-// Leaf: periodise U (H x W, natural, at `U`, stride W) times phi into `B`
-// (HJ x WJ), inverse FFT, real part, unpad, store into the output channel.
+// ------------------------------------------------------------------ geometry
+// A node of H x W with its leaf of HJ x WJ.  DynGeom reads the sizes and
+// FFT plans at run time; StatGeom fixes them at compile time (constant
+// divisors, unrolled passes).  Both use the same twiddle/rev tables.
 template <class T, int MAXP>
-KYM_HD void leaf_body(const Scat2DArgs<T>& a, const cpx<T>* U, const T* phi, int H, int W, int kk, int ch,
-                      int img, cpx<T>* B, int tid, int nth) {
-    const int J = a.J, HJ = a.H[J], WJ = a.W[J];
+struct DynGeom {
+    int H, W, HJ, WJ;
+    const FftLen<T>* fH;
+    const FftLen<T>* fW;
+    const FftLen<T>* fHJ;
+    const FftLen<T>* fWJ;
+    KYM_HD const int* revH() const { return fH->rev; }
+    KYM_HD const int* revW() const { return fW->rev; }
+    KYM_HD const int* revHJ() const { return fHJ->rev; }
+    KYM_HD const int* revWJ() const { return fWJ->rev; }
+    KYM_HD void inv2d(cpx<T>* b, int tid, int nth) const {
+        fft_dif<true, MAXP>(b, *fW, Batch{H, 1, W, false}, tid, nth);
+        fft_dif<true, MAXP>(b, *fH, Batch{W, W, 1, true}, tid, nth);
+    }
+    KYM_HD void fwd2d(cpx<T>* b, int tid, int nth) const {
+        fft_dit<false, MAXP>(b, *fW, Batch{H, 1, W, false}, tid, nth);
+        fft_dit<false, MAXP>(b, *fH, Batch{W, W, 1, true}, tid, nth);
+    }
+    KYM_HD void leaf_inv2d(cpx<T>* b, int tid, int nth) const {
+        fft_dif<true, MAXP>(b, *fWJ, Batch{HJ, 1, WJ, false}, tid, nth);
+        fft_dif<true, MAXP>(b, *fHJ, Batch{WJ, WJ, 1, true}, tid, nth);
+    }
+};
+
+template <class T, int H_, int W_, int HJ_, int WJ_, int NT>
+struct StatGeom {
+    static constexpr int H = H_, W = W_, HJ = HJ_, WJ = WJ_;
+    const FftLen<T>* fH;
+    const FftLen<T>* fW;
+    const FftLen<T>* fHJ;
+    const FftLen<T>* fWJ;
+    KYM_HD const int* revH() const { return fH->rev; }
+    KYM_HD const int* revW() const { return fW->rev; }
+    KYM_HD const int* revHJ() const { return fHJ->rev; }
+    KYM_HD const int* revWJ() const { return fWJ->rev; }
+    KYM_HD void inv2d(cpx<T>* b, int tid, int) const {
+        fft_dif_s<true, W, H, 1, W, false, NT>(b, fW->tw, tid);
+        fft_dif_s<true, H, W, W, 1, true, NT>(b, fH->tw, tid);
+    }
+    KYM_HD void fwd2d(cpx<T>* b, int tid, int) const {
+        fft_dit_s<false, W, H, 1, W, false, NT>(b, fW->tw, tid);
+        fft_dit_s<false, H, W, W, 1, true, NT>(b, fH->tw, tid);
+    }
+    KYM_HD void leaf_inv2d(cpx<T>* b, int tid, int) const {
+        fft_dif_s<true, WJ, HJ, 1, WJ, false, NT>(b, fWJ->tw, tid);
+        fft_dif_s<true, HJ, WJ, WJ, 1, true, NT>(b, fHJ->tw, tid);
+    }
+};
+
+// Leaf: periodise U (H x W, natural, row stride W) times phi into `B`
+// (HJ x WJ), inverse FFT, real part, unpad, store into the output channel.
+template <class T, class G>
+KYM_HD void leaf_body(const Scat2DArgs<T>& a, const G& g, const cpx<T>* U, const T* phi, int kk, int ch, int img,
+                      cpx<T>* B, int tid, int nth) {
     const T inv_k2 = T(1) / T(kk * kk);
-    for (int i = tid; i < HJ * WJ; i += nth) {
-        const int y = i / WJ, x = i - (i / WJ) * WJ;
-        B[i] = gather_periodized(U, phi, W, HJ, WJ, kk, y, x) * inv_k2;
+    for (int i = tid; i < g.HJ * g.WJ; i += nth) {
+        const int y = i / g.WJ, x = i - (i / g.WJ) * g.WJ;
+        B[i] = gather_periodized(U, phi, g.W, g.HJ, g.WJ, kk, y, x) * inv_k2;
     }
     wg_barrier();
-    const FftLen<T>& fH = a.lens[a.lenH[J]];
-    const FftLen<T>& fW = a.lens[a.lenW[J]];
-    fft_dif<true, MAXP>(B, fW, Batch{HJ, 1, WJ, false}, tid, nth);
-    fft_dif<true, MAXP>(B, fH, Batch{WJ, WJ, 1, true}, tid, nth);
-    const T inv_n = T(1) / T(HJ * WJ);
+    g.leaf_inv2d(B, tid, nth);
+    const T inv_n = T(1) / T(g.HJ * g.WJ);
     T* o = a.out + ((long long)img * a.C + ch) * a.oh * a.ow;
-    for (int p = tid; p < HJ * WJ; p += nth) {
-        const int py = p / WJ, px = p - py * WJ;
-        const int y = fH.rev[py], x = fW.rev[px];
-        if (y >= 1 && y <= HJ - 2 && x >= 1 && x <= WJ - 2) o[(y - 1) * a.ow + (x - 1)] = B[p].re * inv_n;
+    const int* rH = g.revHJ();
+    const int* rW = g.revWJ();
+    for (int p = tid; p < g.HJ * g.WJ; p += nth) {
+        const int py = p / g.WJ, px = p - py * g.WJ;
+        const int y = rH[py], x = rW[px];
+        if (y >= 1 && y <= g.HJ - 2 && x >= 1 && x <= g.WJ - 2) o[(y - 1) * a.ow + (x - 1)] = B[p].re * inv_n;
     }
     wg_barrier();
 }
 
 // ------------------------------------------------------------------ bodies
-// A whole node in LDS.  task -> (image of chunk, node of group).
-template <class T, int MAXP>
-KYM_HD void body_lds_node(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
-    const int il = task / a.ngroup, gi = task - il * a.ngroup;
+// A whole node in LDS: root (pad + forward FFT) or order 1/2 (gather +
+// periodise, inverse FFT, modulus, forward FFT); then the optional store of
+// the spectrum for the children and the optional in-kernel leaf.
+template <class T, class G>
+KYM_HD void node_body(const Scat2DArgs<T>& a, const G& g, const Node2D& nd, int il, cpx<T>* lds, int tid, int nth) {
     const int img = a.img0 + il;
-    const Node2D& nd = a.nodes[a.group[gi]];
-    const int r = nd.res, H = a.H[r], W = a.W[r];
-    const FftLen<T>& fH = a.lens[a.lenH[r]];
-    const FftLen<T>& fW = a.lens[a.lenW[r]];
     cpx<T>* wsi = a.ws + il * a.ws_img;
     if (nd.order == 0) {
         const T* xin = a.x + (long long)img * a.M * a.N;
-        for (int i = tid; i < H * W; i += nth) {
-            const int py = i / W, px = i - py * W;
-            lds[i] = {input_pixel(a, xin, fH.rev[py], fW.rev[px]), T(0)};
+        const int* rH = g.revH();
+        const int* rW = g.revW();
+        for (int i = tid; i < g.H * g.W; i += nth) {
+            const int py = i / g.W, px = i - py * g.W;
+            lds[i] = {input_pixel(a, xin, rH[py], rW[px]), T(0)};
         }
         wg_barrier();
-        fft_dit<false, MAXP>(lds, fW, Batch{H, 1, W, false}, tid, nth);
-        fft_dit<false, MAXP>(lds, fH, Batch{W, W, 1, true}, tid, nth);
+        g.fwd2d(lds, tid, nth);
     } else {
         const Node2D& pn = a.nodes[nd.parent];
         const cpx<T>* U = wsi + pn.slot;
         const T* f = a.filt + a.filt_off[nd.filt];
-        const int Wp = a.W[pn.res];
+        const int Wp = g.W * nd.k;
         const T inv_k2 = T(1) / T(nd.k * nd.k);
-        for (int i = tid; i < H * W; i += nth) {
-            const int y = i / W, x = i - (i / W) * W;
-            lds[i] = gather_periodized(U, f, Wp, H, W, nd.k, y, x) * inv_k2;
+        for (int i = tid; i < g.H * g.W; i += nth) {
+            const int y = i / g.W, x = i - (i / g.W) * g.W;
+            lds[i] = gather_periodized(U, f, Wp, g.H, g.W, nd.k, y, x) * inv_k2;
         }
         wg_barrier();
-        fft_dif<true, MAXP>(lds, fW, Batch{H, 1, W, false}, tid, nth);
-        fft_dif<true, MAXP>(lds, fH, Batch{W, W, 1, true}, tid, nth);
-        const T inv_n = T(1) / T(H * W);
-        for (int i = tid; i < H * W; i += nth) {
+        g.inv2d(lds, tid, nth);
+        const T inv_n = T(1) / T(g.H * g.W);
+        for (int i = tid; i < g.H * g.W; i += nth) {
             const cpx<T> v = lds[i];
             lds[i] = {kym_sqrt(v.re * v.re + v.im * v.im) * inv_n, T(0)};
         }
         wg_barrier();
-        fft_dit<false, MAXP>(lds, fW, Batch{H, 1, W, false}, tid, nth);
-        fft_dit<false, MAXP>(lds, fH, Batch{W, W, 1, true}, tid, nth);
+        g.fwd2d(lds, tid, nth);
     }
     if (nd.slot >= 0) {
         cpx<T>* dst = wsi + nd.slot;
-        for (int i = tid; i < H * W; i += nth) dst[i] = lds[i];
+        for (int i = tid; i < g.H * g.W; i += nth) dst[i] = lds[i];
     }
-    if (nd.leaf_mode == 0) {
-        const T* phi = a.filt + a.filt_off[nd.leaf_filt];
-        leaf_body<T, MAXP>(a, lds, phi, H, W, nd.leaf_k, nd.leaf_ch, img, lds + H * W, tid, nth);
-    }
+    if (nd.leaf_mode == 0)
+        leaf_body(a, g, lds, a.filt + a.filt_off[nd.leaf_filt], nd.leaf_k, nd.leaf_ch, img, lds + g.H * g.W, tid,
+                  nth);
+}
+
+template <class T, int MAXP>
+KYM_HD DynGeom<T, MAXP> dyn_geom(const Scat2DArgs<T>& a, int r) {
+    const int J = a.J;
+    return DynGeom<T, MAXP>{a.H[r], a.W[r], a.H[J], a.W[J], &a.lens[a.lenH[r]], &a.lens[a.lenW[r]],
+                            &a.lens[a.lenH[J]], &a.lens[a.lenW[J]]};
+}
+
+// A whole node in LDS.  task -> (image of chunk, node of group).
+template <class T, int MAXP>
+KYM_HD void body_lds_node(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
+    const int il = task / a.ngroup, gi = task - il * a.ngroup;
+    const Node2D& nd = a.nodes[a.group[gi]];
+    node_body(a, dyn_geom<T, MAXP>(a, nd.res), nd, il, lds, tid, nth);
+}
+
+// The same with compile-time sizes (the plan only selects it when the
+// launch's node and leaf sizes equal the template's).
+template <class T, int H, int W, int HJ, int WJ, int NT>
+KYM_HD void body_lds_node_s(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid) {
+    const int il = task / a.ngroup, gi = task - il * a.ngroup;
+    const Node2D& nd = a.nodes[a.group[gi]];
+    const int r = nd.res, J = a.J;
+    const StatGeom<T, H, W, HJ, WJ, NT> g{&a.lens[a.lenH[r]], &a.lens[a.lenW[r]], &a.lens[a.lenH[J]],
+                                          &a.lens[a.lenW[J]]};
+    node_body(a, g, nd, il, lds, tid, NT);
 }
 
 // Leaf from a spectrum kept in the workspace.  task -> (image, node).
@@ -170,7 +244,25 @@ KYM_HD void body_leaf(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, in
     const int r = nd.res;
     const cpx<T>* U = a.ws + il * a.ws_img + nd.slot;
     const T* phi = a.filt + a.filt_off[nd.leaf_filt];
-    leaf_body<T, MAXP>(a, U, phi, a.H[r], a.W[r], nd.leaf_k, nd.leaf_ch, a.img0 + il, lds, tid, nth);
+    leaf_body(a, dyn_geom<T, MAXP>(a, r), U, phi, nd.leaf_k, nd.leaf_ch, a.img0 + il, lds, tid, nth);
+}
+
+template <class T, int HJ, int WJ, int NT>
+KYM_HD void body_leaf_s(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid) {
+    const int il = task / a.ngroup, gi = task - il * a.ngroup;
+    const Node2D& nd = a.nodes[a.group[gi]];
+    const int r = nd.res, J = a.J;
+    // node sizes only enter through g.W (row stride of U): keep them dynamic
+    struct G : StatGeom<T, 1, 1, HJ, WJ, NT> {
+        int W;
+    } g;
+    g.fH = &a.lens[a.lenH[r]];
+    g.fW = &a.lens[a.lenW[r]];
+    g.fHJ = &a.lens[a.lenH[J]];
+    g.fWJ = &a.lens[a.lenW[J]];
+    g.W = a.W[r];
+    const cpx<T>* U = a.ws + il * a.ws_img + nd.slot;
+    leaf_body(a, g, U, a.filt + a.filt_off[nd.leaf_filt], nd.leaf_k, nd.leaf_ch, a.img0 + il, lds, tid, NT);
 }
 
 // Root row pass: rows of the padded image (row storage positions
@@ -308,6 +400,58 @@ KYM_HD void body_fwd_cols(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid
     for (int i = tid; i < H * nc; i += nth) {
         const int y = i / nc, c = i - y * nc;
         buf[(long long)y * W + c] = lds[i];
+    }
+}
+
+// Compile-time specialisations: X(id, H, W, HJ, WJ, threads) for LDS nodes,
+// X(id, HJ, WJ, threads) for leaves.  A launch uses one when its sizes match;
+// every other size runs the run-time-sized bodies.
+#define KYM_NODE_SPECS(X)            \
+    X(0, 136, 136, 34, 34, 1024)     \
+    X(1, 68, 68, 34, 34, 512)        \
+    X(2, 80, 80, 10, 10, 512)        \
+    X(3, 40, 40, 10, 10, 256)        \
+    X(4, 20, 20, 10, 10, 256)
+#define KYM_LEAF_SPECS(X) \
+    X(0, 34, 34, 256)     \
+    X(1, 10, 10, 256)
+
+inline int node_spec_id(int H, int W, int HJ, int WJ, int* threads) {
+#define KYM_NS_MATCH(id, h, w, hj, wj, nt) \
+    if (H == h && W == w && HJ == hj && WJ == wj) { *threads = nt; return id; }
+    KYM_NODE_SPECS(KYM_NS_MATCH)
+#undef KYM_NS_MATCH
+    return -1;
+}
+
+inline int leaf_spec_id(int HJ, int WJ, int* threads) {
+#define KYM_LS_MATCH(id, hj, wj, nt) \
+    if (HJ == hj && WJ == wj) { *threads = nt; return id; }
+    KYM_LEAF_SPECS(KYM_LS_MATCH)
+#undef KYM_LS_MATCH
+    return -1;
+}
+
+// EMU = true: one thread per workgroup (CPU harness), otherwise the spec's count.
+template <class T, bool EMU>
+KYM_HD void dispatch_node_spec(int id, const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid) {
+    switch (id) {
+#define KYM_NS_CASE(sid, h, w, hj, wj, nt) \
+    case sid: body_lds_node_s<T, h, w, hj, wj, (EMU ? 1 : nt)>(a, task, lds, tid); break;
+        KYM_NODE_SPECS(KYM_NS_CASE)
+#undef KYM_NS_CASE
+        default: break;
+    }
+}
+
+template <class T, bool EMU>
+KYM_HD void dispatch_leaf_spec(int id, const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid) {
+    switch (id) {
+#define KYM_LS_CASE(sid, hj, wj, nt) \
+    case sid: body_leaf_s<T, hj, wj, (EMU ? 1 : nt)>(a, task, lds, tid); break;
+        KYM_LEAF_SPECS(KYM_LS_CASE)
+#undef KYM_LS_CASE
+        default: break;
     }
 }
 

@@ -16,6 +16,7 @@ struct Launch2D {
     size_t lds_bytes;
     int res;                // resolution of the launch's nodes
     long long model_bytes;  // algorithmic HBM bytes per image (stage-boundary model)
+    int spec;               // compile-time specialisation id, -1 = run-time sizes
 };
 
 template <class T>
@@ -51,6 +52,8 @@ struct Scat2DPlan {
         while ((1 << r) < v) ++r;
         return r;
     }
+
+    bool use_specs = true;  // select compile-time specialised kernels when sizes match (FP32 only)
 
     bool build(int M_, int N_, int J_, int L_, int max_order_, int pre_pad_, size_t lds_limit_, KymError* e) {
         M = M_; N = N_; J = J_; L = L_; max_order = max_order_; pre_pad = pre_pad_;
@@ -194,6 +197,7 @@ struct Scat2DPlan {
                 if (leaf_here) mb += leaf_model;
             }
             l.model_bytes = mb;
+            l.spec = -1;
             launches.push_back(l);
             max_lds = std::max(max_lds, lds);
             max_threads = std::max(max_threads, threads);
@@ -213,7 +217,12 @@ struct Scat2DPlan {
                 if (nodes[id].leaf_mode == 1) small.push_back(id);
                 if (nodes[id].leaf_mode == 2) big.push_back(id);
             }
-            if (!small.empty()) push(K2_LEAF, small, 1, 0, node_threads(leaf_elems), leaf_elems * cs);
+            if (!small.empty()) {
+                push(K2_LEAF, small, 1, 0, node_threads(leaf_elems), leaf_elems * cs);
+                int nt = 0;
+                const int sid = leaf_spec_id(H[J], W[J], &nt);
+                if (use_specs && sizeof(T) == 4 && sid >= 0) { launches.back().spec = sid; launches.back().threads = nt; }
+            }
             if (!big.empty()) {
                 const int cb = col_blk(H[J]), rb = row_blk(W[J]);
                 push(K2_LEAF_ICOL, big, (W[J] + cb - 1) / cb, cb, 256, (size_t)H[J] * cb * cs);
@@ -228,6 +237,12 @@ struct Scat2DPlan {
             if (!n0.big) {
                 const size_t lds = (elems + (n0.leaf_mode == 0 ? leaf_elems : 0)) * cs;
                 push(K2_LDS_NODE, ids, 1, 0, node_threads(elems), lds);
+                int nt = 0;
+                const int sid = node_spec_id(H[r], W[r], H[J], W[J], &nt);
+                if (use_specs && sizeof(T) == 4 && sid >= 0 && n0.leaf_mode == 0) {
+                    launches.back().spec = sid;
+                    launches.back().threads = nt;
+                }
             } else {
                 const int cb = col_blk(H[r]), rb = row_blk(W[r]);
                 if (is_root) push(K2_ROOT_ROWS, ids, (H[r] + rb - 1) / rb, rb, 256, (size_t)rb * W[r] * cs);

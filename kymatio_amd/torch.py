@@ -1,8 +1,11 @@
 """kymatio_amd.torch: the torch.nn.Module scattering transforms on MI355X
 (drop-in names of kymatio/torch.py:1-27)."""
+from .scattering1d.frontend.torch_frontend import ScatteringTorch1D as Scattering1D
 from .scattering2d.frontend.torch_frontend import ScatteringTorch2D as Scattering2D
 
+Scattering1D.__module__ = 'kymatio_amd.torch'
+Scattering1D.__name__ = 'Scattering1D'
 Scattering2D.__module__ = 'kymatio_amd.torch'
 Scattering2D.__name__ = 'Scattering2D'
 
-__all__ = ['Scattering2D']
+__all__ = ['Scattering1D', 'Scattering2D']

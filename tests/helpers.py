@@ -81,4 +81,36 @@ class Emu:
                                              filters_packed.ctypes.data, out.ctypes.data, B, chunk, info)
         if rc != 0:
             raise RuntimeError(self.lib.kym_emu_last_error().decode())
-        return out, {'launches': info[3], 'big_nodes': info[4]}
+        return out, {'launches': info[3], 'big_nodes': info[4], 'spec_launches': info[5]}
+
+
+def emu_scat1d(emu, module, x, dtype=np.float32):
+    """Run the 1D bodies in the CPU harness with the setup of a (CPU) Scattering1D module."""
+    m = module
+    modes = {'local': 0, False: 1, 'global': 2}
+    i32 = lambda v: (ctypes.c_int * max(len(v), 1))(*v)
+    f = emu.lib.kym_emu_scat1d_forward
+    f.argtypes = ([ctypes.c_int] * 8 + [ctypes.c_void_p] * 2 + [ctypes.c_int] + [ctypes.c_void_p] * 2
+                  + [ctypes.c_int] + [ctypes.c_void_p] * 2 + [ctypes.c_int, ctypes.c_int]
+                  + [ctypes.c_void_p] * 3 + [ctypes.c_longlong, ctypes.c_void_p])
+    f.restype = ctypes.c_int
+    p1, p2 = m.psi1_f, m.psi2_f
+    nind = max(m.ind_start.keys()) + 1
+    bufs = [lev.numpy().reshape(-1) for bank in ([m.phi_f], p1, p2) for flt in bank for lev in flt['levels']]
+    packed = np.concatenate(bufs).astype(dtype)
+    args = [m.shape[0], m._N_padded, m.pad_left, m.log2_stride, modes[m.average], m._oversampling, m.max_order,
+            len(p1), i32([p['j'] for p in p1]), i32([len(p['levels']) for p in p1]),
+            len(p2), i32([p['j'] for p in p2]), i32([len(p['levels']) for p in p2]),
+            len(m.phi_f['levels']), i32([m.ind_start[i] for i in range(nind)]),
+            i32([m.ind_end[i] for i in range(nind)]), nind, 0 if dtype == np.float32 else 1]
+    info = (ctypes.c_longlong * 4)()
+    B = x.shape[0]
+    rc = f(*args, None, None, None, B, info)
+    if rc != 0:
+        raise RuntimeError(emu.lib.kym_emu_last_error().decode())
+    out = np.zeros((B, info[1]), dtype)
+    xx = np.ascontiguousarray(x, dtype)
+    rc = f(*args, xx.ctypes.data, packed.ctypes.data, out.ctypes.data, B, info)
+    if rc != 0:
+        raise RuntimeError(emu.lib.kym_emu_last_error().decode())
+    return out, info[0]

@@ -55,9 +55,9 @@ def test_headline_error_by_order_vs_fp64():
     sc = Scattering2D(3, (256, 256)).to(DEV)
     out = sc(torch.from_numpy(x).to(DEV)).cpu().numpy()
     assert channel_rel_err(out, G['cfg2_Sx']) <= RTOL_CHANNEL
-    fb = {'phi': {'levels': [v.numpy()[..., 0] for v in sc.load_filters()[0]['levels']]},
-          'psi': [{'levels': [v.numpy()[..., 0] for v in p['levels']], 'j': p['j']}
-                  for p in sc.load_filters()[1]]}
+    phi, psi = sc.load_filters()
+    fb = {'phi': {'levels': [v.cpu().numpy()[..., 0] for v in phi['levels']]},
+          'psi': [{'levels': [v.cpu().numpy()[..., 0] for v in p['levels']], 'j': p['j']} for p in psi]}
     ref64 = scat2d.scattering2d_numpy(x, 3, filters=fb)
     orders = scat2d.channel_orders(3, 8)
     for o in (0, 1, 2):
