@@ -50,8 +50,9 @@ int kym_scat2d_create(int M, int N, int J, int L, int max_order, int pre_pad, in
                       long long lds_limit, void** plan);
 int kym_scat2d_destroy(void* plan);
 
-/* info[11] = {C, out_h, out_w, M_padded, N_padded, n_filters, filter_elems,
- *             workspace_bytes_per_image, n_launches, pad_top, pad_left} */
+/* info[12] = {C, out_h, out_w, M_padded, N_padded, n_filters, filter_elems,
+ *             workspace_bytes_per_image, n_launches, pad_top, pad_left,
+ *             training_workspace_bytes_per_image (-1 if not differentiable)} */
 int kym_scat2d_info(void* plan, long long* info);
 
 /* Packed filter layout: filter i (the reference's buffer tensor{i}: phi
@@ -66,6 +67,19 @@ int kym_scat2d_filter_offsets(void* plan, long long* offs);
  * ws_bytes / workspace_bytes_per_image images). */
 int kym_scat2d_forward(void* plan, const void* x, const void* filt, void* out, long long batch,
                        void* ws, long long ws_bytes, void* stream);
+
+/* Differentiation (replaces torch autograd through the reference's ops:
+ * ModulusStable.backward, kymatio/backend/torch_backend.py:63-96, and the
+ * autograd of fft2/ifft2, mean, the real filter product, ReflectionPad2d and
+ * the unpad slice).  forward_train is kym_scat2d_forward that also keeps
+ * every node's pre-modulus field in `ws`, which must hold the whole batch
+ * (training_workspace_bytes_per_image * batch).  backward then reads dS
+ * (= dL/dS, layout of out) and that workspace and writes gx = dL/dx
+ * (layout of x); it must be given the same ws, untouched in between. */
+int kym_scat2d_forward_train(void* plan, const void* x, const void* filt, void* out, long long batch,
+                             void* ws, long long ws_bytes, void* stream);
+int kym_scat2d_backward(void* plan, const void* dS, const void* filt, void* gx, long long batch, void* ws,
+                        long long ws_bytes, void* stream);
 
 /* Launch list introspection: rec[7] = {kind, nodes, tasks_per_node, threads,
  * lds_bytes, model_bytes_per_image, resolution}.  model_bytes is the

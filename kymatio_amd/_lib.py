@@ -28,6 +28,8 @@ _SIGNATURES = {
     'kym_scat2d_info': [_c_vp, ctypes.POINTER(_c_ll)],
     'kym_scat2d_filter_offsets': [_c_vp, ctypes.POINTER(_c_ll)],
     'kym_scat2d_forward': [_c_vp, _c_vp, _c_vp, _c_vp, _c_ll, _c_vp, _c_ll, _c_vp],
+    'kym_scat2d_forward_train': [_c_vp, _c_vp, _c_vp, _c_vp, _c_ll, _c_vp, _c_ll, _c_vp],
+    'kym_scat2d_backward': [_c_vp, _c_vp, _c_vp, _c_vp, _c_ll, _c_vp, _c_ll, _c_vp],
     'kym_scat2d_launch_info': [_c_vp, _c_i, ctypes.POINTER(_c_ll)],
     'kym_scat2d_profile': [_c_vp, _c_i],
     'kym_scat2d_profile_read': [_c_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_c_ll), _c_i],

@@ -21,8 +21,13 @@ HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 COMMON = ['-O3', '-std=c++17', '-fPIC', '-shared', '-Wno-unused-result',
           '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops']
 
-PRODUCT_SOURCES = ['capi.hip']
+# (source, extra defines, object name); compiled in parallel, then linked
+PRODUCT_UNITS = [('capi.hip', [], 'capi.o'), ('kspec2d.hip', [], 'kspec2d.o'), ('kinst1d.hip', [], 'kinst1d.o')] + [
+    ('kinst2d.hip', ['-DKYM_T=%s' % T, '-DKYM_P=%d' % P, '-DKYM_PICK=kym_pick2d_%s_p%d' % (tag, P)],
+     'kinst2d_%s_p%d.o' % (tag, P))
+    for tag, T in (('f32', 'float'), ('f64', 'double')) for P in (7, 17, 31)]
 EMU_SOURCES = ['emu.hip']
+BUILD_DIR = os.path.join(HERE, '_build')
 
 
 def _run(cmd):
@@ -32,10 +37,23 @@ def _run(cmd):
     print('  %.1f s' % (time.time() - t), flush=True)
 
 
-def build_product():
+def build_product(jobs=None):
+    from concurrent.futures import ThreadPoolExecutor
+    os.makedirs(BUILD_DIR, exist_ok=True)
+    jobs = jobs or min(len(PRODUCT_UNITS), max(1, os.cpu_count() or 1))
+    objs = []
+
+    def compile_unit(u):
+        src, defs, obj = u
+        o = os.path.join(BUILD_DIR, obj)
+        _run([HIPCC, '--offload-arch=' + ARCH, '-c'] + [f for f in COMMON if f != '-shared'] + defs + [src, '-o', o])
+        return o
+
+    with ThreadPoolExecutor(jobs) as ex:
+        objs = list(ex.map(compile_unit, PRODUCT_UNITS))
     out = os.path.join(PKG, 'libkymatio_hip.so')
     tmp = out + '.tmp'
-    _run([HIPCC, '--offload-arch=' + ARCH] + COMMON + PRODUCT_SOURCES + ['-o', tmp])
+    _run([HIPCC, '--offload-arch=' + ARCH, '-shared', '-fPIC'] + objs + ['-o', tmp])
     os.replace(tmp, out)
     return out
 

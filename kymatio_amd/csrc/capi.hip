@@ -5,8 +5,7 @@
 #include <string>
 
 #include "../../include/kymatio_hip.h"
-#include "kernels1d.h"
-#include "kernels2d.h"
+#include "picks.h"
 #include "scat1d_plan.h"
 #include "scat2d_plan.h"
 
@@ -76,12 +75,14 @@ static int upload_and_prepare(Scat2DPlan<T>& p, void** dblob) {
     KYM_HIP_TRY(hipMalloc(dblob, bytes));
     p.blob.relocate((uintptr_t)*dblob);
     KYM_HIP_TRY(hipMemcpy(*dblob, p.blob.bytes.data(), bytes, hipMemcpyHostToDevice));
-    for (const Launch2D& l : p.launches) {
-        Scat2DKernel<T> k = pick_scat2d<T>(l.kind, l.threads, p.pclass, l.spec);
-        if (l.lds_bytes > 65536)
-            KYM_HIP_TRY(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                            (int)l.lds_bytes));
-    }
+    for (const std::vector<Launch2D>* ls : {&p.launches, &p.bwd_launches})
+        for (const Launch2D& l : *ls) {
+            Scat2DKernel<T> k = pick_scat2d<T>(l.kind, l.threads, p.pclass, l.spec);
+            if (!k) return fail(KYM_ERR_UNSUPPORTED, "no kernel for launch kind");
+            if (l.lds_bytes > 65536)
+                KYM_HIP_TRY(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                (int)l.lds_bytes));
+        }
     return KYM_OK;
 }
 
@@ -141,6 +142,7 @@ static void fill_info(const Scat2DPlan<T>& p, long long* info) {
     info[8] = (long long)p.launches.size();
     info[9] = p.top;
     info[10] = p.left;
+    info[11] = p.trainable ? p.ws_img_train * (long long)sizeof(cpx<T>) : -1;
 }
 
 extern "C" int kym_scat2d_info(void* plan, long long* info) {
@@ -159,21 +161,32 @@ extern "C" int kym_scat2d_filter_offsets(void* plan, long long* offs) {
     return KYM_OK;
 }
 
+// mode 0: inference forward (batch in chunks); 1: training forward (saves
+// u, whole batch in the workspace); 2: backward from the saved workspace.
 template <class T>
-static int forward2d(const Scat2DPlan<T>& p, const void* x, const void* filt, void* out, long long batch,
-                     void* ws, long long ws_bytes, hipStream_t stream, uintptr_t base, LaunchProfile* prof) {
-    const long long per = p.ws_img * (long long)sizeof(cpx<T>);
+static int run2d(const Scat2DPlan<T>& p, int mode, const void* x, const void* filt, void* out, const void* dS,
+                 void* gx, long long batch, void* ws, long long ws_bytes, hipStream_t stream, uintptr_t base,
+                 LaunchProfile* prof) {
+    const long long img_elems = mode == 0 ? p.ws_img : p.ws_img_train;
+    const long long per = img_elems * (long long)sizeof(cpx<T>);
+    if (mode != 0 && !p.trainable) return fail(KYM_ERR_UNSUPPORTED, p.train_error);
     long long chunk = per > 0 ? ws_bytes / per : batch;
     if (chunk < 1) return fail(KYM_ERR_VALUE, "workspace too small for one image");
     if (chunk > batch) chunk = batch;
+    if (mode != 0 && chunk < batch) return fail(KYM_ERR_VALUE, "training workspace must hold the whole batch");
+    const std::vector<Launch2D>& launches = mode == 2 ? p.bwd_launches : p.launches;
     for (long long i0 = 0; i0 < batch; i0 += chunk) {
         const int nimg = (int)std::min(chunk, batch - i0);
-        for (const Launch2D& l : p.launches) {
+        for (const Launch2D& l : launches) {
             Scat2DArgs<T> a = p.args(l, base);
             a.x = static_cast<const T*>(x);
             a.filt = static_cast<const T*>(filt);
             a.ws = static_cast<cpx<T>*>(ws);
+            a.ws_img = img_elems;
             a.out = static_cast<T*>(out);
+            a.dS = static_cast<const T*>(dS);
+            a.gx = static_cast<T*>(gx);
+            a.save = mode == 1 ? 1 : 0;
             a.img0 = (int)i0;
             const long long ntask = (long long)nimg * l.ngroup * l.nblk;
             if (ntask > (1ll << 30)) return fail(KYM_ERR_VALUE, "too many tasks in one launch");
@@ -190,7 +203,7 @@ static int forward2d(const Scat2DPlan<T>& p, const void* x, const void* filt, vo
             KYM_HIP_TRY(hipGetLastError());
             if (e0 && e1) {
                 KYM_HIP_TRY(hipEventRecord(e1, stream));
-                prof->recs.push_back({(int)(&l - p.launches.data()), {e0, e1}});
+                prof->recs.push_back({(int)(&l - launches.data()), {e0, e1}});
             }
         }
     }
@@ -208,8 +221,38 @@ extern "C" int kym_scat2d_forward(void* plan, const void* x, const void* filt, v
     KYM_HIP_TRY(hipGetDevice(&cur));
     if (cur != h->device) return fail(KYM_ERR_TYPE, "plan was created for another device");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    if (h->pf) return forward2d(*h->pf, x, filt, out, batch, ws, ws_bytes, s, (uintptr_t)h->dblob, &h->prof);
-    return forward2d(*h->pd, x, filt, out, batch, ws, ws_bytes, s, (uintptr_t)h->dblob, &h->prof);
+    if (h->pf) return run2d(*h->pf, 0, x, filt, out, nullptr, nullptr, batch, ws, ws_bytes, s, (uintptr_t)h->dblob, &h->prof);
+    return run2d(*h->pd, 0, x, filt, out, nullptr, nullptr, batch, ws, ws_bytes, s, (uintptr_t)h->dblob, &h->prof);
+}
+
+extern "C" int kym_scat2d_forward_train(void* plan, const void* x, const void* filt, void* out, long long batch,
+                                        void* ws, long long ws_bytes, void* stream) {
+    if (!plan) return fail(KYM_ERR_VALUE, "null plan");
+    if (batch < 0) return fail(KYM_ERR_VALUE, "negative batch");
+    if (batch == 0) return KYM_OK;
+    if (!x || !filt || !out || !ws) return fail(KYM_ERR_VALUE, "null device pointer");
+    KymScat2D* h = static_cast<KymScat2D*>(plan);
+    int cur = 0;
+    KYM_HIP_TRY(hipGetDevice(&cur));
+    if (cur != h->device) return fail(KYM_ERR_TYPE, "plan was created for another device");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (h->pf) return run2d(*h->pf, 1, x, filt, out, nullptr, nullptr, batch, ws, ws_bytes, s, (uintptr_t)h->dblob, &h->prof);
+    return run2d(*h->pd, 1, x, filt, out, nullptr, nullptr, batch, ws, ws_bytes, s, (uintptr_t)h->dblob, &h->prof);
+}
+
+extern "C" int kym_scat2d_backward(void* plan, const void* dS, const void* filt, void* gx, long long batch,
+                                   void* ws, long long ws_bytes, void* stream) {
+    if (!plan) return fail(KYM_ERR_VALUE, "null plan");
+    if (batch < 0) return fail(KYM_ERR_VALUE, "negative batch");
+    if (batch == 0) return KYM_OK;
+    if (!dS || !filt || !gx || !ws) return fail(KYM_ERR_VALUE, "null device pointer");
+    KymScat2D* h = static_cast<KymScat2D*>(plan);
+    int cur = 0;
+    KYM_HIP_TRY(hipGetDevice(&cur));
+    if (cur != h->device) return fail(KYM_ERR_TYPE, "plan was created for another device");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (h->pf) return run2d(*h->pf, 2, nullptr, filt, nullptr, dS, gx, batch, ws, ws_bytes, s, (uintptr_t)h->dblob, nullptr);
+    return run2d(*h->pd, 2, nullptr, filt, nullptr, dS, gx, batch, ws, ws_bytes, s, (uintptr_t)h->dblob, nullptr);
 }
 
 template <class T>

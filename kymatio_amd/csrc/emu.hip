@@ -59,6 +59,55 @@ static int emu_forward2d(int M, int N, int J, int L, int max_order, int pre_pad,
     return 0;
 }
 
+template <class T>
+static int emu_train2d(int M, int N, int J, int L, int max_order, int pre_pad, long long lds_limit, const void* x,
+                       const void* filt, void* out, const void* dS, void* gx, long long batch) {
+    Scat2DPlan<T> p;
+    KymError e = {0, ""};
+    if (!p.build(M, N, J, L, max_order, pre_pad, (size_t)lds_limit, &e)) {
+        g_emu_error = e.msg;
+        return e.code;
+    }
+    if (!p.trainable) {
+        g_emu_error = p.train_error;
+        return KYM_ERR_UNSUPPORTED;
+    }
+    p.blob.relocate((uintptr_t)p.blob.bytes.data());
+    const uintptr_t base = (uintptr_t)p.blob.bytes.data();
+    std::vector<cpx<T>> ws((size_t)(p.ws_img_train * batch) + 1);
+    std::vector<cpx<T>> lds(p.max_lds / sizeof(cpx<T>) + 1);
+    for (int phase = 0; phase < 2; ++phase) {
+        const std::vector<Launch2D>& ls = phase == 0 ? p.launches : p.bwd_launches;
+        for (const Launch2D& l : ls) {
+            Scat2DArgs<T> a = p.args(l, base);
+            a.x = static_cast<const T*>(x);
+            a.filt = static_cast<const T*>(filt);
+            a.ws = ws.data();
+            a.ws_img = p.ws_img_train;
+            a.out = static_cast<T*>(out);
+            a.dS = static_cast<const T*>(dS);
+            a.gx = static_cast<T*>(gx);
+            a.save = 1;
+            a.img0 = 0;
+            a.ntask = (int)(batch * l.ngroup * l.nblk);
+            for (int t = 0; t < a.ntask; ++t) {
+                if (l.kind >= K2B_NODE) dispatch_bwd2d<T, 31>(l.kind, a, t, lds.data(), 0, 1);
+                else if (l.spec >= 0 && l.kind == K2_LDS_NODE) dispatch_node_spec<T, true>(l.spec, a, t, lds.data(), 0);
+                else if (l.spec >= 0 && l.kind == K2_LEAF) dispatch_leaf_spec<T, true>(l.spec, a, t, lds.data(), 0);
+                else dispatch_body2d<T, 31>(l.kind, a, t, lds.data(), 0, 1);
+            }
+        }
+    }
+    return 0;
+}
+
+extern "C" int kym_emu_scat2d_train(int M, int N, int J, int L, int max_order, int pre_pad, int dtype,
+                                    long long lds_limit, const void* x, const void* filt, void* out, const void* dS,
+                                    void* gx, long long batch) {
+    if (dtype == 0) return emu_train2d<float>(M, N, J, L, max_order, pre_pad, lds_limit, x, filt, out, dS, gx, batch);
+    return emu_train2d<double>(M, N, J, L, max_order, pre_pad, lds_limit, x, filt, out, dS, gx, batch);
+}
+
 extern "C" int kym_emu_scat2d_forward(int M, int N, int J, int L, int max_order, int pre_pad, int dtype,
                                       long long lds_limit, const void* x, const void* filt, void* out,
                                       long long batch, long long chunk, long long* info) {

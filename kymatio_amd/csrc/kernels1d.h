@@ -15,7 +15,7 @@ template <class T>
 using Scat1DKernel = void (*)(Scat1DArgs<T>);
 
 template <class T>
-Scat1DKernel<T> pick_scat1d(int threads) {
+Scat1DKernel<T> pick_scat1d_nt(int threads) {
     if (threads >= 1024) return k_scat1d<T, 1024>;
     if (threads >= 512) return k_scat1d<T, 512>;
     if (threads >= 256) return k_scat1d<T, 256>;

@@ -1,6 +1,6 @@
 // __global__ wrappers around the 2D workgroup bodies.
 #pragma once
-#include "scat2d_body.h"
+#include "scat2d_bwd.h"
 
 // Workgroups are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8).
 // Give each XCD a contiguous range of tasks so that all consumers of one
@@ -17,7 +17,8 @@ __global__ void __launch_bounds__(NT) k_scat2d(Scat2DArgs<T> a) {
     const int task = xcd_task(blockIdx.x, gridDim.x);
     if (task >= a.ntask) return;
     cpx<T>* lds = reinterpret_cast<cpx<T>*>(kym_smem2d);
-    dispatch_body2d<T, MAXP>(KIND, a, task, lds, threadIdx.x, NT);
+    if constexpr (KIND >= K2B_NODE) dispatch_bwd2d<T, MAXP>(KIND, a, task, lds, threadIdx.x, NT);
+    else dispatch_body2d<T, MAXP>(KIND, a, task, lds, threadIdx.x, NT);
 }
 
 template <class T, int SPEC, int NT>
@@ -81,17 +82,12 @@ Scat2DKernel<T> pick_scat2d_p(int kind, int threads) {
         case K2_FWD_COLS: return k_scat2d<T, K2_FWD_COLS, 256, MAXP>;
         case K2_LEAF_ICOL: return k_scat2d<T, K2_LEAF_ICOL, 256, MAXP>;
         case K2_LEAF_ROWS: return k_scat2d<T, K2_LEAF_ROWS, 256, MAXP>;
+        case K2B_NODE: return pick_scat2d_nt<T, K2B_NODE, MAXP>(threads);
+        case K2B_LEAFG: return pick_scat2d_nt<T, K2B_LEAFG, MAXP>(threads);
+        case K2B_GCOLS: return k_scat2d<T, K2B_GCOLS, 256, MAXP>;
+        case K2B_GROWS: return k_scat2d<T, K2B_GROWS, 256, MAXP>;
+        case K2B_GFCOLS: return k_scat2d<T, K2B_GFCOLS, 256, MAXP>;
+        case K2B_FOLD: return k_scat2d<T, K2B_FOLD, 256, MAXP>;
         default: return nullptr;
     }
-}
-
-// pclass: 7, 17 or 31 (kym_prime_class of the plan's largest odd prime);
-// spec >= 0 selects a compile-time specialisation of the node / leaf kernel.
-template <class T>
-Scat2DKernel<T> pick_scat2d(int kind, int threads, int pclass, int spec = -1) {
-    if (spec >= 0 && kind == K2_LDS_NODE) return pick_node_spec<T>(spec);
-    if (spec >= 0 && kind == K2_LEAF) return pick_leaf_spec<T>(spec);
-    if (pclass <= 7) return pick_scat2d_p<T, 7>(kind, threads);
-    if (pclass <= 17) return pick_scat2d_p<T, 17>(kind, threads);
-    return pick_scat2d_p<T, 31>(kind, threads);
 }

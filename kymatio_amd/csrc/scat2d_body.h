@@ -35,6 +35,11 @@ struct Node2D {
     int big;        // 1 = does not fit in LDS (column/row passes)
     long long slot;       // offset (complex elements) of U in the per-image workspace, -1 none
     long long leaf_slot;  // scratch for big leaves, -1 none
+    // training (forward with saving + backward)
+    int child_begin, child_end;  // node-index range of this node's children
+    long long save;              // slot of the saved pre-modulus field u (then of dL/dX), -1 none
+    long long gslot;             // leaf dL/dY (natural, scaled) when not formed in the node kernel
+    long long gpad;              // root: dL/d(padded input), real
 };
 
 template <class T>
@@ -58,6 +63,10 @@ struct Scat2DArgs {
     int C, oh, ow;
     int img0;   // first image of this chunk
     int ntask;  // tasks in this launch
+    // training
+    int save;        // forward: store u into each node's save slot
+    const T* dS;     // backward: dL/dS, same layout as out
+    T* gx;           // backward: dL/dx, same layout as x
 };
 
 // ------------------------------------------------------------------ helpers
@@ -111,6 +120,10 @@ struct DynGeom {
         fft_dif<true, MAXP>(b, *fWJ, Batch{HJ, 1, WJ, false}, tid, nth);
         fft_dif<true, MAXP>(b, *fHJ, Batch{WJ, WJ, 1, true}, tid, nth);
     }
+    KYM_HD void leaf_fwd2d(cpx<T>* b, int tid, int nth) const {
+        fft_dit<false, MAXP>(b, *fWJ, Batch{HJ, 1, WJ, false}, tid, nth);
+        fft_dit<false, MAXP>(b, *fHJ, Batch{WJ, WJ, 1, true}, tid, nth);
+    }
 };
 
 template <class T, int H_, int W_, int HJ_, int WJ_, int NT>
@@ -135,6 +148,10 @@ struct StatGeom {
     KYM_HD void leaf_inv2d(cpx<T>* b, int tid, int) const {
         fft_dif_s<true, WJ, HJ, 1, WJ, false, NT>(b, fWJ->tw, tid);
         fft_dif_s<true, HJ, WJ, WJ, 1, true, NT>(b, fHJ->tw, tid);
+    }
+    KYM_HD void leaf_fwd2d(cpx<T>* b, int tid, int) const {
+        fft_dit_s<false, WJ, HJ, 1, WJ, false, NT>(b, fWJ->tw, tid);
+        fft_dit_s<false, HJ, WJ, WJ, 1, true, NT>(b, fHJ->tw, tid);
     }
 };
 
@@ -193,8 +210,10 @@ KYM_HD void node_body(const Scat2DArgs<T>& a, const G& g, const Node2D& nd, int 
         wg_barrier();
         g.inv2d(lds, tid, nth);
         const T inv_n = T(1) / T(g.H * g.W);
+        cpx<T>* sv = (a.save && nd.save >= 0) ? wsi + nd.save : nullptr;
         for (int i = tid; i < g.H * g.W; i += nth) {
             const cpx<T> v = lds[i];
+            if (sv) sv[i] = v;
             lds[i] = {kym_sqrt(v.re * v.re + v.im * v.im) * inv_n, T(0)};
         }
         wg_barrier();
@@ -347,8 +366,10 @@ KYM_HD void body_rows_mod(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid
     wg_barrier();
     fft_dif<true, MAXP>(lds, fW, Batch{nr, 1, W, false}, tid, nth);
     const T inv_n = T(1) / T(H * W);
+    cpx<T>* sv = (a.save && nd.save >= 0) ? a.ws + il * a.ws_img + nd.save + (long long)r0 * W : nullptr;
     for (int i = tid; i < nr * W; i += nth) {
         const cpx<T> v = lds[i];
+        if (sv) sv[i] = v;
         lds[i] = {kym_sqrt(v.re * v.re + v.im * v.im) * inv_n, T(0)};
     }
     wg_barrier();

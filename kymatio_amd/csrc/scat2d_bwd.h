@@ -1,0 +1,261 @@
+// Scattering2D backward: hand-written adjoint of every stage, node by node
+// in reverse order (order 2, order 1, root), each node pulling the input
+// gradients of its children.  With L real and complex gradients written as
+// dL/dRe + i dL/dIm (the convention of torch autograd):
+//   leaf   S = unpad(Re(ifft2(Y)))         dL/dY = fft2(unpad^T dS) / n_J
+//          Y = P_kl(U * phi)               dL/dU += phi * tile(dL/dY) / kl^2
+//   child  X_c = P_kc(U * f_c)             dL/dU += f_c * tile(dL/dX_c) / kc^2
+//   U = fft2(m), m real                    dL/dm = Re(n ifft2(dL/dU))  (unnormalised inverse)
+//   m = |u|                                dL/du = dL/dm * u / |u|, 0 where u = 0
+//                                          (kymatio/backend/torch_backend.py:63-96)
+//   u = ifft2(X)                           dL/dX = fft2(dL/du) / n
+//   root U_0 = fft2(pad(x))                dL/dpad(x) = Re(n ifft2(dL/dU_0)), folded by the
+//                                          reflect-pad adjoint into dL/dx.
+// The forward run in training mode stores u (unnormalised, in the
+// digit-reversed order the inverse DIF leaves it in) in the node's save slot;
+// the backward overwrites it with dL/dX (natural order).
+#pragma once
+#include "scat2d_body.h"
+
+// dL/dU at (y, x) of a node of H x W: leaf term (GY: HJ x WJ, already scaled
+// by 1 / (n_J kl^2)) plus the children's dL/dX (scaled here by 1 / kc^2).
+template <class T>
+KYM_HD KYM_INL cpx<T> grad_u_elem(const Scat2DArgs<T>& a, const Node2D& nd, const cpx<T>* wsi, const cpx<T>* GY,
+                                  int W, int HJ, int WJ, int y, int x) {
+    const long long i = (long long)y * W + x;
+    const T ph = a.filt[a.filt_off[nd.leaf_filt] + i];
+    const cpx<T> gy = GY[(y % HJ) * WJ + (x % WJ)];
+    cpx<T> acc = {gy.re * ph, gy.im * ph};
+    for (int c = nd.child_begin; c < nd.child_end; ++c) {
+        const Node2D& cn = a.nodes[c];
+        const int Hc = a.H[cn.res], Wc = a.W[cn.res];
+        const T fv = a.filt[a.filt_off[cn.filt] + i] / T(cn.k * cn.k);
+        const cpx<T> gx = wsi[cn.save + (long long)(y % Hc) * Wc + (x % Wc)];
+        acc.re += gx.re * fv;
+        acc.im += gx.im * fv;
+    }
+    return acc;
+}
+
+// Leaf gradient into B (HJ x WJ, natural), scaled by 1 / (n_J kl^2).
+template <class T, class G>
+KYM_HD void leaf_grad(const Scat2DArgs<T>& a, const G& g, const Node2D& nd, int img, cpx<T>* B, int tid, int nth) {
+    const T* d = a.dS + ((long long)img * a.C + nd.leaf_ch) * a.oh * a.ow;
+    const int* rH = g.revHJ();
+    const int* rW = g.revWJ();
+    for (int p = tid; p < g.HJ * g.WJ; p += nth) {
+        const int py = p / g.WJ, px = p - py * g.WJ;
+        const int y = rH[py], x = rW[px];
+        T v = T(0);
+        if (y >= 1 && y <= g.HJ - 2 && x >= 1 && x <= g.WJ - 2) v = d[(y - 1) * a.ow + (x - 1)];
+        B[p] = {v, T(0)};
+    }
+    wg_barrier();
+    g.leaf_fwd2d(B, tid, nth);
+    const T sc = T(1) / (T(g.HJ * g.WJ) * T(nd.leaf_k * nd.leaf_k));
+    for (int p = tid; p < g.HJ * g.WJ; p += nth) B[p] = B[p] * sc;
+    wg_barrier();
+}
+
+// A whole node's backward in LDS.
+template <class T, class G>
+KYM_HD void bwd_node_body(const Scat2DArgs<T>& a, const G& g, const Node2D& nd, int il, cpx<T>* lds, int tid,
+                          int nth) {
+    const int img = a.img0 + il;
+    cpx<T>* wsi = a.ws + il * a.ws_img;
+    const int n = g.H * g.W;
+    const cpx<T>* GY;
+    if (nd.leaf_mode == 0) {
+        leaf_grad(a, g, nd, img, lds + n, tid, nth);
+        GY = lds + n;
+    } else {
+        GY = wsi + nd.gslot;
+    }
+    for (int i = tid; i < n; i += nth) {
+        const int y = i / g.W, x = i - (i / g.W) * g.W;
+        lds[i] = grad_u_elem(a, nd, wsi, GY, g.W, g.HJ, g.WJ, y, x);
+    }
+    wg_barrier();
+    g.inv2d(lds, tid, nth);  // unnormalised: Re() is dL/dm (or dL/dpad(x) at the root)
+    if (nd.order == 0) {
+        T* gp = reinterpret_cast<T*>(wsi + nd.gpad);
+        const int* rH = g.revH();
+        const int* rW = g.revW();
+        for (int p = tid; p < n; p += nth) {
+            const int py = p / g.W, px = p - py * g.W;
+            gp[rH[py] * g.W + rW[px]] = lds[p].re;
+        }
+        return;
+    }
+    cpx<T>* sv = wsi + nd.save;
+    for (int p = tid; p < n; p += nth) {
+        const cpx<T> u = sv[p];
+        const T mag = kym_sqrt(u.re * u.re + u.im * u.im);
+        const T gm = lds[p].re;
+        lds[p] = mag > T(0) ? cpx<T>{u.re * (gm / mag), u.im * (gm / mag)} : cpx<T>{T(0), T(0)};
+    }
+    wg_barrier();
+    g.fwd2d(lds, tid, nth);
+    const T inv_n = T(1) / T(n);
+    for (int i = tid; i < n; i += nth) sv[i] = lds[i] * inv_n;
+}
+
+template <class T, int MAXP>
+KYM_HD void body_bwd_node(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
+    const int il = task / a.ngroup, gi = task - il * a.ngroup;
+    const Node2D& nd = a.nodes[a.group[gi]];
+    bwd_node_body(a, dyn_geom<T, MAXP>(a, nd.res), nd, il, lds, tid, nth);
+}
+
+// Leaf gradient of a node whose leaf is not formed in its node kernel.
+template <class T, int MAXP>
+KYM_HD void body_bwd_leafg(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
+    const int il = task / a.ngroup, gi = task - il * a.ngroup;
+    const Node2D& nd = a.nodes[a.group[gi]];
+    const auto g = dyn_geom<T, MAXP>(a, nd.res);
+    leaf_grad(a, g, nd, a.img0 + il, lds, tid, nth);
+    cpx<T>* dst = a.ws + il * a.ws_img + nd.gslot;
+    for (int i = tid; i < g.HJ * g.WJ; i += nth) dst[i] = lds[i];
+}
+
+// Big nodes: dL/dU by column blocks, inverse DIF along y -> U slot (rows digit-reversed).
+template <class T, int MAXP>
+KYM_HD void body_bwd_gcols(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
+    const int per = a.ngroup * a.nblk;
+    const int il = task / per, rem = task - il * per;
+    const int gi = rem / a.nblk, cb = rem - gi * a.nblk;
+    const Node2D& nd = a.nodes[a.group[gi]];
+    const int r = nd.res, H = a.H[r], W = a.W[r], J = a.J;
+    const int c0 = cb * a.blk, nc = (W - c0 < a.blk) ? W - c0 : a.blk;
+    cpx<T>* wsi = a.ws + il * a.ws_img;
+    for (int i = tid; i < H * nc; i += nth) {
+        const int y = i / nc, c = i - (i / nc) * nc;
+        lds[i] = grad_u_elem(a, nd, wsi, wsi + nd.gslot, W, a.H[J], a.W[J], y, c0 + c);
+    }
+    wg_barrier();
+    fft_dif<true, MAXP>(lds, a.lens[a.lenH[r]], Batch{nc, nc, 1, true}, tid, nth);
+    cpx<T>* dst = wsi + nd.slot;
+    for (int i = tid; i < H * nc; i += nth) {
+        const int p = i / nc, c = i - (i / nc) * nc;
+        dst[(long long)p * W + c0 + c] = lds[i];
+    }
+}
+
+// Big nodes: rows of the U slot, inverse DIF along x, then modulus adjoint and
+// forward DIT along x into the save slot (root: dL/dpad(x) into gpad).
+template <class T, int MAXP>
+KYM_HD void body_bwd_grows(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
+    const int per = a.ngroup * a.nblk;
+    const int il = task / per, rem = task - il * per;
+    const int gi = rem / a.nblk, rb = rem - gi * a.nblk;
+    const Node2D& nd = a.nodes[a.group[gi]];
+    const int r = nd.res, H = a.H[r], W = a.W[r];
+    const FftLen<T>& fW = a.lens[a.lenW[r]];
+    const FftLen<T>& fH = a.lens[a.lenH[r]];
+    const int r0 = rb * a.blk, nr = (H - r0 < a.blk) ? H - r0 : a.blk;
+    cpx<T>* wsi = a.ws + il * a.ws_img;
+    const cpx<T>* src = wsi + nd.slot + (long long)r0 * W;
+    for (int i = tid; i < nr * W; i += nth) lds[i] = src[i];
+    wg_barrier();
+    fft_dif<true, MAXP>(lds, fW, Batch{nr, 1, W, false}, tid, nth);
+    if (nd.order == 0) {
+        T* gp = reinterpret_cast<T*>(wsi + nd.gpad);
+        for (int i = tid; i < nr * W; i += nth) {
+            const int rr = i / W, px = i - rr * W;
+            gp[(long long)fH.rev[r0 + rr] * W + fW.rev[px]] = lds[i].re;
+        }
+        return;
+    }
+    cpx<T>* sv = wsi + nd.save + (long long)r0 * W;
+    for (int i = tid; i < nr * W; i += nth) {
+        const cpx<T> u = sv[i];
+        const T mag = kym_sqrt(u.re * u.re + u.im * u.im);
+        const T gm = lds[i].re;
+        lds[i] = mag > T(0) ? cpx<T>{u.re * (gm / mag), u.im * (gm / mag)} : cpx<T>{T(0), T(0)};
+    }
+    wg_barrier();
+    fft_dit<false, MAXP>(lds, fW, Batch{nr, 1, W, false}, tid, nth);
+    for (int i = tid; i < nr * W; i += nth) sv[i] = lds[i];
+}
+
+// Big nodes: forward DIT along y of the save slot -> natural dL/dX (scaled 1/n).
+template <class T, int MAXP>
+KYM_HD void body_bwd_gfcols(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
+    const int per = a.ngroup * a.nblk;
+    const int il = task / per, rem = task - il * per;
+    const int gi = rem / a.nblk, cb = rem - gi * a.nblk;
+    const Node2D& nd = a.nodes[a.group[gi]];
+    const int r = nd.res, H = a.H[r], W = a.W[r];
+    const int c0 = cb * a.blk, nc = (W - c0 < a.blk) ? W - c0 : a.blk;
+    cpx<T>* buf = a.ws + il * a.ws_img + nd.save + c0;
+    for (int i = tid; i < H * nc; i += nth) {
+        const int y = i / nc, c = i - y * nc;
+        lds[i] = buf[(long long)y * W + c];
+    }
+    wg_barrier();
+    fft_dit<false, MAXP>(lds, a.lens[a.lenH[r]], Batch{nc, nc, 1, true}, tid, nth);
+    const T inv_n = T(1) / T(H * W);
+    for (int i = tid; i < H * nc; i += nth) {
+        const int y = i / nc, c = i - y * nc;
+        buf[(long long)y * W + c] = lds[i] * inv_n;
+    }
+}
+
+// Reflect-pad adjoint: dL/dx[i][j] = sum of dL/dpad over every padded
+// position that reads input pixel (i, j).  task -> (image, row block).
+KYM_HD KYM_INL int refl_preimages(int i, int n, int pad, int P, int* out) {
+    // positions y in [0, P) with reflect_index(y - pad, n) == i
+    int c = 0;
+    if (n == 1) {
+        for (int y = 0; y < P && c < 64; ++y) out[c++] = y;
+        return c;
+    }
+    const int T_ = 2 * (n - 1);
+    int y0 = (pad + i) % T_;
+    for (int y = y0; y < P && c < 64; y += T_) out[c++] = y;
+    if (i != 0 && i != n - 1) {
+        int y1 = ((pad - i) % T_ + T_) % T_;
+        for (int y = y1; y < P && c < 64; y += T_) out[c++] = y;
+    }
+    return c;
+}
+
+template <class T, int MAXP>
+KYM_HD void body_bwd_fold(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
+    (void)lds;
+    const int il = task / a.nblk, rb = task - il * a.nblk;
+    const int img = a.img0 + il;
+    const int P = a.H[0], Q = a.W[0];
+    const T* gp = reinterpret_cast<const T*>(a.ws + il * a.ws_img + a.nodes[0].gpad);
+    T* gx = a.gx + (long long)img * a.M * a.N;
+    const int r0 = rb * a.blk, nr = (a.M - r0 < a.blk) ? a.M - r0 : a.blk;
+    for (int e = tid; e < nr * a.N; e += nth) {
+        const int i = r0 + e / a.N, j = e - (e / a.N) * a.N;
+        if (a.pre_pad) {
+            gx[(long long)i * a.N + j] = gp[(long long)i * Q + j];
+            continue;
+        }
+        int ys[64], xs[64];
+        const int ny = refl_preimages(i, a.M, a.top, P, ys);
+        const int nx = refl_preimages(j, a.N, a.left, Q, xs);
+        T acc = T(0);
+        for (int u = 0; u < ny; ++u)
+            for (int v = 0; v < nx; ++v) acc += gp[(long long)ys[u] * Q + xs[v]];
+        gx[(long long)i * a.N + j] = acc;
+    }
+}
+
+enum Kind2DBwd { K2B_NODE = 16, K2B_LEAFG, K2B_GCOLS, K2B_GROWS, K2B_GFCOLS, K2B_FOLD };
+
+template <class T, int MAXP>
+KYM_HD KYM_INL void dispatch_bwd2d(int kind, const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
+    switch (kind) {
+        case K2B_NODE: body_bwd_node<T, MAXP>(a, task, lds, tid, nth); break;
+        case K2B_LEAFG: body_bwd_leafg<T, MAXP>(a, task, lds, tid, nth); break;
+        case K2B_GCOLS: body_bwd_gcols<T, MAXP>(a, task, lds, tid, nth); break;
+        case K2B_GROWS: body_bwd_grows<T, MAXP>(a, task, lds, tid, nth); break;
+        case K2B_GFCOLS: body_bwd_gfcols<T, MAXP>(a, task, lds, tid, nth); break;
+        case K2B_FOLD: body_bwd_fold<T, MAXP>(a, task, lds, tid, nth); break;
+        default: break;
+    }
+}

@@ -6,7 +6,7 @@
 #include <algorithm>
 
 #include "plan_common.h"
-#include "scat2d_body.h"
+#include "scat2d_bwd.h"
 
 struct Launch2D {
     int kind;
@@ -34,6 +34,11 @@ struct Scat2DPlan {
     std::vector<Node2D> nodes;
     long long ws_img = 0;  // complex elements of workspace per image
     std::vector<Launch2D> launches;
+    // training: forward with saving, then the backward launches
+    long long ws_img_train = 0;  // complex elements per image (layout extends the inference one)
+    std::vector<Launch2D> bwd_launches;
+    bool trainable = true;
+    std::string train_error;
     LenTable<T> lens;
     int lenH[KYM_MAX_RES] = {0}, lenW[KYM_MAX_RES] = {0};
     // constants blob (relocated to its device or host home)
@@ -152,6 +157,27 @@ struct Scat2DPlan {
             const long long align = 256 / (long long)cs;
             ws_img = (ws_img + align - 1) / align * align;
         }
+        // training layout: save slot (u, then dL/dX) for every non-root node,
+        // leaf dL/dY slot where the leaf is not formed in the node kernel, and
+        // the root's dL/dpad(x); children as node-index ranges.
+        {
+            for (auto& nd : nodes) { nd.child_begin = nd.child_end = 0; nd.save = nd.gslot = nd.gpad = -1; }
+            for (size_t i = 1; i < nodes.size(); ++i) {
+                Node2D& pn = nodes[nodes[i].parent];
+                if (pn.child_end == 0) pn.child_begin = (int)i;
+                pn.child_end = (int)i + 1;
+            }
+            const long long align = 256 / (long long)cs;
+            long long w = ws_img;
+            for (auto& nd : nodes) {
+                if (nd.order > 0) { nd.save = w; w += (long long)H[nd.res] * W[nd.res]; }
+                if (nd.leaf_mode != 0) { nd.gslot = w; w += (long long)leaf_elems; }
+                if (nd.order == 0) { nd.gpad = w; w += ((long long)H[0] * W[0] * (long long)sizeof(T) + cs - 1) / cs; }
+                w = (w + align - 1) / align * align;
+                if (nd.leaf_mode == 2) { trainable = false; train_error = "leaf does not fit LDS: backward unsupported"; }
+            }
+            ws_img_train = w;
+        }
         // blob: nodes, lens, filter offsets, group lists
         blob = Blob();
         std::string err;
@@ -263,6 +289,43 @@ struct Scat2DPlan {
                         if (nodes[i].order == order && nodes[i].res == r && nodes[i].big == big) ids.push_back((int)i);
                     emit_nodes(ids, false);
                 }
+        // backward: order 2, order 1, root; then the pad fold
+        std::vector<Launch2D> fwd = launches;
+        launches.clear();
+        auto emit_bwd = [&](const std::vector<int>& ids) {
+            if (ids.empty()) return;
+            const Node2D& n0 = nodes[ids[0]];
+            const int r = n0.res;
+            const size_t elems = (size_t)H[r] * W[r];
+            std::vector<int> leafg;
+            for (int id : ids)
+                if (nodes[id].leaf_mode != 0) leafg.push_back(id);
+            if (!leafg.empty()) push(K2B_LEAFG, leafg, 1, 0, node_threads(leaf_elems), leaf_elems * cs);
+            if (!n0.big) {
+                const size_t lds = (elems + (n0.leaf_mode == 0 ? leaf_elems : 0)) * cs;
+                push(K2B_NODE, ids, 1, 0, node_threads(elems), lds);
+            } else {
+                const int cb = col_blk(H[r]), rb = row_blk(W[r]);
+                push(K2B_GCOLS, ids, (W[r] + cb - 1) / cb, cb, 256, (size_t)H[r] * cb * cs);
+                push(K2B_GROWS, ids, (H[r] + rb - 1) / rb, rb, 256, (size_t)rb * W[r] * cs);
+                if (n0.order > 0) push(K2B_GFCOLS, ids, (W[r] + cb - 1) / cb, cb, 256, (size_t)H[r] * cb * cs);
+            }
+        };
+        for (int order = max_order; order >= 1; --order)
+            for (int r = 0; r < J; ++r)
+                for (int big = 0; big <= 1; ++big) {
+                    std::vector<int> ids;
+                    for (size_t i = 0; i < nodes.size(); ++i)
+                        if (nodes[i].order == order && nodes[i].res == r && nodes[i].big == big) ids.push_back((int)i);
+                    emit_bwd(ids);
+                }
+        emit_bwd({0});
+        {
+            const int rb = 8;
+            push(K2B_FOLD, {0}, ((pre_pad ? Mp : M) + rb - 1) / rb, rb, 256, 0);
+        }
+        bwd_launches = launches;
+        launches = fwd;
         return true;
     }
 

@@ -14,10 +14,10 @@ class Scat2DPlanHandle:
                                          int(lds_limit), ctypes.byref(h)))
         self._h = h
         self._lib = lib
-        info = (ctypes.c_longlong * 11)()
+        info = (ctypes.c_longlong * 12)()
         _lib.check(lib.kym_scat2d_info(h, info))
         (self.C, self.oh, self.ow, self.Mp, self.Np, self.n_filters, self.filter_elems,
-         self.ws_bytes_per_image, self.n_launches, self.top, self.left) = list(info)
+         self.ws_bytes_per_image, self.n_launches, self.top, self.left, self.train_ws_bytes_per_image) = list(info)
         offs = (ctypes.c_longlong * (self.n_filters + 1))()
         _lib.check(lib.kym_scat2d_filter_offsets(h, offs))
         self.filter_offsets = list(offs)
@@ -52,6 +52,18 @@ class Scat2DPlanHandle:
             self._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(filt.data_ptr()),
             ctypes.c_void_p(out.data_ptr()), int(batch), ctypes.c_void_p(ws.data_ptr()), ws.numel(),
             _lib.stream_of(x.device)))
+
+    def forward_train(self, x, filt, out, batch, ws):
+        _lib.check(self._lib.kym_scat2d_forward_train(
+            self._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(filt.data_ptr()),
+            ctypes.c_void_p(out.data_ptr()), int(batch), ctypes.c_void_p(ws.data_ptr()), ws.numel(),
+            _lib.stream_of(x.device)))
+
+    def backward(self, dS, filt, gx, batch, ws):
+        _lib.check(self._lib.kym_scat2d_backward(
+            self._h, ctypes.c_void_p(dS.data_ptr()), ctypes.c_void_p(filt.data_ptr()),
+            ctypes.c_void_p(gx.data_ptr()), int(batch), ctypes.c_void_p(ws.data_ptr()), ws.numel(),
+            _lib.stream_of(dS.device)))
 
     def __del__(self):
         try:

@@ -190,16 +190,40 @@ class ScatteringTorch2D(ScatteringTorch, ScatteringBase):
 
 
 class ScatteringFunction2D(torch.autograd.Function):
-    """Forward through the fused HIP pipeline."""
+    """Fused HIP forward; when a gradient is needed, the training forward keeps
+    every node's pre-modulus field in a workspace that the hand-written
+    backward (kym_scat2d_backward) consumes."""
 
     @staticmethod
     def forward(ctx, x, module):
-        ctx.module = module
-        return module._forward_hip(x.contiguous())
+        x = x.contiguous()
+        if not ctx.needs_input_grad[0]:
+            return module._forward_hip(x)
+        device, dtype = x.device, x.dtype
+        plan = module._plan(device, dtype)
+        if plan.train_ws_bytes_per_image < 0:
+            raise NotImplementedError('this configuration has no backward on the HIP path')
+        filt = module._packed_filters(plan, device, dtype)
+        B = x.shape[0]
+        out = torch.empty((B, plan.C, plan.oh, plan.ow), dtype=dtype, device=device)
+        ws = torch.empty(max(B, 1) * plan.train_ws_bytes_per_image, dtype=torch.uint8, device=device)
+        if B:
+            with torch.cuda.device(device):
+                plan.forward_train(x, filt, out, B, ws)
+        ctx.save_for_backward(filt, ws)
+        ctx.plan, ctx.B, ctx.xshape = plan, B, x.shape
+        return out
 
     @staticmethod
+    @torch.autograd.function.once_differentiable
     def backward(ctx, grad):
-        raise NotImplementedError('Scattering2D backward is not available in this build yet.')
+        filt, ws = ctx.saved_tensors
+        g = grad.contiguous()
+        gx = torch.empty(ctx.xshape, dtype=g.dtype, device=g.device)
+        if ctx.B:
+            with torch.cuda.device(g.device):
+                ctx.plan.backward(g, filt, gx, ctx.B, ws)
+        return gx, None
 
 
 __all__ = ['ScatteringTorch2D']

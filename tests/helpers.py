@@ -114,3 +114,21 @@ def emu_scat1d(emu, module, x, dtype=np.float32):
     if rc != 0:
         raise RuntimeError(emu.lib.kym_emu_last_error().decode())
     return out, info[0]
+
+
+def emu_train2d(emu, x, J, shape, filters_packed, dS, L=8, max_order=2, pre_pad=0, lds_limit=160 * 1024):
+    """Training forward + hand-written backward of the 2D bodies on the CPU
+    harness; returns (S, dL/dx) for the upstream gradient dS."""
+    f = emu.lib.kym_emu_scat2d_train
+    f.argtypes = [ctypes.c_int] * 7 + [ctypes.c_longlong] + [ctypes.c_void_p] * 5 + [ctypes.c_longlong]
+    f.restype = ctypes.c_int
+    dt = filters_packed.dtype
+    xx = np.ascontiguousarray(x, dt)
+    dd = np.ascontiguousarray(dS, dt)
+    out = np.zeros(dS.shape, dt)
+    gx = np.zeros(xx.shape, dt)
+    rc = f(shape[0], shape[1], J, L, max_order, pre_pad, 0 if dt == np.float32 else 1, lds_limit, xx.ctypes.data,
+           filters_packed.ctypes.data, out.ctypes.data, dd.ctypes.data, gx.ctypes.data, x.shape[0])
+    if rc != 0:
+        raise RuntimeError(emu.lib.kym_emu_last_error().decode())
+    return out, gx

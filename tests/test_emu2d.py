@@ -79,3 +79,36 @@ def test_unsupported_prime_raises(emu):
 def test_small_j_error(emu):
     with pytest.raises(RuntimeError, match='smallest dimension'):
         emu.scat2d(np.zeros((1, 10, 10), np.float32), 4, (10, 10), np.zeros(1, np.float32))
+
+
+@pytest.mark.parametrize('shape,J,kw,lds', [
+    ((20, 24), 2, {}, 160 * 1024),
+    ((20, 24), 2, {}, 4096),
+    ((16, 16), 2, {'pre_pad': 1}, 160 * 1024),
+    ((31, 33), 3, {'max_order': 1}, 160 * 1024),
+    ((32, 32), 4, {}, 8192),
+    ((32, 32), 5, {}, 160 * 1024),
+    ((8, 8), 2, {'L': 4}, 160 * 1024),
+], ids=['lds', 'multipass', 'prepad', 'order1', 'J4-multipass', 'J5-fold', 'L4'])
+def test_backward_matches_autograd_through_oracle(emu, shape, J, kw, lds):
+    """FP64: the hand-written adjoint equals torch autograd through the
+    op-for-op restatement of the reference (what the reference's own
+    gradients are)."""
+    L = kw.get('L', 8)
+    fb, f = _filters(shape, J, L, np.float64)
+    rng = np.random.default_rng(0)
+    from kymatio_amd.scattering2d.utils import compute_padding
+    xs = compute_padding(shape[0], shape[1], J) if kw.get('pre_pad') else shape
+    x = rng.standard_normal((2,) + tuple(xs))
+    sc = scat2d.Scattering2DTorchCPU(J, shape, L=L, max_order=kw.get('max_order', 2),
+                                     pre_pad=bool(kw.get('pre_pad')), dtype=torch.float64, filters=fb)
+    xt = torch.from_numpy(x).requires_grad_()
+    S = sc(xt)
+    w = rng.standard_normal(S.shape)
+    (S * torch.from_numpy(w)).sum().backward()
+    from tests.helpers import emu_train2d
+    out, gx = emu_train2d(emu, x, J, shape, f, w, L=L, max_order=kw.get('max_order', 2),
+                          pre_pad=kw.get('pre_pad', 0), lds_limit=lds)
+    g = xt.grad.numpy()
+    assert np.abs(out - S.detach().numpy()).max() < 1e-13
+    assert np.abs(gx - g).max() <= 1e-12 * np.abs(g).max()

@@ -1,0 +1,100 @@
+"""Scattering2D gradients on the GPU: the hand-written backward against
+torch.autograd.gradcheck (the reference's own test), FP64 central finite
+differences on 16x16 patches, torch autograd through the oracle, and the
+gradients the reference produced (golden, J=4 128x128)."""
+import numpy as np
+import pytest
+import torch
+from torch.autograd import gradcheck
+
+from kymatio_amd.torch import Scattering2D
+from oracle import scat2d
+from tests.helpers import golden
+
+pytestmark = pytest.mark.gpu
+DEV = 'cuda'
+
+
+def test_gradcheck_reference_case():
+    """tests/scattering2d/test_torch_scattering2d.py:237-248."""
+    sc = Scattering2D(2, shape=(8, 8)).double().to(DEV)
+    x = torch.rand(2, 1, 8, 8).double().to(DEV).requires_grad_()
+    assert gradcheck(sc, x, nondet_tol=1e-5, fast_mode=True)
+
+
+def test_gradcheck_full_small():
+    sc = Scattering2D(2, shape=(12, 10), L=4).double().to(DEV)
+    x = torch.randn(1, 12, 10, dtype=torch.float64, device=DEV, requires_grad=True)
+    assert gradcheck(sc, x, nondet_tol=1e-9)
+
+
+def _loss(sc, x, y):
+    return ((sc(x) - sc(y)) ** 2).sum()
+
+
+def test_finite_differences_fp64_patches():
+    """J=4, 128x128 FP64: directional derivatives along random directions
+    supported on 16x16 patches, central differences with h = 1e-6."""
+    G = golden('golden_scat2d.npz')
+    x = torch.from_numpy(G['grad_x'].astype(np.float64)).to(DEV)
+    y = torch.from_numpy(G['grad_y'].astype(np.float64)).to(DEV)
+    sc = Scattering2D(4, (128, 128)).double().to(DEV)
+    xg = x.clone().requires_grad_()
+    _loss(sc, xg, y).backward()
+    g = xg.grad
+    rng = np.random.default_rng(0)
+    h = 1e-6
+    with torch.no_grad():
+        for trial in range(4):
+            i0, j0 = rng.integers(0, 112, size=2)
+            v = torch.zeros_like(x)
+            v[..., i0:i0 + 16, j0:j0 + 16] = torch.from_numpy(rng.standard_normal((16, 16))).to(DEV)
+            fd = (_loss(sc, x + h * v, y) - _loss(sc, x - h * v, y)) / (2 * h)
+            an = (g * v).sum()
+            print('patch (%d,%d): fd %.10e analytic %.10e' % (i0, j0, fd.item(), an.item()))
+            assert abs(fd.item() - an.item()) <= 1e-6 * max(abs(an.item()), 1e-12) + 1e-9
+
+
+@pytest.mark.parametrize('dtype,tol', [(np.float64, 1e-10), (np.float32, 2e-4)])
+def test_matches_reference_autograd_golden(dtype, tol):
+    """d/dx sum((S(x)-S(y))^2) at J=4 128x128, as torch autograd through the
+    reference computed it (tests/golden/make_golden.py)."""
+    G = golden('golden_scat2d.npz')
+    tag = 'f64' if dtype == np.float64 else 'f32'
+    tdt = torch.float64 if dtype == np.float64 else torch.float32
+    x = torch.from_numpy(G['grad_x'].astype(dtype)).to(DEV).requires_grad_()
+    y = torch.from_numpy(G['grad_y'].astype(dtype)).to(DEV)
+    sc = Scattering2D(4, (128, 128)).to(tdt).to(DEV)
+    loss = _loss(sc, x, y)
+    loss.backward()
+    ref = G['grad_' + tag + '_dx']
+    got = x.grad.cpu().numpy()
+    rel = np.abs(got - ref).max() / np.abs(ref).max()
+    print('loss %.10e ref %.10e, grad max rel err %.3e' % (loss.item(), float(G['grad_' + tag + '_loss']), rel))
+    assert abs(loss.item() - float(G['grad_' + tag + '_loss'])) <= tol * abs(float(G['grad_' + tag + '_loss']))
+    assert rel <= tol
+
+
+def test_matches_oracle_autograd_multipass():
+    """Multi-pass (big-node) backward route forced by a small LDS cap."""
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((2, 40, 36))
+    w = None
+    sc = Scattering2D(3, (40, 36)).double().to(DEV)
+    sc.lds_limit = 8192
+    xt = torch.from_numpy(x).to(DEV).requires_grad_()
+    S = sc(xt)
+    w = torch.from_numpy(rng.standard_normal(tuple(S.shape))).to(DEV)
+    (S * w).sum().backward()
+    ref_sc = scat2d.Scattering2DTorchCPU(3, (40, 36), dtype=torch.float64)
+    xr = torch.from_numpy(x).requires_grad_()
+    (ref_sc(xr) * w.cpu()).sum().backward()
+    assert torch.allclose(xt.grad.cpu(), xr.grad, rtol=1e-10, atol=1e-12 * xr.grad.abs().max().item())
+
+
+def test_no_grad_path_unchanged():
+    sc = Scattering2D(3, (32, 32)).to(DEV)
+    x = torch.randn(3, 32, 32, device=DEV)
+    a = sc(x)
+    b = sc(x.clone().requires_grad_())
+    assert torch.allclose(a, b.detach(), rtol=1e-6, atol=1e-7)
