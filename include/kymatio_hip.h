@@ -118,8 +118,9 @@ int kym_scat1d_create(int N, int N_pad, int pad_left, int log2_stride, int avera
                       const int* ind_end, int n_ind, int dtype, long long lds_limit, void** plan);
 int kym_scat1d_destroy(void* plan);
 
-/* info[6] = {n_paths, out_row_elems, n_filters, filter_elems,
- *            workspace_bytes_per_signal, n_launches} */
+/* info[7] = {n_paths, out_row_elems, n_filters, filter_elems,
+ *            workspace_bytes_per_signal, n_launches,
+ *            training_workspace_bytes_per_signal} */
 int kym_scat1d_info(void* plan, long long* info);
 
 /* filt_offs[n_filters + 1]: packed filter layout in the reference's buffer
@@ -132,6 +133,17 @@ int kym_scat1d_layout(void* plan, long long* filt_offs, long long* path_len);
  * [sum(path_len[:c]), ...); ws: workspace (>= 1 signal). */
 int kym_scat1d_forward(void* plan, const void* x, const void* filt, void* out, long long batch, void* ws,
                        long long ws_bytes, void* stream);
+
+/* Differentiation of the 1D path, as kym_scat2d_forward_train /
+ * kym_scat2d_backward: the training forward keeps every node's pre-modulus
+ * field in ws (training_workspace_bytes_per_signal * batch); backward reads
+ * dS (layout of out) and writes gx (layout of x).  Covers all three
+ * averaging modes (the un-averaged and global outputs feed the modulus
+ * gradient directly). */
+int kym_scat1d_forward_train(void* plan, const void* x, const void* filt, void* out, long long batch, void* ws,
+                             long long ws_bytes, void* stream);
+int kym_scat1d_backward(void* plan, const void* dS, const void* filt, void* gx, long long batch, void* ws,
+                        long long ws_bytes, void* stream);
 
 #ifdef __cplusplus
 }

@@ -310,10 +310,11 @@ static int upload1d(Scat1DPlan<T>& p, void** dblob) {
     KYM_HIP_TRY(hipMalloc(dblob, bytes));
     p.blob.relocate((uintptr_t)*dblob);
     KYM_HIP_TRY(hipMemcpy(*dblob, p.blob.bytes.data(), bytes, hipMemcpyHostToDevice));
-    for (const Launch1D& l : p.launches)
-        if (l.lds_bytes > 65536)
-            KYM_HIP_TRY(hipFuncSetAttribute((const void*)pick_scat1d<T>(l.threads),
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.lds_bytes));
+    for (const std::vector<Launch1D>* ls : {&p.launches, &p.bwd_launches})
+        for (const Launch1D& l : *ls)
+            if (l.lds_bytes > 65536)
+                KYM_HIP_TRY(hipFuncSetAttribute((const void*)pick_scat1d<T>(l.threads, l.kind),
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.lds_bytes));
     return KYM_OK;
 }
 
@@ -374,6 +375,7 @@ static void fill_info1d(const Scat1DPlan<T>& p, long long* info) {
     info[3] = p.filt_off.back();
     info[4] = p.ws_sig * (long long)sizeof(cpx<T>);
     info[5] = (long long)p.launches.size();
+    info[6] = p.ws_sig_train * (long long)sizeof(cpx<T>);
 }
 
 extern "C" int kym_scat1d_info(void* plan, long long* info) {
@@ -396,21 +398,30 @@ extern "C" int kym_scat1d_layout(void* plan, long long* filt_offs, long long* pa
     return KYM_OK;
 }
 
+// mode 0 inference, 1 training forward (saves u, whole batch), 2 backward
 template <class T>
-static int forward1d(const Scat1DPlan<T>& p, const void* x, const void* filt, void* out, long long batch, void* ws,
-                     long long ws_bytes, hipStream_t stream, uintptr_t base, LaunchProfile* prof) {
-    const long long per = p.ws_sig * (long long)sizeof(cpx<T>);
+static int run1d(const Scat1DPlan<T>& p, int mode, const void* x, const void* filt, void* out, const void* dS,
+                 void* gx, long long batch, void* ws, long long ws_bytes, hipStream_t stream, uintptr_t base,
+                 LaunchProfile* prof) {
+    const long long sig_elems = mode == 0 ? p.ws_sig : p.ws_sig_train;
+    const long long per = sig_elems * (long long)sizeof(cpx<T>);
     long long chunk = ws_bytes / per;
     if (chunk < 1) return fail(KYM_ERR_VALUE, "workspace too small for one signal");
     if (chunk > batch) chunk = batch;
+    if (mode != 0 && chunk < batch) return fail(KYM_ERR_VALUE, "training workspace must hold the whole batch");
+    const std::vector<Launch1D>& launches = mode == 2 ? p.bwd_launches : p.launches;
     for (long long i0 = 0; i0 < batch; i0 += chunk) {
         const int nimg = (int)std::min(chunk, batch - i0);
-        for (const Launch1D& l : p.launches) {
+        for (const Launch1D& l : launches) {
             Scat1DArgs<T> a = p.args(l, base);
             a.x = static_cast<const T*>(x);
             a.filt = static_cast<const T*>(filt);
             a.ws = static_cast<cpx<T>*>(ws);
+            a.ws_sig = sig_elems;
             a.out = static_cast<T*>(out);
+            a.dS = static_cast<const T*>(dS);
+            a.gx = static_cast<T*>(gx);
+            a.save = mode == 1 ? 1 : 0;
             a.img0 = (int)i0;
             const long long ntask = (long long)nimg * l.ngroup;
             if (ntask > (1ll << 30)) return fail(KYM_ERR_VALUE, "too many tasks in one launch");
@@ -422,11 +433,12 @@ static int forward1d(const Scat1DPlan<T>& p, const void* x, const void* filt, vo
                 e1 = prof->get();
                 if (e0) KYM_HIP_TRY(hipEventRecord(e0, stream));
             }
-            hipLaunchKernelGGL(pick_scat1d<T>(l.threads), dim3(grid), dim3(l.threads), l.lds_bytes, stream, a);
+            hipLaunchKernelGGL(pick_scat1d<T>(l.threads, l.kind), dim3(grid), dim3(l.threads), l.lds_bytes, stream,
+                               a);
             KYM_HIP_TRY(hipGetLastError());
             if (e0 && e1) {
                 KYM_HIP_TRY(hipEventRecord(e1, stream));
-                prof->recs.push_back({(int)(&l - p.launches.data()), {e0, e1}});
+                prof->recs.push_back({(int)(&l - launches.data()), {e0, e1}});
             }
         }
     }
@@ -444,6 +456,36 @@ extern "C" int kym_scat1d_forward(void* plan, const void* x, const void* filt, v
     KYM_HIP_TRY(hipGetDevice(&cur));
     if (cur != h->device) return fail(KYM_ERR_TYPE, "plan was created for another device");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    if (h->pf) return forward1d(*h->pf, x, filt, out, batch, ws, ws_bytes, s, (uintptr_t)h->dblob, &h->prof);
-    return forward1d(*h->pd, x, filt, out, batch, ws, ws_bytes, s, (uintptr_t)h->dblob, &h->prof);
+    if (h->pf) return run1d(*h->pf, 0, x, filt, out, nullptr, nullptr, batch, ws, ws_bytes, s, (uintptr_t)h->dblob, &h->prof);
+    return run1d(*h->pd, 0, x, filt, out, nullptr, nullptr, batch, ws, ws_bytes, s, (uintptr_t)h->dblob, &h->prof);
+}
+
+extern "C" int kym_scat1d_forward_train(void* plan, const void* x, const void* filt, void* out, long long batch,
+                                        void* ws, long long ws_bytes, void* stream) {
+    if (!plan) return fail(KYM_ERR_VALUE, "null plan");
+    if (batch < 0) return fail(KYM_ERR_VALUE, "negative batch");
+    if (batch == 0) return KYM_OK;
+    if (!x || !filt || !out || !ws) return fail(KYM_ERR_VALUE, "null device pointer");
+    KymScat1D* h = static_cast<KymScat1D*>(plan);
+    int cur = 0;
+    KYM_HIP_TRY(hipGetDevice(&cur));
+    if (cur != h->device) return fail(KYM_ERR_TYPE, "plan was created for another device");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (h->pf) return run1d(*h->pf, 1, x, filt, out, nullptr, nullptr, batch, ws, ws_bytes, s, (uintptr_t)h->dblob, &h->prof);
+    return run1d(*h->pd, 1, x, filt, out, nullptr, nullptr, batch, ws, ws_bytes, s, (uintptr_t)h->dblob, &h->prof);
+}
+
+extern "C" int kym_scat1d_backward(void* plan, const void* dS, const void* filt, void* gx, long long batch, void* ws,
+                                   long long ws_bytes, void* stream) {
+    if (!plan) return fail(KYM_ERR_VALUE, "null plan");
+    if (batch < 0) return fail(KYM_ERR_VALUE, "negative batch");
+    if (batch == 0) return KYM_OK;
+    if (!dS || !filt || !gx || !ws) return fail(KYM_ERR_VALUE, "null device pointer");
+    KymScat1D* h = static_cast<KymScat1D*>(plan);
+    int cur = 0;
+    KYM_HIP_TRY(hipGetDevice(&cur));
+    if (cur != h->device) return fail(KYM_ERR_TYPE, "plan was created for another device");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (h->pf) return run1d(*h->pf, 2, nullptr, filt, nullptr, dS, gx, batch, ws, ws_bytes, s, (uintptr_t)h->dblob, nullptr);
+    return run1d(*h->pd, 2, nullptr, filt, nullptr, dS, gx, batch, ws, ws_bytes, s, (uintptr_t)h->dblob, nullptr);
 }

@@ -121,7 +121,7 @@ extern "C" int kym_emu_scat1d_forward(int N, int N_pad, int pad_left, int log2_s
                                       const int* psi1_nlev, int n_psi2, const int* psi2_j, const int* psi2_nlev,
                                       int phi_nlev, const int* ind_start, const int* ind_end, int n_ind, int dtype,
                                       const void* x, const void* filt, void* out, long long batch,
-                                      long long* info) {
+                                      long long* info, const void* dS, void* gx) {
     Scat1DDesc d = {N, N_pad, pad_left, log2_stride, average_mode, oversampling, max_order, n_psi1, n_psi2, phi_nlev,
                     psi1_j, psi1_nlev, psi2_j, psi2_nlev, ind_start, ind_end, n_ind};
     auto run = [&](auto tag) -> int {
@@ -136,18 +136,29 @@ extern "C" int kym_emu_scat1d_forward(int N, int N_pad, int pad_left, int log2_s
         const uintptr_t base = (uintptr_t)p.blob.bytes.data();
         if (info) { info[0] = p.C; info[1] = p.out_row; info[2] = (long long)p.filt_off.size() - 1; info[3] = p.filt_off.back(); }
         if (!x) return 0;
-        std::vector<cpx<T>> ws((size_t)(p.ws_sig * batch) + 1);
+        const bool train = dS != nullptr;
+        const long long stride = train ? p.ws_sig_train : p.ws_sig;
+        std::vector<cpx<T>> ws((size_t)(stride * batch) + 1);
         std::vector<cpx<T>> lds(p.max_lds / sizeof(cpx<T>) + 1);
-        for (const Launch1D& l : p.launches) {
-            Scat1DArgs<T> a = p.args(l, base);
-            a.x = static_cast<const T*>(x);
-            a.filt = static_cast<const T*>(filt);
-            a.ws = ws.data();
-            a.out = static_cast<T*>(out);
-            a.img0 = 0;
-            a.ntask = (int)(batch * l.ngroup);
-            for (int t = 0; t < a.ntask; ++t) body_node1d<T, 31>(a, t, lds.data(), 0, 1);
-        }
+        for (int phase = 0; phase < (train ? 2 : 1); ++phase)
+            for (const Launch1D& l : (phase ? p.bwd_launches : p.launches)) {
+                Scat1DArgs<T> a = p.args(l, base);
+                a.x = static_cast<const T*>(x);
+                a.filt = static_cast<const T*>(filt);
+                a.ws = ws.data();
+                a.ws_sig = stride;
+                a.out = static_cast<T*>(out);
+                a.dS = static_cast<const T*>(dS);
+                a.gx = static_cast<T*>(gx);
+                a.save = train ? 1 : 0;
+                a.img0 = 0;
+                a.ntask = (int)(batch * l.ngroup);
+                for (int t = 0; t < a.ntask; ++t) {
+                    if (l.kind == 0) body_node1d<T, 31>(a, t, lds.data(), 0, 1);
+                    else if (l.kind == 1) body_bwd_node1d<T, 31>(a, t, lds.data(), 0, 1);
+                    else body_bwd_fold1d<T>(a, t, 0, 1);
+                }
+            }
         return 0;
     };
     if (dtype == 0) return run(float(0));

@@ -1,5 +1,5 @@
 // 1D kernels (all dtypes).
 #include "kernels1d.h"
 
-Scat1DKernel<float> kym_pick1d_f32(int threads) { return pick_scat1d_nt<float>(threads); }
-Scat1DKernel<double> kym_pick1d_f64(int threads) { return pick_scat1d_nt<double>(threads); }
+Scat1DKernel<float> kym_pick1d_f32(int threads, int kind) { return pick_scat1d_nt<float>(threads, kind); }
+Scat1DKernel<double> kym_pick1d_f64(int threads, int kind) { return pick_scat1d_nt<double>(threads, kind); }

@@ -15,8 +15,8 @@ Scat2DKernel<double> kym_pick2d_f64_p17(int kind, int threads);
 Scat2DKernel<double> kym_pick2d_f64_p31(int kind, int threads);
 Scat2DKernel<float> kym_pick_node_spec_f32(int id);
 Scat2DKernel<float> kym_pick_leaf_spec_f32(int id);
-Scat1DKernel<float> kym_pick1d_f32(int threads);
-Scat1DKernel<double> kym_pick1d_f64(int threads);
+Scat1DKernel<float> kym_pick1d_f32(int threads, int kind);
+Scat1DKernel<double> kym_pick1d_f64(int threads, int kind);
 
 // pclass: 7, 17 or 31 (kym_prime_class of the plan's largest odd prime);
 // spec >= 0 selects a compile-time specialisation of the node / leaf kernel.
@@ -36,7 +36,7 @@ inline Scat2DKernel<T> pick_scat2d(int kind, int threads, int pclass, int spec =
 }
 
 template <class T>
-inline Scat1DKernel<T> pick_scat1d(int threads) {
-    if constexpr (sizeof(T) == 4) return kym_pick1d_f32(threads);
-    else return kym_pick1d_f64(threads);
+inline Scat1DKernel<T> pick_scat1d(int threads, int kind = 0) {
+    if constexpr (sizeof(T) == 4) return kym_pick1d_f32(threads, kind);
+    else return kym_pick1d_f64(threads, kind);
 }

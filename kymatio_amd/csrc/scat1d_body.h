@@ -31,6 +31,10 @@ struct Node1D {
     int unpad_res;   // border-index resolution of the unpad
     int need_fwd;    // forward FFT of |U| needed (children or local leaf or global sum)
     long long slot;  // spectrum slot in the per-signal workspace, -1 none
+    // training
+    int child_begin, child_end;  // node-index range of the children
+    long long save;              // saved u (digit-reversed, unnormalised), then dL/dX (natural)
+    long long gpad;              // root: dL/dpad(x), real
 };
 
 template <class T>
@@ -51,6 +55,10 @@ struct Scat1DArgs {
     T* out;
     long long out_row;
     int img0, ntask;
+    // training
+    int save;
+    const T* dS;
+    T* gx;
 };
 
 // Output of one path from the spectrum / modulus in LDS.
@@ -128,8 +136,10 @@ KYM_HD void body_node1d(const Scat1DArgs<T>& a, int task, cpx<T>* lds, int tid, 
         fft_dif<true, MAXP>(lds, fl, Batch{1, 1, n, false}, tid, nth);
         const T inv_n = T(1) / T(n);
         const int s = a.ind_start[nd.unpad_res], e = a.ind_end[nd.unpad_res];
+        cpx<T>* sv = (a.save && nd.save >= 0) ? wsi + nd.save : nullptr;
         for (int p = tid; p < n; p += nth) {
             const cpx<T> v = lds[p];
+            if (sv) sv[p] = v;
             const T m = kym_sqrt(v.re * v.re + v.im * v.im) * inv_n;
             lds[p] = {m, T(0)};
             if (a.mode == AVG1_NONE) {
@@ -145,4 +155,100 @@ KYM_HD void body_node1d(const Scat1DArgs<T>& a, int task, cpx<T>* lds, int tid, 
         for (int i = tid; i < n; i += nth) dst[i] = lds[i];
     }
     if (a.mode != AVG1_NONE) leaf1d<T, MAXP>(a, nd, lds, n, img, lds + n, tid, nth);
+}
+
+// ------------------------------------------------------------------ backward
+// Adjoint of body_node1d (same conventions as the 2D backward):
+//   local leaf  dL/dY = fft(unpad^T dS) / nl, dL/dU += phi * tile(dL/dY) / kl
+//   children    dL/dU += f_c * tile(dL/dX_c) / kc
+//   dL/dm = Re(n ifft(dL/dU)) (+ dS on the unpadded window without averaging,
+//           + dS everywhere with global averaging)
+//   dL/du = dL/dm u / |u|,  dL/dX = fft(dL/du) / n;  root: dL/dpad(x) = dL/dm.
+template <class T, int MAXP>
+KYM_HD void body_bwd_node1d(const Scat1DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
+    const int il = task / a.ngroup, gi = task - il * a.ngroup;
+    const int img = a.img0 + il;
+    const Node1D& nd = a.nodes[a.group[gi]];
+    const int r = nd.res, n = a.n[r];
+    const FftLen<T>& fl = a.lens[a.len[r]];
+    cpx<T>* wsi = a.ws + il * a.ws_sig;
+    const T* d = a.dS + (long long)img * a.out_row + nd.out_off;
+    // leaf gradient (local average) into B
+    cpx<T>* B = lds + n;
+    int nl = 1, kk = 1;
+    if (a.mode == AVG1_LOCAL) {
+        kk = nd.leaf_k;
+        nl = n / kk;
+        const int lr = nd.res > a.log2_stride ? nd.res : a.log2_stride;
+        const FftLen<T>& fB = a.lens[a.len[lr]];
+        const int s = a.ind_start[a.log2_stride], e = a.ind_end[a.log2_stride];
+        for (int p = tid; p < nl; p += nth) {
+            const int t = fB.rev[p];
+            B[p] = {(t >= s && t < e) ? d[t - s] : T(0), T(0)};
+        }
+        wg_barrier();
+        fft_dit<false, MAXP>(B, fB, Batch{1, 1, nl, false}, tid, nth);
+    }
+    const bool has_u_grad = (a.mode == AVG1_LOCAL) || (nd.child_end > nd.child_begin);
+    const T* phi = a.mode == AVG1_LOCAL ? a.filt + a.filt_off[nd.leaf_filt] : nullptr;
+    const T sc_leaf = T(1) / (T(nl) * T(kk));
+    for (int i = tid; i < n; i += nth) {
+        cpx<T> acc = {T(0), T(0)};
+        if (phi) {
+            const cpx<T> gy = B[i % nl];
+            acc = {gy.re * phi[i] * sc_leaf, gy.im * phi[i] * sc_leaf};
+        }
+        for (int c = nd.child_begin; c < nd.child_end; ++c) {
+            const Node1D& cn = a.nodes[c];
+            const int nc = a.n[cn.res];
+            const T fv = a.filt[a.filt_off[cn.filt] + i] / T(cn.k);
+            const cpx<T> gx = wsi[cn.save + (i % nc)];
+            acc.re += gx.re * fv;
+            acc.im += gx.im * fv;
+        }
+        lds[i] = acc;
+    }
+    wg_barrier();
+    if (has_u_grad) fft_dif<true, MAXP>(lds, fl, Batch{1, 1, n, false}, tid, nth);
+    // direct terms on dL/dm (positions p hold natural index rev[p])
+    const int us = a.ind_start[nd.unpad_res], ue = a.ind_end[nd.unpad_res];
+    for (int p = tid; p < n; p += nth) {
+        T gm = has_u_grad ? lds[p].re : T(0);
+        const int t = fl.rev[p];
+        if (a.mode == AVG1_NONE && t >= us && t < ue) gm += d[t - us];
+        if (a.mode == AVG1_GLOBAL) gm += d[0];
+        if (nd.order == 0) {
+            reinterpret_cast<T*>(wsi + nd.gpad)[t] = gm;
+        } else {
+            const cpx<T> u = wsi[nd.save + p];
+            const T mag = kym_sqrt(u.re * u.re + u.im * u.im);
+            lds[p] = mag > T(0) ? cpx<T>{u.re * (gm / mag), u.im * (gm / mag)} : cpx<T>{T(0), T(0)};
+        }
+    }
+    if (nd.order == 0) return;
+    wg_barrier();
+    fft_dit<false, MAXP>(lds, fl, Batch{1, 1, n, false}, tid, nth);
+    const T inv_n = T(1) / T(n);
+    for (int i = tid; i < n; i += nth) wsi[nd.save + i] = lds[i] * inv_n;
+}
+
+// Reflect-pad adjoint (1D).  task -> signal.
+template <class T>
+KYM_HD void body_bwd_fold1d(const Scat1DArgs<T>& a, int task, int tid, int nth) {
+    const int img = a.img0 + task;
+    const T* gp = reinterpret_cast<const T*>(a.ws + task * a.ws_sig + a.nodes[0].gpad);
+    T* gx = a.gx + (long long)img * a.N;
+    const int P = a.n[0];
+    for (int i = tid; i < a.N; i += nth) {
+        T acc = T(0);
+        if (a.N == 1) {
+            for (int t = 0; t < P; ++t) acc += gp[t];
+        } else {
+            const int T_ = 2 * (a.N - 1);
+            for (int t = (a.pad_left + i) % T_; t < P; t += T_) acc += gp[t];
+            if (i != 0 && i != a.N - 1)
+                for (int t = ((a.pad_left - i) % T_ + T_) % T_; t < P; t += T_) acc += gp[t];
+        }
+        gx[i] = acc;
+    }
 }

@@ -9,6 +9,7 @@
 
 struct Launch1D {
     size_t group_off;
+    int kind;  // 0 forward node, 1 backward node, 2 backward pad fold
     int ngroup, threads, res;
     size_t lds_bytes;
     long long model_bytes;  // algorithmic bytes per signal (stage-boundary model)
@@ -37,7 +38,8 @@ struct Scat1DPlan {
     long long out_row = 0, ws_sig = 0;
     int C = 0, pclass = 7;
     LenTable<T> lens;
-    std::vector<Launch1D> launches;
+    std::vector<Launch1D> launches, bwd_launches;
+    long long ws_sig_train = 0;
     Blob blob;
     size_t off_nodes = 0, off_lens = 0, off_filt = 0, max_lds = 0;
 
@@ -154,6 +156,23 @@ struct Scat1DPlan {
             ws_sig = (ws_sig + align - 1) / align * align;
         }
         if (ws_sig == 0) ws_sig = align;
+        // training layout (extends the inference one): saved u per non-root
+        // node, the root's dL/dpad(x); children as node-index ranges
+        {
+            for (auto& nd : nodes) { nd.child_begin = nd.child_end = 0; nd.save = nd.gpad = -1; }
+            for (size_t i = 1; i < nodes.size(); ++i) {
+                Node1D& pn = nodes[nodes[i].parent];
+                if (pn.child_end == 0) pn.child_begin = (int)i;
+                pn.child_end = (int)i + 1;
+            }
+            long long w = ws_sig;
+            for (auto& nd : nodes) {
+                if (nd.order > 0) { nd.save = w; w += n[nd.res]; }
+                else { nd.gpad = w; w += ((long long)N_pad * (long long)sizeof(T) + sizeof(cpx<T>) - 1) / sizeof(cpx<T>); }
+                w = (w + align - 1) / align * align;
+            }
+            ws_sig_train = w;
+        }
         // LDS: node + leaf
         const size_t cs = sizeof(cpx<T>);
         for (auto& nd : nodes) {
@@ -176,9 +195,10 @@ struct Scat1DPlan {
         launches.clear();
         max_lds = 0;
         const long long leaf_model = local ? 16ll * (N_pad >> ls) + 4ll * Tlocal : 4;
-        auto push = [&](const std::vector<int>& ids) {
+        auto push = [&](const std::vector<int>& ids, int kind) {
             if (ids.empty()) return;
             Launch1D l;
+            l.kind = kind;
             l.group_off = blob.add(sizeof(int) * ids.size());
             std::memcpy(blob.at<char>(l.group_off), ids.data(), sizeof(int) * ids.size());
             l.ngroup = (int)ids.size();
@@ -201,14 +221,29 @@ struct Scat1DPlan {
             launches.push_back(l);
             max_lds = std::max(max_lds, l.lds_bytes);
         };
-        push({0});
+        push({0}, 0);
         for (int order = 1; order <= max_order; ++order)
             for (int r = 0; r < nres; ++r) {
                 std::vector<int> ids;
                 for (size_t i = 0; i < nodes.size(); ++i)
                     if (nodes[i].order == order && nodes[i].res == r) ids.push_back((int)i);
-                push(ids);
+                push(ids, 0);
             }
+        std::vector<Launch1D> fwd = launches;
+        launches.clear();
+        for (int order = max_order; order >= 1; --order)
+            for (int r = 0; r < nres; ++r) {
+                std::vector<int> ids;
+                for (size_t i = 0; i < nodes.size(); ++i)
+                    if (nodes[i].order == order && nodes[i].res == r) ids.push_back((int)i);
+                push(ids, 1);
+            }
+        push({0}, 1);
+        push({0}, 2);
+        launches.back().threads = 256;
+        launches.back().lds_bytes = 0;
+        bwd_launches = launches;
+        launches = fwd;
         return true;
     }
 

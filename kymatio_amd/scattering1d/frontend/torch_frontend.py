@@ -45,9 +45,10 @@ class Scat1DPlanHandle:
             i32([m.ind_end[i] for i in range(nind)]), nind, _lib.DTYPE_CODE[dtype], int(lds_limit),
             ctypes.byref(h)))
         self._h, self._lib = h, lib
-        info = (ctypes.c_longlong * 6)()
+        info = (ctypes.c_longlong * 7)()
         _lib.check(lib.kym_scat1d_info(h, info))
-        self.C, self.out_row, self.n_filters, self.filter_elems, self.ws_bytes_per_signal, self.n_launches = list(info)
+        (self.C, self.out_row, self.n_filters, self.filter_elems, self.ws_bytes_per_signal, self.n_launches,
+         self.train_ws_bytes_per_signal) = list(info)
         fo = (ctypes.c_longlong * (self.n_filters + 1))()
         pl = (ctypes.c_longlong * self.C)()
         _lib.check(lib.kym_scat1d_layout(h, fo, pl))
@@ -58,6 +59,18 @@ class Scat1DPlanHandle:
             self._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(filt.data_ptr()),
             ctypes.c_void_p(out.data_ptr()), int(batch), ctypes.c_void_p(ws.data_ptr()), ws.numel(),
             _lib.stream_of(x.device)))
+
+    def forward_train(self, x, filt, out, batch, ws):
+        _lib.check(self._lib.kym_scat1d_forward_train(
+            self._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(filt.data_ptr()),
+            ctypes.c_void_p(out.data_ptr()), int(batch), ctypes.c_void_p(ws.data_ptr()), ws.numel(),
+            _lib.stream_of(x.device)))
+
+    def backward(self, dS, filt, gx, batch, ws):
+        _lib.check(self._lib.kym_scat1d_backward(
+            self._h, ctypes.c_void_p(dS.data_ptr()), ctypes.c_void_p(filt.data_ptr()),
+            ctypes.c_void_p(gx.data_ptr()), int(batch), ctypes.c_void_p(ws.data_ptr()), ws.numel(),
+            _lib.stream_of(dS.device)))
 
     def __del__(self):
         try:
@@ -318,14 +331,36 @@ class ScatteringTorch1D(ScatteringTorch, ScatteringBase):
 
 
 class ScatteringFunction1D(torch.autograd.Function):
-    @staticmethod
-    def forward(ctx, x, module):
-        ctx.module = module
-        return module._forward_hip(x)
+    """Fused HIP forward; with a gradient needed, the training forward keeps the
+    pre-modulus fields for the hand-written backward (kym_scat1d_backward)."""
 
     @staticmethod
+    def forward(ctx, x, module):
+        if not ctx.needs_input_grad[0]:
+            return module._forward_hip(x)
+        device, dtype = x.device, x.dtype
+        plan = module._plan(device, dtype)
+        filt = module._packed_filters(plan, device, dtype)
+        B = x.shape[0]
+        out = torch.empty((B, plan.out_row), dtype=dtype, device=device)
+        ws = torch.empty(max(B, 1) * plan.train_ws_bytes_per_signal, dtype=torch.uint8, device=device)
+        if B:
+            with torch.cuda.device(device):
+                plan.forward_train(x, filt, out, B, ws)
+        ctx.save_for_backward(filt, ws)
+        ctx.plan, ctx.B, ctx.xshape = plan, B, x.shape
+        return out
+
+    @staticmethod
+    @torch.autograd.function.once_differentiable
     def backward(ctx, grad):
-        raise NotImplementedError('Scattering1D backward is not available in this build yet.')
+        filt, ws = ctx.saved_tensors
+        g = grad.contiguous()
+        gx = torch.empty(ctx.xshape, dtype=g.dtype, device=g.device)
+        if ctx.B:
+            with torch.cuda.device(g.device):
+                ctx.plan.backward(g, filt, gx, ctx.B, ws)
+        return gx, None
 
 
 __all__ = ['ScatteringTorch1D']

@@ -84,15 +84,16 @@ class Emu:
         return out, {'launches': info[3], 'big_nodes': info[4], 'spec_launches': info[5]}
 
 
-def emu_scat1d(emu, module, x, dtype=np.float32):
-    """Run the 1D bodies in the CPU harness with the setup of a (CPU) Scattering1D module."""
+def emu_scat1d(emu, module, x, dtype=np.float32, dS=None):
+    """Run the 1D bodies in the CPU harness with the setup of a (CPU)
+    Scattering1D module; with dS, also the backward (returns S, C, dL/dx)."""
     m = module
     modes = {'local': 0, False: 1, 'global': 2}
     i32 = lambda v: (ctypes.c_int * max(len(v), 1))(*v)
     f = emu.lib.kym_emu_scat1d_forward
     f.argtypes = ([ctypes.c_int] * 8 + [ctypes.c_void_p] * 2 + [ctypes.c_int] + [ctypes.c_void_p] * 2
                   + [ctypes.c_int] + [ctypes.c_void_p] * 2 + [ctypes.c_int, ctypes.c_int]
-                  + [ctypes.c_void_p] * 3 + [ctypes.c_longlong, ctypes.c_void_p])
+                  + [ctypes.c_void_p] * 3 + [ctypes.c_longlong, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p])
     f.restype = ctypes.c_int
     p1, p2 = m.psi1_f, m.psi2_f
     nind = max(m.ind_start.keys()) + 1
@@ -105,15 +106,22 @@ def emu_scat1d(emu, module, x, dtype=np.float32):
             i32([m.ind_end[i] for i in range(nind)]), nind, 0 if dtype == np.float32 else 1]
     info = (ctypes.c_longlong * 4)()
     B = x.shape[0]
-    rc = f(*args, None, None, None, B, info)
+    rc = f(*args, None, None, None, B, info, None, None)
     if rc != 0:
         raise RuntimeError(emu.lib.kym_emu_last_error().decode())
     out = np.zeros((B, info[1]), dtype)
     xx = np.ascontiguousarray(x, dtype)
-    rc = f(*args, xx.ctypes.data, packed.ctypes.data, out.ctypes.data, B, info)
+    if dS is None:
+        rc = f(*args, xx.ctypes.data, packed.ctypes.data, out.ctypes.data, B, info, None, None)
+        if rc != 0:
+            raise RuntimeError(emu.lib.kym_emu_last_error().decode())
+        return out, info[0]
+    dd = np.ascontiguousarray(dS, dtype).reshape(B, -1)
+    gx = np.zeros_like(xx)
+    rc = f(*args, xx.ctypes.data, packed.ctypes.data, out.ctypes.data, B, info, dd.ctypes.data, gx.ctypes.data)
     if rc != 0:
         raise RuntimeError(emu.lib.kym_emu_last_error().decode())
-    return out, info[0]
+    return out, info[0], gx
 
 
 def emu_train2d(emu, x, J, shape, filters_packed, dS, L=8, max_order=2, pre_pad=0, lds_limit=160 * 1024):
