@@ -145,6 +145,39 @@ int kym_scat1d_forward_train(void* plan, const void* x, const void* filt, void* 
 int kym_scat1d_backward(void* plan, const void* dS, const void* filt, void* gx, long long batch, void* ws,
                         long long ws_bytes, void* stream);
 
+/* ---------------------------------------------------------- per-op ----
+ * The backend namespace as single kernels, for the backend plugin route and
+ * op-level tests (forward only).  Complex arrays are interleaved (re, im)
+ * pairs, i.e. the reference's (..., 2) layout; real arrays its (..., 1).
+ */
+/* Pad (kymatio/scattering2d/backend/torch_backend.py:19-86): reflect pad of
+ * (batch, M, N) to (batch, M+top+bottom, N+left+right), numpy 'reflect'
+ * semantics (so pad == size matches the reference's special case). */
+int kym_op_pad2d(const void* x, void* y, long long batch, int M, int N, int top, int bottom, int left, int right,
+                 int dtype, void* stream);
+/* pad (kymatio/scattering1d/backend/torch_backend.py:50-82); status 3
+ * 'Indefinite padding size (larger than tensor).' when a pad >= N. */
+int kym_op_pad1d(const void* x, void* y, long long batch, int N, int left, int right, int dtype, void* stream);
+/* cdgmm (kymatio/backend/torch_backend.py:147-219): C = A * B, A complex
+ * (batch, n), B real (n) or complex (n) when b_complex. */
+int kym_op_cdgmm(const void* A, const void* B, void* C, long long batch, long long n, int b_complex, int dtype,
+                 void* stream);
+/* modulus (kymatio/backend/torch_backend.py:137-141): y = |x|, n complex in, n real out. */
+int kym_op_modulus(const void* x, void* y, long long n, int dtype, void* stream);
+/* subsample_fourier (kymatio/scattering2d/backend/torch_backend.py:92-129;
+ * 1D: kymatio/scattering1d/backend/torch_backend.py:18-48 with W = 1, kw = 1):
+ * y[b, i, j] = mean over the kh x kw aliases of x (batch, H, W) complex. */
+int kym_op_subsample(const void* x, void* y, long long batch, int H, int W, int kh, int kw, int dtype, void* stream);
+/* unpad (kymatio/scattering2d/backend/torch_backend.py:157-176): y = x[..., 1:-1, 1:-1]. */
+int kym_op_unpad2d(const void* x, void* y, long long batch, int H, int W, int dtype, void* stream);
+/* fft2 / ifft2 / rfft / irfft (kymatio/scattering2d/backend/torch_backend.py:
+ * 133-155; 1D with W = 1, kymatio/scattering1d/backend/torch_backend.py:
+ * 108-141) over (batch, H, W): kind 0 forward c2c, 1 inverse c2c (1/(HW)),
+ * 2 forward from real input, 3 inverse keeping the real part.  lds_limit as
+ * in kym_scat2d_create (0 = device maximum). */
+int kym_op_fft(const void* x, void* y, long long batch, int H, int W, int kind, int dtype, long long lds_limit,
+               void* stream);
+
 #ifdef __cplusplus
 }
 #endif

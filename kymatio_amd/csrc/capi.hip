@@ -27,6 +27,8 @@ static int fail(int code, const std::string& msg) {
 
 extern "C" const char* kym_last_error(void) { return g_last_error.c_str(); }
 
+int kym_ops_fail(int code, const std::string& msg) { return fail(code, msg); }
+
 extern "C" int kym_version(void) { return KYM_ABI_VERSION; }
 
 static size_t device_lds_limit() {

@@ -1,0 +1,405 @@
+// Per-op entry points: the backend namespace of the reference
+// (kymatio/scattering2d/backend/torch_backend.py, kymatio/scattering1d/
+// backend/torch_backend.py, kymatio/backend/torch_backend.py) as single HIP
+// kernels, for the plugin route and for op-level tests.  The fused pipelines
+// in capi.hip do not use these.
+#include <string>
+
+#include "../../include/kymatio_hip.h"
+#include "fft_core.h"
+#include "plan_common.h"
+
+int kym_ops_fail(int code, const std::string& msg);  // capi.hip
+
+#define OPS_HIP_TRY(expr)                                                                          \
+    do {                                                                                           \
+        hipError_t _e = (expr);                                                                    \
+        if (_e != hipSuccess) return kym_ops_fail(KYM_ERR_HIP, std::string(#expr " failed: ") + hipGetErrorString(_e)); \
+    } while (0)
+
+static inline unsigned grid_for(long long n, int block) {
+    long long g = (n + block - 1) / block;
+    if (g > 65535ll * 64) g = 65535ll * 64;
+    return (unsigned)(g < 1 ? 1 : g);
+}
+
+// ------------------------------------------------------------ elementwise
+template <class T>
+__global__ void k_pad2d(const T* x, T* y, long long batch, int M, int N, int top, int left, int P, int Q) {
+    const long long total = batch * P * Q;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total; i += (long long)gridDim.x * blockDim.x) {
+        const long long b = i / ((long long)P * Q);
+        const int r = (int)(i - b * P * Q);
+        const int yy = r / Q, xx = r - (r / Q) * Q;
+        y[i] = x[b * M * N + (long long)reflect_index(yy - top, M) * N + reflect_index(xx - left, N)];
+    }
+}
+
+template <class T>
+__global__ void k_pad1d(const T* x, T* y, long long batch, int N, int left, int P) {
+    const long long total = batch * P;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total; i += (long long)gridDim.x * blockDim.x) {
+        const long long b = i / P;
+        const int t = (int)(i - b * P);
+        y[i] = x[b * N + reflect_index(t - left, N)];
+    }
+}
+
+template <class T>
+__global__ void k_cdgmm(const cpx<T>* A, const T* B, cpx<T>* C, long long total, long long n, int bcomplex) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total; i += (long long)gridDim.x * blockDim.x) {
+        const long long j = i % n;
+        const cpx<T> a = A[i];
+        if (bcomplex) {
+            const cpx<T> b = reinterpret_cast<const cpx<T>*>(B)[j];
+            C[i] = cmul(a, b);
+        } else {
+            const T b = B[j];
+            C[i] = {a.re * b, a.im * b};
+        }
+    }
+}
+
+template <class T>
+__global__ void k_modulus(const cpx<T>* x, T* y, long long n) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const cpx<T> v = x[i];
+        y[i] = kym_sqrt(v.re * v.re + v.im * v.im);
+    }
+}
+
+// out[b, i, j] = mean over a, c < k of x[b, i + a H/k, j + c W/k]   (W = 1 for 1D)
+template <class T>
+__global__ void k_subsample(const cpx<T>* x, cpx<T>* y, long long batch, int H, int W, int kh, int kw) {
+    const int h = H / kh, w = W / kw;
+    const long long total = batch * h * w;
+    const T inv = T(1) / T(kh * kw);
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total; i += (long long)gridDim.x * blockDim.x) {
+        const long long b = i / ((long long)h * w);
+        const int r = (int)(i - b * h * w);
+        const int yy = r / w, xx = r - (r / w) * w;
+        cpx<T> acc = {T(0), T(0)};
+        for (int a = 0; a < kh; ++a)
+            for (int c = 0; c < kw; ++c) acc = acc + x[b * H * W + (long long)(yy + a * h) * W + xx + c * w];
+        y[i] = acc * inv;
+    }
+}
+
+template <class T>
+__global__ void k_unpad2d(const T* x, T* y, long long batch, int H, int W) {
+    const int h = H - 2, w = W - 2;
+    const long long total = batch * h * w;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total; i += (long long)gridDim.x * blockDim.x) {
+        const long long b = i / ((long long)h * w);
+        const int r = (int)(i - b * h * w);
+        const int yy = r / w, xx = r - (r / w) * w;
+        y[i] = x[b * H * W + (long long)(yy + 1) * W + xx + 1];
+    }
+}
+
+// ------------------------------------------------------------ FFT op
+// kind: 0 c2c forward, 1 c2c inverse (1/n), 2 r2c forward (real input),
+// 3 inverse, real part out (1/n).  A batch of 2D transforms of H x W (W = 1
+// gives 1D along H).  Small sizes: one workgroup per transform in LDS.
+// Large sizes: row pass then column pass through `y` (in place).
+struct FftOpArgs {
+    const void* x;
+    void* y;
+    long long batch;
+    int H, W, kind;
+    int r0;     // row/column block start (pass kernels)
+    int nrow;   // rows or columns per task
+    int nblk;   // tasks per transform (pass kernels)
+};
+
+template <class T, bool INV>
+__device__ void load_2d(const FftOpArgs& o, const FftLen<T>& fH, const FftLen<T>& fW, long long b, cpx<T>* lds, int tid,
+                        int nth) {
+    const int H = o.H, W = o.W, n = H * W;
+    for (int p = tid; p < n; p += nth) {
+        const int py = p / W, px = p - (p / W) * W;
+        // forward (DIT) wants digit-reversed storage; inverse (DIF) natural
+        const int y = INV ? py : fH.rev[py];
+        const int x = INV ? px : fW.rev[px];
+        const long long src = b * n + (long long)y * W + x;
+        if (o.kind == 2) lds[p] = {static_cast<const T*>(o.x)[src], T(0)};
+        else lds[p] = static_cast<const cpx<T>*>(o.x)[src];
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_fft2_lds(FftOpArgs o, const FftLen<T>* lens, int lH, int lW) {
+    extern __shared__ __attribute__((aligned(16))) char kym_smem_op[];
+    cpx<T>* lds = reinterpret_cast<cpx<T>*>(kym_smem_op);
+    const long long b = blockIdx.x;
+    if (b >= o.batch) return;
+    const FftLen<T>& fH = lens[lH];
+    const FftLen<T>& fW = lens[lW];
+    const int H = o.H, W = o.W, n = H * W, tid = threadIdx.x, nth = blockDim.x;
+    const bool inv = o.kind == 1 || o.kind == 3;
+    if (inv) load_2d<T, true>(o, fH, fW, b, lds, tid, nth);
+    else load_2d<T, false>(o, fH, fW, b, lds, tid, nth);
+    __syncthreads();
+    if (inv) {
+        if (W > 1) fft_dif<true, 31>(lds, fW, Batch{H, 1, W, false}, tid, nth);
+        fft_dif<true, 31>(lds, fH, Batch{W, W, 1, true}, tid, nth);
+        const T inv_n = T(1) / T(n);
+        for (int p = tid; p < n; p += nth) {
+            const int py = p / W, px = p - (p / W) * W;
+            const long long dst = b * n + (long long)fH.rev[py] * W + fW.rev[px];
+            if (o.kind == 3) static_cast<T*>(o.y)[dst] = lds[p].re * inv_n;
+            else static_cast<cpx<T>*>(o.y)[dst] = lds[p] * inv_n;
+        }
+    } else {
+        if (W > 1) fft_dit<false, 31>(lds, fW, Batch{H, 1, W, false}, tid, nth);
+        fft_dit<false, 31>(lds, fH, Batch{W, W, 1, true}, tid, nth);
+        for (int p = tid; p < n; p += nth) static_cast<cpx<T>*>(o.y)[b * n + p] = lds[p];
+    }
+}
+
+// Pass kernels for transforms that do not fit: axis 1 = rows (along W),
+// axis 0 = columns (along H).  Rows run first reading x, columns in place on y.
+template <class T>
+__global__ void __launch_bounds__(256) k_fft2_pass(FftOpArgs o, const FftLen<T>* lens, int lH, int lW, int axis,
+                                                   int first, int last) {
+    extern __shared__ __attribute__((aligned(16))) char kym_smem_op2[];
+    cpx<T>* lds = reinterpret_cast<cpx<T>*>(kym_smem_op2);
+    const long long task = blockIdx.x;
+    const long long b = task / o.nblk;
+    const int blk = (int)(task - b * o.nblk);
+    if (b >= o.batch) return;
+    const int H = o.H, W = o.W, tid = threadIdx.x, nth = blockDim.x;
+    const long long n = (long long)H * W;
+    const bool inv = o.kind == 1 || o.kind == 3;
+    const FftLen<T>& f = lens[axis == 1 ? lW : lH];
+    const int len = f.n, c0 = blk * o.nrow;
+    const int nt = (axis == 1 ? H : W) - c0 < o.nrow ? (axis == 1 ? H : W) - c0 : o.nrow;
+    // lds[t * len + e]: transform t of this task, element e
+    for (int i = tid; i < nt * len; i += nth) {
+        const int t = i / len, e = i - (i / len) * len;
+        const int src_e = inv ? e : f.rev[e];
+        const long long idx = axis == 1 ? (long long)(c0 + t) * W + src_e : (long long)src_e * W + c0 + t;
+        if (first && o.kind == 2) lds[i] = {static_cast<const T*>(o.x)[b * n + idx], T(0)};
+        else if (first) lds[i] = static_cast<const cpx<T>*>(o.x)[b * n + idx];
+        else lds[i] = static_cast<const cpx<T>*>(o.y)[b * n + idx];
+    }
+    __syncthreads();
+    if (inv) fft_dif<true, 31>(lds, f, Batch{nt, 1, len, false}, tid, nth);
+    else fft_dit<false, 31>(lds, f, Batch{nt, 1, len, false}, tid, nth);
+    const T sc = (inv && last) ? T(1) / T(n) : T(1);
+    for (int i = tid; i < nt * len; i += nth) {
+        const int t = i / len, e = i - (i / len) * len;
+        const int dst_e = inv ? f.rev[e] : e;
+        const long long idx = axis == 1 ? (long long)(c0 + t) * W + dst_e : (long long)dst_e * W + c0 + t;
+        cpx<T> v = lds[i] * sc;
+        if (last && o.kind == 3) static_cast<T*>(o.y)[b * n + idx] = v.re;
+        else static_cast<cpx<T>*>(o.y)[b * n + idx] = v;
+    }
+}
+
+// FFT length tables, built once per (length set, dtype) and kept for the
+// process: a small cache keyed by (H, W, dtype, device).
+#include <map>
+#include <mutex>
+#include <tuple>
+
+struct OpTables {
+    void* dev = nullptr;
+    size_t lens_off = 0;
+    int lH = 0, lW = 0;
+};
+
+static std::mutex g_op_mu;
+static std::map<std::tuple<int, int, int, int>, OpTables> g_op_tables;
+
+template <class T>
+static int op_tables(int H, int W, OpTables* out) {
+    int dev = 0;
+    OPS_HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> g(g_op_mu);
+    auto key = std::make_tuple(H, W, (int)sizeof(T), dev);
+    auto it = g_op_tables.find(key);
+    if (it != g_op_tables.end()) {
+        *out = it->second;
+        return KYM_OK;
+    }
+    LenTable<T> lt;
+    OpTables t;
+    t.lH = lt.add(H);
+    t.lW = lt.add(W);
+    Blob b;
+    std::string err;
+    if (!lt.emit(b, &t.lens_off, &err)) return kym_ops_fail(KYM_ERR_UNSUPPORTED, err);
+    OPS_HIP_TRY(hipMalloc(&t.dev, b.bytes.size()));
+    b.relocate((uintptr_t)t.dev);
+    OPS_HIP_TRY(hipMemcpy(t.dev, b.bytes.data(), b.bytes.size(), hipMemcpyHostToDevice));
+    g_op_tables[key] = t;
+    *out = t;
+    return KYM_OK;
+}
+
+static size_t op_lds_limit() {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 65536;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeSharedMemPerBlockOptin, dev) == hipSuccess && v > 0) return v;
+    return 65536;
+}
+
+template <class T>
+static int fft_op(const void* x, void* y, long long batch, int H, int W, int kind, hipStream_t s, long long lds_cap) {
+    OpTables t;
+    int rc = op_tables<T>(H, W, &t);
+    if (rc != KYM_OK) return rc;
+    const FftLen<T>* lens = reinterpret_cast<const FftLen<T>*>((char*)t.dev + t.lens_off);
+    FftOpArgs o = {x, y, batch, H, W, kind, 0, 0, 0};
+    size_t lim = op_lds_limit();
+    if (lds_cap > 0 && (size_t)lds_cap < lim) lim = (size_t)lds_cap;
+    const size_t need = (size_t)H * W * sizeof(cpx<T>);
+    if (need <= lim) {
+        if (need > 65536)
+            OPS_HIP_TRY(hipFuncSetAttribute((const void*)k_fft2_lds<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+        hipLaunchKernelGGL(k_fft2_lds<T>, dim3((unsigned)batch), dim3(256), need, s, o, lens, t.lH, t.lW);
+        OPS_HIP_TRY(hipGetLastError());
+        return KYM_OK;
+    }
+    // passes: rows (along W) then columns (along H); the first reads x, the
+    // last scales and writes the final layout, both in place on y
+    int axes[2], npass = 0;
+    if (W > 1) axes[npass++] = 1;
+    if (H > 1) axes[npass++] = 0;
+    for (int q = 0; q < npass; ++q) {
+        const int axis = axes[q];
+        const int len = axis == 1 ? W : H, ntr = axis == 1 ? H : W;
+        if ((size_t)len * sizeof(cpx<T>) > lim) return kym_ops_fail(KYM_ERR_UNSUPPORTED, "FFT length too large for LDS");
+        const int per = (int)std::max<size_t>(1, std::min<size_t>(16, lim / ((size_t)len * sizeof(cpx<T>))));
+        o.nrow = per;
+        o.nblk = (ntr + per - 1) / per;
+        const size_t lds = (size_t)per * len * sizeof(cpx<T>);
+        if (lds > 65536)
+            OPS_HIP_TRY(hipFuncSetAttribute((const void*)k_fft2_pass<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(k_fft2_pass<T>, dim3((unsigned)(batch * o.nblk)), dim3(256), lds, s, o, lens, t.lH, t.lW,
+                           axis, q == 0 ? 1 : 0, q == npass - 1 ? 1 : 0);
+        OPS_HIP_TRY(hipGetLastError());
+    }
+    return KYM_OK;
+}
+
+// ------------------------------------------------------------ C ABI
+template <class T>
+static int launch_pad2d(const void* x, void* y, long long batch, int M, int N, int top, int left, int P, int Q,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(k_pad2d<T>, dim3(grid_for(batch * P * Q, 256)), dim3(256), 0, s, (const T*)x, (T*)y, batch, M,
+                       N, top, left, P, Q);
+    OPS_HIP_TRY(hipGetLastError());
+    return KYM_OK;
+}
+
+template <class T>
+static int launch_pad1d(const void* x, void* y, long long batch, int N, int left, int P, hipStream_t s) {
+    hipLaunchKernelGGL(k_pad1d<T>, dim3(grid_for(batch * P, 256)), dim3(256), 0, s, (const T*)x, (T*)y, batch, N, left,
+                       P);
+    OPS_HIP_TRY(hipGetLastError());
+    return KYM_OK;
+}
+
+template <class T>
+static int launch_cdgmm(const void* A, const void* B, void* C, long long total, long long n, int bc, hipStream_t s) {
+    hipLaunchKernelGGL(k_cdgmm<T>, dim3(grid_for(total, 256)), dim3(256), 0, s, (const cpx<T>*)A, (const T*)B,
+                       (cpx<T>*)C, total, n, bc);
+    OPS_HIP_TRY(hipGetLastError());
+    return KYM_OK;
+}
+
+template <class T>
+static int launch_modulus(const void* x, void* y, long long n, hipStream_t s) {
+    hipLaunchKernelGGL(k_modulus<T>, dim3(grid_for(n, 256)), dim3(256), 0, s, (const cpx<T>*)x, (T*)y, n);
+    OPS_HIP_TRY(hipGetLastError());
+    return KYM_OK;
+}
+
+template <class T>
+static int launch_subsample(const void* x, void* y, long long batch, int H, int W, int kh, int kw, hipStream_t s) {
+    const long long total = batch * (H / kh) * (W / kw);
+    hipLaunchKernelGGL(k_subsample<T>, dim3(grid_for(total, 256)), dim3(256), 0, s, (const cpx<T>*)x, (cpx<T>*)y, batch,
+                       H, W, kh, kw);
+    OPS_HIP_TRY(hipGetLastError());
+    return KYM_OK;
+}
+
+template <class T>
+static int launch_unpad2d(const void* x, void* y, long long batch, int H, int W, hipStream_t s) {
+    hipLaunchKernelGGL(k_unpad2d<T>, dim3(grid_for(batch * (H - 2) * (W - 2), 256)), dim3(256), 0, s, (const T*)x,
+                       (T*)y, batch, H, W);
+    OPS_HIP_TRY(hipGetLastError());
+    return KYM_OK;
+}
+
+static int bad_dtype() { return kym_ops_fail(KYM_ERR_TYPE, "dtype must be float32 or float64"); }
+
+extern "C" int kym_op_pad2d(const void* x, void* y, long long batch, int M, int N, int top, int bottom, int left,
+                            int right, int dtype, void* stream) {
+    if (batch <= 0) return KYM_OK;
+    const int P = M + top + bottom, Q = N + left + right;
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == KYM_F32) return launch_pad2d<float>(x, y, batch, M, N, top, left, P, Q, s);
+    if (dtype == KYM_F64) return launch_pad2d<double>(x, y, batch, M, N, top, left, P, Q, s);
+    return bad_dtype();
+}
+
+extern "C" int kym_op_pad1d(const void* x, void* y, long long batch, int N, int left, int right, int dtype,
+                            void* stream) {
+    if (left >= N || right >= N) return kym_ops_fail(KYM_ERR_VALUE, "Indefinite padding size (larger than tensor).");
+    if (batch <= 0) return KYM_OK;
+    const int P = N + left + right;
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == KYM_F32) return launch_pad1d<float>(x, y, batch, N, left, P, s);
+    if (dtype == KYM_F64) return launch_pad1d<double>(x, y, batch, N, left, P, s);
+    return bad_dtype();
+}
+
+extern "C" int kym_op_cdgmm(const void* A, const void* B, void* C, long long batch, long long n, int b_complex,
+                            int dtype, void* stream) {
+    const long long total = batch * n;
+    if (total <= 0) return KYM_OK;
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == KYM_F32) return launch_cdgmm<float>(A, B, C, total, n, b_complex, s);
+    if (dtype == KYM_F64) return launch_cdgmm<double>(A, B, C, total, n, b_complex, s);
+    return bad_dtype();
+}
+
+extern "C" int kym_op_modulus(const void* x, void* y, long long n, int dtype, void* stream) {
+    if (n <= 0) return KYM_OK;
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == KYM_F32) return launch_modulus<float>(x, y, n, s);
+    if (dtype == KYM_F64) return launch_modulus<double>(x, y, n, s);
+    return bad_dtype();
+}
+
+extern "C" int kym_op_subsample(const void* x, void* y, long long batch, int H, int W, int kh, int kw, int dtype,
+                                void* stream) {
+    if (kh < 1 || kw < 1 || H % kh || W % kw)
+        return kym_ops_fail(KYM_ERR_RUNTIME, "subsampling factor must divide the size");
+    if (batch <= 0) return KYM_OK;
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == KYM_F32) return launch_subsample<float>(x, y, batch, H, W, kh, kw, s);
+    if (dtype == KYM_F64) return launch_subsample<double>(x, y, batch, H, W, kh, kw, s);
+    return bad_dtype();
+}
+
+extern "C" int kym_op_unpad2d(const void* x, void* y, long long batch, int H, int W, int dtype, void* stream) {
+    if (batch <= 0 || H < 3 || W < 3) return KYM_OK;
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == KYM_F32) return launch_unpad2d<float>(x, y, batch, H, W, s);
+    if (dtype == KYM_F64) return launch_unpad2d<double>(x, y, batch, H, W, s);
+    return bad_dtype();
+}
+
+extern "C" int kym_op_fft(const void* x, void* y, long long batch, int H, int W, int kind, int dtype,
+                          long long lds_limit, void* stream) {
+    if (kind < 0 || kind > 3) return kym_ops_fail(KYM_ERR_VALUE, "kind must be 0..3");
+    if (batch <= 0) return KYM_OK;
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == KYM_F32) return fft_op<float>(x, y, batch, H, W, kind, s, lds_limit);
+    if (dtype == KYM_F64) return fft_op<double>(x, y, batch, H, W, kind, s, lds_limit);
+    return bad_dtype();
+}
