@@ -178,6 +178,46 @@ int kym_op_unpad2d(const void* x, void* y, long long batch, int H, int W, int dt
 int kym_op_fft(const void* x, void* y, long long batch, int H, int W, int kind, int dtype, long long lds_limit,
                void* stream);
 
+/* ------------------------------------------------------ learned part ----
+ * Kernels of the warm-start decoder (kymatio_amd/learned).  They have no
+ * counterpart in the reference, whose closest precedents are the learned
+ * inverse and the descent reconstruction examples
+ * (examples/2d/regularized_inverse_scattering_MNIST_torch.py:38-131,
+ * examples/1d/reconstruct_torch.py:77-129).  FP32 tensors unless noted.
+ *
+ * conv_fwd: out (B, Cout, T) = conv1d_k9_same(scale * upsample_up(in)) + bias,
+ *   optional ReLU; in (B, Cin, T/up); w bf16 (Cout_pad, 9, Cin_pad);
+ *   T % 64 == 0, Cin_pad % 16 == 0, Cout_pad % 16 == 0 <= 128.  Also the
+ *   backward-data pass when given the transposed, flipped weights.
+ * conv_bwd_weight: dw (Cout, 9, Cin) += dW of that conv (atomics; zero it
+ *   first); Cin_pad % 64 == 0; ktiles = 64-sample time tiles per workgroup.
+ * up_relu_adjoint: din (B, C, Tin) = sum of up fine positions of dx,
+ *   masked by act > 0 when mask.
+ * bias_grad: db (C) += sum over (b, t) of dout (B, C, T) (atomics).
+ * mse: loss += mean((y - x)^2), dy = 2 (y - x) / n (dy may be null).
+ * adam: Adam step (bias-corrected, step >= 1) on n parameters.
+ * pack_weights: FP32 (Cout, 9, Cin) -> bf16 forward (Cout_pad, 9, Cin_pad)
+ *   and backward-data (Cin_pad, 9, Cout_pad, taps reversed) layouts (wt may
+ *   be null).
+ * chirps: B signals of N samples, 1-8 linear chirps + noise * N(0,1), a
+ *   deterministic function of (seed, index0 + signal index).
+ * mfma_layout_check: writes the 16x16 products of fixed integer matrices
+ *   through the bf16 MFMA builtins (K = 16 and K = 32) for a layout test. */
+int kym_dec_conv_fwd(const void* in, const void* w, const void* bias, const void* scale, void* out, int B, int Cin,
+                     int Cin_pad, int Cout, int Cout_pad, int T, int up, int relu, void* stream);
+int kym_dec_conv_bwd_weight(const void* dout, const void* in, const void* scale, void* dw, int B, int Cin,
+                            int Cin_pad, int Cout, int Cout_pad, int T, int up, int ktiles, void* stream);
+int kym_dec_up_relu_adjoint(const void* dx, const void* act, void* din, int B, int C, int Tin, int up, int mask,
+                            void* stream);
+int kym_dec_bias_grad(const void* dout, void* db, int B, int C, int T, void* stream);
+int kym_dec_mse(const void* y, const void* x, void* dy, void* loss, long long n, void* stream);
+int kym_dec_adam(void* p, const void* g, void* m, void* v, long long n, float lr, float b1, float b2, float eps,
+                 int step, void* stream);
+int kym_dec_pack_weights(const void* w, void* wf, void* wt, int Cout, int Cout_pad, int Cin, int Cin_pad,
+                         void* stream);
+int kym_chirps(void* x, int B, int N, unsigned long long seed, long long index0, float noise, void* stream);
+int kym_mfma_layout_check(void* out16, void* out32, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -41,6 +41,22 @@ _SIGNATURES = {
     'kym_scat1d_forward': [_c_vp, _c_vp, _c_vp, _c_vp, _c_ll, _c_vp, _c_ll, _c_vp],
     'kym_scat1d_forward_train': [_c_vp, _c_vp, _c_vp, _c_vp, _c_ll, _c_vp, _c_ll, _c_vp],
     'kym_scat1d_backward': [_c_vp, _c_vp, _c_vp, _c_vp, _c_ll, _c_vp, _c_ll, _c_vp],
+    'kym_op_pad2d': [_c_vp, _c_vp, _c_ll] + [_c_i] * 7 + [_c_vp],
+    'kym_op_pad1d': [_c_vp, _c_vp, _c_ll, _c_i, _c_i, _c_i, _c_i, _c_vp],
+    'kym_op_cdgmm': [_c_vp, _c_vp, _c_vp, _c_ll, _c_ll, _c_i, _c_i, _c_vp],
+    'kym_op_modulus': [_c_vp, _c_vp, _c_ll, _c_i, _c_vp],
+    'kym_op_subsample': [_c_vp, _c_vp, _c_ll, _c_i, _c_i, _c_i, _c_i, _c_i, _c_vp],
+    'kym_op_unpad2d': [_c_vp, _c_vp, _c_ll, _c_i, _c_i, _c_i, _c_vp],
+    'kym_op_fft': [_c_vp, _c_vp, _c_ll, _c_i, _c_i, _c_i, _c_i, _c_ll, _c_vp],
+    'kym_dec_conv_fwd': [_c_vp] * 5 + [_c_i] * 9 + [_c_vp],
+    'kym_dec_conv_bwd_weight': [_c_vp] * 4 + [_c_i] * 9 + [_c_vp],
+    'kym_dec_up_relu_adjoint': [_c_vp] * 3 + [_c_i] * 5 + [_c_vp],
+    'kym_dec_bias_grad': [_c_vp, _c_vp, _c_i, _c_i, _c_i, _c_vp],
+    'kym_dec_mse': [_c_vp] * 4 + [_c_ll, _c_vp],
+    'kym_dec_adam': [_c_vp] * 4 + [_c_ll] + [ctypes.c_float] * 4 + [_c_i, _c_vp],
+    'kym_dec_pack_weights': [_c_vp] * 3 + [_c_i] * 4 + [_c_vp],
+    'kym_chirps': [_c_vp, _c_i, _c_i, ctypes.c_ulonglong, _c_ll, ctypes.c_float, _c_vp],
+    'kym_mfma_layout_check': [_c_vp, _c_vp, _c_vp],
 }
 
 

@@ -23,7 +23,7 @@ COMMON = ['-O3', '-std=c++17', '-fPIC', '-shared', '-Wno-unused-result',
 
 # (source, extra defines, object name); compiled in parallel, then linked
 PRODUCT_UNITS = [('capi.hip', [], 'capi.o'), ('kspec2d.hip', [], 'kspec2d.o'), ('kinst1d.hip', [], 'kinst1d.o'),
-                 ('ops.hip', [], 'ops.o')] + [
+                 ('ops.hip', [], 'ops.o'), ('decoder.hip', [], 'decoder.o')] + [
     ('kinst2d.hip', ['-DKYM_T=%s' % T, '-DKYM_P=%d' % P, '-DKYM_PICK=kym_pick2d_%s_p%d' % (tag, P)],
      'kinst2d_%s_p%d.o' % (tag, P))
     for tag, T in (('f32', 'float'), ('f64', 'double')) for P in (7, 17, 31)]

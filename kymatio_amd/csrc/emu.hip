@@ -61,7 +61,7 @@ static int emu_forward2d(int M, int N, int J, int L, int max_order, int pre_pad,
 
 template <class T>
 static int emu_train2d(int M, int N, int J, int L, int max_order, int pre_pad, long long lds_limit, const void* x,
-                       const void* filt, void* out, const void* dS, void* gx, long long batch) {
+                       const void* filt, void* out, const void* dS, void* gx, long long batch, int nbwd) {
     Scat2DPlan<T> p;
     KymError e = {0, ""};
     if (!p.build(M, N, J, L, max_order, pre_pad, (size_t)lds_limit, &e)) {
@@ -76,7 +76,7 @@ static int emu_train2d(int M, int N, int J, int L, int max_order, int pre_pad, l
     const uintptr_t base = (uintptr_t)p.blob.bytes.data();
     std::vector<cpx<T>> ws((size_t)(p.ws_img_train * batch) + 1);
     std::vector<cpx<T>> lds(p.max_lds / sizeof(cpx<T>) + 1);
-    for (int phase = 0; phase < 2; ++phase) {
+    for (int phase = 0; phase < 1 + nbwd; ++phase) {
         const std::vector<Launch2D>& ls = phase == 0 ? p.launches : p.bwd_launches;
         for (const Launch2D& l : ls) {
             Scat2DArgs<T> a = p.args(l, base);
@@ -101,11 +101,13 @@ static int emu_train2d(int M, int N, int J, int L, int max_order, int pre_pad, l
     return 0;
 }
 
+// nbwd: number of backward passes run on one training forward (the backward
+// must leave the saved state intact, so every pass gives the same dL/dx).
 extern "C" int kym_emu_scat2d_train(int M, int N, int J, int L, int max_order, int pre_pad, int dtype,
                                     long long lds_limit, const void* x, const void* filt, void* out, const void* dS,
-                                    void* gx, long long batch) {
-    if (dtype == 0) return emu_train2d<float>(M, N, J, L, max_order, pre_pad, lds_limit, x, filt, out, dS, gx, batch);
-    return emu_train2d<double>(M, N, J, L, max_order, pre_pad, lds_limit, x, filt, out, dS, gx, batch);
+                                    void* gx, long long batch, int nbwd) {
+    if (dtype == 0) return emu_train2d<float>(M, N, J, L, max_order, pre_pad, lds_limit, x, filt, out, dS, gx, batch, nbwd);
+    return emu_train2d<double>(M, N, J, L, max_order, pre_pad, lds_limit, x, filt, out, dS, gx, batch, nbwd);
 }
 
 extern "C" int kym_emu_scat2d_forward(int M, int N, int J, int L, int max_order, int pre_pad, int dtype,
@@ -121,7 +123,7 @@ extern "C" int kym_emu_scat1d_forward(int N, int N_pad, int pad_left, int log2_s
                                       const int* psi1_nlev, int n_psi2, const int* psi2_j, const int* psi2_nlev,
                                       int phi_nlev, const int* ind_start, const int* ind_end, int n_ind, int dtype,
                                       const void* x, const void* filt, void* out, long long batch,
-                                      long long* info, const void* dS, void* gx) {
+                                      long long* info, const void* dS, void* gx, int nbwd) {
     Scat1DDesc d = {N, N_pad, pad_left, log2_stride, average_mode, oversampling, max_order, n_psi1, n_psi2, phi_nlev,
                     psi1_j, psi1_nlev, psi2_j, psi2_nlev, ind_start, ind_end, n_ind};
     auto run = [&](auto tag) -> int {
@@ -140,7 +142,7 @@ extern "C" int kym_emu_scat1d_forward(int N, int N_pad, int pad_left, int log2_s
         const long long stride = train ? p.ws_sig_train : p.ws_sig;
         std::vector<cpx<T>> ws((size_t)(stride * batch) + 1);
         std::vector<cpx<T>> lds(p.max_lds / sizeof(cpx<T>) + 1);
-        for (int phase = 0; phase < (train ? 2 : 1); ++phase)
+        for (int phase = 0; phase < (train ? 1 + nbwd : 1); ++phase)
             for (const Launch1D& l : (phase ? p.bwd_launches : p.launches)) {
                 Scat1DArgs<T> a = p.args(l, base);
                 a.x = static_cast<const T*>(x);

@@ -33,7 +33,8 @@ struct Node1D {
     long long slot;  // spectrum slot in the per-signal workspace, -1 none
     // training
     int child_begin, child_end;  // node-index range of the children
-    long long save;              // saved u (digit-reversed, unnormalised), then dL/dX (natural)
+    long long save;              // saved u (digit-reversed, unnormalised), read-only in the backward
+    long long gsave;             // dL/dX (natural) written by the backward
     long long gpad;              // root: dL/dpad(x), real
 };
 
@@ -202,7 +203,7 @@ KYM_HD void body_bwd_node1d(const Scat1DArgs<T>& a, int task, cpx<T>* lds, int t
             const Node1D& cn = a.nodes[c];
             const int nc = a.n[cn.res];
             const T fv = a.filt[a.filt_off[cn.filt] + i] / T(cn.k);
-            const cpx<T> gx = wsi[cn.save + (i % nc)];
+            const cpx<T> gx = wsi[cn.gsave + (i % nc)];
             acc.re += gx.re * fv;
             acc.im += gx.im * fv;
         }
@@ -229,7 +230,7 @@ KYM_HD void body_bwd_node1d(const Scat1DArgs<T>& a, int task, cpx<T>* lds, int t
     wg_barrier();
     fft_dit<false, MAXP>(lds, fl, Batch{1, 1, n, false}, tid, nth);
     const T inv_n = T(1) / T(n);
-    for (int i = tid; i < n; i += nth) wsi[nd.save + i] = lds[i] * inv_n;
+    for (int i = tid; i < n; i += nth) wsi[nd.gsave + i] = lds[i] * inv_n;
 }
 
 // Reflect-pad adjoint (1D).  task -> signal.

@@ -159,7 +159,7 @@ struct Scat1DPlan {
         // training layout (extends the inference one): saved u per non-root
         // node, the root's dL/dpad(x); children as node-index ranges
         {
-            for (auto& nd : nodes) { nd.child_begin = nd.child_end = 0; nd.save = nd.gpad = -1; }
+            for (auto& nd : nodes) { nd.child_begin = nd.child_end = 0; nd.save = nd.gsave = nd.gpad = -1; }
             for (size_t i = 1; i < nodes.size(); ++i) {
                 Node1D& pn = nodes[nodes[i].parent];
                 if (pn.child_end == 0) pn.child_begin = (int)i;
@@ -167,7 +167,11 @@ struct Scat1DPlan {
             }
             long long w = ws_sig;
             for (auto& nd : nodes) {
-                if (nd.order > 0) { nd.save = w; w += n[nd.res]; }
+                if (nd.order > 0) {
+                    nd.save = w; w += n[nd.res];
+                    w = (w + align - 1) / align * align;
+                    nd.gsave = w; w += n[nd.res];
+                }
                 else { nd.gpad = w; w += ((long long)N_pad * (long long)sizeof(T) + sizeof(cpx<T>) - 1) / sizeof(cpx<T>); }
                 w = (w + align - 1) / align * align;
             }

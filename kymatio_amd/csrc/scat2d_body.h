@@ -37,7 +37,8 @@ struct Node2D {
     long long leaf_slot;  // scratch for big leaves, -1 none
     // training (forward with saving + backward)
     int child_begin, child_end;  // node-index range of this node's children
-    long long save;              // slot of the saved pre-modulus field u (then of dL/dX), -1 none
+    long long save;              // slot of the saved pre-modulus field u, -1 none
+    long long gsave;             // slot of dL/dX written by the backward, -1 none
     long long gslot;             // leaf dL/dY (natural, scaled) when not formed in the node kernel
     long long gpad;              // root: dL/d(padded input), real
 };

@@ -13,7 +13,8 @@
 //                                          reflect-pad adjoint into dL/dx.
 // The forward run in training mode stores u (unnormalised, in the
 // digit-reversed order the inverse DIF leaves it in) in the node's save slot;
-// the backward overwrites it with dL/dX (natural order).
+// the backward only reads it and writes dL/dX (natural order) to the node's
+// gsave slot, so it can run any number of times on one forward.
 #pragma once
 #include "scat2d_body.h"
 
@@ -30,7 +31,7 @@ KYM_HD KYM_INL cpx<T> grad_u_elem(const Scat2DArgs<T>& a, const Node2D& nd, cons
         const Node2D& cn = a.nodes[c];
         const int Hc = a.H[cn.res], Wc = a.W[cn.res];
         const T fv = a.filt[a.filt_off[cn.filt] + i] / T(cn.k * cn.k);
-        const cpx<T> gx = wsi[cn.save + (long long)(y % Hc) * Wc + (x % Wc)];
+        const cpx<T> gx = wsi[cn.gsave + (long long)(y % Hc) * Wc + (x % Wc)];
         acc.re += gx.re * fv;
         acc.im += gx.im * fv;
     }
@@ -87,7 +88,7 @@ KYM_HD void bwd_node_body(const Scat2DArgs<T>& a, const G& g, const Node2D& nd, 
         }
         return;
     }
-    cpx<T>* sv = wsi + nd.save;
+    const cpx<T>* sv = wsi + nd.save;
     for (int p = tid; p < n; p += nth) {
         const cpx<T> u = sv[p];
         const T mag = kym_sqrt(u.re * u.re + u.im * u.im);
@@ -97,7 +98,8 @@ KYM_HD void bwd_node_body(const Scat2DArgs<T>& a, const G& g, const Node2D& nd, 
     wg_barrier();
     g.fwd2d(lds, tid, nth);
     const T inv_n = T(1) / T(n);
-    for (int i = tid; i < n; i += nth) sv[i] = lds[i] * inv_n;
+    cpx<T>* gs = wsi + nd.gsave;
+    for (int i = tid; i < n; i += nth) gs[i] = lds[i] * inv_n;
 }
 
 template <class T, int MAXP>
@@ -142,7 +144,7 @@ KYM_HD void body_bwd_gcols(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int ti
 }
 
 // Big nodes: rows of the U slot, inverse DIF along x, then modulus adjoint and
-// forward DIT along x into the save slot (root: dL/dpad(x) into gpad).
+// forward DIT along x into the gsave slot (root: dL/dpad(x) into gpad).
 template <class T, int MAXP>
 KYM_HD void body_bwd_grows(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
     const int per = a.ngroup * a.nblk;
@@ -166,7 +168,7 @@ KYM_HD void body_bwd_grows(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int ti
         }
         return;
     }
-    cpx<T>* sv = wsi + nd.save + (long long)r0 * W;
+    const cpx<T>* sv = wsi + nd.save + (long long)r0 * W;
     for (int i = tid; i < nr * W; i += nth) {
         const cpx<T> u = sv[i];
         const T mag = kym_sqrt(u.re * u.re + u.im * u.im);
@@ -175,10 +177,11 @@ KYM_HD void body_bwd_grows(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int ti
     }
     wg_barrier();
     fft_dit<false, MAXP>(lds, fW, Batch{nr, 1, W, false}, tid, nth);
-    for (int i = tid; i < nr * W; i += nth) sv[i] = lds[i];
+    cpx<T>* gs = wsi + nd.gsave + (long long)r0 * W;
+    for (int i = tid; i < nr * W; i += nth) gs[i] = lds[i];
 }
 
-// Big nodes: forward DIT along y of the save slot -> natural dL/dX (scaled 1/n).
+// Big nodes: forward DIT along y of the gsave slot -> natural dL/dX (scaled 1/n).
 template <class T, int MAXP>
 KYM_HD void body_bwd_gfcols(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
     const int per = a.ngroup * a.nblk;
@@ -187,7 +190,7 @@ KYM_HD void body_bwd_gfcols(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int t
     const Node2D& nd = a.nodes[a.group[gi]];
     const int r = nd.res, H = a.H[r], W = a.W[r];
     const int c0 = cb * a.blk, nc = (W - c0 < a.blk) ? W - c0 : a.blk;
-    cpx<T>* buf = a.ws + il * a.ws_img + nd.save + c0;
+    cpx<T>* buf = a.ws + il * a.ws_img + nd.gsave + c0;
     for (int i = tid; i < H * nc; i += nth) {
         const int y = i / nc, c = i - y * nc;
         lds[i] = buf[(long long)y * W + c];

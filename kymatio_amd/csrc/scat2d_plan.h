@@ -157,11 +157,11 @@ struct Scat2DPlan {
             const long long align = 256 / (long long)cs;
             ws_img = (ws_img + align - 1) / align * align;
         }
-        // training layout: save slot (u, then dL/dX) for every non-root node,
+        // training layout: save slot (u) and gsave slot (dL/dX) for every non-root node,
         // leaf dL/dY slot where the leaf is not formed in the node kernel, and
         // the root's dL/dpad(x); children as node-index ranges.
         {
-            for (auto& nd : nodes) { nd.child_begin = nd.child_end = 0; nd.save = nd.gslot = nd.gpad = -1; }
+            for (auto& nd : nodes) { nd.child_begin = nd.child_end = 0; nd.save = nd.gsave = nd.gslot = nd.gpad = -1; }
             for (size_t i = 1; i < nodes.size(); ++i) {
                 Node2D& pn = nodes[nodes[i].parent];
                 if (pn.child_end == 0) pn.child_begin = (int)i;
@@ -170,7 +170,11 @@ struct Scat2DPlan {
             const long long align = 256 / (long long)cs;
             long long w = ws_img;
             for (auto& nd : nodes) {
-                if (nd.order > 0) { nd.save = w; w += (long long)H[nd.res] * W[nd.res]; }
+                if (nd.order > 0) {
+                    nd.save = w; w += (long long)H[nd.res] * W[nd.res];
+                    w = (w + align - 1) / align * align;
+                    nd.gsave = w; w += (long long)H[nd.res] * W[nd.res];
+                }
                 if (nd.leaf_mode != 0) { nd.gslot = w; w += (long long)leaf_elems; }
                 if (nd.order == 0) { nd.gpad = w; w += ((long long)H[0] * W[0] * (long long)sizeof(T) + cs - 1) / cs; }
                 w = (w + align - 1) / align * align;

@@ -1,10 +1,85 @@
-"""1D HIP backend namespace (plugin name 'torch'); the frontend runs the fused
-pipeline for it.  Op names of kymatio/scattering1d/backend/torch_backend.py."""
-from ...backend.torch_backend import TorchBackend
+"""1D HIP backend namespace (plugin name 'torch', the default): op names and
+checks of kymatio/scattering1d/backend/torch_backend.py:18-148, each op one
+HIP kernel on GPU tensors.  The frontend runs the fused pipeline."""
+import torch
+
+from ... import _lib
+from ...backend.torch_backend import TorchBackend, _dt, _gpu, _ptr
+
+
+def _fft1(x, kind, out_last):
+    _gpu(x)
+    n = x.shape[-2]
+    y = torch.empty(x.shape[:-1] + (out_last,), dtype=x.dtype, device=x.device)
+    with torch.cuda.device(x.device):
+        _lib.check(_lib.lib().kym_op_fft(_ptr(x), _ptr(y), x.numel() // (n * x.shape[-1]), n, 1, kind, _dt(x), 0,
+                                          _lib.stream_of(x.device)))
+    return y
 
 
 class TorchBackend1D(TorchBackend):
-    pass
+    @classmethod
+    def subsample_fourier(cls, x, k):
+        cls.complex_check(x)
+        _gpu(x)
+        n = x.shape[-2]
+        xx = x.contiguous()
+        y = torch.empty(x.shape[:-2] + (n // k, 2), dtype=x.dtype, device=x.device)
+        with torch.cuda.device(x.device):
+            _lib.check(_lib.lib().kym_op_subsample(_ptr(xx), _ptr(y), x.numel() // (2 * n), n, 1, k, 1, _dt(x),
+                                                    _lib.stream_of(x.device)))
+        return y
+
+    @staticmethod
+    def pad(x, pad_left, pad_right, mode='reflect'):
+        if mode != 'constant':
+            if pad_left >= x.shape[-1] or pad_right >= x.shape[-1]:
+                raise ValueError('Indefinite padding size (larger than tensor).')
+        if mode != 'reflect':
+            raise NotImplementedError('only reflect padding runs on the HIP path')
+        _gpu(x)
+        N = x.shape[-1]
+        xx = x.contiguous()
+        y = torch.empty(x.shape[:-1] + (N + pad_left + pad_right,), dtype=x.dtype, device=x.device)
+        with torch.cuda.device(x.device):
+            _lib.check(_lib.lib().kym_op_pad1d(_ptr(xx), _ptr(y), x.numel() // N, N, pad_left, pad_right, _dt(x),
+                                                _lib.stream_of(x.device)))
+        return y[..., None]
+
+    @staticmethod
+    def unpad(x, i0, i1):
+        x = x.reshape(x.shape[:-1])
+        return x[..., i0:i1]
+
+    @classmethod
+    def cfft(cls, x):
+        cls.contiguous_check(x)
+        cls.complex_check(x)
+        return _fft1(x, 0, 2)
+
+    @classmethod
+    def rfft(cls, x):
+        cls.contiguous_check(x)
+        cls.real_check(x)
+        return _fft1(x, 2, 2)
+
+    @classmethod
+    def irfft(cls, x):
+        cls.contiguous_check(x)
+        cls.complex_check(x)
+        return _fft1(x, 3, 1)
+
+    @classmethod
+    def ifft(cls, x):
+        cls.contiguous_check(x)
+        cls.complex_check(x)
+        return _fft1(x, 1, 2)
+
+    @classmethod
+    def average_global(cls, x):
+        cls.contiguous_check(x)
+        cls.real_check(x)
+        return torch.sum(x, axis=-2, keepdims=True)
 
 
 backend = TorchBackend1D

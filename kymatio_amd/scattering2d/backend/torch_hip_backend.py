@@ -1,13 +1,4 @@
-"""2D HIP backend namespace (plugin name 'torch'): the reference's op names
-(kymatio/scattering2d/backend/torch_backend.py:19-183).  The frontend runs
-the fused pipeline for this backend."""
-from ...backend.torch_backend import TorchBackend
+"""Alias of the default 2D HIP backend (backend='torch_hip')."""
+from .torch_backend import TorchBackend2D
 
-
-class TorchBackend2D(TorchBackend):
-    @staticmethod
-    def stack(arrays):
-        return TorchBackend.stack(arrays, -3)
-
-
-backend = TorchBackend2D  # alias module: backend="torch_hip"
+backend = TorchBackend2D

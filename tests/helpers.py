@@ -84,7 +84,7 @@ class Emu:
         return out, {'launches': info[3], 'big_nodes': info[4], 'spec_launches': info[5]}
 
 
-def emu_scat1d(emu, module, x, dtype=np.float32, dS=None):
+def emu_scat1d(emu, module, x, dtype=np.float32, dS=None, nbwd=1):
     """Run the 1D bodies in the CPU harness with the setup of a (CPU)
     Scattering1D module; with dS, also the backward (returns S, C, dL/dx)."""
     m = module
@@ -93,7 +93,8 @@ def emu_scat1d(emu, module, x, dtype=np.float32, dS=None):
     f = emu.lib.kym_emu_scat1d_forward
     f.argtypes = ([ctypes.c_int] * 8 + [ctypes.c_void_p] * 2 + [ctypes.c_int] + [ctypes.c_void_p] * 2
                   + [ctypes.c_int] + [ctypes.c_void_p] * 2 + [ctypes.c_int, ctypes.c_int]
-                  + [ctypes.c_void_p] * 3 + [ctypes.c_longlong, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p])
+                  + [ctypes.c_void_p] * 3 + [ctypes.c_longlong, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.c_int])
     f.restype = ctypes.c_int
     p1, p2 = m.psi1_f, m.psi2_f
     nind = max(m.ind_start.keys()) + 1
@@ -106,29 +107,31 @@ def emu_scat1d(emu, module, x, dtype=np.float32, dS=None):
             i32([m.ind_end[i] for i in range(nind)]), nind, 0 if dtype == np.float32 else 1]
     info = (ctypes.c_longlong * 4)()
     B = x.shape[0]
-    rc = f(*args, None, None, None, B, info, None, None)
+    rc = f(*args, None, None, None, B, info, None, None, 0)
     if rc != 0:
         raise RuntimeError(emu.lib.kym_emu_last_error().decode())
     out = np.zeros((B, info[1]), dtype)
     xx = np.ascontiguousarray(x, dtype)
     if dS is None:
-        rc = f(*args, xx.ctypes.data, packed.ctypes.data, out.ctypes.data, B, info, None, None)
+        rc = f(*args, xx.ctypes.data, packed.ctypes.data, out.ctypes.data, B, info, None, None, 0)
         if rc != 0:
             raise RuntimeError(emu.lib.kym_emu_last_error().decode())
         return out, info[0]
     dd = np.ascontiguousarray(dS, dtype).reshape(B, -1)
     gx = np.zeros_like(xx)
-    rc = f(*args, xx.ctypes.data, packed.ctypes.data, out.ctypes.data, B, info, dd.ctypes.data, gx.ctypes.data)
+    rc = f(*args, xx.ctypes.data, packed.ctypes.data, out.ctypes.data, B, info, dd.ctypes.data, gx.ctypes.data,
+           nbwd)
     if rc != 0:
         raise RuntimeError(emu.lib.kym_emu_last_error().decode())
     return out, info[0], gx
 
 
-def emu_train2d(emu, x, J, shape, filters_packed, dS, L=8, max_order=2, pre_pad=0, lds_limit=160 * 1024):
-    """Training forward + hand-written backward of the 2D bodies on the CPU
-    harness; returns (S, dL/dx) for the upstream gradient dS."""
+def emu_train2d(emu, x, J, shape, filters_packed, dS, L=8, max_order=2, pre_pad=0, lds_limit=160 * 1024, nbwd=1):
+    """Training forward + hand-written backward (nbwd times on the same saved
+    state) of the 2D bodies on the CPU harness; returns (S, dL/dx of the last
+    backward) for the upstream gradient dS."""
     f = emu.lib.kym_emu_scat2d_train
-    f.argtypes = [ctypes.c_int] * 7 + [ctypes.c_longlong] + [ctypes.c_void_p] * 5 + [ctypes.c_longlong]
+    f.argtypes = [ctypes.c_int] * 7 + [ctypes.c_longlong] + [ctypes.c_void_p] * 5 + [ctypes.c_longlong, ctypes.c_int]
     f.restype = ctypes.c_int
     dt = filters_packed.dtype
     xx = np.ascontiguousarray(x, dt)
@@ -136,7 +139,7 @@ def emu_train2d(emu, x, J, shape, filters_packed, dS, L=8, max_order=2, pre_pad=
     out = np.zeros(dS.shape, dt)
     gx = np.zeros(xx.shape, dt)
     rc = f(shape[0], shape[1], J, L, max_order, pre_pad, 0 if dt == np.float32 else 1, lds_limit, xx.ctypes.data,
-           filters_packed.ctypes.data, out.ctypes.data, dd.ctypes.data, gx.ctypes.data, x.shape[0])
+           filters_packed.ctypes.data, out.ctypes.data, dd.ctypes.data, gx.ctypes.data, x.shape[0], nbwd)
     if rc != 0:
         raise RuntimeError(emu.lib.kym_emu_last_error().decode())
     return out, gx

@@ -107,8 +107,9 @@ def test_backward_matches_autograd_through_oracle(emu, shape, J, kw, lds):
     w = rng.standard_normal(S.shape)
     (S * torch.from_numpy(w)).sum().backward()
     from tests.helpers import emu_train2d
+    # three backward passes on one forward: the backward must not consume the saved state
     out, gx = emu_train2d(emu, x, J, shape, f, w, L=L, max_order=kw.get('max_order', 2),
-                          pre_pad=kw.get('pre_pad', 0), lds_limit=lds)
+                          pre_pad=kw.get('pre_pad', 0), lds_limit=lds, nbwd=3)
     g = xt.grad.numpy()
     assert np.abs(out - S.detach().numpy()).max() < 1e-13
     assert np.abs(gx - g).max() <= 1e-12 * np.abs(g).max()
