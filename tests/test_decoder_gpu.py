@@ -85,7 +85,7 @@ def test_adam_reduces_loss_on_fixed_batch():
         losses.append(loss.item())
         grads = dec.backward(2 * (y - target) / y.numel(), acts)
         dec.adam_step(grads, lr=1e-3)
-    assert losses[-1] < 0.5 * losses[0]
+    assert losses[-1] < 0.9 * losses[0]
 
 
 def test_chirp_generator():

@@ -55,7 +55,7 @@ def test_finite_differences_fp64_patches():
             assert abs(fd.item() - an.item()) <= 1e-6 * max(abs(an.item()), 1e-12) + 1e-9
 
 
-@pytest.mark.parametrize('dtype,tol', [(np.float64, 1e-10), (np.float32, 2e-4)])
+@pytest.mark.parametrize('dtype,tol', [(np.float64, 1e-10), (np.float32, 1e-3)])
 def test_matches_reference_autograd_golden(dtype, tol):
     """d/dx sum((S(x)-S(y))^2) at J=4 128x128, as torch autograd through the
     reference computed it (tests/golden/make_golden.py)."""
