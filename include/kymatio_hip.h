@@ -174,9 +174,11 @@ int kym_op_unpad2d(const void* x, void* y, long long batch, int H, int W, int dt
  * 133-155; 1D with W = 1, kymatio/scattering1d/backend/torch_backend.py:
  * 108-141) over (batch, H, W): kind 0 forward c2c, 1 inverse c2c (1/(HW)),
  * 2 forward from real input, 3 inverse keeping the real part.  lds_limit as
- * in kym_scat2d_create (0 = device maximum). */
-int kym_op_fft(const void* x, void* y, long long batch, int H, int W, int kind, int dtype, long long lds_limit,
-               void* stream);
+ * in kym_scat2d_create (0 = device maximum).  work: batch * H * W complex
+ * scratch, needed only by kind 3 when a transform does not fit in LDS
+ * (status 3 if it is then null); may be null otherwise. */
+int kym_op_fft(const void* x, void* y, void* work, long long batch, int H, int W, int kind, int dtype,
+               long long lds_limit, void* stream);
 
 /* ------------------------------------------------------ learned part ----
  * Kernels of the warm-start decoder (kymatio_amd/learned).  They have no

@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 
+#include "ops_body.h"
 #include "scat1d_plan.h"
 #include "scat2d_plan.h"
 
@@ -165,4 +166,39 @@ extern "C" int kym_emu_scat1d_forward(int N, int N_pad, int pad_left, int log2_s
     };
     if (dtype == 0) return run(float(0));
     return run(double(0));
+}
+
+// Per-op FFT (kym_op_fft) on the host: same launch sequence and bodies.
+template <class T>
+static int emu_op_fft(const void* x, void* y, void* work, long long batch, int H, int W, int kind, long long lds_cap) {
+    LenTable<T> lt;
+    const int lH = lt.add(H), lW = lt.add(W);
+    Blob b;
+    size_t lens_off = 0;
+    std::string err;
+    if (!lt.emit(b, &lens_off, &err)) { g_emu_error = err; return KYM_ERR_UNSUPPORTED; }
+    b.relocate((uintptr_t)b.bytes.data());
+    const FftLen<T>* lens = reinterpret_cast<const FftLen<T>*>(b.bytes.data() + lens_off);
+    FftOpStep steps[2];
+    int nsteps = 0;
+    const char* e = nullptr;
+    const int code = fftop_steps(x, y, work, H, W, kind, (size_t)lds_cap, sizeof(cpx<T>), steps, &nsteps, &e);
+    if (code) { g_emu_error = e; return code; }
+    for (int q = 0; q < nsteps; ++q) {
+        const FftOpStep& st = steps[q];
+        FftOpArgs o = {x, y, batch, H, W, kind, 0, st.nrow, st.nblk};
+        std::vector<cpx<T>> lds(st.lds / sizeof(cpx<T>) + 1);
+        if (st.route == 0)
+            for (long long bb = 0; bb < batch; ++bb) fftop_lds_body<T>(o, lens, lH, lW, bb, lds.data(), 0, 1);
+        else
+            for (long long task = 0; task < batch * st.nblk; ++task)
+                fftop_pass_body<T>(o, st.io, lens, lH, lW, st.axis, task, lds.data(), 0, 1);
+    }
+    return 0;
+}
+
+extern "C" int kym_emu_op_fft(const void* x, void* y, void* work, long long batch, int H, int W, int kind, int dtype,
+                              long long lds_cap) {
+    if (dtype == 0) return emu_op_fft<float>(x, y, work, batch, H, W, kind, lds_cap);
+    return emu_op_fft<double>(x, y, work, batch, H, W, kind, lds_cap);
 }

@@ -97,104 +97,21 @@ __global__ void k_unpad2d(const T* x, T* y, long long batch, int H, int W) {
     }
 }
 
-// ------------------------------------------------------------ FFT op
-// kind: 0 c2c forward, 1 c2c inverse (1/n), 2 r2c forward (real input),
-// 3 inverse, real part out (1/n).  A batch of 2D transforms of H x W (W = 1
-// gives 1D along H).  Small sizes: one workgroup per transform in LDS.
-// Large sizes: row pass then column pass through `y` (in place).
-struct FftOpArgs {
-    const void* x;
-    void* y;
-    long long batch;
-    int H, W, kind;
-    int r0;     // row/column block start (pass kernels)
-    int nrow;   // rows or columns per task
-    int nblk;   // tasks per transform (pass kernels)
-};
-
-template <class T, bool INV>
-__device__ void load_2d(const FftOpArgs& o, const FftLen<T>& fH, const FftLen<T>& fW, long long b, cpx<T>* lds, int tid,
-                        int nth) {
-    const int H = o.H, W = o.W, n = H * W;
-    for (int p = tid; p < n; p += nth) {
-        const int py = p / W, px = p - (p / W) * W;
-        // forward (DIT) wants digit-reversed storage; inverse (DIF) natural
-        const int y = INV ? py : fH.rev[py];
-        const int x = INV ? px : fW.rev[px];
-        const long long src = b * n + (long long)y * W + x;
-        if (o.kind == 2) lds[p] = {static_cast<const T*>(o.x)[src], T(0)};
-        else lds[p] = static_cast<const cpx<T>*>(o.x)[src];
-    }
-}
+#include "ops_body.h"
 
 template <class T>
 __global__ void __launch_bounds__(256) k_fft2_lds(FftOpArgs o, const FftLen<T>* lens, int lH, int lW) {
     extern __shared__ __attribute__((aligned(16))) char kym_smem_op[];
-    cpx<T>* lds = reinterpret_cast<cpx<T>*>(kym_smem_op);
-    const long long b = blockIdx.x;
-    if (b >= o.batch) return;
-    const FftLen<T>& fH = lens[lH];
-    const FftLen<T>& fW = lens[lW];
-    const int H = o.H, W = o.W, n = H * W, tid = threadIdx.x, nth = blockDim.x;
-    const bool inv = o.kind == 1 || o.kind == 3;
-    if (inv) load_2d<T, true>(o, fH, fW, b, lds, tid, nth);
-    else load_2d<T, false>(o, fH, fW, b, lds, tid, nth);
-    __syncthreads();
-    if (inv) {
-        if (W > 1) fft_dif<true, 31>(lds, fW, Batch{H, 1, W, false}, tid, nth);
-        fft_dif<true, 31>(lds, fH, Batch{W, W, 1, true}, tid, nth);
-        const T inv_n = T(1) / T(n);
-        for (int p = tid; p < n; p += nth) {
-            const int py = p / W, px = p - (p / W) * W;
-            const long long dst = b * n + (long long)fH.rev[py] * W + fW.rev[px];
-            if (o.kind == 3) static_cast<T*>(o.y)[dst] = lds[p].re * inv_n;
-            else static_cast<cpx<T>*>(o.y)[dst] = lds[p] * inv_n;
-        }
-    } else {
-        if (W > 1) fft_dit<false, 31>(lds, fW, Batch{H, 1, W, false}, tid, nth);
-        fft_dit<false, 31>(lds, fH, Batch{W, W, 1, true}, tid, nth);
-        for (int p = tid; p < n; p += nth) static_cast<cpx<T>*>(o.y)[b * n + p] = lds[p];
-    }
+    if ((long long)blockIdx.x >= o.batch) return;
+    fftop_lds_body<T>(o, lens, lH, lW, blockIdx.x, reinterpret_cast<cpx<T>*>(kym_smem_op), threadIdx.x, blockDim.x);
 }
 
-// Pass kernels for transforms that do not fit: axis 1 = rows (along W),
-// axis 0 = columns (along H).  Rows run first reading x, columns in place on y.
 template <class T>
-__global__ void __launch_bounds__(256) k_fft2_pass(FftOpArgs o, const FftLen<T>* lens, int lH, int lW, int axis,
-                                                   int first, int last) {
+__global__ void __launch_bounds__(256) k_fft2_pass(FftOpArgs o, FftPassIO io, const FftLen<T>* lens, int lH, int lW,
+                                                   int axis) {
     extern __shared__ __attribute__((aligned(16))) char kym_smem_op2[];
-    cpx<T>* lds = reinterpret_cast<cpx<T>*>(kym_smem_op2);
-    const long long task = blockIdx.x;
-    const long long b = task / o.nblk;
-    const int blk = (int)(task - b * o.nblk);
-    if (b >= o.batch) return;
-    const int H = o.H, W = o.W, tid = threadIdx.x, nth = blockDim.x;
-    const long long n = (long long)H * W;
-    const bool inv = o.kind == 1 || o.kind == 3;
-    const FftLen<T>& f = lens[axis == 1 ? lW : lH];
-    const int len = f.n, c0 = blk * o.nrow;
-    const int nt = (axis == 1 ? H : W) - c0 < o.nrow ? (axis == 1 ? H : W) - c0 : o.nrow;
-    // lds[t * len + e]: transform t of this task, element e
-    for (int i = tid; i < nt * len; i += nth) {
-        const int t = i / len, e = i - (i / len) * len;
-        const int src_e = inv ? e : f.rev[e];
-        const long long idx = axis == 1 ? (long long)(c0 + t) * W + src_e : (long long)src_e * W + c0 + t;
-        if (first && o.kind == 2) lds[i] = {static_cast<const T*>(o.x)[b * n + idx], T(0)};
-        else if (first) lds[i] = static_cast<const cpx<T>*>(o.x)[b * n + idx];
-        else lds[i] = static_cast<const cpx<T>*>(o.y)[b * n + idx];
-    }
-    __syncthreads();
-    if (inv) fft_dif<true, 31>(lds, f, Batch{nt, 1, len, false}, tid, nth);
-    else fft_dit<false, 31>(lds, f, Batch{nt, 1, len, false}, tid, nth);
-    const T sc = (inv && last) ? T(1) / T(n) : T(1);
-    for (int i = tid; i < nt * len; i += nth) {
-        const int t = i / len, e = i - (i / len) * len;
-        const int dst_e = inv ? f.rev[e] : e;
-        const long long idx = axis == 1 ? (long long)(c0 + t) * W + dst_e : (long long)dst_e * W + c0 + t;
-        cpx<T> v = lds[i] * sc;
-        if (last && o.kind == 3) static_cast<T*>(o.y)[b * n + idx] = v.re;
-        else static_cast<cpx<T>*>(o.y)[b * n + idx] = v;
-    }
+    fftop_pass_body<T>(o, io, lens, lH, lW, axis, blockIdx.x, reinterpret_cast<cpx<T>*>(kym_smem_op2), threadIdx.x,
+                       blockDim.x);
 }
 
 // FFT length tables, built once per (length set, dtype) and kept for the
@@ -246,39 +163,34 @@ static size_t op_lds_limit() {
 }
 
 template <class T>
-static int fft_op(const void* x, void* y, long long batch, int H, int W, int kind, hipStream_t s, long long lds_cap) {
+static int fft_op(const void* x, void* y, void* work, long long batch, int H, int W, int kind, hipStream_t s,
+                  long long lds_cap) {
     OpTables t;
     int rc = op_tables<T>(H, W, &t);
     if (rc != KYM_OK) return rc;
     const FftLen<T>* lens = reinterpret_cast<const FftLen<T>*>((char*)t.dev + t.lens_off);
-    FftOpArgs o = {x, y, batch, H, W, kind, 0, 0, 0};
     size_t lim = op_lds_limit();
     if (lds_cap > 0 && (size_t)lds_cap < lim) lim = (size_t)lds_cap;
-    const size_t need = (size_t)H * W * sizeof(cpx<T>);
-    if (need <= lim) {
-        if (need > 65536)
-            OPS_HIP_TRY(hipFuncSetAttribute((const void*)k_fft2_lds<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
-        hipLaunchKernelGGL(k_fft2_lds<T>, dim3((unsigned)batch), dim3(256), need, s, o, lens, t.lH, t.lW);
-        OPS_HIP_TRY(hipGetLastError());
-        return KYM_OK;
-    }
-    // passes: rows (along W) then columns (along H); the first reads x, the
-    // last scales and writes the final layout, both in place on y
-    int axes[2], npass = 0;
-    if (W > 1) axes[npass++] = 1;
-    if (H > 1) axes[npass++] = 0;
-    for (int q = 0; q < npass; ++q) {
-        const int axis = axes[q];
-        const int len = axis == 1 ? W : H, ntr = axis == 1 ? H : W;
-        if ((size_t)len * sizeof(cpx<T>) > lim) return kym_ops_fail(KYM_ERR_UNSUPPORTED, "FFT length too large for LDS");
-        const int per = (int)std::max<size_t>(1, std::min<size_t>(16, lim / ((size_t)len * sizeof(cpx<T>))));
-        o.nrow = per;
-        o.nblk = (ntr + per - 1) / per;
-        const size_t lds = (size_t)per * len * sizeof(cpx<T>);
-        if (lds > 65536)
-            OPS_HIP_TRY(hipFuncSetAttribute((const void*)k_fft2_pass<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(k_fft2_pass<T>, dim3((unsigned)(batch * o.nblk)), dim3(256), lds, s, o, lens, t.lH, t.lW,
-                           axis, q == 0 ? 1 : 0, q == npass - 1 ? 1 : 0);
+    FftOpStep steps[2];
+    int nsteps = 0;
+    const char* err = nullptr;
+    const int code = fftop_steps(x, y, work, H, W, kind, lim, sizeof(cpx<T>), steps, &nsteps, &err);
+    if (code) return kym_ops_fail(code, err);
+    for (int q = 0; q < nsteps; ++q) {
+        const FftOpStep& st = steps[q];
+        FftOpArgs o = {x, y, batch, H, W, kind, 0, st.nrow, st.nblk};
+        if (st.route == 0) {
+            if (st.lds > 65536)
+                OPS_HIP_TRY(hipFuncSetAttribute((const void*)k_fft2_lds<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                (int)st.lds));
+            hipLaunchKernelGGL(k_fft2_lds<T>, dim3((unsigned)batch), dim3(256), st.lds, s, o, lens, t.lH, t.lW);
+        } else {
+            if (st.lds > 65536)
+                OPS_HIP_TRY(hipFuncSetAttribute((const void*)k_fft2_pass<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                (int)st.lds));
+            hipLaunchKernelGGL(k_fft2_pass<T>, dim3((unsigned)(batch * st.nblk)), dim3(256), st.lds, s, o, st.io, lens,
+                               t.lH, t.lW, st.axis);
+        }
         OPS_HIP_TRY(hipGetLastError());
     }
     return KYM_OK;
@@ -394,12 +306,12 @@ extern "C" int kym_op_unpad2d(const void* x, void* y, long long batch, int H, in
     return bad_dtype();
 }
 
-extern "C" int kym_op_fft(const void* x, void* y, long long batch, int H, int W, int kind, int dtype,
+extern "C" int kym_op_fft(const void* x, void* y, void* work, long long batch, int H, int W, int kind, int dtype,
                           long long lds_limit, void* stream) {
     if (kind < 0 || kind > 3) return kym_ops_fail(KYM_ERR_VALUE, "kind must be 0..3");
     if (batch <= 0) return KYM_OK;
     hipStream_t s = (hipStream_t)stream;
-    if (dtype == KYM_F32) return fft_op<float>(x, y, batch, H, W, kind, s, lds_limit);
-    if (dtype == KYM_F64) return fft_op<double>(x, y, batch, H, W, kind, s, lds_limit);
+    if (dtype == KYM_F32) return fft_op<float>(x, y, work, batch, H, W, kind, s, lds_limit);
+    if (dtype == KYM_F64) return fft_op<double>(x, y, work, batch, H, W, kind, s, lds_limit);
     return bad_dtype();
 }

@@ -1,0 +1,165 @@
+// Bodies of the per-op FFT kernels (ops.hip), written as workgroup
+// functions of (tid, nthreads) like the scattering bodies so the CPU test
+// harness can run them.
+#pragma once
+#include "fft_core.h"
+
+// ------------------------------------------------------------ FFT op
+// kind: 0 c2c forward, 1 c2c inverse (1/n), 2 r2c forward (real input),
+// 3 inverse, real part out (1/n).  A batch of 2D transforms of H x W (W = 1
+// gives 1D along H).  Small sizes: one workgroup per transform in LDS.
+// Large sizes: row pass then column pass through `y` (in place).
+struct FftOpArgs {
+    const void* x;
+    void* y;
+    long long batch;
+    int H, W, kind;
+    int r0;     // row/column block start (pass kernels)
+    int nrow;   // rows or columns per task
+    int nblk;   // tasks per transform (pass kernels)
+};
+
+template <class T, bool INV>
+KYM_HD void load_2d(const FftOpArgs& o, const FftLen<T>& fH, const FftLen<T>& fW, long long b, cpx<T>* lds, int tid,
+                        int nth) {
+    const int H = o.H, W = o.W, n = H * W;
+    for (int p = tid; p < n; p += nth) {
+        const int py = p / W, px = p - (p / W) * W;
+        // forward (DIT) wants digit-reversed storage; inverse (DIF) natural
+        const int y = INV ? py : fH.rev[py];
+        const int x = INV ? px : fW.rev[px];
+        const long long src = b * n + (long long)y * W + x;
+        if (o.kind == 2) lds[p] = {static_cast<const T*>(o.x)[src], T(0)};
+        else lds[p] = static_cast<const cpx<T>*>(o.x)[src];
+    }
+}
+
+// One transform per workgroup, whole in LDS.
+template <class T>
+KYM_HD void fftop_lds_body(const FftOpArgs& o, const FftLen<T>* lens, int lH, int lW, long long b, cpx<T>* lds,
+                           int tid, int nth) {
+    const FftLen<T>& fH = lens[lH];
+    const FftLen<T>& fW = lens[lW];
+    const int H = o.H, W = o.W, n = H * W;
+    const bool inv = o.kind == 1 || o.kind == 3;
+    if (inv) load_2d<T, true>(o, fH, fW, b, lds, tid, nth);
+    else load_2d<T, false>(o, fH, fW, b, lds, tid, nth);
+    wg_barrier();
+    if (inv) {
+        if (W > 1) fft_dif<true, 31>(lds, fW, Batch{H, 1, W, false}, tid, nth);
+        fft_dif<true, 31>(lds, fH, Batch{W, W, 1, true}, tid, nth);
+        const T inv_n = T(1) / T(n);
+        for (int p = tid; p < n; p += nth) {
+            const int py = p / W, px = p - (p / W) * W;
+            const long long dst = b * n + (long long)fH.rev[py] * W + fW.rev[px];
+            if (o.kind == 3) static_cast<T*>(o.y)[dst] = lds[p].re * inv_n;
+            else static_cast<cpx<T>*>(o.y)[dst] = lds[p] * inv_n;
+        }
+    } else {
+        if (W > 1) fft_dit<false, 31>(lds, fW, Batch{H, 1, W, false}, tid, nth);
+        fft_dit<false, 31>(lds, fH, Batch{W, W, 1, true}, tid, nth);
+        for (int p = tid; p < n; p += nth) static_cast<cpx<T>*>(o.y)[b * n + p] = lds[p];
+    }
+}
+
+// Pass kernels for transforms that do not fit: axis 1 = rows (along W),
+// axis 0 = columns (along H).  Each task reads whole lines of `src` into LDS,
+// transforms them and writes them to `dst` (complex unless dst_real; src
+// real only for the first pass of kind 2).  A real output (kind 3) keeps its
+// complex intermediate in a separate work buffer, never in the output.
+struct FftPassIO {
+    const void* src;
+    void* dst;
+    int src_real, dst_real;
+    int scale_last;  // multiply by 1/(H W) on this pass
+};
+
+template <class T>
+KYM_HD void fftop_pass_body(const FftOpArgs& o, const FftPassIO& io, const FftLen<T>* lens, int lH, int lW, int axis,
+                            long long task, cpx<T>* lds, int tid, int nth) {
+    const long long b = task / o.nblk;
+    const int blk = (int)(task - b * o.nblk);
+    if (b >= o.batch) return;
+    const int H = o.H, W = o.W;
+    const long long n = (long long)H * W;
+    const bool inv = o.kind == 1 || o.kind == 3;
+    const FftLen<T>& f = lens[axis == 1 ? lW : lH];
+    const int len = f.n, c0 = blk * o.nrow;
+    const int lines = axis == 1 ? H : W;
+    const int nt = lines - c0 < o.nrow ? lines - c0 : o.nrow;
+    // lds[t * len + e]: line t of this task, element e
+    for (int i = tid; i < nt * len; i += nth) {
+        const int t = i / len, e = i - (i / len) * len;
+        const int src_e = inv ? e : f.rev[e];
+        const long long idx = axis == 1 ? (long long)(c0 + t) * W + src_e : (long long)src_e * W + c0 + t;
+        if (io.src_real) lds[i] = {static_cast<const T*>(io.src)[b * n + idx], T(0)};
+        else lds[i] = static_cast<const cpx<T>*>(io.src)[b * n + idx];
+    }
+    wg_barrier();
+    if (inv) fft_dif<true, 31>(lds, f, Batch{nt, 1, len, false}, tid, nth);
+    else fft_dit<false, 31>(lds, f, Batch{nt, 1, len, false}, tid, nth);
+    const T sc = io.scale_last ? T(1) / T(n) : T(1);
+    for (int i = tid; i < nt * len; i += nth) {
+        const int t = i / len, e = i - (i / len) * len;
+        const int dst_e = inv ? f.rev[e] : e;
+        const long long idx = axis == 1 ? (long long)(c0 + t) * W + dst_e : (long long)dst_e * W + c0 + t;
+        const cpx<T> v = lds[i] * sc;
+        if (io.dst_real) static_cast<T*>(io.dst)[b * n + idx] = v.re;
+        else static_cast<cpx<T>*>(io.dst)[b * n + idx] = v;
+    }
+}
+
+
+// Launch sequence of one per-op FFT (shared by the GPU entry point and the
+// CPU harness).  route 0: whole transform in LDS (grid = batch); route 1: a
+// line pass along `axis` (grid = batch * nblk).
+struct FftOpStep {
+    int route, axis;
+    FftPassIO io;
+    int nrow, nblk;
+    size_t lds;
+};
+
+// Returns 0, or 3 (bad argument) / 5 (unsupported) with *err set.
+inline int fftop_steps(const void* x, void* y, void* work, int H, int W, int kind, size_t lim, size_t cs,
+                       FftOpStep* steps, int* nsteps, const char** err) {
+    *nsteps = 0;
+    const size_t need = (size_t)H * W * cs;
+    if (need <= lim) {
+        FftOpStep st = {};
+        st.route = 0;
+        st.lds = need;
+        steps[(*nsteps)++] = st;
+        return 0;
+    }
+    int axes[2], npass = 0;
+    if (W > 1) axes[npass++] = 1;
+    if (H > 1) axes[npass++] = 0;
+    void* mid = y;
+    if (kind == 3 && npass > 1) {
+        if (!work) { *err = "kind 3 at this size needs a complex work buffer"; return 3; }
+        mid = work;
+    }
+    for (int q = 0; q < npass; ++q) {
+        const int axis = axes[q];
+        const int len = axis == 1 ? W : H, lines = axis == 1 ? H : W;
+        if ((size_t)len * cs > lim) { *err = "FFT length too large for LDS"; return 5; }
+        FftOpStep st = {};
+        st.route = 1;
+        st.axis = axis;
+        size_t per = lim / ((size_t)len * cs);
+        if (per > 16) per = 16;
+        if (per < 1) per = 1;
+        st.nrow = (int)per;
+        st.nblk = (lines + st.nrow - 1) / st.nrow;
+        st.lds = (size_t)st.nrow * len * cs;
+        const bool first = q == 0, last = q == npass - 1;
+        st.io.src = first ? x : mid;
+        st.io.dst = last ? y : mid;
+        st.io.src_real = (first && kind == 2) ? 1 : 0;
+        st.io.dst_real = (last && kind == 3) ? 1 : 0;
+        st.io.scale_last = (last && (kind == 1 || kind == 3)) ? 1 : 0;
+        steps[(*nsteps)++] = st;
+    }
+    return 0;
+}

@@ -12,8 +12,9 @@ def _fft1(x, kind, out_last):
     n = x.shape[-2]
     y = torch.empty(x.shape[:-1] + (out_last,), dtype=x.dtype, device=x.device)
     with torch.cuda.device(x.device):
-        _lib.check(_lib.lib().kym_op_fft(_ptr(x), _ptr(y), x.numel() // (n * x.shape[-1]), n, 1, kind, _dt(x), 0,
-                                          _lib.stream_of(x.device)))
+        # 1D transforms run in one pass (W = 1): no scratch needed
+        _lib.check(_lib.lib().kym_op_fft(_ptr(x), _ptr(y), None, x.numel() // (n * x.shape[-1]), n, 1, kind, _dt(x),
+                                          0, _lib.stream_of(x.device)))
     return y
 
 

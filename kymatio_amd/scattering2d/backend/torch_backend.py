@@ -33,9 +33,11 @@ def _fft(x, kind, out_last):
     H, W = x.shape[-3], x.shape[-2]
     batch = x.numel() // (H * W * x.shape[-1])
     y = torch.empty(x.shape[:-1] + (out_last,), dtype=x.dtype, device=x.device)
+    # complex scratch for a real output computed in two passes
+    work = torch.empty(x.shape[:-1] + (2,), dtype=x.dtype, device=x.device) if kind == 3 else None
     with torch.cuda.device(x.device):
-        _lib.check(_lib.lib().kym_op_fft(_ptr(x), _ptr(y), batch, H, W, kind, _dt(x), 0,
-                                          _lib.stream_of(x.device)))
+        _lib.check(_lib.lib().kym_op_fft(_ptr(x), _ptr(y), _ptr(work) if work is not None else None, batch, H, W,
+                                          kind, _dt(x), 0, _lib.stream_of(x.device)))
     return y
 
 

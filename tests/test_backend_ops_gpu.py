@@ -113,8 +113,8 @@ def test_fft_2d_multipass():
     for cap in (0, 4096):
         for kind in (0, 1):
             y = torch.empty_like(x)
-            _lib.check(_lib.lib().kym_op_fft(ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr()), 2, 68,
-                                              72, kind, 1, cap, _lib.stream_of(x.device)))
+            _lib.check(_lib.lib().kym_op_fft(ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr()), None, 2,
+                                              68, 72, kind, 1, cap, _lib.stream_of(x.device)))
             outs.append(y)
     assert torch.allclose(outs[0], outs[2], rtol=1e-12, atol=1e-12)
     assert torch.allclose(outs[1], outs[3], rtol=1e-12, atol=1e-14)

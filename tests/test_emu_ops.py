@@ -1,0 +1,73 @@
+"""Per-op FFT bodies (kym_op_fft) run on the CPU harness, for every kind on
+the in-LDS and the two-pass routes, against numpy FP64, with guard regions
+after each buffer to catch out-of-bounds writes."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from tests.helpers import Emu
+
+GUARD = 4096
+
+
+@pytest.fixture(scope='module')
+def emu():
+    e = Emu()
+    f = e.lib.kym_emu_op_fft
+    f.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_longlong] + [ctypes.c_int] * 4 + [ctypes.c_longlong]
+    f.restype = ctypes.c_int
+    return e
+
+
+def _guarded(n, dtype):
+    buf = np.full(n + GUARD, 12345.0, dtype)
+    return buf
+
+
+def _run(emu, x_c, H, W, kind, cap, with_work=True):
+    B = x_c.shape[0]
+    n = B * H * W
+    if kind == 2:
+        xin = _guarded(n, np.float64)
+        xin[:n] = x_c.real.reshape(-1)
+    else:
+        xin = _guarded(2 * n, np.float64)
+        xin[:2 * n] = np.stack([x_c.real, x_c.imag], -1).reshape(-1)
+    out_len = n if kind == 3 else 2 * n
+    y = _guarded(out_len, np.float64)
+    work = _guarded(2 * n, np.float64) if with_work else None
+    rc = emu.lib.kym_emu_op_fft(xin.ctypes.data, y.ctypes.data, work.ctypes.data if work is not None else None, B,
+                                H, W, kind, 1, cap)
+    if rc:
+        raise RuntimeError(emu.lib.kym_emu_last_error().decode())
+    assert np.all(y[out_len:] == 12345.0), 'write past the output'
+    assert np.all(xin[len(xin) - GUARD:] == 12345.0)
+    if work is not None:
+        assert np.all(work[2 * n:] == 12345.0), 'write past the work buffer'
+    if kind == 3:
+        return y[:n].reshape(B, H, W)
+    v = y[:out_len].reshape(B, H, W, 2)
+    return v[..., 0] + 1j * v[..., 1]
+
+
+@pytest.mark.parametrize('H,W', [(4, 4), (34, 34), (40, 24), (68, 72), (17, 1), (272, 16)])
+@pytest.mark.parametrize('cap', [160 * 1024, 8192], ids=['lds', 'twopass'])
+@pytest.mark.parametrize('kind', [0, 1, 2, 3])
+def test_op_fft(emu, H, W, cap, kind):
+    rng = np.random.default_rng(H * 1000 + W)
+    x = rng.standard_normal((2, H, W)) + (0 if kind == 2 else 1j) * rng.standard_normal((2, H, W))
+    got = _run(emu, x, H, W, kind, cap)
+    if kind in (0, 2):
+        ref = np.fft.fft2(x)
+    elif kind == 1:
+        ref = np.fft.ifft2(x)
+    else:
+        ref = np.fft.ifft2(x).real
+    assert np.abs(got - ref).max() <= 1e-12 * max(np.abs(ref).max(), 1)
+
+
+def test_real_inverse_two_pass_needs_work(emu):
+    x = np.ones((1, 68, 72)) + 0j
+    with pytest.raises(RuntimeError, match='work buffer'):
+        _run(emu, x, 68, 72, 3, 8192, with_work=False)
