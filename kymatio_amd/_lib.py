@@ -48,8 +48,8 @@ _SIGNATURES = {
     'kym_op_subsample': [_c_vp, _c_vp, _c_ll, _c_i, _c_i, _c_i, _c_i, _c_i, _c_vp],
     'kym_op_unpad2d': [_c_vp, _c_vp, _c_ll, _c_i, _c_i, _c_i, _c_vp],
     'kym_op_fft': [_c_vp, _c_vp, _c_vp, _c_ll, _c_i, _c_i, _c_i, _c_i, _c_ll, _c_vp],
-    'kym_dec_conv_fwd': [_c_vp] * 5 + [_c_i] * 9 + [_c_vp],
-    'kym_dec_conv_bwd_weight': [_c_vp] * 4 + [_c_i] * 9 + [_c_vp],
+    'kym_dec_conv_fwd': [_c_vp] * 5 + [_c_i] * 8 + [_c_vp],
+    'kym_dec_conv_bwd_weight': [_c_vp] * 4 + [_c_i] * 8 + [_c_vp],
     'kym_dec_up_relu_adjoint': [_c_vp] * 3 + [_c_i] * 5 + [_c_vp],
     'kym_dec_bias_grad': [_c_vp, _c_vp, _c_i, _c_i, _c_i, _c_vp],
     'kym_dec_mse': [_c_vp] * 4 + [_c_ll, _c_vp],
@@ -58,6 +58,11 @@ _SIGNATURES = {
     'kym_chirps': [_c_vp, _c_i, _c_i, ctypes.c_ulonglong, _c_ll, ctypes.c_float, _c_vp],
     'kym_mfma_layout_check': [_c_vp, _c_vp, _c_vp],
 }
+
+
+def signatures():
+    """name -> ctypes argtypes of every bound entry point."""
+    return dict(_SIGNATURES)
 
 
 def exported_symbols():

@@ -43,3 +43,36 @@ def test_version_and_error_mapping():
     assert _lib.STATUS_EXC[1] is TypeError
     assert _lib.STATUS_EXC[2] is RuntimeError
     assert _lib.STATUS_EXC[3] is ValueError
+
+
+def _prototypes():
+    protos = {}
+    for h in glob.glob(os.path.join(ROOT, 'include', '*.h')):
+        txt = re.sub(r'/\*.*?\*/', '', open(h).read(), flags=re.S)
+        for m in re.finditer(r'(kym_\w+)\s*\(([^)]*)\)\s*;', txt):
+            params = [p.strip() for p in m.group(2).split(',') if p.strip() and p.strip() != 'void']
+            protos[m.group(1)] = params
+    return protos
+
+
+def test_binding_arity_matches_header():
+    """Every ctypes binding has exactly as many arguments as the C prototype."""
+    protos = _prototypes()
+    for name, argtypes in _lib.signatures().items():
+        assert name in protos, name
+        assert len(argtypes) == len(protos[name]), (name, len(argtypes), protos[name])
+
+
+def test_binding_kinds_match_header():
+    """Pointer parameters bind as pointers, 64-bit integers as c_longlong."""
+    protos = _prototypes()
+    for name, argtypes in _lib.signatures().items():
+        for decl, at in zip(protos[name], argtypes):
+            if '*' in decl:
+                assert at is ctypes.c_void_p or issubclass(at, ctypes._Pointer), (name, decl, at)
+            elif 'long long' in decl:
+                assert at in (ctypes.c_longlong, ctypes.c_ulonglong), (name, decl, at)
+            elif decl.startswith('float'):
+                assert at is ctypes.c_float, (name, decl, at)
+            elif decl.startswith('int'):
+                assert at is ctypes.c_int, (name, decl, at)
