@@ -38,16 +38,37 @@ def _small_decoder():
     return dec
 
 
-def test_forward_matches_oracle():
+def _cpu_layers(dec):
+    return [{k: (v.cpu() if torch.is_tensor(v) else v) for k, v in L.items()} for L in dec.layers]
+
+
+def test_forward_layers_match_oracle():
+    """Each layer from the same FP32 input: both sides round the same values
+    to bf16 (round to nearest even), so only the FP32 accumulation order
+    separates them."""
+    dec = _small_decoder()
+    S = torch.rand(3, 20, 64, device=DEV)
+    dec.fit_scale(S)
+    _, acts = dec.forward(S, keep=True)
+    for i, L in enumerate(_cpu_layers(dec)):
+        ref = odec.layer(acts[i].cpu(), L, dec.scale.cpu() if i == 0 else None)
+        rel = (acts[i + 1].cpu().double() - ref).abs().max() / ref.abs().max()
+        print('layer %d max rel err %.3e' % (i, rel))
+        assert rel < 2e-5
+
+
+def test_forward_end_to_end_matches_oracle():
+    """Whole network: an FP32 sum that lands a hair either side of a bf16
+    rounding boundary moves that activation by one bf16 step (2^-8
+    relative), so the end-to-end bar is at bf16 resolution."""
     dec = _small_decoder()
     S = torch.rand(3, 20, 64, device=DEV)
     dec.fit_scale(S)
     y = dec.forward(S)
-    ref, _ = odec.forward(S.cpu(), [{k: (v.cpu() if torch.is_tensor(v) else v) for k, v in L.items()}
-                                    for L in dec.layers], dec.scale.cpu())
+    ref, _ = odec.forward(S.cpu(), _cpu_layers(dec), dec.scale.cpu())
     rel = (y.cpu().double() - ref).abs().max() / ref.abs().max()
     print('forward max rel err %.3e' % rel)
-    assert rel < 2e-3
+    assert rel < 2e-2
 
 
 def test_backward_matches_oracle():
