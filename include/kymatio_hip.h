@@ -58,8 +58,22 @@ int kym_scat2d_info(void* plan, long long* info);
 /* Packed filter layout: filter i (the reference's buffer tensor{i}: phi
  * levels first, then each psi's levels in (j, theta) order,
  * kymatio/scattering2d/frontend/torch_frontend.py:28-43) occupies elements
- * [offs[i], offs[i+1]) of one contiguous real array of the plan's dtype. */
+ * [offs[i], offs[i+1]) of one contiguous real array of the plan's dtype,
+ * of filter_elems elements in all (info[6]); the low-pass taps follow. */
 int kym_scat2d_filter_offsets(void* plan, long long* offs);
+
+/* Low-pass taps (FP32 plans).  The leaf of resolution r,
+ * unpad(Re(ifft2(P_kl(U * phi.levels[r])))) of the reference
+ * (kymatio/scattering2d/core/scattering2d.py:19-23, 42-46, 69-73), equals a
+ * separable real filter-and-subsample of the field behind U, because
+ * phi.levels[r] is separable: Re(ifft2(phi.levels[r]))[a][b] = g_r[a] h_r[b].
+ * desc (4 + 6 J entries, n its capacity) = {taps_off, taps_len, M_padded,
+ * N_padded}, then per r < J: {off_y, lo_y, cnt_y, off_x, lo_x, cnt_x}.  The
+ * caller fills packed[off_y + i] = g_r[(lo_y + i) mod H_r] for i < cnt_y and
+ * packed[off_x + i] = h_r[(lo_x + i) mod W_r] for i < cnt_x, with
+ * H_r = M_padded / 2^r, W_r = N_padded / 2^r; taps_len = 0 (FP64 plans) means
+ * no taps.  No counterpart in the reference (derived filter data). */
+int kym_scat2d_leaf_taps(void* plan, long long* desc, int n);
 
 /* x: (batch, M, N) real (or padded size with pre_pad); filt: packed filters;
  * out: (batch, C, out_h, out_w); ws: workspace of ws_bytes (any multiple of

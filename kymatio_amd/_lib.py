@@ -27,6 +27,7 @@ _SIGNATURES = {
     'kym_scat2d_destroy': [_c_vp],
     'kym_scat2d_info': [_c_vp, ctypes.POINTER(_c_ll)],
     'kym_scat2d_filter_offsets': [_c_vp, ctypes.POINTER(_c_ll)],
+    'kym_scat2d_leaf_taps': [_c_vp, ctypes.POINTER(_c_ll), _c_i],
     'kym_scat2d_forward': [_c_vp, _c_vp, _c_vp, _c_vp, _c_ll, _c_vp, _c_ll, _c_vp],
     'kym_scat2d_forward_train': [_c_vp, _c_vp, _c_vp, _c_vp, _c_ll, _c_vp, _c_ll, _c_vp],
     'kym_scat2d_backward': [_c_vp, _c_vp, _c_vp, _c_vp, _c_ll, _c_vp, _c_ll, _c_vp],

@@ -139,7 +139,7 @@ static void fill_info(const Scat2DPlan<T>& p, long long* info) {
     info[3] = p.Mp;
     info[4] = p.Np;
     info[5] = (long long)p.filt_off.size() - 1;
-    info[6] = p.filt_off.back();
+    info[6] = p.taps_off + p.taps_len;
     info[7] = p.ws_img * (long long)sizeof(cpx<T>);
     info[8] = (long long)p.launches.size();
     info[9] = p.top;
@@ -161,6 +161,27 @@ extern "C" int kym_scat2d_filter_offsets(void* plan, long long* offs) {
     const std::vector<long long>& v = h->pf ? h->pf->filt_off : h->pd->filt_off;
     for (size_t i = 0; i < v.size(); ++i) offs[i] = v[i];
     return KYM_OK;
+}
+
+template <class T>
+static int leaf_taps(const Scat2DPlan<T>& p, long long* desc, int n) {
+    std::vector<long long> d = {p.taps_off, p.taps_len, p.Mp, p.Np};
+    for (int r = 0; r < p.J; ++r)
+        for (int ax = 0; ax < 2; ++ax) {
+            const FirAxis& f = p.fir_axes[2 * r + ax];
+            d.push_back(f.off);
+            d.push_back(f.lo);
+            d.push_back(f.cnt);
+        }
+    if (n < (int)d.size()) return fail(KYM_ERR_VALUE, "leaf taps descriptor needs 4 + 6 J entries");
+    for (size_t i = 0; i < d.size(); ++i) desc[i] = d[i];
+    return KYM_OK;
+}
+
+extern "C" int kym_scat2d_leaf_taps(void* plan, long long* desc, int n) {
+    if (!plan || !desc) return fail(KYM_ERR_VALUE, "null argument");
+    KymScat2D* h = static_cast<KymScat2D*>(plan);
+    return h->pf ? leaf_taps(*h->pf, desc, n) : leaf_taps(*h->pd, desc, n);
 }
 
 // mode 0: inference forward (batch in chunks); 1: training forward (saves

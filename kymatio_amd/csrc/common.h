@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #define KYM_HD __host__ __device__
 #define KYM_INL __forceinline__
 

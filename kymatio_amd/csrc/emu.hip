@@ -52,7 +52,6 @@ static int emu_forward2d(int M, int N, int J, int L, int max_order, int pre_pad,
             a.ntask = nimg * l.ngroup * l.nblk;
             for (int t = 0; t < a.ntask; ++t) {
                 if (l.spec >= 0 && l.kind == K2_LDS_NODE) dispatch_node_spec<T, true>(l.spec, a, t, lds.data(), 0);
-                else if (l.spec >= 0 && l.kind == K2_LEAF) dispatch_leaf_spec<T, true>(l.spec, a, t, lds.data(), 0);
                 else dispatch_body2d<T, 31>(l.kind, a, t, lds.data(), 0, 1);
             }
         }
@@ -94,7 +93,6 @@ static int emu_train2d(int M, int N, int J, int L, int max_order, int pre_pad, l
             for (int t = 0; t < a.ntask; ++t) {
                 if (l.kind >= K2B_NODE) dispatch_bwd2d<T, 31>(l.kind, a, t, lds.data(), 0, 1);
                 else if (l.spec >= 0 && l.kind == K2_LDS_NODE) dispatch_node_spec<T, true>(l.spec, a, t, lds.data(), 0);
-                else if (l.spec >= 0 && l.kind == K2_LEAF) dispatch_leaf_spec<T, true>(l.spec, a, t, lds.data(), 0);
                 else dispatch_body2d<T, 31>(l.kind, a, t, lds.data(), 0, 1);
             }
         }
@@ -109,6 +107,37 @@ extern "C" int kym_emu_scat2d_train(int M, int N, int J, int L, int max_order, i
                                     void* gx, long long batch, int nbwd) {
     if (dtype == 0) return emu_train2d<float>(M, N, J, L, max_order, pre_pad, lds_limit, x, filt, out, dS, gx, batch, nbwd);
     return emu_train2d<double>(M, N, J, L, max_order, pre_pad, lds_limit, x, filt, out, dS, gx, batch, nbwd);
+}
+
+// Leaf-tap descriptor of the plan (kym_scat2d_leaf_taps).
+template <class T>
+static int emu_taps2d(int M, int N, int J, int L, int max_order, int pre_pad, long long lds_limit, long long* desc,
+                      int n) {
+    Scat2DPlan<T> p;
+    KymError e = {0, ""};
+    if (!p.build(M, N, J, L, max_order, pre_pad, (size_t)lds_limit, &e)) {
+        g_emu_error = e.msg;
+        return e.code;
+    }
+    std::vector<long long> d = {p.taps_off, p.taps_len, p.Mp, p.Np};
+    for (int r = 0; r < J; ++r)
+        for (int ax = 0; ax < 2; ++ax) {
+            d.push_back(p.fir_axes[2 * r + ax].off);
+            d.push_back(p.fir_axes[2 * r + ax].lo);
+            d.push_back(p.fir_axes[2 * r + ax].cnt);
+        }
+    if (n < (int)d.size()) {
+        g_emu_error = "descriptor too small";
+        return KYM_ERR_VALUE;
+    }
+    for (size_t i = 0; i < d.size(); ++i) desc[i] = d[i];
+    return 0;
+}
+
+extern "C" int kym_emu_scat2d_taps(int M, int N, int J, int L, int max_order, int pre_pad, int dtype,
+                                   long long lds_limit, long long* desc, int n) {
+    if (dtype == 0) return emu_taps2d<float>(M, N, J, L, max_order, pre_pad, lds_limit, desc, n);
+    return emu_taps2d<double>(M, N, J, L, max_order, pre_pad, lds_limit, desc, n);
 }
 
 extern "C" int kym_emu_scat2d_forward(int M, int N, int J, int L, int max_order, int pre_pad, int dtype,

@@ -29,14 +29,6 @@ __global__ void __launch_bounds__(NT) k_node2d_spec(Scat2DArgs<T> a) {
     dispatch_node_spec<T, false>(SPEC, a, task, reinterpret_cast<cpx<T>*>(kym_smem2d_ns), threadIdx.x);
 }
 
-template <class T, int SPEC, int NT>
-__global__ void __launch_bounds__(NT) k_leaf2d_spec(Scat2DArgs<T> a) {
-    extern __shared__ __attribute__((aligned(16))) char kym_smem2d_ls[];
-    const int task = xcd_task(blockIdx.x, gridDim.x);
-    if (task >= a.ntask) return;
-    dispatch_leaf_spec<T, false>(SPEC, a, task, reinterpret_cast<cpx<T>*>(kym_smem2d_ls), threadIdx.x);
-}
-
 template <class T>
 using Scat2DKernel = void (*)(Scat2DArgs<T>);
 
@@ -48,18 +40,6 @@ Scat2DKernel<T> pick_node_spec(int id) {
     case sid: return k_node2d_spec<T, sid, nt>;
         KYM_NODE_SPECS(KYM_NS_K)
 #undef KYM_NS_K
-        default: return nullptr;
-    }
-}
-
-template <class T>
-Scat2DKernel<T> pick_leaf_spec(int id) {
-    if constexpr (sizeof(T) != 4) return nullptr;
-    else switch (id) {
-#define KYM_LS_K(sid, hj, wj, nt) \
-    case sid: return k_leaf2d_spec<T, sid, nt>;
-        KYM_LEAF_SPECS(KYM_LS_K)
-#undef KYM_LS_K
         default: return nullptr;
     }
 }
@@ -82,6 +62,7 @@ Scat2DKernel<T> pick_scat2d_p(int kind, int threads) {
         case K2_FWD_COLS: return k_scat2d<T, K2_FWD_COLS, 256, MAXP>;
         case K2_LEAF_ICOL: return k_scat2d<T, K2_LEAF_ICOL, 256, MAXP>;
         case K2_LEAF_ROWS: return k_scat2d<T, K2_LEAF_ROWS, 256, MAXP>;
+        case K2_FIR_COLS: return k_scat2d<T, K2_FIR_COLS, 256, MAXP>;
         case K2B_NODE: return pick_scat2d_nt<T, K2B_NODE, MAXP>(threads);
         case K2B_LEAFG: return pick_scat2d_nt<T, K2B_LEAFG, MAXP>(threads);
         case K2B_GCOLS: return k_scat2d<T, K2B_GCOLS, 256, MAXP>;

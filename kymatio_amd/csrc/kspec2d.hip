@@ -1,5 +1,4 @@
-// Compile-time specialised 2D node / leaf kernels (FP32).
+// Compile-time specialised 2D node kernels (FP32).
 #include "kernels2d.h"
 
 Scat2DKernel<float> kym_pick_node_spec_f32(int id) { return pick_node_spec<float>(id); }
-Scat2DKernel<float> kym_pick_leaf_spec_f32(int id) { return pick_leaf_spec<float>(id); }

@@ -15,8 +15,22 @@
 // spectrum (kept in a workspace slot, usually resident in L2/MALL) while
 // loading, so the filter multiply and the periodisation are fused into the
 // load of the next transform.  Nodes that fit in LDS run as one workgroup
-// (gather, inverse FFT, modulus, forward FFT, leaf); bigger ones run as
-// column / row / column passes through their workspace slot.
+// (gather, inverse FFT, modulus, leaf, forward FFT when the node has
+// children); bigger ones run as column / row / column passes through their
+// workspace slot.
+//
+// Leaves in space (FP32 plans).  With m = |u| the real field whose spectrum
+// is U (m = pad(x) at the root), the periodisation identity
+//   ifft2_{HJ,WJ}(P_kl(fft2(m) * phi_hat))[y, x] = (m (*) phi)[kl y, kl x],
+//   phi = Re(ifft2(phi_hat)) (circular convolution on H_r x W_r),
+// turns the leaf into a filter-and-subsample of m.  phi is a periodised
+// Gaussian (kymatio/scattering2d/filter_bank.py: gabor_2d with xi = 0 and an
+// isotropic envelope, then a separable crop), so phi = g (x) h and
+//   S[y][x] = sum_a g[a] sum_b h[b] m[kl (y+1) + a][kl (x+1) + b]   (indices mod H_r, W_r)
+// a row pass and a column pass of short real FIRs ("taps", filled by the
+// frontend from the same filter bank).  The node with no children then needs
+// no forward FFT at all.  FP64 plans keep the spectral leaf, so they follow
+// the reference's arithmetic to FP64 rounding.
 #pragma once
 #include "fft_static.h"
 
@@ -41,6 +55,22 @@ struct Node2D {
     long long gsave;             // slot of dL/dX written by the backward, -1 none
     long long gslot;             // leaf dL/dY (natural, scaled) when not formed in the node kernel
     long long gpad;              // root: dL/d(padded input), real
+    // leaf in space (FP32 plans)
+    int fir;         // 0 spectral leaf; 1 FIR leaf in the LDS node kernel; 2 FIR row pass + K2_FIR_COLS (big nodes)
+    int fwd;         // 1 = forward FFT of the modulus is formed (children or spectral leaf need it)
+    long long tslot; // fir == 2: row-filtered field (H_r x ow reals), -1 none
+};
+
+// FP32 plans form every leaf in space, FP64 plans spectrally (compile-time,
+// so each kernel carries one of the two code paths).
+template <class T>
+inline constexpr bool kym_fir = sizeof(T) == 4;
+
+// Taps of one axis of a leaf filter: coefficient c[i] (at filt + off) for the
+// offset lo + i from the output's centre sample.
+struct FirAxis {
+    long long off;
+    int lo, cnt;
 };
 
 template <class T>
@@ -49,6 +79,7 @@ struct Scat2DArgs {
     const FftLen<T>* lens;
     const long long* filt_off;  // element offsets into `filt`
     const int* group;           // node ids of this launch
+    const FirAxis* fir;         // [2 r]: along y (H_r), [2 r + 1]: along x (W_r), r < J
     int ngroup, nblk, blk;      // nodes per image, tasks per node, rows/cols per task
     int lenH[KYM_MAX_RES], lenW[KYM_MAX_RES], H[KYM_MAX_RES], W[KYM_MAX_RES];
     int J;
@@ -94,12 +125,32 @@ KYM_HD KYM_INL T input_pixel(const Scat2DArgs<T>& a, const T* xin, int y, int x)
     return xin[(long long)reflect_index(y - a.top, a.M) * a.N + reflect_index(x - a.left, a.N)];
 }
 
+// ------------------------------------------------------------------ leaf FIR
+// sum_i c[i] src[((base + lo + i) mod n) * stride], with base + lo in (-n, n).
+// c is uniform across the workgroup (scalar loads).
+template <class T>
+KYM_HD KYM_INL T fir_sum(const T* src, int stride, int n, int base, const T* c, int lo, int cnt) {
+    int q = base + lo;
+    if (q < 0) q += n;
+    T acc = T(0);
+    for (int i = 0; i < cnt; ++i) {
+        acc += c[i] * src[q * stride];
+        if (++q == n) q = 0;
+    }
+    return acc;
+}
+
 // ------------------------------------------------------------------ geometry
 // A node of H x W with its leaf of HJ x WJ.  DynGeom reads the sizes and
 // FFT plans at run time; StatGeom fixes them at compile time (constant
 // divisors, unrolled passes).  Both use the same twiddle/rev tables.
+// Registers per thread when a body stages the whole H x W field through
+// registers (see stage_modulus): the plan keeps H W <= KYM_STAGE_MAX * threads.
+#define KYM_STAGE_MAX 20
+
 template <class T, int MAXP>
 struct DynGeom {
+    static constexpr int STAGE = KYM_STAGE_MAX;
     int H, W, HJ, WJ;
     const FftLen<T>* fH;
     const FftLen<T>* fW;
@@ -130,6 +181,7 @@ struct DynGeom {
 template <class T, int H_, int W_, int HJ_, int WJ_, int NT>
 struct StatGeom {
     static constexpr int H = H_, W = W_, HJ = HJ_, WJ = WJ_;
+    static constexpr int STAGE = (H_ * W_ + NT - 1) / NT;
     const FftLen<T>* fH;
     const FftLen<T>* fW;
     const FftLen<T>* fHJ;
@@ -180,6 +232,114 @@ KYM_HD void leaf_body(const Scat2DArgs<T>& a, const G& g, const cpx<T>* U, const
     wg_barrier();
 }
 
+// ------------------------------------------------------------------ leaf in space
+// The real field of a node lives in LDS as R[y * (W + 1) + x] (natural order,
+// odd row stride), overlaying the node's complex buffer; the leaf's row pass
+// writes Tb[y * (ow + 1) + x] right after R.  R + Tb take
+// H (W + 1) + H (ow + 1) <= 2 H W reals, i.e. fit in the complex buffer.
+//
+// Moving between the digit-reversed complex layout and R permutes the whole
+// buffer in place, so every thread first reads its elements into registers,
+// then all write after a barrier.  The CPU harness runs one thread, which
+// stages through a host array instead.
+template <class T, class G>
+KYM_HD void stage_modulus(const G& g, cpx<T>* lds, cpx<T>* sv, T inv_n, int tid, int nth) {
+    const int n = g.H * g.W, rs = g.W + 1;
+    const int* rH = g.revH();
+    const int* rW = g.revW();
+    T* R = reinterpret_cast<T*>(lds);
+#if !defined(__HIP_DEVICE_COMPILE__)
+    if (nth == 1) {
+        std::vector<T> m(n);
+        for (int s = 0; s < n; ++s) {
+            const cpx<T> v = lds[s];
+            if (sv) sv[s] = v;
+            m[s] = kym_sqrt(v.re * v.re + v.im * v.im) * inv_n;
+        }
+        for (int s = 0; s < n; ++s) R[rH[s / g.W] * rs + rW[s % g.W]] = m[s];
+        return;
+    }
+#endif
+    constexpr int S = G::STAGE;
+    T m[S];
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+        const int s = tid + k * nth;
+        if (s < n) {
+            const cpx<T> v = lds[s];
+            if (sv) sv[s] = v;
+            m[k] = kym_sqrt(v.re * v.re + v.im * v.im) * inv_n;
+        }
+    }
+    wg_barrier();
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+        const int s = tid + k * nth;
+        if (s < n) {
+            const int py = s / g.W, px = s - py * g.W;
+            R[rH[py] * rs + rW[px]] = m[k];
+        }
+    }
+    wg_barrier();
+}
+
+// R (natural reals) -> digit-reversed complex buffer, ready for a forward DIT.
+template <class T, class G>
+KYM_HD void restage_complex(const G& g, cpx<T>* lds, int tid, int nth) {
+    const int n = g.H * g.W, rs = g.W + 1;
+    const int* rH = g.revH();
+    const int* rW = g.revW();
+    const T* R = reinterpret_cast<const T*>(lds);
+#if !defined(__HIP_DEVICE_COMPILE__)
+    if (nth == 1) {
+        std::vector<T> m(n);
+        for (int s = 0; s < n; ++s) m[s] = R[rH[s / g.W] * rs + rW[s % g.W]];
+        for (int s = 0; s < n; ++s) lds[s] = {m[s], T(0)};
+        return;
+    }
+#endif
+    constexpr int S = G::STAGE;
+    T m[S];
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+        const int s = tid + k * nth;
+        if (s < n) {
+            const int py = s / g.W, px = s - py * g.W;
+            m[k] = R[rH[py] * rs + rW[px]];
+        }
+    }
+    wg_barrier();
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+        const int s = tid + k * nth;
+        if (s < n) lds[s] = {m[k], T(0)};
+    }
+    wg_barrier();
+}
+
+// Leaf of the node from R: row pass (rows fastest across threads: the odd
+// row stride keeps the LDS reads conflict-free) into Tb, column pass into
+// the output channel.  Callers barrier before reusing Tb's memory.
+template <class T, class G>
+KYM_HD void fir_leaf_lds(const Scat2DArgs<T>& a, const G& g, const Node2D& nd, T* R, int img, int tid, int nth) {
+    const int r = nd.res, ow = a.ow, oh = a.oh, kl = nd.leaf_k;
+    const int rs = g.W + 1, ts = ow + 1;
+    T* Tb = R + g.H * rs;
+    const FirAxis fy = a.fir[2 * r], fx = a.fir[2 * r + 1];
+    const T* cy = a.filt + fy.off;
+    const T* cx = a.filt + fx.off;
+    for (int e = tid; e < g.H * ow; e += nth) {
+        const int x = e / g.H, p = e - x * g.H;
+        Tb[p * ts + x] = fir_sum(R + p * rs, 1, g.W, kl * (x + 1), cx, fx.lo, fx.cnt);
+    }
+    wg_barrier();
+    T* o = a.out + ((long long)img * a.C + nd.leaf_ch) * oh * ow;
+    for (int e = tid; e < oh * ow; e += nth) {
+        const int y = e / ow, x = e - y * ow;
+        o[e] = fir_sum(Tb + x, ts, g.H, kl * (y + 1), cy, fy.lo, fy.cnt);
+    }
+}
+
 // ------------------------------------------------------------------ bodies
 // A whole node in LDS: root (pad + forward FFT) or order 1/2 (gather +
 // periodise, inverse FFT, modulus, forward FFT); then the optional store of
@@ -190,13 +350,25 @@ KYM_HD void node_body(const Scat2DArgs<T>& a, const G& g, const Node2D& nd, int 
     cpx<T>* wsi = a.ws + il * a.ws_img;
     if (nd.order == 0) {
         const T* xin = a.x + (long long)img * a.M * a.N;
-        const int* rH = g.revH();
-        const int* rW = g.revW();
-        for (int i = tid; i < g.H * g.W; i += nth) {
-            const int py = i / g.W, px = i - py * g.W;
-            lds[i] = {input_pixel(a, xin, rH[py], rW[px]), T(0)};
+        if constexpr (kym_fir<T>) {
+            T* R = reinterpret_cast<T*>(lds);
+            const int rs = g.W + 1;
+            for (int i = tid; i < g.H * g.W; i += nth) {
+                const int y = i / g.W, x = i - y * g.W;
+                R[y * rs + x] = input_pixel(a, xin, y, x);
+            }
+            wg_barrier();
+            fir_leaf_lds(a, g, nd, R, img, tid, nth);
+            restage_complex<T>(g, lds, tid, nth);
+        } else {
+            const int* rH = g.revH();
+            const int* rW = g.revW();
+            for (int i = tid; i < g.H * g.W; i += nth) {
+                const int py = i / g.W, px = i - py * g.W;
+                lds[i] = {input_pixel(a, xin, rH[py], rW[px]), T(0)};
+            }
+            wg_barrier();
         }
-        wg_barrier();
         g.fwd2d(lds, tid, nth);
     } else {
         const Node2D& pn = a.nodes[nd.parent];
@@ -212,19 +384,28 @@ KYM_HD void node_body(const Scat2DArgs<T>& a, const G& g, const Node2D& nd, int 
         g.inv2d(lds, tid, nth);
         const T inv_n = T(1) / T(g.H * g.W);
         cpx<T>* sv = (a.save && nd.save >= 0) ? wsi + nd.save : nullptr;
-        for (int i = tid; i < g.H * g.W; i += nth) {
-            const cpx<T> v = lds[i];
-            if (sv) sv[i] = v;
-            lds[i] = {kym_sqrt(v.re * v.re + v.im * v.im) * inv_n, T(0)};
+        if constexpr (kym_fir<T>) {
+            stage_modulus<T>(g, lds, sv, inv_n, tid, nth);
+            fir_leaf_lds(a, g, nd, reinterpret_cast<T*>(lds), img, tid, nth);
+            if (nd.fwd) {
+                restage_complex<T>(g, lds, tid, nth);
+                g.fwd2d(lds, tid, nth);
+            }
+        } else {
+            for (int i = tid; i < g.H * g.W; i += nth) {
+                const cpx<T> v = lds[i];
+                if (sv) sv[i] = v;
+                lds[i] = {kym_sqrt(v.re * v.re + v.im * v.im) * inv_n, T(0)};
+            }
+            wg_barrier();
+            g.fwd2d(lds, tid, nth);
         }
-        wg_barrier();
-        g.fwd2d(lds, tid, nth);
     }
-    if (nd.slot >= 0) {
+    if (nd.slot >= 0 && nd.fwd) {
         cpx<T>* dst = wsi + nd.slot;
         for (int i = tid; i < g.H * g.W; i += nth) dst[i] = lds[i];
     }
-    if (nd.leaf_mode == 0)
+    if (!kym_fir<T> && nd.leaf_mode == 0)
         leaf_body(a, g, lds, a.filt + a.filt_off[nd.leaf_filt], nd.leaf_k, nd.leaf_ch, img, lds + g.H * g.W, tid,
                   nth);
 }
@@ -267,26 +448,25 @@ KYM_HD void body_leaf(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, in
     leaf_body(a, dyn_geom<T, MAXP>(a, r), U, phi, nd.leaf_k, nd.leaf_ch, a.img0 + il, lds, tid, nth);
 }
 
-template <class T, int HJ, int WJ, int NT>
-KYM_HD void body_leaf_s(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid) {
-    const int il = task / a.ngroup, gi = task - il * a.ngroup;
-    const Node2D& nd = a.nodes[a.group[gi]];
-    const int r = nd.res, J = a.J;
-    // node sizes only enter through g.W (row stride of U): keep them dynamic
-    struct G : StatGeom<T, 1, 1, HJ, WJ, NT> {
-        int W;
-    } g;
-    g.fH = &a.lens[a.lenH[r]];
-    g.fW = &a.lens[a.lenW[r]];
-    g.fHJ = &a.lens[a.lenH[J]];
-    g.fWJ = &a.lens[a.lenW[J]];
-    g.W = a.W[r];
-    const cpx<T>* U = a.ws + il * a.ws_img + nd.slot;
-    leaf_body(a, g, U, a.filt + a.filt_off[nd.leaf_filt], nd.leaf_k, nd.leaf_ch, a.img0 + il, lds, tid, NT);
+// Row pass of a big node's FIR leaf: rows rr < nr of R (natural columns,
+// row stride rs) are the node's rows rev[r0 + rr]; T[y][x] goes to the tslot.
+template <class T>
+KYM_HD void fir_rows_to_slot(const Scat2DArgs<T>& a, const Node2D& nd, const T* R, int rs, int nr, int r0,
+                             const int* rev, cpx<T>* wsi, int tid, int nth) {
+    const int r = nd.res, W = a.W[r], ow = a.ow, kl = nd.leaf_k;
+    const FirAxis fx = a.fir[2 * r + 1];
+    const T* cx = a.filt + fx.off;
+    T* Tg = reinterpret_cast<T*>(wsi + nd.tslot);
+    for (int e = tid; e < nr * ow; e += nth) {
+        const int rr = e / ow, x = e - rr * ow;
+        Tg[(long long)rev[r0 + rr] * ow + x] = fir_sum(R + rr * rs, 1, W, kl * (x + 1), cx, fx.lo, fx.cnt);
+    }
 }
 
 // Root row pass: rows of the padded image (row storage positions
-// digit-reversed), forward DIT along x.  task -> (image, row block).
+// digit-reversed), forward DIT along x.  task -> (image, row block).  With a
+// FIR leaf the block's rows are first loaded in natural order as reals (after
+// the complex rows in LDS); the leaf's row pass goes to the node's tslot.
 template <class T, int MAXP>
 KYM_HD void body_root_rows(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
     const int il = task / a.nblk, rb = task - il * a.nblk;
@@ -297,9 +477,24 @@ KYM_HD void body_root_rows(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int ti
     const FftLen<T>& fW = a.lens[a.lenW[0]];
     const int r0 = rb * a.blk, nr = (H - r0 < a.blk) ? H - r0 : a.blk;
     const T* xin = a.x + (long long)img * a.M * a.N;
-    for (int i = tid; i < nr * W; i += nth) {
-        const int rr = i / W, px = i - rr * W;
-        lds[i] = {input_pixel(a, xin, fH.rev[r0 + rr], fW.rev[px]), T(0)};
+    if constexpr (kym_fir<T>) {
+        T* R = reinterpret_cast<T*>(lds + nr * W);
+        const int rs = W + 1;
+        for (int i = tid; i < nr * W; i += nth) {
+            const int rr = i / W, x = i - rr * W;
+            R[rr * rs + x] = input_pixel(a, xin, fH.rev[r0 + rr], x);
+        }
+        wg_barrier();
+        for (int i = tid; i < nr * W; i += nth) {
+            const int rr = i / W, px = i - rr * W;
+            lds[i] = {R[rr * rs + fW.rev[px]], T(0)};
+        }
+        fir_rows_to_slot(a, nd, R, rs, nr, r0, fH.rev, a.ws + il * a.ws_img, tid, nth);
+    } else {
+        for (int i = tid; i < nr * W; i += nth) {
+            const int rr = i / W, px = i - rr * W;
+            lds[i] = {input_pixel(a, xin, fH.rev[r0 + rr], fW.rev[px]), T(0)};
+        }
     }
     wg_barrier();
     fft_dit<false, MAXP>(lds, fW, Batch{nr, 1, W, false}, tid, nth);
@@ -352,7 +547,9 @@ KYM_HD void body_leaf_icol(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int ti
                 a.lens[a.lenH[J]], wsi + nd.leaf_slot, c0, nc, lds, tid, nth);
 }
 
-// Rows: inverse DIF along x, modulus, forward DIT along x (in place).
+// Rows: inverse DIF along x, modulus, forward DIT along x (in place).  With
+// a FIR leaf the modulus rows are also kept as natural reals after the complex
+// rows, for the leaf's row pass; nodes without children stop there.
 template <class T, int MAXP>
 KYM_HD void body_rows_mod(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
     const int per = a.ngroup * a.nblk;
@@ -362,20 +559,48 @@ KYM_HD void body_rows_mod(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid
     const int r = nd.res, H = a.H[r], W = a.W[r];
     const FftLen<T>& fW = a.lens[a.lenW[r]];
     const int r0 = rb * a.blk, nr = (H - r0 < a.blk) ? H - r0 : a.blk;
-    cpx<T>* buf = a.ws + il * a.ws_img + nd.slot + (long long)r0 * W;
+    cpx<T>* wsi = a.ws + il * a.ws_img;
+    cpx<T>* buf = wsi + nd.slot + (long long)r0 * W;
     for (int i = tid; i < nr * W; i += nth) lds[i] = buf[i];
     wg_barrier();
     fft_dif<true, MAXP>(lds, fW, Batch{nr, 1, W, false}, tid, nth);
     const T inv_n = T(1) / T(H * W);
-    cpx<T>* sv = (a.save && nd.save >= 0) ? a.ws + il * a.ws_img + nd.save + (long long)r0 * W : nullptr;
+    cpx<T>* sv = (a.save && nd.save >= 0) ? wsi + nd.save + (long long)r0 * W : nullptr;
+    T* R = reinterpret_cast<T*>(lds + nr * W);
+    const int rs = W + 1;
     for (int i = tid; i < nr * W; i += nth) {
         const cpx<T> v = lds[i];
         if (sv) sv[i] = v;
-        lds[i] = {kym_sqrt(v.re * v.re + v.im * v.im) * inv_n, T(0)};
+        const T m = kym_sqrt(v.re * v.re + v.im * v.im) * inv_n;
+        lds[i] = {m, T(0)};
+        if constexpr (kym_fir<T>) {
+            const int rr = i / W, px = i - rr * W;
+            R[rr * rs + fW.rev[px]] = m;
+        }
     }
     wg_barrier();
+    if constexpr (kym_fir<T>) fir_rows_to_slot(a, nd, R, rs, nr, r0, a.lens[a.lenH[r]].rev, wsi, tid, nth);
+    if (!nd.fwd) return;
     fft_dit<false, MAXP>(lds, fW, Batch{nr, 1, W, false}, tid, nth);
     for (int i = tid; i < nr * W; i += nth) buf[i] = lds[i];
+}
+
+// Column pass of a big node's FIR leaf: tslot -> output channel.
+// task -> (image, node).
+template <class T, int MAXP>
+KYM_HD void body_fir_cols(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
+    (void)lds;
+    const int il = task / a.ngroup, gi = task - il * a.ngroup;
+    const Node2D& nd = a.nodes[a.group[gi]];
+    const int r = nd.res, H = a.H[r], ow = a.ow, oh = a.oh, kl = nd.leaf_k;
+    const FirAxis fy = a.fir[2 * r];
+    const T* cy = a.filt + fy.off;
+    const T* Tg = reinterpret_cast<const T*>(a.ws + il * a.ws_img + nd.tslot);
+    T* o = a.out + ((long long)(a.img0 + il) * a.C + nd.leaf_ch) * oh * ow;
+    for (int e = tid; e < oh * ow; e += nth) {
+        const int y = e / ow, x = e - y * ow;
+        o[e] = fir_sum(Tg + x, ow, H, kl * (y + 1), cy, fy.lo, fy.cnt);
+    }
 }
 
 // Rows of a big leaf: inverse DIF along x, real part, unpad, store.
@@ -425,32 +650,21 @@ KYM_HD void body_fwd_cols(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid
     }
 }
 
-// Compile-time specialisations: X(id, H, W, HJ, WJ, threads) for LDS nodes,
-// X(id, HJ, WJ, threads) for leaves.  A launch uses one when its sizes match;
-// every other size runs the run-time-sized bodies.
+// Compile-time specialisations X(id, H, W, HJ, WJ, threads) of the LDS node
+// kernel (FP32).  A launch uses one when its sizes match; every other size
+// runs the run-time-sized bodies.
 #define KYM_NODE_SPECS(X)            \
     X(0, 136, 136, 34, 34, 1024)     \
     X(1, 68, 68, 34, 34, 512)        \
     X(2, 80, 80, 10, 10, 512)        \
     X(3, 40, 40, 10, 10, 256)        \
     X(4, 20, 20, 10, 10, 256)
-#define KYM_LEAF_SPECS(X) \
-    X(0, 34, 34, 256)     \
-    X(1, 10, 10, 256)
 
 inline int node_spec_id(int H, int W, int HJ, int WJ, int* threads) {
 #define KYM_NS_MATCH(id, h, w, hj, wj, nt) \
     if (H == h && W == w && HJ == hj && WJ == wj) { *threads = nt; return id; }
     KYM_NODE_SPECS(KYM_NS_MATCH)
 #undef KYM_NS_MATCH
-    return -1;
-}
-
-inline int leaf_spec_id(int HJ, int WJ, int* threads) {
-#define KYM_LS_MATCH(id, hj, wj, nt) \
-    if (HJ == hj && WJ == wj) { *threads = nt; return id; }
-    KYM_LEAF_SPECS(KYM_LS_MATCH)
-#undef KYM_LS_MATCH
     return -1;
 }
 
@@ -466,18 +680,9 @@ KYM_HD void dispatch_node_spec(int id, const Scat2DArgs<T>& a, int task, cpx<T>*
     }
 }
 
-template <class T, bool EMU>
-KYM_HD void dispatch_leaf_spec(int id, const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid) {
-    switch (id) {
-#define KYM_LS_CASE(sid, hj, wj, nt) \
-    case sid: body_leaf_s<T, hj, wj, (EMU ? 1 : nt)>(a, task, lds, tid); break;
-        KYM_LEAF_SPECS(KYM_LS_CASE)
-#undef KYM_LS_CASE
-        default: break;
-    }
-}
-
-enum Kind2D { K2_LDS_NODE = 0, K2_LEAF, K2_ROOT_ROWS, K2_ICOL, K2_ROWS_MOD, K2_FWD_COLS, K2_LEAF_ICOL, K2_LEAF_ROWS };
+enum Kind2D {
+    K2_LDS_NODE = 0, K2_LEAF, K2_ROOT_ROWS, K2_ICOL, K2_ROWS_MOD, K2_FWD_COLS, K2_LEAF_ICOL, K2_LEAF_ROWS, K2_FIR_COLS
+};
 
 template <class T, int MAXP>
 KYM_HD KYM_INL void dispatch_body2d(int kind, const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
@@ -490,6 +695,7 @@ KYM_HD KYM_INL void dispatch_body2d(int kind, const Scat2DArgs<T>& a, int task, 
         case K2_FWD_COLS: body_fwd_cols<T, MAXP>(a, task, lds, tid, nth); break;
         case K2_LEAF_ICOL: body_leaf_icol<T, MAXP>(a, task, lds, tid, nth); break;
         case K2_LEAF_ROWS: body_leaf_rows<T, MAXP>(a, task, lds, tid, nth); break;
+        case K2_FIR_COLS: body_fir_cols<T, MAXP>(a, task, lds, tid, nth); break;
         default: break;
     }
 }

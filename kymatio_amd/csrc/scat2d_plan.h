@@ -4,6 +4,7 @@
 // filter order, core/scattering2d.py:1-88 for the path order).
 #pragma once
 #include <algorithm>
+#include <cmath>
 
 #include "plan_common.h"
 #include "scat2d_bwd.h"
@@ -59,6 +60,30 @@ struct Scat2DPlan {
     }
 
     bool use_specs = true;  // select compile-time specialised kernels when sizes match (FP32 only)
+    // leaf taps: 2 J axes ([2 r] along y, [2 r + 1] along x) in the packed
+    // filter buffer after the filters, at taps_off, taps_len reals in all
+    std::vector<FirAxis> fir_axes;
+    long long taps_off = 0, taps_len = 0;
+    size_t off_fir = 0;
+
+    // Taps of the low-pass filter phi.levels[r] along an axis of n samples.
+    // phi.levels[0] is a periodised Gaussian of sigma0 = 0.8 2^(J-1)
+    // (kymatio/scattering2d/filter_bank.py: phi = gabor_2d(M, N, 0.8 * 2 ** (J - 1), 0, 0));
+    // its taps beyond 7 sigma0 are below 1e-10 of the filter's mass.  Level
+    // r >= 1 is its crop to the band |k| < n / 2, which cuts the Gaussian
+    // spectrum at c = pi sigma0 / 2^r of its standard deviations and leaves
+    // a ringing tail of relative size ~exp(-c^2 / 2): a 7 sigma band is kept
+    // only when c >= 7, otherwise every sample of the circle is a tap.
+    FirAxis fir_axis(int r, int n, long long off) const {
+        const double sigma0 = 0.8 * std::ldexp(1.0, J - 1);
+        const double c = r == 0 ? 1e30 : 3.14159265358979 * sigma0 / std::ldexp(1.0, r);
+        FirAxis f = {off, -(n / 2), n};
+        if (c >= 7.0) {
+            const int R = (int)std::ceil(7.0 * sigma0 / std::ldexp(1.0, r));
+            if (2 * R + 1 < n) { f.lo = -R; f.cnt = 2 * R + 1; }
+        }
+        return f;
+    }
 
     bool build(int M_, int N_, int J_, int L_, int max_order_, int pre_pad_, size_t lds_limit_, KymError* e) {
         M = M_; N = N_; J = J_; L = L_; max_order = max_order_; pre_pad = pre_pad_;
@@ -108,6 +133,18 @@ struct Scat2DPlan {
                 }
             }
         filt_off.push_back(off);
+        // leaf taps (FP32 plans)
+        const bool fir = kym_fir<T>;
+        fir_axes.assign(2 * (size_t)std::max(J, 1), FirAxis{0, 0, 0});
+        taps_off = off;
+        taps_len = 0;
+        if (fir)
+            for (int r = 0; r < J; ++r) {
+                fir_axes[2 * r] = fir_axis(r, H[r], taps_off + taps_len);
+                taps_len += fir_axes[2 * r].cnt;
+                fir_axes[2 * r + 1] = fir_axis(r, W[r], taps_off + taps_len);
+                taps_len += fir_axes[2 * r + 1].cnt;
+            }
         // nodes (path order of kymatio/scattering2d/core/scattering2d.py:25-78)
         nodes.clear();
         const int npsi = J * L;
@@ -141,18 +178,28 @@ struct Scat2DPlan {
         std::vector<char> has_children(nodes.size(), 0);
         for (auto& nd : nodes)
             if (nd.parent >= 0) has_children[nd.parent] = 1;
+        auto node_threads = [&](size_t elems) { return elems >= 8192 ? 1024 : elems >= 2048 ? 512 : 256; };
         ws_img = 0;
         for (size_t i = 0; i < nodes.size(); ++i) {
             Node2D& nd = nodes[i];
             const size_t elems = (size_t)H[nd.res] * W[nd.res];
+            // leaf_mode places the spectral leaf and, in training, the leaf's gradient
             if ((elems + leaf_elems) * cs <= lds_limit) { nd.big = 0; nd.leaf_mode = 0; }
             else if (elems * cs <= lds_limit) { nd.big = 0; nd.leaf_mode = 1; }
             else { nd.big = 1; nd.leaf_mode = (leaf_elems * cs <= lds_limit) ? 1 : 2; }
-            const bool need_slot = has_children[i] || nd.big || nd.leaf_mode != 0;
+            if (fir && !nd.big && elems > (size_t)KYM_STAGE_MAX * node_threads(elems)) nd.big = 1;
+            nd.fir = fir ? (nd.big ? 2 : 1) : 0;
+            nd.fwd = (has_children[i] || !fir) ? 1 : 0;
+            const bool need_slot = has_children[i] || nd.big || (!fir && nd.leaf_mode != 0);
             nd.slot = need_slot ? ws_img : -1;
             if (need_slot) ws_img += (long long)elems;
             nd.leaf_slot = -1;
-            if (nd.leaf_mode == 2) { nd.leaf_slot = ws_img; ws_img += (long long)leaf_elems; }
+            if (!fir && nd.leaf_mode == 2) { nd.leaf_slot = ws_img; ws_img += (long long)leaf_elems; }
+            nd.tslot = -1;
+            if (nd.fir == 2) {
+                nd.tslot = ws_img;
+                ws_img += ((long long)H[nd.res] * ow * (long long)sizeof(T) + cs - 1) / cs;
+            }
             // keep slots 256-byte aligned
             const long long align = 256 / (long long)cs;
             ws_img = (ws_img + align - 1) / align * align;
@@ -190,6 +237,8 @@ struct Scat2DPlan {
         std::memcpy(blob.at<char>(off_nodes), nodes.data(), sizeof(Node2D) * nodes.size());
         off_filt = blob.add(sizeof(long long) * filt_off.size());
         std::memcpy(blob.at<char>(off_filt), filt_off.data(), sizeof(long long) * filt_off.size());
+        off_fir = blob.add(sizeof(FirAxis) * fir_axes.size());
+        std::memcpy(blob.at<char>(off_fir), fir_axes.data(), sizeof(FirAxis) * fir_axes.size());
         // launches
         launches.clear();
         max_lds = 0;
@@ -199,7 +248,6 @@ struct Scat2DPlan {
             std::memcpy(blob.at<char>(o), ids.data(), sizeof(int) * ids.size());
             return o;
         };
-        auto node_threads = [&](size_t elems) { return elems >= 8192 ? 1024 : elems >= 2048 ? 512 : 256; };
         // Stage-boundary byte model per node (per image): every tensor that
         // crosses a stage boundary is written once and read once; complex 8 B,
         // real 4 B.  Root: 4 M N input + 16 P_M P_N (U_0 spectrum); other
@@ -219,11 +267,12 @@ struct Scat2DPlan {
             long long mb = 0;
             for (int id : ids) {
                 const Node2D& nd = nodes[id];
-                const bool leaf_here = (kind == K2_LDS_NODE && nd.leaf_mode == 0) || kind == K2_LEAF ||
-                                       kind == K2_LEAF_ROWS;
+                const bool leaf_here = (kind == K2_LDS_NODE && (nd.fir == 1 || nd.leaf_mode == 0)) ||
+                                       kind == K2_LEAF || kind == K2_LEAF_ROWS || kind == K2_FIR_COLS;
+                const int npass = nd.order == 0 ? 2 : nd.fwd ? 3 : 2;  // big-node passes sharing the node's bytes
                 if (kind == K2_LDS_NODE) mb += node_model(id);
-                if (kind == K2_ROOT_ROWS || kind == K2_FWD_COLS) mb += node_model(id) / (nd.order == 0 ? 2 : 3);
-                if (kind == K2_ICOL || kind == K2_ROWS_MOD) mb += node_model(id) / 3;
+                if (kind == K2_ROOT_ROWS || kind == K2_FWD_COLS || kind == K2_ICOL || kind == K2_ROWS_MOD)
+                    mb += node_model(id) / npass;
                 if (leaf_here) mb += leaf_model;
             }
             l.model_bytes = mb;
@@ -237,24 +286,23 @@ struct Scat2DPlan {
             int cb = (int)(pass_budget / (cs * Hh));
             return std::max(1, std::min(16, cb));
         };
+        // row passes of FIR nodes also keep the block's rows as reals (stride W + 1)
+        auto row_bytes = [&](int Ww) { return cs * Ww + (fir ? sizeof(T) * (Ww + 1) : 0); };
         auto row_blk = [&](int Ww) {
-            int rb = (int)(pass_budget / (cs * Ww));
+            int rb = (int)(pass_budget / row_bytes(Ww));
             return std::max(1, std::min(16, rb));
         };
         auto emit_leaves = [&](const std::vector<int>& ids) {
-            std::vector<int> small, big;
+            std::vector<int> small, big, firc;
             for (int id : ids) {
-                if (nodes[id].leaf_mode == 1) small.push_back(id);
-                if (nodes[id].leaf_mode == 2) big.push_back(id);
+                if (nodes[id].fir == 2) firc.push_back(id);
+                else if (nodes[id].fir == 0 && nodes[id].leaf_mode == 1) small.push_back(id);
+                else if (nodes[id].fir == 0 && nodes[id].leaf_mode == 2) big.push_back(id);
             }
-            if (!small.empty()) {
-                push(K2_LEAF, small, 1, 0, node_threads(leaf_elems), leaf_elems * cs);
-                int nt = 0;
-                const int sid = leaf_spec_id(H[J], W[J], &nt);
-                if (use_specs && sizeof(T) == 4 && sid >= 0) { launches.back().spec = sid; launches.back().threads = nt; }
-            }
+            if (!firc.empty()) push(K2_FIR_COLS, firc, 1, 0, 256, 0);
+            if (!small.empty()) push(K2_LEAF, small, 1, 0, node_threads(leaf_elems), leaf_elems * cs);
             if (!big.empty()) {
-                const int cb = col_blk(H[J]), rb = row_blk(W[J]);
+                const int cb = col_blk(H[J]), rb = std::max(1, std::min(16, (int)(pass_budget / (cs * W[J]))));
                 push(K2_LEAF_ICOL, big, (W[J] + cb - 1) / cb, cb, 256, (size_t)H[J] * cb * cs);
                 push(K2_LEAF_ROWS, big, (H[J] + rb - 1) / rb, rb, 256, (size_t)rb * W[J] * cs);
             }
@@ -265,22 +313,22 @@ struct Scat2DPlan {
             const int r = n0.res;
             const size_t elems = (size_t)H[r] * W[r];
             if (!n0.big) {
-                const size_t lds = (elems + (n0.leaf_mode == 0 ? leaf_elems : 0)) * cs;
+                const size_t lds = (elems + (n0.fir == 0 && n0.leaf_mode == 0 ? leaf_elems : 0)) * cs;
                 push(K2_LDS_NODE, ids, 1, 0, node_threads(elems), lds);
                 int nt = 0;
                 const int sid = node_spec_id(H[r], W[r], H[J], W[J], &nt);
-                if (use_specs && sizeof(T) == 4 && sid >= 0 && n0.leaf_mode == 0) {
+                if (use_specs && sizeof(T) == 4 && sid >= 0 && (n0.fir == 1 || n0.leaf_mode == 0)) {
                     launches.back().spec = sid;
                     launches.back().threads = nt;
                 }
             } else {
                 const int cb = col_blk(H[r]), rb = row_blk(W[r]);
-                if (is_root) push(K2_ROOT_ROWS, ids, (H[r] + rb - 1) / rb, rb, 256, (size_t)rb * W[r] * cs);
+                if (is_root) push(K2_ROOT_ROWS, ids, (H[r] + rb - 1) / rb, rb, 256, (size_t)rb * row_bytes(W[r]));
                 else {
                     push(K2_ICOL, ids, (W[r] + cb - 1) / cb, cb, 256, (size_t)H[r] * cb * cs);
-                    push(K2_ROWS_MOD, ids, (H[r] + rb - 1) / rb, rb, 256, (size_t)rb * W[r] * cs);
+                    push(K2_ROWS_MOD, ids, (H[r] + rb - 1) / rb, rb, 256, (size_t)rb * row_bytes(W[r]));
                 }
-                push(K2_FWD_COLS, ids, (W[r] + cb - 1) / cb, cb, 256, (size_t)H[r] * cb * cs);
+                if (n0.fwd) push(K2_FWD_COLS, ids, (W[r] + cb - 1) / cb, cb, 256, (size_t)H[r] * cb * cs);
             }
             emit_leaves(ids);
         };
@@ -339,6 +387,7 @@ struct Scat2DPlan {
         a.nodes = reinterpret_cast<const Node2D*>(base + off_nodes);
         a.lens = reinterpret_cast<const FftLen<T>*>(base + off_lens);
         a.filt_off = reinterpret_cast<const long long*>(base + off_filt);
+        a.fir = reinterpret_cast<const FirAxis*>(base + off_fir);
         a.group = reinterpret_cast<const int*>(base + l.group_off);
         a.ngroup = l.ngroup; a.nblk = l.nblk; a.blk = l.blk;
         for (int r = 0; r <= J; ++r) { a.lenH[r] = lenH[r]; a.lenW[r] = lenW[r]; a.H[r] = H[r]; a.W[r] = W[r]; }

@@ -70,3 +70,43 @@ def filter_bank(M, N, J, L=8):
     phi_f = np.real(fft2(gabor_2d(M, N, 0.8 * 2 ** (J - 1), 0, 0)))
     phi = {'levels': [periodize_filter_fft(phi_f, r) for r in range(J)], 'j': J}
     return {'phi': phi, 'psi': psi}
+
+
+def leaf_taps(phi_levels, desc):
+    """Taps of the separable low-pass leaves (kym_scat2d_leaf_taps in
+    include/kymatio_hip.h).
+
+    phi_levels[r] is the Fourier-domain low-pass filter at resolution r (the
+    buffers the spectral leaf multiplies with); its spatial form
+    phi_r = Re(ifft2(phi_levels[r])) is separable, phi_r[a][b] = g[a] h[b]
+    with g = phi_r[:, 0] / sqrt(phi_r[0, 0]) and h = phi_r[0, :] / sqrt(phi_r[0, 0]).
+    desc = (taps_off, taps_len, M_padded, N_padded, then per r: off_y, lo_y,
+    cnt_y, off_x, lo_x, cnt_x).  Returns the taps_len reals to store at
+    taps_off of the packed filter buffer (float64).  Raises RuntimeError if a
+    filter is not separable or the plan's tap window drops more than 1e-6 of
+    its mass, which does not happen for the filter bank of this module.
+    """
+    taps_off, taps_len = int(desc[0]), int(desc[1])
+    out = np.zeros(taps_len, np.float64)
+    if taps_len == 0:
+        return out
+    for r in range((len(desc) - 4) // 6):
+        ph = np.asarray(phi_levels[r], np.float64)
+        sp = np.real(np.fft.ifft2(ph))
+        c00 = sp[0, 0]
+        if not c00 > 0:
+            raise RuntimeError('low-pass filter %d has no positive centre tap' % r)
+        g = sp[:, 0] / np.sqrt(c00)
+        h = sp[0, :] / np.sqrt(c00)
+        resid = np.abs(sp - np.outer(g, h)).max() / np.abs(sp).max()
+        if resid > 1e-5:
+            raise RuntimeError('low-pass filter %d is not separable (residual %.2e)' % (r, resid))
+        for v, base in ((g, 4 + 6 * r), (h, 4 + 6 * r + 3)):
+            off, lo, cnt = int(desc[base]), int(desc[base + 1]), int(desc[base + 2])
+            idx = (lo + np.arange(cnt)) % v.shape[0]
+            kept = v[idx]
+            dropped = 1.0 - np.abs(kept).sum() / np.abs(v).sum()
+            if dropped > 1e-6:
+                raise RuntimeError('leaf taps of level %d drop %.2e of the filter' % (r, dropped))
+            out[off - taps_off:off - taps_off + cnt] = kept
+    return out

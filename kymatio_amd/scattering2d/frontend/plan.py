@@ -21,9 +21,14 @@ class Scat2DPlanHandle:
         offs = (ctypes.c_longlong * (self.n_filters + 1))()
         _lib.check(lib.kym_scat2d_filter_offsets(h, offs))
         self.filter_offsets = list(offs)
+        J = int(J)
+        desc = (ctypes.c_longlong * (4 + 6 * max(J, 1)))()
+        _lib.check(lib.kym_scat2d_leaf_taps(h, desc, len(desc)))
+        self.leaf_taps_desc = list(desc)[:4 + 6 * J]
         self.dtype = dtype
 
-    KIND_NAMES = ['lds_node', 'leaf', 'root_rows', 'icol', 'rows_mod', 'fwd_cols', 'leaf_icol', 'leaf_rows']
+    KIND_NAMES = ['lds_node', 'leaf', 'root_rows', 'icol', 'rows_mod', 'fwd_cols', 'leaf_icol', 'leaf_rows',
+                  'fir_cols']
 
     def launches(self):
         """Per-launch records: kind, nodes, tasks per node, threads, LDS bytes,

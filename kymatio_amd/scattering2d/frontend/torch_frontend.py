@@ -14,7 +14,7 @@ import torch
 from ... import _lib
 from ...frontend.base_frontend import ScatteringBase
 from ...frontend.torch_frontend import ScatteringTorch
-from ..filter_bank import filter_bank
+from ..filter_bank import filter_bank, leaf_taps
 from ..utils import compute_padding, pad_amounts
 from .plan import Scat2DPlanHandle
 
@@ -95,9 +95,14 @@ class ScatteringTorch2D(ScatteringTorch, ScatteringBase):
         if hit is not None and hit[0] == stamp:
             return hit[1]
         offs = plan.filter_offsets
-        packed = torch.empty(int(offs[-1]), dtype=dtype, device=device)
+        packed = torch.empty(int(plan.filter_elems), dtype=dtype, device=device)
         for i, b in enumerate(bufs):
             packed[int(offs[i]):int(offs[i + 1])].copy_(b.reshape(-1))
+        desc = plan.leaf_taps_desc
+        if desc[1] > 0:
+            phi = [b.detach().cpu().numpy()[..., 0] for b in bufs[:self.J]]
+            taps = leaf_taps(phi, desc)
+            packed[int(desc[0]):int(desc[0] + desc[1])].copy_(torch.from_numpy(taps).to(dtype))
         self._packed[key] = (stamp, packed)
         return packed
 

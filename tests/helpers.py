@@ -4,6 +4,8 @@ import os
 
 import numpy as np
 
+from kymatio_amd.scattering2d.filter_bank import leaf_taps
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLDEN = os.path.join(ROOT, 'tests', 'golden')
 
@@ -64,6 +66,33 @@ class Emu:
         f.restype = ctypes.c_int
         self.lib.kym_emu_last_error.restype = ctypes.c_char_p
 
+    def with_taps(self, filters_packed, J, shape, L=8, max_order=2, pre_pad=0, lds_limit=160 * 1024):
+        """The packed filters followed by the plan's low-pass taps
+        (filter_bank.leaf_taps from the phi levels at the buffer's start)."""
+        dt = filters_packed.dtype
+        code = 0 if dt == np.float32 else 1
+        f = self.lib.kym_emu_scat2d_taps
+        f.argtypes = [ctypes.c_int] * 7 + [ctypes.c_longlong, ctypes.c_void_p, ctypes.c_int]
+        f.restype = ctypes.c_int
+        n = 4 + 6 * max(J, 1)
+        desc = (ctypes.c_longlong * n)()
+        rc = f(shape[0], shape[1], J, L, max_order, pre_pad, code, lds_limit, desc, n)
+        if rc != 0:
+            raise RuntimeError(self.lib.kym_emu_last_error().decode())
+        desc = list(desc)[:4 + 6 * J]
+        if desc[1] == 0:
+            return filters_packed
+        Mp, Np = desc[2], desc[3]
+        phi, o = [], 0
+        for r in range(J):
+            h, w = Mp >> r, Np >> r
+            phi.append(filters_packed[o:o + h * w].reshape(h, w))
+            o += h * w
+        out = np.zeros(desc[0] + desc[1], dt)
+        out[:desc[0]] = filters_packed[:desc[0]]
+        out[desc[0]:] = leaf_taps(phi, desc).astype(dt)
+        return out
+
     def scat2d(self, x, J, shape, filters_packed, L=8, max_order=2, pre_pad=0, lds_limit=160 * 1024,
                chunk=0):
         dt = filters_packed.dtype
@@ -77,6 +106,7 @@ class Emu:
             raise RuntimeError(self.lib.kym_emu_last_error().decode())
         out = np.zeros((B, info[0], info[1], info[2]), dt)
         xx = np.ascontiguousarray(x, dt)
+        filters_packed = self.with_taps(filters_packed, J, shape, L, max_order, pre_pad, lds_limit)
         rc = self.lib.kym_emu_scat2d_forward(M, N, J, L, max_order, pre_pad, code, lds_limit, xx.ctypes.data,
                                              filters_packed.ctypes.data, out.ctypes.data, B, chunk, info)
         if rc != 0:
@@ -134,6 +164,7 @@ def emu_train2d(emu, x, J, shape, filters_packed, dS, L=8, max_order=2, pre_pad=
     f.argtypes = [ctypes.c_int] * 7 + [ctypes.c_longlong] + [ctypes.c_void_p] * 5 + [ctypes.c_longlong, ctypes.c_int]
     f.restype = ctypes.c_int
     dt = filters_packed.dtype
+    filters_packed = emu.with_taps(filters_packed, J, shape, L, max_order, pre_pad, lds_limit)
     xx = np.ascontiguousarray(x, dt)
     dd = np.ascontiguousarray(dS, dt)
     out = np.zeros(dS.shape, dt)
