@@ -33,6 +33,7 @@
 // the reference's arithmetic to FP64 rounding.
 #pragma once
 #include "fft_static.h"
+#include "mma.h"
 
 #define KYM_MAX_RES 16
 
@@ -117,6 +118,64 @@ KYM_HD KYM_INL cpx<T> gather_periodized(const cpx<T>* U, const T* f, int Wp, int
         }
     }
     return acc;
+}
+
+// The same with the alias count K fixed at compile time (unrolled loads).
+template <int K, class T>
+KYM_HD KYM_INL cpx<T> gather_periodized_k(const cpx<T>* U, const T* f, int Wp, int H, int W, int y, int x) {
+    cpx<T> acc = {T(0), T(0)};
+#pragma unroll
+    for (int a = 0; a < K; ++a) {
+        const int row = (y + a * H) * Wp + x;
+#pragma unroll
+        for (int b = 0; b < K; ++b) {
+            const T fv = f[row + b * W];
+            const cpx<T> u = U[row + b * W];
+            acc.re += u.re * fv;
+            acc.im += u.im * fv;
+        }
+    }
+    return acc;
+}
+
+// dst[y * ncols + c] = scale * P_k(U * f)[y][c0 + c] for y < H, c < ncols.
+// Four elements per thread and round: their loads are all issued before the
+// first use, so the memory latency is paid once per four.
+template <int K, class T>
+KYM_HD void gather_block_k(cpx<T>* dst, int ncols, int c0, const cpx<T>* U, const T* f, int Wp, int H, int W,
+                           T scale, int tid, int nth) {
+    const int n = H * ncols;
+    int i = tid;
+    for (; i + 3 * nth < n; i += 4 * nth) {
+        cpx<T> v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = i + u * nth, y = e / ncols, c = e - y * ncols;
+            v[u] = gather_periodized_k<K>(U, f, Wp, H, W, y, c0 + c);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) dst[i + u * nth] = v[u] * scale;
+    }
+    for (; i < n; i += nth) {
+        const int y = i / ncols, c = i - y * ncols;
+        dst[i] = gather_periodized_k<K>(U, f, Wp, H, W, y, c0 + c) * scale;
+    }
+}
+
+template <class T>
+KYM_HD void gather_block(cpx<T>* dst, int ncols, int c0, const cpx<T>* U, const T* f, int Wp, int H, int W, int k,
+                         int tid, int nth) {
+    const T scale = T(1) / T(k * k);
+    switch (k) {
+        case 1: gather_block_k<1>(dst, ncols, c0, U, f, Wp, H, W, scale, tid, nth); break;
+        case 2: gather_block_k<2>(dst, ncols, c0, U, f, Wp, H, W, scale, tid, nth); break;
+        case 4: gather_block_k<4>(dst, ncols, c0, U, f, Wp, H, W, scale, tid, nth); break;
+        default:
+            for (int i = tid; i < H * ncols; i += nth) {
+                const int y = i / ncols, c = i - y * ncols;
+                dst[i] = gather_periodized(U, f, Wp, H, W, k, y, c0 + c) * scale;
+            }
+    }
 }
 
 template <class T>
@@ -234,9 +293,9 @@ KYM_HD void leaf_body(const Scat2DArgs<T>& a, const G& g, const cpx<T>* U, const
 
 // ------------------------------------------------------------------ leaf in space
 // The real field of a node lives in LDS as R[y * (W + 1) + x] (natural order,
-// odd row stride), overlaying the node's complex buffer; the leaf's row pass
-// writes Tb[y * (ow + 1) + x] right after R.  R + Tb take
-// H (W + 1) + H (ow + 1) <= 2 H W reals, i.e. fit in the complex buffer.
+// odd row stride), overlaying the node's complex buffer; the leaf's tables
+// follow it (fir_lds_reals; the plan sizes the LDS for the larger of the
+// two layouts).
 //
 // Moving between the digit-reversed complex layout and R permutes the whole
 // buffer in place, so every thread first reads its elements into registers,
@@ -317,26 +376,111 @@ KYM_HD void restage_complex(const G& g, cpx<T>* lds, int tid, int nth) {
     wg_barrier();
 }
 
-// Leaf of the node from R: row pass (rows fastest across threads: the odd
-// row stride keeps the LDS reads conflict-free) into Tb, column pass into
-// the output channel.  Callers barrier before reusing Tb's memory.
+// Row stride of Tb: 16 mod 32 floats (column reads of the column pass hit
+// distinct banks) when that still fits in the complex buffer, else ow + 1.
+KYM_HD KYM_INL int fir_tstride(int W, int ow) {
+    const int ts = (ow + 15) / 32 * 32 + 16;
+    return (W + 1 + ts <= 2 * W) ? ts : ow + 1;
+}
+
+// Reals the in-LDS leaf uses: R, Tb and the two band matrices Bx (Kx x 16)
+// and Ay (Ky x 16) with K = min(n, 15 kl + cnt).
+KYM_HD KYM_INL long long fir_lds_reals(int H, int W, int ow, int kl, int cnt_y, int cnt_x) {
+    const int kx = (15 * kl + cnt_x < W) ? 15 * kl + cnt_x : W;
+    const int ky = (15 * kl + cnt_y < H) ? 15 * kl + cnt_y : H;
+    return (long long)H * (W + 1) + (long long)H * fir_tstride(W, ow) + 16ll * (kx + ky);
+}
+
+// Leaf of the node from R on the matrix cores (FP32).  Both passes are
+// GEMMs with a banded circulant factor:
+//   row pass     Tb[p][x] = sum_q R[p][q] c_x[(q - kl (x+1) - lo_x) mod W]
+//   column pass  S[y][x]  = sum_p c_y[(p - kl (y+1) - lo_y) mod H] Tb[p][x]
+// Each 16 x 16 output tile only needs the window of 15 kl + cnt input
+// samples its 16 outputs touch (the whole circle when the taps cover it);
+// inside a window the factor depends only on (k, output offset d < 16), so
+// it is tabulated once per node as band[k][d] = c[(k - kl d) mod n] (zero
+// beyond the taps).  Callers barrier before reusing the buffer.
 template <class T, class G>
 KYM_HD void fir_leaf_lds(const Scat2DArgs<T>& a, const G& g, const Node2D& nd, T* R, int img, int tid, int nth) {
     const int r = nd.res, ow = a.ow, oh = a.oh, kl = nd.leaf_k;
-    const int rs = g.W + 1, ts = ow + 1;
-    T* Tb = R + g.H * rs;
+    const int H = g.H, W = g.W, rs = W + 1, ts = fir_tstride(W, ow);
     const FirAxis fy = a.fir[2 * r], fx = a.fir[2 * r + 1];
     const T* cy = a.filt + fy.off;
     const T* cx = a.filt + fx.off;
-    for (int e = tid; e < g.H * ow; e += nth) {
-        const int x = e / g.H, p = e - x * g.H;
-        Tb[p * ts + x] = fir_sum(R + p * rs, 1, g.W, kl * (x + 1), cx, fx.lo, fx.cnt);
+    const int kxm = (15 * kl + fx.cnt < W) ? 15 * kl + fx.cnt : W;
+    const int kym = (15 * kl + fy.cnt < H) ? 15 * kl + fy.cnt : H;
+    T* Tb = R + H * rs;
+    T* Bx = Tb + H * ts;
+    T* Ay = Bx + 16 * kxm;
+    for (int i = tid; i < 16 * (kxm + kym); i += nth) {
+        const bool isx = i < 16 * kxm;
+        const int j = isx ? i : i - 16 * kxm;
+        const int k = j >> 4, d = j & 15;
+        const int n = isx ? W : H, cnt = isx ? fx.cnt : fy.cnt;
+        int t = (k - kl * d) % n;
+        if (t < 0) t += n;
+        const T v = t < cnt ? (isx ? cx[t] : cy[t]) : T(0);
+        (isx ? Bx : Ay)[j] = v;
+    }
+    wg_barrier();
+    const int lane = lane_id(tid), wv = wave_id(tid), nw = wave_count(nth);
+    {
+        const int nmt = (H + 15) >> 4, nnt = (ow + 15) >> 4;
+        for (int t = wv; t < nmt * nnt; t += nw) {
+            const int p0 = (t / nnt) << 4, x0 = (t % nnt) << 4;
+            const int span = kl * ((ow - 1 - x0 < 15) ? ow - 1 - x0 : 15) + fx.cnt;
+            const int len = span < W ? span : W;
+            int qs = kl * (x0 + 1) + fx.lo;
+            if (qs < 0) qs += W;
+            wave_mma16(
+                lane, (len + 3) >> 2,
+                [&](int l, int s) -> float {
+                    const int kk = 4 * s + (l >> 4), p = p0 + (l & 15);
+                    int q = qs + kk;
+                    if (q >= W) q -= W;
+                    const float v = R[(p < H ? p : H - 1) * rs + q];
+                    return (kk < len && p < H) ? v : 0.f;
+                },
+                [&](int l, int s) -> float {
+                    const int kk = 4 * s + (l >> 4), x = x0 + (l & 15);
+                    const float v = Bx[(kk < len ? kk : len - 1) * 16 + (l & 15)];
+                    return (kk < len && x < ow) ? v : 0.f;
+                },
+                [&](int l, int v, float val) {
+                    const int p = p0 + 4 * (l >> 4) + v, x = x0 + (l & 15);
+                    if (p < H && x < ow) Tb[p * ts + x] = val;
+                });
+        }
     }
     wg_barrier();
     T* o = a.out + ((long long)img * a.C + nd.leaf_ch) * oh * ow;
-    for (int e = tid; e < oh * ow; e += nth) {
-        const int y = e / ow, x = e - y * ow;
-        o[e] = fir_sum(Tb + x, ts, g.H, kl * (y + 1), cy, fy.lo, fy.cnt);
+    {
+        const int nmt = (oh + 15) >> 4, nnt = (ow + 15) >> 4;
+        for (int t = wv; t < nmt * nnt; t += nw) {
+            const int y0 = (t / nnt) << 4, x0 = (t % nnt) << 4;
+            const int span = kl * ((oh - 1 - y0 < 15) ? oh - 1 - y0 : 15) + fy.cnt;
+            const int len = span < H ? span : H;
+            int ps = kl * (y0 + 1) + fy.lo;
+            if (ps < 0) ps += H;
+            wave_mma16(
+                lane, (len + 3) >> 2,
+                [&](int l, int s) -> float {
+                    const int kk = 4 * s + (l >> 4), y = y0 + (l & 15);
+                    const float v = Ay[(kk < len ? kk : len - 1) * 16 + (l & 15)];
+                    return (kk < len && y < oh) ? v : 0.f;
+                },
+                [&](int l, int s) -> float {
+                    const int kk = 4 * s + (l >> 4), x = x0 + (l & 15);
+                    int p = ps + kk;
+                    if (p >= H) p -= H;
+                    const float v = Tb[p * ts + (x < ow ? x : ow - 1)];
+                    return (kk < len && x < ow) ? v : 0.f;
+                },
+                [&](int l, int v, float val) {
+                    const int y = y0 + 4 * (l >> 4) + v, x = x0 + (l & 15);
+                    if (y < oh && x < ow) o[y * ow + x] = val;
+                });
+        }
     }
 }
 
@@ -372,14 +516,7 @@ KYM_HD void node_body(const Scat2DArgs<T>& a, const G& g, const Node2D& nd, int 
         g.fwd2d(lds, tid, nth);
     } else {
         const Node2D& pn = a.nodes[nd.parent];
-        const cpx<T>* U = wsi + pn.slot;
-        const T* f = a.filt + a.filt_off[nd.filt];
-        const int Wp = g.W * nd.k;
-        const T inv_k2 = T(1) / T(nd.k * nd.k);
-        for (int i = tid; i < g.H * g.W; i += nth) {
-            const int y = i / g.W, x = i - (i / g.W) * g.W;
-            lds[i] = gather_periodized(U, f, Wp, g.H, g.W, nd.k, y, x) * inv_k2;
-        }
+        gather_block(lds, g.W, 0, wsi + pn.slot, a.filt + a.filt_off[nd.filt], g.W * nd.k, g.H, g.W, nd.k, tid, nth);
         wg_barrier();
         g.inv2d(lds, tid, nth);
         const T inv_n = T(1) / T(g.H * g.W);
@@ -507,11 +644,7 @@ KYM_HD void body_root_rows(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int ti
 template <class T, int MAXP>
 KYM_HD void icol_common(const Scat2DArgs<T>& a, const cpx<T>* U, const T* f, int Wp, int H, int W, int k,
                         const FftLen<T>& fH, cpx<T>* dst, int c0, int nc, cpx<T>* lds, int tid, int nth) {
-    const T inv_k2 = T(1) / T(k * k);
-    for (int i = tid; i < H * nc; i += nth) {
-        const int y = i / nc, c = i - (i / nc) * nc;
-        lds[i] = gather_periodized(U, f, Wp, H, W, k, y, c0 + c) * inv_k2;
-    }
+    gather_block(lds, nc, c0, U, f, Wp, H, W, k, tid, nth);
     wg_barrier();
     fft_dif<true, MAXP>(lds, fH, Batch{nc, nc, 1, true}, tid, nth);
     for (int i = tid; i < H * nc; i += nth) {

@@ -179,15 +179,27 @@ struct Scat2DPlan {
         for (auto& nd : nodes)
             if (nd.parent >= 0) has_children[nd.parent] = 1;
         auto node_threads = [&](size_t elems) { return elems >= 8192 ? 1024 : elems >= 2048 ? 512 : 256; };
+        // LDS bytes of a node run in one workgroup (leaf not counted for the spectral route)
+        auto node_lds = [&](const Node2D& nd) {
+            const int r = nd.res;
+            size_t b = (size_t)H[r] * W[r] * cs;
+            if (fir) {
+                const long long reals = fir_lds_reals(H[r], W[r], ow, nd.leaf_k, fir_axes[2 * r].cnt,
+                                                      fir_axes[2 * r + 1].cnt);
+                b = std::max(b, (size_t)reals * sizeof(T));
+            }
+            return b;
+        };
         ws_img = 0;
         for (size_t i = 0; i < nodes.size(); ++i) {
             Node2D& nd = nodes[i];
             const size_t elems = (size_t)H[nd.res] * W[nd.res];
-            // leaf_mode places the spectral leaf and, in training, the leaf's gradient
-            if ((elems + leaf_elems) * cs <= lds_limit) { nd.big = 0; nd.leaf_mode = 0; }
-            else if (elems * cs <= lds_limit) { nd.big = 0; nd.leaf_mode = 1; }
+            // LDS-resident if the node (and, FP32, its in-LDS leaf) fits; leaf_mode
+            // places the spectral leaf and, in training, the leaf's gradient
+            const bool fits = node_lds(nd) <= lds_limit &&
+                              (!fir || elems <= (size_t)KYM_STAGE_MAX * node_threads(elems));
+            if (fits) { nd.big = 0; nd.leaf_mode = ((elems + leaf_elems) * cs <= lds_limit) ? 0 : 1; }
             else { nd.big = 1; nd.leaf_mode = (leaf_elems * cs <= lds_limit) ? 1 : 2; }
-            if (fir && !nd.big && elems > (size_t)KYM_STAGE_MAX * node_threads(elems)) nd.big = 1;
             nd.fir = fir ? (nd.big ? 2 : 1) : 0;
             nd.fwd = (has_children[i] || !fir) ? 1 : 0;
             const bool need_slot = has_children[i] || nd.big || (!fir && nd.leaf_mode != 0);
@@ -313,7 +325,7 @@ struct Scat2DPlan {
             const int r = n0.res;
             const size_t elems = (size_t)H[r] * W[r];
             if (!n0.big) {
-                const size_t lds = (elems + (n0.fir == 0 && n0.leaf_mode == 0 ? leaf_elems : 0)) * cs;
+                const size_t lds = fir ? node_lds(n0) : (elems + (n0.leaf_mode == 0 ? leaf_elems : 0)) * cs;
                 push(K2_LDS_NODE, ids, 1, 0, node_threads(elems), lds);
                 int nt = 0;
                 const int sid = node_spec_id(H[r], W[r], H[J], W[J], &nt);
