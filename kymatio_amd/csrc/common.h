@@ -37,6 +37,31 @@ template <bool INV, class T> KYM_HD KYM_INL cpx<T> mul_mi(cpx<T> a) {
     return {a.im, -a.re};
 }
 
+// Load through a pointer that is known to address global memory.  Table
+// pointers read from the plan's blob are generic to the compiler, which then
+// emits flat loads (tied to the LDS counter); this keeps them global.
+// (Scalars and vector types only: a cast of a struct pointer to the global
+// address space is dropped by the compiler, so complex values load as a
+// two-element vector.)
+template <class U>
+KYM_HD KYM_INL U kym_ldg(const U* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return *(const __attribute__((address_space(1))) U*)p;
+#else
+    return *p;
+#endif
+}
+template <class T>
+KYM_HD KYM_INL cpx<T> kym_ldg(const cpx<T>* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    const v2 v = *(const __attribute__((address_space(1))) v2*)(const void*)p;
+    return {v.x, v.y};
+#else
+    return *p;
+#endif
+}
+
 // Workgroup barrier; a no-op when a body runs on the host with one thread.
 KYM_HD KYM_INL void wg_barrier() {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -45,7 +70,15 @@ KYM_HD KYM_INL void wg_barrier() {
 }
 
 template <class T> KYM_HD KYM_INL T kym_sqrt(T v);
-template <> KYM_HD KYM_INL float kym_sqrt<float>(float v) { return sqrtf(v); }
+// FP32: the hardware square root (v_sqrt_f32, within 1 ulp) rather than the
+// correctly rounded expansion, which costs a dozen instructions per call.
+template <> KYM_HD KYM_INL float kym_sqrt<float>(float v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_sqrtf(v);
+#else
+    return sqrtf(v);
+#endif
+}
 template <> KYM_HD KYM_INL double kym_sqrt<double>(double v) { return sqrt(v); }
 
 // numpy-style 'reflect' index (edge sample not repeated), any pad width.

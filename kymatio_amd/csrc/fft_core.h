@@ -176,7 +176,7 @@ KYM_HD void dit_pass(cpx<T>* buf, const FftLen<T>& fl, int Lp, const Batch& bt, 
         if (j != 0) {
 #pragma unroll
             for (int m = 1; m < R; ++m) {
-                const cpx<T> w = tw[j * m * tstep];
+                const cpx<T> w = kym_ldg(tw + j * m * tstep);
                 v[m] = INV ? cmulc(v[m], w) : cmul(v[m], w);
             }
         }
@@ -205,7 +205,7 @@ KYM_HD void dif_pass(cpx<T>* buf, const FftLen<T>& fl, int L, const Batch& bt, i
         if (j != 0) {
 #pragma unroll
             for (int m = 1; m < R; ++m) {
-                const cpx<T> w = tw[j * m * tstep];
+                const cpx<T> w = kym_ldg(tw + j * m * tstep);
                 v[m] = INV ? cmulc(v[m], w) : cmul(v[m], w);
             }
         }

@@ -22,7 +22,7 @@ KYM_HD KYM_INL void dit_pass_s(cpx<T>* buf, const cpx<T>* tw, int tid) {
             if (j != 0) {
 #pragma unroll
                 for (int m = 1; m < R; ++m) {
-                    const cpx<T> w = tw[j * m * TSTEP];
+                    const cpx<T> w = kym_ldg(tw + j * m * TSTEP);
                     v[m] = INV ? cmulc(v[m], w) : cmul(v[m], w);
                 }
             }
@@ -50,7 +50,7 @@ KYM_HD KYM_INL void dif_pass_s(cpx<T>* buf, const cpx<T>* tw, int tid) {
             if (j != 0) {
 #pragma unroll
                 for (int m = 1; m < R; ++m) {
-                    const cpx<T> w = tw[j * m * TSTEP];
+                    const cpx<T> w = kym_ldg(tw + j * m * TSTEP);
                     v[m] = INV ? cmulc(v[m], w) : cmul(v[m], w);
                 }
             }

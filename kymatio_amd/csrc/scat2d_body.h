@@ -129,8 +129,8 @@ KYM_HD KYM_INL cpx<T> gather_periodized_k(const cpx<T>* U, const T* f, int Wp, i
         const int row = (y + a * H) * Wp + x;
 #pragma unroll
         for (int b = 0; b < K; ++b) {
-            const T fv = f[row + b * W];
-            const cpx<T> u = U[row + b * W];
+            const T fv = kym_ldg(f + row + b * W);
+            const cpx<T> u = kym_ldg(U + row + b * W);
             acc.re += u.re * fv;
             acc.im += u.im * fv;
         }
@@ -336,7 +336,7 @@ KYM_HD void stage_modulus(const G& g, cpx<T>* lds, cpx<T>* sv, T inv_n, int tid,
         const int s = tid + k * nth;
         if (s < n) {
             const int py = s / g.W, px = s - py * g.W;
-            R[rH[py] * rs + rW[px]] = m[k];
+            R[kym_ldg(rH + py) * rs + kym_ldg(rW + px)] = m[k];
         }
     }
     wg_barrier();
@@ -364,7 +364,7 @@ KYM_HD void restage_complex(const G& g, cpx<T>* lds, int tid, int nth) {
         const int s = tid + k * nth;
         if (s < n) {
             const int py = s / g.W, px = s - py * g.W;
-            m[k] = R[rH[py] * rs + rW[px]];
+            m[k] = R[kym_ldg(rH + py) * rs + kym_ldg(rW + px)];
         }
     }
     wg_barrier();
@@ -391,97 +391,113 @@ KYM_HD KYM_INL long long fir_lds_reals(int H, int W, int ow, int kl, int cnt_y, 
     return (long long)H * (W + 1) + (long long)H * fir_tstride(W, ow) + 16ll * (kx + ky);
 }
 
-// Leaf of the node from R on the matrix cores (FP32).  Both passes are
-// GEMMs with a banded circulant factor:
+// Band length of a leaf pass: the samples one 16-output tile touches.
+KYM_HD KYM_INL int fir_kmax(int n, int kl, int cnt) { return (15 * kl + cnt < n) ? 15 * kl + cnt : n; }
+
+// band[k][d] = c[(k - kl d) mod n] (zero beyond the taps), k < kmax, d < 16:
+// inside a tile's window the circulant factor depends only on (k, d).
+template <class T>
+KYM_HD void fir_band(T* band, int kmax, int n, int kl, const T* c, int cnt, int tid, int nth) {
+    for (int i = tid; i < 16 * kmax; i += nth) {
+        const int k = i >> 4, d = i & 15;
+        int t = (k - kl * d) % n;
+        if (t < 0) t += n;
+        band[i] = t < cnt ? c[t] : T(0);
+    }
+}
+
+// Row pass on the matrix cores: for rows p < nrows of R (row stride rs,
+// natural columns q < W) and outputs x < ow,
+//   T[p][x] = sum_q R[p][q] c[(q - kl (x+1) - lo) mod W]
+// handed to out(p, x, value).  Bx = fir_band(.., W, kl, c, cnt).
+template <class T, class OUT>
+KYM_HD void fir_rows_mma(const T* R, int rs, int nrows, int W, int kl, int ow, const FirAxis& fx, const T* Bx,
+                         OUT out, int tid, int nth) {
+    const int lane = lane_id(tid), wv = wave_id(tid), nw = wave_count(nth);
+    const int nmt = (nrows + 15) >> 4, nnt = (ow + 15) >> 4;
+    for (int t = wv; t < nmt * nnt; t += nw) {
+        const int p0 = (t / nnt) << 4, x0 = (t % nnt) << 4;
+        const int span = kl * ((ow - 1 - x0 < 15) ? ow - 1 - x0 : 15) + fx.cnt;
+        const int len = span < W ? span : W;
+        int qs = kl * (x0 + 1) + fx.lo;
+        if (qs < 0) qs += W;
+        wave_mma16(
+            lane, (len + 3) >> 2,
+            [&](int l, int s) -> float {
+                const int kk = 4 * s + (l >> 4), p = p0 + (l & 15);
+                int q = qs + kk;
+                if (q >= W) q -= W;
+                const float v = R[(p < nrows ? p : nrows - 1) * rs + q];
+                return (kk < len && p < nrows) ? v : 0.f;
+            },
+            [&](int l, int s) -> float {
+                const int kk = 4 * s + (l >> 4), x = x0 + (l & 15);
+                const float v = Bx[(kk < len ? kk : len - 1) * 16 + (l & 15)];
+                return (kk < len && x < ow) ? v : 0.f;
+            },
+            [&](int l, int v, float val) {
+                const int p = p0 + 4 * (l >> 4) + v, x = x0 + (l & 15);
+                if (p < nrows && x < ow) out(p, x, val);
+            });
+    }
+}
+
+// Column pass on the matrix cores: S[y][x] = sum_p c[(p - kl (y+1) - lo) mod H] Tb[p][x]
+// for y < oh, x < ow (Tb row stride ts), into o[y * ow + x].  Ay = fir_band(.., H, kl, c, cnt).
+template <class T>
+KYM_HD void fir_cols_mma(const T* Tb, int ts, int H, int kl, int oh, int ow, const FirAxis& fy, const T* Ay, T* o,
+                         int tid, int nth) {
+    const int lane = lane_id(tid), wv = wave_id(tid), nw = wave_count(nth);
+    const int nmt = (oh + 15) >> 4, nnt = (ow + 15) >> 4;
+    for (int t = wv; t < nmt * nnt; t += nw) {
+        const int y0 = (t / nnt) << 4, x0 = (t % nnt) << 4;
+        const int span = kl * ((oh - 1 - y0 < 15) ? oh - 1 - y0 : 15) + fy.cnt;
+        const int len = span < H ? span : H;
+        int ps = kl * (y0 + 1) + fy.lo;
+        if (ps < 0) ps += H;
+        wave_mma16(
+            lane, (len + 3) >> 2,
+            [&](int l, int s) -> float {
+                const int kk = 4 * s + (l >> 4), y = y0 + (l & 15);
+                const float v = Ay[(kk < len ? kk : len - 1) * 16 + (l & 15)];
+                return (kk < len && y < oh) ? v : 0.f;
+            },
+            [&](int l, int s) -> float {
+                const int kk = 4 * s + (l >> 4), x = x0 + (l & 15);
+                int p = ps + kk;
+                if (p >= H) p -= H;
+                const float v = Tb[p * ts + (x < ow ? x : ow - 1)];
+                return (kk < len && x < ow) ? v : 0.f;
+            },
+            [&](int l, int v, float val) {
+                const int y = y0 + 4 * (l >> 4) + v, x = x0 + (l & 15);
+                if (y < oh && x < ow) o[y * ow + x] = val;
+            });
+    }
+}
+
+// Leaf of an LDS node from R (FP32, matrix cores).  Both passes are GEMMs
+// with a banded circulant factor:
 //   row pass     Tb[p][x] = sum_q R[p][q] c_x[(q - kl (x+1) - lo_x) mod W]
 //   column pass  S[y][x]  = sum_p c_y[(p - kl (y+1) - lo_y) mod H] Tb[p][x]
 // Each 16 x 16 output tile only needs the window of 15 kl + cnt input
-// samples its 16 outputs touch (the whole circle when the taps cover it);
-// inside a window the factor depends only on (k, output offset d < 16), so
-// it is tabulated once per node as band[k][d] = c[(k - kl d) mod n] (zero
-// beyond the taps).  Callers barrier before reusing the buffer.
+// samples its 16 outputs touch (the whole circle when the taps cover it).
+// Callers barrier before reusing the buffer.
 template <class T, class G>
 KYM_HD void fir_leaf_lds(const Scat2DArgs<T>& a, const G& g, const Node2D& nd, T* R, int img, int tid, int nth) {
     const int r = nd.res, ow = a.ow, oh = a.oh, kl = nd.leaf_k;
     const int H = g.H, W = g.W, rs = W + 1, ts = fir_tstride(W, ow);
     const FirAxis fy = a.fir[2 * r], fx = a.fir[2 * r + 1];
-    const T* cy = a.filt + fy.off;
-    const T* cx = a.filt + fx.off;
-    const int kxm = (15 * kl + fx.cnt < W) ? 15 * kl + fx.cnt : W;
-    const int kym = (15 * kl + fy.cnt < H) ? 15 * kl + fy.cnt : H;
+    const int kxm = fir_kmax(W, kl, fx.cnt), kym = fir_kmax(H, kl, fy.cnt);
     T* Tb = R + H * rs;
     T* Bx = Tb + H * ts;
     T* Ay = Bx + 16 * kxm;
-    for (int i = tid; i < 16 * (kxm + kym); i += nth) {
-        const bool isx = i < 16 * kxm;
-        const int j = isx ? i : i - 16 * kxm;
-        const int k = j >> 4, d = j & 15;
-        const int n = isx ? W : H, cnt = isx ? fx.cnt : fy.cnt;
-        int t = (k - kl * d) % n;
-        if (t < 0) t += n;
-        const T v = t < cnt ? (isx ? cx[t] : cy[t]) : T(0);
-        (isx ? Bx : Ay)[j] = v;
-    }
+    fir_band(Bx, kxm, W, kl, a.filt + fx.off, fx.cnt, tid, nth);
+    fir_band(Ay, kym, H, kl, a.filt + fy.off, fy.cnt, tid, nth);
     wg_barrier();
-    const int lane = lane_id(tid), wv = wave_id(tid), nw = wave_count(nth);
-    {
-        const int nmt = (H + 15) >> 4, nnt = (ow + 15) >> 4;
-        for (int t = wv; t < nmt * nnt; t += nw) {
-            const int p0 = (t / nnt) << 4, x0 = (t % nnt) << 4;
-            const int span = kl * ((ow - 1 - x0 < 15) ? ow - 1 - x0 : 15) + fx.cnt;
-            const int len = span < W ? span : W;
-            int qs = kl * (x0 + 1) + fx.lo;
-            if (qs < 0) qs += W;
-            wave_mma16(
-                lane, (len + 3) >> 2,
-                [&](int l, int s) -> float {
-                    const int kk = 4 * s + (l >> 4), p = p0 + (l & 15);
-                    int q = qs + kk;
-                    if (q >= W) q -= W;
-                    const float v = R[(p < H ? p : H - 1) * rs + q];
-                    return (kk < len && p < H) ? v : 0.f;
-                },
-                [&](int l, int s) -> float {
-                    const int kk = 4 * s + (l >> 4), x = x0 + (l & 15);
-                    const float v = Bx[(kk < len ? kk : len - 1) * 16 + (l & 15)];
-                    return (kk < len && x < ow) ? v : 0.f;
-                },
-                [&](int l, int v, float val) {
-                    const int p = p0 + 4 * (l >> 4) + v, x = x0 + (l & 15);
-                    if (p < H && x < ow) Tb[p * ts + x] = val;
-                });
-        }
-    }
+    fir_rows_mma(R, rs, H, W, kl, ow, fx, Bx, [&](int p, int x, float v) { Tb[p * ts + x] = v; }, tid, nth);
     wg_barrier();
-    T* o = a.out + ((long long)img * a.C + nd.leaf_ch) * oh * ow;
-    {
-        const int nmt = (oh + 15) >> 4, nnt = (ow + 15) >> 4;
-        for (int t = wv; t < nmt * nnt; t += nw) {
-            const int y0 = (t / nnt) << 4, x0 = (t % nnt) << 4;
-            const int span = kl * ((oh - 1 - y0 < 15) ? oh - 1 - y0 : 15) + fy.cnt;
-            const int len = span < H ? span : H;
-            int ps = kl * (y0 + 1) + fy.lo;
-            if (ps < 0) ps += H;
-            wave_mma16(
-                lane, (len + 3) >> 2,
-                [&](int l, int s) -> float {
-                    const int kk = 4 * s + (l >> 4), y = y0 + (l & 15);
-                    const float v = Ay[(kk < len ? kk : len - 1) * 16 + (l & 15)];
-                    return (kk < len && y < oh) ? v : 0.f;
-                },
-                [&](int l, int s) -> float {
-                    const int kk = 4 * s + (l >> 4), x = x0 + (l & 15);
-                    int p = ps + kk;
-                    if (p >= H) p -= H;
-                    const float v = Tb[p * ts + (x < ow ? x : ow - 1)];
-                    return (kk < len && x < ow) ? v : 0.f;
-                },
-                [&](int l, int v, float val) {
-                    const int y = y0 + 4 * (l >> 4) + v, x = x0 + (l & 15);
-                    if (y < oh && x < ow) o[y * ow + x] = val;
-                });
-        }
-    }
+    fir_cols_mma(Tb, ts, H, kl, oh, ow, fy, Ay, a.out + ((long long)img * a.C + nd.leaf_ch) * oh * ow, tid, nth);
 }
 
 // ------------------------------------------------------------------ bodies
@@ -586,18 +602,18 @@ KYM_HD void body_leaf(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, in
 }
 
 // Row pass of a big node's FIR leaf: rows rr < nr of R (natural columns,
-// row stride rs) are the node's rows rev[r0 + rr]; T[y][x] goes to the tslot.
+// row stride rs) are the node's rows rev[r0 + rr]; T[y][x] goes to the
+// tslot.  Bx: band table space after R (fir_kmax(W, ..) x 16 reals).
 template <class T>
 KYM_HD void fir_rows_to_slot(const Scat2DArgs<T>& a, const Node2D& nd, const T* R, int rs, int nr, int r0,
-                             const int* rev, cpx<T>* wsi, int tid, int nth) {
+                             const int* rev, cpx<T>* wsi, T* Bx, int tid, int nth) {
     const int r = nd.res, W = a.W[r], ow = a.ow, kl = nd.leaf_k;
     const FirAxis fx = a.fir[2 * r + 1];
-    const T* cx = a.filt + fx.off;
+    fir_band(Bx, fir_kmax(W, kl, fx.cnt), W, kl, a.filt + fx.off, fx.cnt, tid, nth);
+    wg_barrier();
     T* Tg = reinterpret_cast<T*>(wsi + nd.tslot);
-    for (int e = tid; e < nr * ow; e += nth) {
-        const int rr = e / ow, x = e - rr * ow;
-        Tg[(long long)rev[r0 + rr] * ow + x] = fir_sum(R + rr * rs, 1, W, kl * (x + 1), cx, fx.lo, fx.cnt);
-    }
+    fir_rows_mma(R, rs, nr, W, kl, ow, fx, Bx, [&](int p, int x, float v) { Tg[(long long)rev[r0 + p] * ow + x] = v; },
+                 tid, nth);
 }
 
 // Root row pass: rows of the padded image (row storage positions
@@ -626,7 +642,7 @@ KYM_HD void body_root_rows(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int ti
             const int rr = i / W, px = i - rr * W;
             lds[i] = {R[rr * rs + fW.rev[px]], T(0)};
         }
-        fir_rows_to_slot(a, nd, R, rs, nr, r0, fH.rev, a.ws + il * a.ws_img, tid, nth);
+        fir_rows_to_slot(a, nd, R, rs, nr, r0, fH.rev, a.ws + il * a.ws_img, R + nr * rs, tid, nth);
     } else {
         for (int i = tid; i < nr * W; i += nth) {
             const int rr = i / W, px = i - rr * W;
@@ -712,27 +728,35 @@ KYM_HD void body_rows_mod(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid
         }
     }
     wg_barrier();
-    if constexpr (kym_fir<T>) fir_rows_to_slot(a, nd, R, rs, nr, r0, a.lens[a.lenH[r]].rev, wsi, tid, nth);
+    if constexpr (kym_fir<T>) fir_rows_to_slot(a, nd, R, rs, nr, r0, a.lens[a.lenH[r]].rev, wsi, R + nr * rs, tid, nth);
     if (!nd.fwd) return;
     fft_dit<false, MAXP>(lds, fW, Batch{nr, 1, W, false}, tid, nth);
     for (int i = tid; i < nr * W; i += nth) buf[i] = lds[i];
 }
 
-// Column pass of a big node's FIR leaf: tslot -> output channel.
-// task -> (image, node).
+// Column pass of a big node's FIR leaf: tslot -> LDS (row stride
+// fir_tstride) -> output channel.  task -> (image, node).
 template <class T, int MAXP>
 KYM_HD void body_fir_cols(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
-    (void)lds;
-    const int il = task / a.ngroup, gi = task - il * a.ngroup;
-    const Node2D& nd = a.nodes[a.group[gi]];
-    const int r = nd.res, H = a.H[r], ow = a.ow, oh = a.oh, kl = nd.leaf_k;
-    const FirAxis fy = a.fir[2 * r];
-    const T* cy = a.filt + fy.off;
-    const T* Tg = reinterpret_cast<const T*>(a.ws + il * a.ws_img + nd.tslot);
-    T* o = a.out + ((long long)(a.img0 + il) * a.C + nd.leaf_ch) * oh * ow;
-    for (int e = tid; e < oh * ow; e += nth) {
-        const int y = e / ow, x = e - y * ow;
-        o[e] = fir_sum(Tg + x, ow, H, kl * (y + 1), cy, fy.lo, fy.cnt);
+    if constexpr (kym_fir<T>) {
+        const int il = task / a.ngroup, gi = task - il * a.ngroup;
+        const Node2D& nd = a.nodes[a.group[gi]];
+        const int r = nd.res, H = a.H[r], W = a.W[r], ow = a.ow, oh = a.oh, kl = nd.leaf_k;
+        const FirAxis fy = a.fir[2 * r];
+        const int ts = fir_tstride(W, ow), kym = fir_kmax(H, kl, fy.cnt);
+        T* Tb = reinterpret_cast<T*>(lds);
+        T* Ay = Tb + H * ts;
+        const T* Tg = reinterpret_cast<const T*>(a.ws + il * a.ws_img + nd.tslot);
+        for (int i = tid; i < H * ow; i += nth) {
+            const int p = i / ow, x = i - p * ow;
+            Tb[p * ts + x] = Tg[i];
+        }
+        fir_band(Ay, kym, H, kl, a.filt + fy.off, fy.cnt, tid, nth);
+        wg_barrier();
+        fir_cols_mma(Tb, ts, H, kl, oh, ow, fy, Ay, a.out + ((long long)(a.img0 + il) * a.C + nd.leaf_ch) * oh * ow,
+                     tid, nth);
+    } else {
+        (void)a; (void)task; (void)lds; (void)tid; (void)nth;
     }
 }
 

@@ -299,10 +299,20 @@ struct Scat2DPlan {
             return std::max(1, std::min(16, cb));
         };
         // row passes of FIR nodes also keep the block's rows as reals (stride W + 1)
+        // and the leaf's band table along x
         auto row_bytes = [&](int Ww) { return cs * Ww + (fir ? sizeof(T) * (Ww + 1) : 0); };
-        auto row_blk = [&](int Ww) {
-            int rb = (int)(pass_budget / row_bytes(Ww));
+        auto band_bytes = [&](int r) {
+            return fir ? sizeof(T) * 16 * (size_t)fir_kmax(W[r], 1 << (J - r), fir_axes[2 * r + 1].cnt) : 0;
+        };
+        auto row_blk = [&](int r) {
+            int rb = (int)((pass_budget - std::min(pass_budget / 2, band_bytes(r))) / row_bytes(W[r]));
             return std::max(1, std::min(16, rb));
+        };
+        auto row_lds = [&](int r, int rb) { return (size_t)rb * row_bytes(W[r]) + band_bytes(r); };
+        // column pass of a big node's leaf: its row-filtered field plus the band table along y
+        auto fir_cols_lds = [&](int r) {
+            return sizeof(T) * ((size_t)H[r] * fir_tstride(W[r], ow) +
+                                16 * (size_t)fir_kmax(H[r], 1 << (J - r), fir_axes[2 * r].cnt));
         };
         auto emit_leaves = [&](const std::vector<int>& ids) {
             std::vector<int> small, big, firc;
@@ -311,7 +321,7 @@ struct Scat2DPlan {
                 else if (nodes[id].fir == 0 && nodes[id].leaf_mode == 1) small.push_back(id);
                 else if (nodes[id].fir == 0 && nodes[id].leaf_mode == 2) big.push_back(id);
             }
-            if (!firc.empty()) push(K2_FIR_COLS, firc, 1, 0, 256, 0);
+            if (!firc.empty()) push(K2_FIR_COLS, firc, 1, 0, 256, fir_cols_lds(nodes[firc[0]].res));
             if (!small.empty()) push(K2_LEAF, small, 1, 0, node_threads(leaf_elems), leaf_elems * cs);
             if (!big.empty()) {
                 const int cb = col_blk(H[J]), rb = std::max(1, std::min(16, (int)(pass_budget / (cs * W[J]))));
@@ -334,11 +344,11 @@ struct Scat2DPlan {
                     launches.back().threads = nt;
                 }
             } else {
-                const int cb = col_blk(H[r]), rb = row_blk(W[r]);
-                if (is_root) push(K2_ROOT_ROWS, ids, (H[r] + rb - 1) / rb, rb, 256, (size_t)rb * row_bytes(W[r]));
+                const int cb = col_blk(H[r]), rb = row_blk(r);
+                if (is_root) push(K2_ROOT_ROWS, ids, (H[r] + rb - 1) / rb, rb, 256, row_lds(r, rb));
                 else {
                     push(K2_ICOL, ids, (W[r] + cb - 1) / cb, cb, 256, (size_t)H[r] * cb * cs);
-                    push(K2_ROWS_MOD, ids, (H[r] + rb - 1) / rb, rb, 256, (size_t)rb * row_bytes(W[r]));
+                    push(K2_ROWS_MOD, ids, (H[r] + rb - 1) / rb, rb, 256, row_lds(r, rb));
                 }
                 if (n0.fwd) push(K2_FWD_COLS, ids, (W[r] + cb - 1) / cb, cb, 256, (size_t)H[r] * cb * cs);
             }
@@ -369,7 +379,7 @@ struct Scat2DPlan {
                 const size_t lds = (elems + (n0.leaf_mode == 0 ? leaf_elems : 0)) * cs;
                 push(K2B_NODE, ids, 1, 0, node_threads(elems), lds);
             } else {
-                const int cb = col_blk(H[r]), rb = row_blk(W[r]);
+                const int cb = col_blk(H[r]), rb = std::max(1, std::min(16, (int)(pass_budget / (cs * W[r]))));
                 push(K2B_GCOLS, ids, (W[r] + cb - 1) / cb, cb, 256, (size_t)H[r] * cb * cs);
                 push(K2B_GROWS, ids, (H[r] + rb - 1) / rb, rb, 256, (size_t)rb * W[r] * cs);
                 if (n0.order > 0) push(K2B_GFCOLS, ids, (W[r] + cb - 1) / cb, cb, 256, (size_t)H[r] * cb * cs);
