@@ -12,7 +12,7 @@ __device__ KYM_INL int xcd_task(int bid, int nblocks) {
 }
 
 template <class T, int KIND, int NT, int MAXP>
-__global__ void __launch_bounds__(NT) k_scat2d(Scat2DArgs<T> a) {
+__global__ void __launch_bounds__(NT, 2) k_scat2d(Scat2DArgs<T> a) {
     extern __shared__ __attribute__((aligned(16))) char kym_smem2d[];
     const int task = xcd_task(blockIdx.x, gridDim.x);
     if (task >= a.ntask) return;

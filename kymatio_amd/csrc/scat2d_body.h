@@ -139,22 +139,23 @@ KYM_HD KYM_INL cpx<T> gather_periodized_k(const cpx<T>* U, const T* f, int Wp, i
 }
 
 // dst[y * ncols + c] = scale * P_k(U * f)[y][c0 + c] for y < H, c < ncols.
-// Four elements per thread and round: their loads are all issued before the
-// first use, so the memory latency is paid once per four.
+// E = 4 / K elements per thread and round: their 2 K^2 loads each are all
+// issued before the first use, so the memory latency is paid once per round.
 template <int K, class T>
-KYM_HD void gather_block_k(cpx<T>* dst, int ncols, int c0, const cpx<T>* U, const T* f, int Wp, int H, int W,
-                           T scale, int tid, int nth) {
+KYM_HD KYM_INL void gather_block_k(cpx<T>* dst, int ncols, int c0, const cpx<T>* U, const T* f, int Wp, int H,
+                                   int W, T scale, int tid, int nth) {
+    constexpr int E = K >= 4 ? 1 : 4 / K;
     const int n = H * ncols;
     int i = tid;
-    for (; i + 3 * nth < n; i += 4 * nth) {
-        cpx<T> v[4];
+    for (; i + (E - 1) * nth < n; i += E * nth) {
+        cpx<T> v[E];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < E; ++u) {
             const int e = i + u * nth, y = e / ncols, c = e - y * ncols;
             v[u] = gather_periodized_k<K>(U, f, Wp, H, W, y, c0 + c);
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) dst[i + u * nth] = v[u] * scale;
+        for (int u = 0; u < E; ++u) dst[i + u * nth] = v[u] * scale;
     }
     for (; i < n; i += nth) {
         const int y = i / ncols, c = i - y * ncols;
@@ -163,7 +164,7 @@ KYM_HD void gather_block_k(cpx<T>* dst, int ncols, int c0, const cpx<T>* U, cons
 }
 
 template <class T>
-KYM_HD void gather_block(cpx<T>* dst, int ncols, int c0, const cpx<T>* U, const T* f, int Wp, int H, int W, int k,
+KYM_HD KYM_INL void gather_block(cpx<T>* dst, int ncols, int c0, const cpx<T>* U, const T* f, int Wp, int H, int W, int k,
                          int tid, int nth) {
     const T scale = T(1) / T(k * k);
     switch (k) {
@@ -658,7 +659,7 @@ KYM_HD void body_root_rows(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int ti
 // Gather + inverse column DIF.  Shared by big nodes (filter psi, slot) and
 // big leaves (filter phi, leaf_slot).  task -> (image, node, column block).
 template <class T, int MAXP>
-KYM_HD void icol_common(const Scat2DArgs<T>& a, const cpx<T>* U, const T* f, int Wp, int H, int W, int k,
+KYM_HD KYM_INL void icol_common(const Scat2DArgs<T>& a, const cpx<T>* U, const T* f, int Wp, int H, int W, int k,
                         const FftLen<T>& fH, cpx<T>* dst, int c0, int nc, cpx<T>* lds, int tid, int nth) {
     gather_block(lds, nc, c0, U, f, Wp, H, W, k, tid, nth);
     wg_barrier();
