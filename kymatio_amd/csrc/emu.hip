@@ -17,6 +17,19 @@ void kym_set_error(const std::string& msg) { g_emu_error = msg; }
 
 extern "C" const char* kym_emu_last_error(void) { return g_emu_error.c_str(); }
 
+// A launch with a compile-time specialisation, one thread per workgroup.
+template <class T>
+static void emu_spec(const Launch2D& l, const Scat2DArgs<T>& a, int t, cpx<T>* lds) {
+    switch (l.kind) {
+        case K2_LDS_NODE: dispatch_node_spec<T, true>(l.spec, a, t, lds, 0); break;
+        case K2_ROOT_ROWS: dispatch_pass_spec<T, K2_ROOT_ROWS, true>(l.spec, a, t, lds, 0); break;
+        case K2_ICOL: dispatch_pass_spec<T, K2_ICOL, true>(l.spec, a, t, lds, 0); break;
+        case K2_ROWS_MOD: dispatch_pass_spec<T, K2_ROWS_MOD, true>(l.spec, a, t, lds, 0); break;
+        case K2_FWD_COLS: dispatch_pass_spec<T, K2_FWD_COLS, true>(l.spec, a, t, lds, 0); break;
+        default: break;
+    }
+}
+
 template <class T>
 static int emu_forward2d(int M, int N, int J, int L, int max_order, int pre_pad, long long lds_limit, const void* x,
                          const void* filt, void* out, long long batch, long long chunk, long long* info) {
@@ -51,7 +64,7 @@ static int emu_forward2d(int M, int N, int J, int L, int max_order, int pre_pad,
             a.img0 = (int)i0;
             a.ntask = nimg * l.ngroup * l.nblk;
             for (int t = 0; t < a.ntask; ++t) {
-                if (l.spec >= 0 && l.kind == K2_LDS_NODE) dispatch_node_spec<T, true>(l.spec, a, t, lds.data(), 0);
+                if (l.spec >= 0) emu_spec(l, a, t, lds.data());
                 else dispatch_body2d<T, 31>(l.kind, a, t, lds.data(), 0, 1);
             }
         }
@@ -92,7 +105,7 @@ static int emu_train2d(int M, int N, int J, int L, int max_order, int pre_pad, l
             a.ntask = (int)(batch * l.ngroup * l.nblk);
             for (int t = 0; t < a.ntask; ++t) {
                 if (l.kind >= K2B_NODE) dispatch_bwd2d<T, 31>(l.kind, a, t, lds.data(), 0, 1);
-                else if (l.spec >= 0 && l.kind == K2_LDS_NODE) dispatch_node_spec<T, true>(l.spec, a, t, lds.data(), 0);
+                else if (l.spec >= 0) emu_spec(l, a, t, lds.data());
                 else dispatch_body2d<T, 31>(l.kind, a, t, lds.data(), 0, 1);
             }
         }

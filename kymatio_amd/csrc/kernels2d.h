@@ -29,6 +29,14 @@ __global__ void __launch_bounds__(NT) k_node2d_spec(Scat2DArgs<T> a) {
     dispatch_node_spec<T, false>(SPEC, a, task, reinterpret_cast<cpx<T>*>(kym_smem2d_ns), threadIdx.x);
 }
 
+template <class T, int KIND, int SPEC, int NT>
+__global__ void __launch_bounds__(NT, 2) k_pass2d_spec(Scat2DArgs<T> a) {
+    extern __shared__ __attribute__((aligned(16))) char kym_smem2d_ps[];
+    const int task = xcd_task(blockIdx.x, gridDim.x);
+    if (task >= a.ntask) return;
+    dispatch_pass_spec<T, KIND, false>(SPEC, a, task, reinterpret_cast<cpx<T>*>(kym_smem2d_ps), threadIdx.x);
+}
+
 template <class T>
 using Scat2DKernel = void (*)(Scat2DArgs<T>);
 
@@ -40,6 +48,29 @@ Scat2DKernel<T> pick_node_spec(int id) {
     case sid: return k_node2d_spec<T, sid, nt>;
         KYM_NODE_SPECS(KYM_NS_K)
 #undef KYM_NS_K
+        default: return nullptr;
+    }
+}
+
+template <class T, int KIND>
+Scat2DKernel<T> pick_pass_spec_k(int id) {
+    if constexpr (sizeof(T) != 4) return nullptr;
+    else switch (id) {
+#define KYM_PS_K(sid, h, w, c, r, nt) \
+    case sid: return k_pass2d_spec<T, KIND, sid, nt>;
+        KYM_PASS_SPECS(KYM_PS_K)
+#undef KYM_PS_K
+        default: return nullptr;
+    }
+}
+
+template <class T>
+Scat2DKernel<T> pick_pass_spec(int kind, int id) {
+    switch (kind) {
+        case K2_ROOT_ROWS: return pick_pass_spec_k<T, K2_ROOT_ROWS>(id);
+        case K2_ICOL: return pick_pass_spec_k<T, K2_ICOL>(id);
+        case K2_ROWS_MOD: return pick_pass_spec_k<T, K2_ROWS_MOD>(id);
+        case K2_FWD_COLS: return pick_pass_spec_k<T, K2_FWD_COLS>(id);
         default: return nullptr;
     }
 }

@@ -401,9 +401,9 @@ template <class T>
 KYM_HD void fir_band(T* band, int kmax, int n, int kl, const T* c, int cnt, int tid, int nth) {
     for (int i = tid; i < 16 * kmax; i += nth) {
         const int k = i >> 4, d = i & 15;
-        int t = (k - kl * d) % n;
-        if (t < 0) t += n;
-        band[i] = t < cnt ? c[t] : T(0);
+        int t = k - kl * d;  // k < n, so one or two additions of n make it non-negative in practice
+        while (t < 0) t += n;
+        band[i] = t < cnt ? kym_ldg(c + t) : T(0);
     }
 }
 
@@ -602,6 +602,43 @@ KYM_HD void body_leaf(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, in
     leaf_body(a, dyn_geom<T, MAXP>(a, r), U, phi, nd.leaf_k, nd.leaf_ch, a.img0 + il, lds, tid, nth);
 }
 
+// ------------------------------------------------------------------ pass FFTs
+// Transforms of the big-node passes over a block of nc columns (stride nc,
+// column pass) or nr rows (row pass).  DynPass takes the lengths at run time;
+// StatPass fixes lengths, block sizes and thread count at compile time (the
+// plan selects it only when every block of the launch is full).
+template <class T, int MAXP>
+struct DynPass {
+    KYM_HD void col_inv(cpx<T>* b, const FftLen<T>& f, int nc, int tid, int nth) const {
+        fft_dif<true, MAXP>(b, f, Batch{nc, nc, 1, true}, tid, nth);
+    }
+    KYM_HD void col_fwd(cpx<T>* b, const FftLen<T>& f, int nc, int tid, int nth) const {
+        fft_dit<false, MAXP>(b, f, Batch{nc, nc, 1, true}, tid, nth);
+    }
+    KYM_HD void row_inv(cpx<T>* b, const FftLen<T>& f, int nr, int tid, int nth) const {
+        fft_dif<true, MAXP>(b, f, Batch{nr, 1, f.n, false}, tid, nth);
+    }
+    KYM_HD void row_fwd(cpx<T>* b, const FftLen<T>& f, int nr, int tid, int nth) const {
+        fft_dit<false, MAXP>(b, f, Batch{nr, 1, f.n, false}, tid, nth);
+    }
+};
+
+template <class T, int H, int W, int CB, int RB, int NT>
+struct StatPass {
+    KYM_HD void col_inv(cpx<T>* b, const FftLen<T>& f, int, int tid, int) const {
+        fft_dif_s<true, H, CB, CB, 1, true, NT>(b, f.tw, tid);
+    }
+    KYM_HD void col_fwd(cpx<T>* b, const FftLen<T>& f, int, int tid, int) const {
+        fft_dit_s<false, H, CB, CB, 1, true, NT>(b, f.tw, tid);
+    }
+    KYM_HD void row_inv(cpx<T>* b, const FftLen<T>& f, int, int tid, int) const {
+        fft_dif_s<true, W, RB, 1, W, false, NT>(b, f.tw, tid);
+    }
+    KYM_HD void row_fwd(cpx<T>* b, const FftLen<T>& f, int, int tid, int) const {
+        fft_dit_s<false, W, RB, 1, W, false, NT>(b, f.tw, tid);
+    }
+};
+
 // Row pass of a big node's FIR leaf: rows rr < nr of R (natural columns,
 // row stride rs) are the node's rows rev[r0 + rr]; T[y][x] goes to the
 // tslot.  Bx: band table space after R (fir_kmax(W, ..) x 16 reals).
@@ -621,8 +658,8 @@ KYM_HD void fir_rows_to_slot(const Scat2DArgs<T>& a, const Node2D& nd, const T* 
 // digit-reversed), forward DIT along x.  task -> (image, row block).  With a
 // FIR leaf the block's rows are first loaded in natural order as reals (after
 // the complex rows in LDS); the leaf's row pass goes to the node's tslot.
-template <class T, int MAXP>
-KYM_HD void body_root_rows(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
+template <class T, class P>
+KYM_HD void body_root_rows(const P& fp, const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
     const int il = task / a.nblk, rb = task - il * a.nblk;
     const int img = a.img0 + il;
     const Node2D& nd = a.nodes[0];
@@ -651,27 +688,27 @@ KYM_HD void body_root_rows(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int ti
         }
     }
     wg_barrier();
-    fft_dit<false, MAXP>(lds, fW, Batch{nr, 1, W, false}, tid, nth);
+    fp.row_fwd(lds, fW, nr, tid, nth);
     cpx<T>* dst = a.ws + il * a.ws_img + nd.slot + (long long)r0 * W;
     for (int i = tid; i < nr * W; i += nth) dst[i] = lds[i];
 }
 
 // Gather + inverse column DIF.  Shared by big nodes (filter psi, slot) and
 // big leaves (filter phi, leaf_slot).  task -> (image, node, column block).
-template <class T, int MAXP>
-KYM_HD KYM_INL void icol_common(const Scat2DArgs<T>& a, const cpx<T>* U, const T* f, int Wp, int H, int W, int k,
+template <class T, class P>
+KYM_HD KYM_INL void icol_common(const P& fp, const Scat2DArgs<T>& a, const cpx<T>* U, const T* f, int Wp, int H, int W, int k,
                         const FftLen<T>& fH, cpx<T>* dst, int c0, int nc, cpx<T>* lds, int tid, int nth) {
     gather_block(lds, nc, c0, U, f, Wp, H, W, k, tid, nth);
     wg_barrier();
-    fft_dif<true, MAXP>(lds, fH, Batch{nc, nc, 1, true}, tid, nth);
+    fp.col_inv(lds, fH, nc, tid, nth);
     for (int i = tid; i < H * nc; i += nth) {
         const int p = i / nc, c = i - (i / nc) * nc;
         dst[(long long)p * W + c0 + c] = lds[i];
     }
 }
 
-template <class T, int MAXP>
-KYM_HD void body_icol(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
+template <class T, class P>
+KYM_HD void body_icol(const P& fp, const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
     const int per = a.ngroup * a.nblk;
     const int il = task / per, rem = task - il * per;
     const int gi = rem / a.nblk, cb = rem - gi * a.nblk;
@@ -680,12 +717,13 @@ KYM_HD void body_icol(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, in
     const int r = nd.res, H = a.H[r], W = a.W[r];
     const int c0 = cb * a.blk, nc = (W - c0 < a.blk) ? W - c0 : a.blk;
     cpx<T>* wsi = a.ws + il * a.ws_img;
-    icol_common<T, MAXP>(a, wsi + pn.slot, a.filt + a.filt_off[nd.filt], a.W[pn.res], H, W, nd.k, a.lens[a.lenH[r]],
+    icol_common<T>(fp, a, wsi + pn.slot, a.filt + a.filt_off[nd.filt], a.W[pn.res], H, W, nd.k, a.lens[a.lenH[r]],
                 wsi + nd.slot, c0, nc, lds, tid, nth);
 }
 
 template <class T, int MAXP>
 KYM_HD void body_leaf_icol(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
+    const DynPass<T, MAXP> fp;
     const int per = a.ngroup * a.nblk;
     const int il = task / per, rem = task - il * per;
     const int gi = rem / a.nblk, cb = rem - gi * a.nblk;
@@ -693,15 +731,15 @@ KYM_HD void body_leaf_icol(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int ti
     const int J = a.J, HJ = a.H[J], WJ = a.W[J];
     const int c0 = cb * a.blk, nc = (WJ - c0 < a.blk) ? WJ - c0 : a.blk;
     cpx<T>* wsi = a.ws + il * a.ws_img;
-    icol_common<T, MAXP>(a, wsi + nd.slot, a.filt + a.filt_off[nd.leaf_filt], a.W[nd.res], HJ, WJ, nd.leaf_k,
+    icol_common<T>(fp, a, wsi + nd.slot, a.filt + a.filt_off[nd.leaf_filt], a.W[nd.res], HJ, WJ, nd.leaf_k,
                 a.lens[a.lenH[J]], wsi + nd.leaf_slot, c0, nc, lds, tid, nth);
 }
 
 // Rows: inverse DIF along x, modulus, forward DIT along x (in place).  With
 // a FIR leaf the modulus rows are also kept as natural reals after the complex
 // rows, for the leaf's row pass; nodes without children stop there.
-template <class T, int MAXP>
-KYM_HD void body_rows_mod(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
+template <class T, class P>
+KYM_HD void body_rows_mod(const P& fp, const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
     const int per = a.ngroup * a.nblk;
     const int il = task / per, rem = task - il * per;
     const int gi = rem / a.nblk, rb = rem - gi * a.nblk;
@@ -713,7 +751,7 @@ KYM_HD void body_rows_mod(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid
     cpx<T>* buf = wsi + nd.slot + (long long)r0 * W;
     for (int i = tid; i < nr * W; i += nth) lds[i] = buf[i];
     wg_barrier();
-    fft_dif<true, MAXP>(lds, fW, Batch{nr, 1, W, false}, tid, nth);
+    fp.row_inv(lds, fW, nr, tid, nth);
     const T inv_n = T(1) / T(H * W);
     cpx<T>* sv = (a.save && nd.save >= 0) ? wsi + nd.save + (long long)r0 * W : nullptr;
     T* R = reinterpret_cast<T*>(lds + nr * W);
@@ -731,7 +769,7 @@ KYM_HD void body_rows_mod(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid
     wg_barrier();
     if constexpr (kym_fir<T>) fir_rows_to_slot(a, nd, R, rs, nr, r0, a.lens[a.lenH[r]].rev, wsi, R + nr * rs, tid, nth);
     if (!nd.fwd) return;
-    fft_dit<false, MAXP>(lds, fW, Batch{nr, 1, W, false}, tid, nth);
+    fp.row_fwd(lds, fW, nr, tid, nth);
     for (int i = tid; i < nr * W; i += nth) buf[i] = lds[i];
 }
 
@@ -786,8 +824,8 @@ KYM_HD void body_leaf_rows(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int ti
 }
 
 // Forward DIT along y over column blocks (rows stored digit-reversed) -> natural.
-template <class T, int MAXP>
-KYM_HD void body_fwd_cols(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
+template <class T, class P>
+KYM_HD void body_fwd_cols(const P& fp, const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
     const int per = a.ngroup * a.nblk;
     const int il = task / per, rem = task - il * per;
     const int gi = rem / a.nblk, cb = rem - gi * a.nblk;
@@ -801,7 +839,7 @@ KYM_HD void body_fwd_cols(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid
         lds[i] = buf[(long long)y * W + c];
     }
     wg_barrier();
-    fft_dit<false, MAXP>(lds, fH, Batch{nc, nc, 1, true}, tid, nth);
+    fp.col_fwd(lds, fH, nc, tid, nth);
     for (int i = tid; i < H * nc; i += nth) {
         const int y = i / nc, c = i - y * nc;
         buf[(long long)y * W + c] = lds[i];
@@ -838,19 +876,54 @@ KYM_HD void dispatch_node_spec(int id, const Scat2DArgs<T>& a, int task, cpx<T>*
     }
 }
 
+// Compile-time specialisations X(id, H, W, col block, row block, threads) of
+// the big-node passes (FP32): root rows, inverse columns, rows + modulus,
+// forward columns.
+#define KYM_PASS_SPECS(X) X(0, 272, 272, 16, 8, 256)
+
+inline int pass_spec_id(int H, int W, int cb, int rb, int* threads) {
+#define KYM_PS_MATCH(id, h, w, c, r, nt) \
+    if (H == h && W == w && cb == c && rb == r) { *threads = nt; return id; }
+    KYM_PASS_SPECS(KYM_PS_MATCH)
+#undef KYM_PS_MATCH
+    return -1;
+}
+
 enum Kind2D {
     K2_LDS_NODE = 0, K2_LEAF, K2_ROOT_ROWS, K2_ICOL, K2_ROWS_MOD, K2_FWD_COLS, K2_LEAF_ICOL, K2_LEAF_ROWS, K2_FIR_COLS
 };
+
+template <class T, int KIND, class P>
+KYM_HD KYM_INL void dispatch_pass(const P& fp, const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
+    if constexpr (KIND == K2_ROOT_ROWS) body_root_rows<T>(fp, a, task, lds, tid, nth);
+    else if constexpr (KIND == K2_ICOL) body_icol<T>(fp, a, task, lds, tid, nth);
+    else if constexpr (KIND == K2_ROWS_MOD) body_rows_mod<T>(fp, a, task, lds, tid, nth);
+    else if constexpr (KIND == K2_FWD_COLS) body_fwd_cols<T>(fp, a, task, lds, tid, nth);
+}
+
+// Static pass body: EMU = true runs it with one thread (CPU harness).
+template <class T, int KIND, bool EMU>
+KYM_HD void dispatch_pass_spec(int id, const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid) {
+    switch (id) {
+#define KYM_PS_CASE(sid, h, w, c, r, nt)                                                          \
+    case sid:                                                                                     \
+        dispatch_pass<T, KIND>(StatPass<T, h, w, c, r, (EMU ? 1 : nt)>(), a, task, lds, tid, EMU ? 1 : nt); \
+        break;
+        KYM_PASS_SPECS(KYM_PS_CASE)
+#undef KYM_PS_CASE
+        default: break;
+    }
+}
 
 template <class T, int MAXP>
 KYM_HD KYM_INL void dispatch_body2d(int kind, const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
     switch (kind) {
         case K2_LDS_NODE: body_lds_node<T, MAXP>(a, task, lds, tid, nth); break;
         case K2_LEAF: body_leaf<T, MAXP>(a, task, lds, tid, nth); break;
-        case K2_ROOT_ROWS: body_root_rows<T, MAXP>(a, task, lds, tid, nth); break;
-        case K2_ICOL: body_icol<T, MAXP>(a, task, lds, tid, nth); break;
-        case K2_ROWS_MOD: body_rows_mod<T, MAXP>(a, task, lds, tid, nth); break;
-        case K2_FWD_COLS: body_fwd_cols<T, MAXP>(a, task, lds, tid, nth); break;
+        case K2_ROOT_ROWS: body_root_rows<T>(DynPass<T, MAXP>(), a, task, lds, tid, nth); break;
+        case K2_ICOL: body_icol<T>(DynPass<T, MAXP>(), a, task, lds, tid, nth); break;
+        case K2_ROWS_MOD: body_rows_mod<T>(DynPass<T, MAXP>(), a, task, lds, tid, nth); break;
+        case K2_FWD_COLS: body_fwd_cols<T>(DynPass<T, MAXP>(), a, task, lds, tid, nth); break;
         case K2_LEAF_ICOL: body_leaf_icol<T, MAXP>(a, task, lds, tid, nth); break;
         case K2_LEAF_ROWS: body_leaf_rows<T, MAXP>(a, task, lds, tid, nth); break;
         case K2_FIR_COLS: body_fir_cols<T, MAXP>(a, task, lds, tid, nth); break;

@@ -304,9 +304,10 @@ struct Scat2DPlan {
         auto band_bytes = [&](int r) {
             return fir ? sizeof(T) * 16 * (size_t)fir_kmax(W[r], 1 << (J - r), fir_axes[2 * r + 1].cnt) : 0;
         };
+        // FIR row passes hold 8 rows at most, so that four of their workgroups share a CU
         auto row_blk = [&](int r) {
             int rb = (int)((pass_budget - std::min(pass_budget / 2, band_bytes(r))) / row_bytes(W[r]));
-            return std::max(1, std::min(16, rb));
+            return std::max(1, std::min(fir ? 8 : 16, rb));
         };
         auto row_lds = [&](int r, int rb) { return (size_t)rb * row_bytes(W[r]) + band_bytes(r); };
         // column pass of a big node's leaf: its row-filtered field plus the band table along y
@@ -345,12 +346,20 @@ struct Scat2DPlan {
                 }
             } else {
                 const int cb = col_blk(H[r]), rb = row_blk(r);
-                if (is_root) push(K2_ROOT_ROWS, ids, (H[r] + rb - 1) / rb, rb, 256, row_lds(r, rb));
+                // compile-time passes when the sizes match and every block is full
+                int pnt = 0;
+                const int psid = (use_specs && sizeof(T) == 4 && H[r] % rb == 0 && W[r] % cb == 0)
+                                     ? pass_spec_id(H[r], W[r], cb, rb, &pnt) : -1;
+                auto pass = [&](int kind, int nblk, int blk, size_t lds) {
+                    push(kind, ids, nblk, blk, 256, lds);
+                    if (psid >= 0) { launches.back().spec = psid; launches.back().threads = pnt; }
+                };
+                if (is_root) pass(K2_ROOT_ROWS, H[r] / rb + (H[r] % rb != 0), rb, row_lds(r, rb));
                 else {
-                    push(K2_ICOL, ids, (W[r] + cb - 1) / cb, cb, 256, (size_t)H[r] * cb * cs);
-                    push(K2_ROWS_MOD, ids, (H[r] + rb - 1) / rb, rb, 256, row_lds(r, rb));
+                    pass(K2_ICOL, W[r] / cb + (W[r] % cb != 0), cb, (size_t)H[r] * cb * cs);
+                    pass(K2_ROWS_MOD, H[r] / rb + (H[r] % rb != 0), rb, row_lds(r, rb));
                 }
-                if (n0.fwd) push(K2_FWD_COLS, ids, (W[r] + cb - 1) / cb, cb, 256, (size_t)H[r] * cb * cs);
+                if (n0.fwd) pass(K2_FWD_COLS, W[r] / cb + (W[r] % cb != 0), cb, (size_t)H[r] * cb * cs);
             }
             emit_leaves(ids);
         };
