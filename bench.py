@@ -31,6 +31,21 @@ METRIC = 'images/s Scattering2D J=3 L=8 256² fp32 (%HBM roofline); recon time t
 J, L, SIDE, BATCH, SEED = 3, 8, 256, 64, 2
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, 'profiles', 'scat2d_j3_256_hbm_traffic.md')
+
+
+def measured_traffic_per_node_image():
+    """HBM bytes per (image, node) of the dominant launch, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None."""
+    try:
+        for line in open(TRAFFIC_PROFILE):
+            if line.startswith('traffic_bytes_per_image_per_node:'):
+                return int(line.split(':', 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(budget_s=15.0, max_images=2048):
     from oracle import scat2d
     from kymatio_amd import workloads
@@ -129,6 +144,10 @@ def main():
                            'gbps': bytes_per_launch / (mean_ms * 1e-3) / 1e9})
     dom = max(per_kernel, key=lambda r: r['total_ms'])
     achieved = dom['model_bytes_per_launch'] / (dom['mean_us'] * 1e-6)
+    per_node_image = measured_traffic_per_node_image()
+    traffic = None
+    if per_node_image is not None and dom['kind'] == 'lds_node' and dom['res'] == 1 and dom['nodes'] == 64:
+        traffic = per_node_image * dom['nodes'] * BATCH * args.steps / dom['launches']
     total_model = sum(l['model_bytes_per_image'] for l in launches) * BATCH
     gpu_ms = sum(r['total_ms'] for r in per_kernel) / args.steps
     res = {
@@ -140,7 +159,9 @@ def main():
                    'batch': BATCH, 'J': J, 'L': L, 'shape': [SIDE, SIDE], 'images_per_chunk': chunk,
                    'seed': SEED},
         'roofline': {'bound': 'hbm', 'achieved': achieved / 1e9, 'peak': HBM_PEAK / 1e9, 'unit': 'GB/s',
-                     'frac': achieved / HBM_PEAK, 'traffic': None,
+                     'frac': achieved / HBM_PEAK, 'traffic': traffic,
+                     'traffic_source': 'profiles/scat2d_j3_256_hbm_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, '
+                                       'copy-calibrated), bytes per launch' if traffic is not None else None,
                      'kernel': '%s (res %d, %d nodes/image)' % (dom['kind'], dom['res'], dom['nodes']),
                      'mean_launch_us': dom['mean_us'], 'model_bytes_per_launch': dom['model_bytes_per_launch']},
         'pipeline_roofline': {'model_bytes_per_image': total_model / BATCH,
