@@ -48,6 +48,14 @@ int kym_version(void);
  */
 int kym_scat2d_create(int M, int N, int J, int L, int max_order, int pre_pad, int dtype,
                       long long lds_limit, void** plan);
+
+/* kym_scat2d_create with tuning choices (the autotuner's knobs; no
+ * counterpart in the reference): tune[0..ntune) = {largest power-of-two FFT
+ * radix (16/8/4/2), LDS-node threads (256/512/1024), big-node column block,
+ * big-node row block}; 0 = built-in heuristic.  Results are the same for
+ * every choice up to FP32 rounding. */
+int kym_scat2d_create_tuned(int M, int N, int J, int L, int max_order, int pre_pad, int dtype,
+                            long long lds_limit, const int* tune, int ntune, void** plan);
 int kym_scat2d_destroy(void* plan);
 
 /* info[12] = {C, out_h, out_w, M_padded, N_padded, n_filters, filter_elems,
@@ -102,6 +110,20 @@ int kym_scat2d_backward(void* plan, const void* dS, const void* filt, void* gx, 
  * once); it feeds the roofline figure of the benchmark.  No counterpart in
  * the reference (measurement support). */
 int kym_scat2d_launch_info(void* plan, int index, long long* rec);
+
+/* Mixed-size batches (no counterpart in the reference, whose modules take one
+ * image size each): one executable HIP graph over the inference forwards of
+ * nplans plans (one per image size) bound to fixed device buffers: x[i],
+ * filt[i], out[i], batch[i] images and a workspace ws[i] of ws_bytes[i] per
+ * plan, as kym_scat2d_forward takes them.  Each plan's launches form one
+ * dependency chain; the chains are independent branches of the graph, so
+ * sizes run concurrently.  graph_launch replays it on a stream; the plans
+ * and buffers must outlive the graph. */
+int kym_scat2d_graph_create(void* const* plans, int nplans, const void* const* x, const void* const* filt,
+                            void* const* out, const long long* batch, void* const* ws, const long long* ws_bytes,
+                            void** graph);
+int kym_scat2d_graph_launch(void* graph, void* stream);
+int kym_scat2d_graph_destroy(void* graph);
 
 /* Per-launch HIP-event timing of subsequent kym_scat2d_forward calls on this
  * plan (enable 1/0; enabling clears earlier records).  profile_read waits

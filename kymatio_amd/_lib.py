@@ -28,6 +28,12 @@ _SIGNATURES = {
     'kym_scat2d_info': [_c_vp, ctypes.POINTER(_c_ll)],
     'kym_scat2d_filter_offsets': [_c_vp, ctypes.POINTER(_c_ll)],
     'kym_scat2d_leaf_taps': [_c_vp, ctypes.POINTER(_c_ll), _c_i],
+    'kym_scat2d_create_tuned': [_c_i] * 7 + [_c_ll, ctypes.POINTER(_c_i), _c_i, ctypes.POINTER(_c_vp)],
+    'kym_scat2d_graph_create': [ctypes.POINTER(_c_vp), _c_i] + [ctypes.POINTER(_c_vp)] * 3
+                               + [ctypes.POINTER(_c_ll), ctypes.POINTER(_c_vp), ctypes.POINTER(_c_ll),
+                                  ctypes.POINTER(_c_vp)],
+    'kym_scat2d_graph_launch': [_c_vp, _c_vp],
+    'kym_scat2d_graph_destroy': [_c_vp],
     'kym_scat2d_forward': [_c_vp, _c_vp, _c_vp, _c_vp, _c_ll, _c_vp, _c_ll, _c_vp],
     'kym_scat2d_forward_train': [_c_vp, _c_vp, _c_vp, _c_vp, _c_ll, _c_vp, _c_ll, _c_vp],
     'kym_scat2d_backward': [_c_vp, _c_vp, _c_vp, _c_vp, _c_ll, _c_vp, _c_ll, _c_vp],

@@ -88,11 +88,16 @@ static int upload_and_prepare(Scat2DPlan<T>& p, void** dblob) {
     return KYM_OK;
 }
 
-extern "C" int kym_scat2d_create(int M, int N, int J, int L, int max_order, int pre_pad, int dtype,
-                                 long long lds_limit, void** out) {
+extern "C" int kym_scat2d_create_tuned(int M, int N, int J, int L, int max_order, int pre_pad, int dtype,
+                                       long long lds_limit, const int* tune, int ntune, void** out) {
     if (!out) return fail(KYM_ERR_VALUE, "null plan pointer");
     *out = nullptr;
     if (dtype != KYM_F32 && dtype != KYM_F64) return fail(KYM_ERR_TYPE, "dtype must be float32 or float64");
+    if (ntune < 0 || ntune > 4 || (ntune > 0 && !tune)) return fail(KYM_ERR_VALUE, "tuning array must hold 0-4 values");
+    Scat2DTune t;
+    int* fields[4] = {&t.max_pow2, &t.node_threads, &t.col_block, &t.row_block};
+    for (int i = 0; i < ntune; ++i)
+        if (tune[i] != 0) *fields[i] = tune[i];
     auto h = std::make_unique<KymScat2D>();
     h->dtype = dtype;
     KYM_HIP_TRY(hipGetDevice(&h->device));
@@ -102,10 +107,12 @@ extern "C" int kym_scat2d_create(int M, int N, int J, int L, int max_order, int 
     int rc;
     if (dtype == KYM_F32) {
         h->pf = std::make_unique<Scat2DPlan<float>>();
+        h->pf->tune = t;
         if (!h->pf->build(M, N, J, L, max_order, pre_pad, lim, &e)) return fail(e.code, e.msg);
         rc = upload_and_prepare(*h->pf, &h->dblob);
     } else {
         h->pd = std::make_unique<Scat2DPlan<double>>();
+        h->pd->tune = t;
         if (!h->pd->build(M, N, J, L, max_order, pre_pad, lim, &e)) return fail(e.code, e.msg);
         rc = upload_and_prepare(*h->pd, &h->dblob);
     }
@@ -115,6 +122,11 @@ extern "C" int kym_scat2d_create(int M, int N, int J, int L, int max_order, int 
     }
     *out = h.release();
     return KYM_OK;
+}
+
+extern "C" int kym_scat2d_create(int M, int N, int J, int L, int max_order, int pre_pad, int dtype,
+                                 long long lds_limit, void** out) {
+    return kym_scat2d_create_tuned(M, N, J, L, max_order, pre_pad, dtype, lds_limit, nullptr, 0, out);
 }
 
 extern "C" int kym_scat2d_destroy(void* plan) {
@@ -186,43 +198,79 @@ extern "C" int kym_scat2d_leaf_taps(void* plan, long long* desc, int n) {
 
 // mode 0: inference forward (batch in chunks); 1: training forward (saves
 // u, whole batch in the workspace); 2: backward from the saved workspace.
+// The kernel, grid and arguments of one launch of a forward chunk.
+template <class T>
+struct Launch2DCall {
+    Scat2DKernel<T> k;
+    unsigned grid, threads;
+    size_t lds;
+    Scat2DArgs<T> a;
+};
+
+template <class T>
+static int launch_call(const Scat2DPlan<T>& p, const Launch2D& l, int mode, const void* x, const void* filt,
+                       void* out, const void* dS, void* gx, long long img_elems, void* ws, long long i0, int nimg,
+                       uintptr_t base, Launch2DCall<T>* c) {
+    Scat2DArgs<T> a = p.args(l, base);
+    a.x = static_cast<const T*>(x);
+    a.filt = static_cast<const T*>(filt);
+    a.ws = static_cast<cpx<T>*>(ws);
+    a.ws_img = img_elems;
+    a.out = static_cast<T*>(out);
+    a.dS = static_cast<const T*>(dS);
+    a.gx = static_cast<T*>(gx);
+    a.save = mode == 1 ? 1 : 0;
+    a.img0 = (int)i0;
+    const long long ntask = (long long)nimg * l.ngroup * l.nblk;
+    if (ntask > (1ll << 30)) return fail(KYM_ERR_VALUE, "too many tasks in one launch");
+    a.ntask = (int)ntask;
+    c->grid = (unsigned)((ntask + 7) / 8 * 8);
+    c->threads = (unsigned)l.threads;
+    c->lds = l.lds_bytes;
+    c->k = pick_scat2d<T>(l.kind, l.threads, p.pclass, l.spec);
+    if (!c->k) return fail(KYM_ERR_UNSUPPORTED, "no kernel for launch kind");
+    c->a = a;
+    return KYM_OK;
+}
+
+// Images per chunk of a forward over `batch` images with ws_bytes of workspace.
+template <class T>
+static int chunk_of(const Scat2DPlan<T>& p, int mode, long long batch, long long ws_bytes, long long* chunk) {
+    const long long img_elems = mode == 0 ? p.ws_img : p.ws_img_train;
+    const long long per = img_elems * (long long)sizeof(cpx<T>);
+    if (mode != 0 && !p.trainable) return fail(KYM_ERR_UNSUPPORTED, p.train_error);
+    long long c = per > 0 ? ws_bytes / per : batch;
+    if (c < 1) return fail(KYM_ERR_VALUE, "workspace too small for one image");
+    if (c > batch) c = batch;
+    if (mode != 0 && c < batch) return fail(KYM_ERR_VALUE, "training workspace must hold the whole batch");
+    *chunk = c;
+    return KYM_OK;
+}
+
+// mode 0: inference forward (batch in chunks); 1: training forward (saves
+// u, whole batch in the workspace); 2: backward from the saved workspace.
 template <class T>
 static int run2d(const Scat2DPlan<T>& p, int mode, const void* x, const void* filt, void* out, const void* dS,
                  void* gx, long long batch, void* ws, long long ws_bytes, hipStream_t stream, uintptr_t base,
                  LaunchProfile* prof) {
     const long long img_elems = mode == 0 ? p.ws_img : p.ws_img_train;
-    const long long per = img_elems * (long long)sizeof(cpx<T>);
-    if (mode != 0 && !p.trainable) return fail(KYM_ERR_UNSUPPORTED, p.train_error);
-    long long chunk = per > 0 ? ws_bytes / per : batch;
-    if (chunk < 1) return fail(KYM_ERR_VALUE, "workspace too small for one image");
-    if (chunk > batch) chunk = batch;
-    if (mode != 0 && chunk < batch) return fail(KYM_ERR_VALUE, "training workspace must hold the whole batch");
+    long long chunk = 0;
+    int rc = chunk_of(p, mode, batch, ws_bytes, &chunk);
+    if (rc != KYM_OK) return rc;
     const std::vector<Launch2D>& launches = mode == 2 ? p.bwd_launches : p.launches;
     for (long long i0 = 0; i0 < batch; i0 += chunk) {
         const int nimg = (int)std::min(chunk, batch - i0);
         for (const Launch2D& l : launches) {
-            Scat2DArgs<T> a = p.args(l, base);
-            a.x = static_cast<const T*>(x);
-            a.filt = static_cast<const T*>(filt);
-            a.ws = static_cast<cpx<T>*>(ws);
-            a.ws_img = img_elems;
-            a.out = static_cast<T*>(out);
-            a.dS = static_cast<const T*>(dS);
-            a.gx = static_cast<T*>(gx);
-            a.save = mode == 1 ? 1 : 0;
-            a.img0 = (int)i0;
-            const long long ntask = (long long)nimg * l.ngroup * l.nblk;
-            if (ntask > (1ll << 30)) return fail(KYM_ERR_VALUE, "too many tasks in one launch");
-            a.ntask = (int)ntask;
-            const unsigned grid = (unsigned)((ntask + 7) / 8 * 8);
-            Scat2DKernel<T> k = pick_scat2d<T>(l.kind, l.threads, p.pclass, l.spec);
+            Launch2DCall<T> c;
+            rc = launch_call(p, l, mode, x, filt, out, dS, gx, img_elems, ws, i0, nimg, base, &c);
+            if (rc != KYM_OK) return rc;
             hipEvent_t e0 = nullptr, e1 = nullptr;
             if (prof && prof->enabled) {
                 e0 = prof->get();
                 e1 = prof->get();
                 if (e0) KYM_HIP_TRY(hipEventRecord(e0, stream));
             }
-            hipLaunchKernelGGL(k, dim3(grid), dim3(l.threads), l.lds_bytes, stream, a);
+            hipLaunchKernelGGL(c.k, dim3(c.grid), dim3(c.threads), c.lds, stream, c.a);
             KYM_HIP_TRY(hipGetLastError());
             if (e0 && e1) {
                 KYM_HIP_TRY(hipEventRecord(e1, stream));
@@ -314,6 +362,98 @@ extern "C" int kym_scat2d_profile_read(void* plan, double* ms, long long* count,
         if (r.first >= 0 && r.first < n) { ms[r.first] += t; count[r.first] += 1; }
     }
     h->prof.reset();
+    return KYM_OK;
+}
+
+// ----------------------------------------------------------------- 2D graphs
+// One executable HIP graph over the inference forwards of several plans (one
+// per image size), bound to fixed buffers: each plan's launches form one
+// dependency chain, chains are independent branches.
+struct KymScat2DGraph {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int device = 0;
+    ~KymScat2DGraph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (g) (void)hipGraphDestroy(g);
+    }
+};
+
+template <class T>
+static int add_plan_chain(hipGraph_t g, const Scat2DPlan<T>& p, const void* x, const void* filt, void* out,
+                          long long batch, void* ws, long long ws_bytes, uintptr_t base) {
+    long long chunk = 0;
+    int rc = chunk_of(p, 0, batch, ws_bytes, &chunk);
+    if (rc != KYM_OK) return rc;
+    hipGraphNode_t prev = nullptr;
+    for (long long i0 = 0; i0 < batch; i0 += chunk) {
+        const int nimg = (int)std::min(chunk, batch - i0);
+        for (const Launch2D& l : p.launches) {
+            Launch2DCall<T> c;
+            rc = launch_call(p, l, 0, x, filt, out, nullptr, nullptr, p.ws_img, ws, i0, nimg, base, &c);
+            if (rc != KYM_OK) return rc;
+            void* args[1] = {&c.a};
+            hipKernelNodeParams kp = {};
+            kp.func = reinterpret_cast<void*>(c.k);
+            kp.gridDim = dim3(c.grid);
+            kp.blockDim = dim3(c.threads);
+            kp.sharedMemBytes = (unsigned)c.lds;
+            kp.kernelParams = args;
+            kp.extra = nullptr;
+            hipGraphNode_t node;
+            KYM_HIP_TRY(hipGraphAddKernelNode(&node, g, prev ? &prev : nullptr, prev ? 1 : 0, &kp));
+            prev = node;
+        }
+    }
+    return KYM_OK;
+}
+
+extern "C" int kym_scat2d_graph_create(void* const* plans, int nplans, const void* const* x, const void* const* filt,
+                                       void* const* out, const long long* batch, void* const* ws,
+                                       const long long* ws_bytes, void** graph) {
+    if (!graph) return fail(KYM_ERR_VALUE, "null graph pointer");
+    *graph = nullptr;
+    if (nplans < 1 || !plans || !x || !filt || !out || !batch || !ws || !ws_bytes)
+        return fail(KYM_ERR_VALUE, "null argument");
+    auto G = std::make_unique<KymScat2DGraph>();
+    KYM_HIP_TRY(hipGetDevice(&G->device));
+    KYM_HIP_TRY(hipGraphCreate(&G->g, 0));
+    for (int i = 0; i < nplans; ++i) {
+        KymScat2D* h = static_cast<KymScat2D*>(plans[i]);
+        if (!h) return fail(KYM_ERR_VALUE, "null plan");
+        if (h->device != G->device) return fail(KYM_ERR_TYPE, "plan was created for another device");
+        if (batch[i] < 0) return fail(KYM_ERR_VALUE, "negative batch");
+        if (batch[i] == 0) continue;
+        if (!x[i] || !filt[i] || !out[i] || !ws[i]) return fail(KYM_ERR_VALUE, "null device pointer");
+        const int rc = h->pf ? add_plan_chain(G->g, *h->pf, x[i], filt[i], out[i], batch[i], ws[i], ws_bytes[i],
+                                              (uintptr_t)h->dblob)
+                             : add_plan_chain(G->g, *h->pd, x[i], filt[i], out[i], batch[i], ws[i], ws_bytes[i],
+                                              (uintptr_t)h->dblob);
+        if (rc != KYM_OK) return rc;
+    }
+    KYM_HIP_TRY(hipGraphInstantiate(&G->exec, G->g, nullptr, nullptr, 0));
+    *graph = G.release();
+    return KYM_OK;
+}
+
+extern "C" int kym_scat2d_graph_launch(void* graph, void* stream) {
+    if (!graph) return fail(KYM_ERR_VALUE, "null graph");
+    KymScat2DGraph* G = static_cast<KymScat2DGraph*>(graph);
+    int cur = 0;
+    KYM_HIP_TRY(hipGetDevice(&cur));
+    if (cur != G->device) return fail(KYM_ERR_TYPE, "graph was created for another device");
+    KYM_HIP_TRY(hipGraphLaunch(G->exec, static_cast<hipStream_t>(stream)));
+    return KYM_OK;
+}
+
+extern "C" int kym_scat2d_graph_destroy(void* graph) {
+    if (!graph) return KYM_OK;
+    KymScat2DGraph* G = static_cast<KymScat2DGraph*>(graph);
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(G->device);
+    delete G;
+    (void)hipSetDevice(cur);
     return KYM_OK;
 }
 

@@ -32,8 +32,12 @@ static void emu_spec(const Launch2D& l, const Scat2DArgs<T>& a, int t, cpx<T>* l
 
 template <class T>
 static int emu_forward2d(int M, int N, int J, int L, int max_order, int pre_pad, long long lds_limit, const void* x,
-                         const void* filt, void* out, long long batch, long long chunk, long long* info) {
+                         const void* filt, void* out, long long batch, long long chunk, long long* info,
+                         const int* tune = nullptr, int ntune = 0) {
     Scat2DPlan<T> p;
+    int* fields[4] = {&p.tune.max_pow2, &p.tune.node_threads, &p.tune.col_block, &p.tune.row_block};
+    for (int i = 0; i < ntune && i < 4; ++i)
+        if (tune[i] != 0) *fields[i] = tune[i];
     KymError e = {0, ""};
     if (!p.build(M, N, J, L, max_order, pre_pad, (size_t)lds_limit, &e)) {
         g_emu_error = e.msg;
@@ -151,6 +155,18 @@ extern "C" int kym_emu_scat2d_taps(int M, int N, int J, int L, int max_order, in
                                    long long lds_limit, long long* desc, int n) {
     if (dtype == 0) return emu_taps2d<float>(M, N, J, L, max_order, pre_pad, lds_limit, desc, n);
     return emu_taps2d<double>(M, N, J, L, max_order, pre_pad, lds_limit, desc, n);
+}
+
+// As kym_emu_scat2d_forward with plan tuning choices (kym_scat2d_create_tuned).
+extern "C" int kym_emu_scat2d_forward_tuned(int M, int N, int J, int L, int max_order, int pre_pad, int dtype,
+                                            long long lds_limit, const void* x, const void* filt, void* out,
+                                            long long batch, long long chunk, long long* info, const int* tune,
+                                            int ntune) {
+    if (dtype == 0)
+        return emu_forward2d<float>(M, N, J, L, max_order, pre_pad, lds_limit, x, filt, out, batch, chunk, info,
+                                    tune, ntune);
+    return emu_forward2d<double>(M, N, J, L, max_order, pre_pad, lds_limit, x, filt, out, batch, chunk, info, tune,
+                                 ntune);
 }
 
 extern "C" int kym_emu_scat2d_forward(int M, int N, int J, int L, int max_order, int pre_pad, int dtype,

@@ -279,6 +279,8 @@ KYM_HD void fft_dif(cpx<T>* buf, const FftLen<T>& fl, const Batch& bt, int tid, 
 
 // Factor n into supported radices (DIT order): odd primes first (their
 // butterflies then need no twiddles), then powers of two as 16/8/4/2.
+// max_pow2 (16, 8, 4 or 2) caps the power-of-two radices (a tuning choice;
+// 16 is the default every compile-time specialisation assumes).
 // constexpr so compile-time specialisations use exactly the runtime order.
 struct RadList {
     int n = 0;
@@ -287,7 +289,7 @@ struct RadList {
     bool ok = false;
 };
 
-KYM_HD constexpr RadList kym_factor_list(int n) {
+KYM_HD constexpr RadList kym_factor_list(int n, int max_pow2 = 16) {
     RadList rl;
     rl.n = n;
     int k = 0;
@@ -304,13 +306,19 @@ KYM_HD constexpr RadList kym_factor_list(int n) {
     int a = 0;
     while (m % 2 == 0) { m /= 2; ++a; }
     if (m != 1) return rl;
+    const int dmax = max_pow2 >= 16 ? 4 : max_pow2 >= 8 ? 3 : max_pow2 >= 4 ? 2 : 1;
     while (a > 0) {
-        int r = 2, d = 1;
-        if (a >= 8 || a == 4 || a == 7) { r = 16; d = 4; }
-        else if (a == 6 || a == 3 || a == 5) { r = 8; d = 3; }
-        else if (a == 2) { r = 4; d = 2; }
+        int d = 1;
+        if (dmax == 4) {
+            if (a >= 8 || a == 4 || a == 7) d = 4;
+            else if (a == 6 || a == 3 || a == 5) d = 3;
+            else if (a == 2) d = 2;
+        } else {
+            d = a < dmax ? a : dmax;
+            if (a - d == 1 && d > 2) d -= 1;  // 4 x 4 rather than 8 x 2
+        }
         if (k >= KYM_MAX_RADICES) return rl;
-        rl.rad[k++] = r;
+        rl.rad[k++] = 1 << d;
         a -= d;
     }
     if (n == 1) rl.rad[k++] = 1;
@@ -319,8 +327,8 @@ KYM_HD constexpr RadList kym_factor_list(int n) {
     return rl;
 }
 
-inline bool kym_factor(int n, int* rad, int* nrad) {
-    const RadList rl = kym_factor_list(n);
+inline bool kym_factor(int n, int* rad, int* nrad, int max_pow2 = 16) {
+    const RadList rl = kym_factor_list(n, max_pow2);
     if (!rl.ok) return false;
     for (int i = 0; i < rl.nrad; ++i) rad[i] = rl.rad[i];
     *nrad = rl.nrad;

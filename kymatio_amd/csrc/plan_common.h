@@ -87,6 +87,7 @@ struct Blob {
 template <class T>
 struct LenTable {
     std::vector<int> ns;
+    int max_pow2 = 16;  // largest power-of-two radix (tuning)
     int index_of(int n) const {
         for (size_t i = 0; i < ns.size(); ++i)
             if (ns[i] == n) return (int)i;
@@ -106,7 +107,7 @@ struct LenTable {
             const int n = ns[i];
             FftLen<T>& f = fl[i];
             f.n = n;
-            if (!kym_factor(n, f.rad, &f.nrad)) {
+            if (!kym_factor(n, f.rad, &f.nrad, max_pow2)) {
                 *err = "FFT length " + std::to_string(n) + " has a prime factor above 31 (unsupported)";
                 return false;
             }

@@ -20,6 +20,15 @@ struct Launch2D {
     int spec;               // compile-time specialisation id, -1 = run-time sizes
 };
 
+// Tuning choices of a plan (0 = the built-in heuristic).  Any non-default
+// choice turns the compile-time specialisations off for what it affects.
+struct Scat2DTune {
+    int max_pow2 = 16;      // largest power-of-two FFT radix: 16, 8, 4 or 2
+    int node_threads = 0;   // threads of LDS-node workgroups: 256, 512 or 1024
+    int col_block = 0;      // columns per workgroup of the big-node column passes
+    int row_block = 0;      // rows per workgroup of the big-node row passes
+};
+
 template <class T>
 struct Scat2DPlan {
     // configuration
@@ -60,6 +69,7 @@ struct Scat2DPlan {
     }
 
     bool use_specs = true;  // select compile-time specialised kernels when sizes match (FP32 only)
+    Scat2DTune tune;
     // leaf taps: 2 J axes ([2 r] along y, [2 r + 1] along x) in the packed
     // filter buffer after the filters, at taps_off, taps_len reals in all
     std::vector<FirAxis> fir_axes;
@@ -88,6 +98,16 @@ struct Scat2DPlan {
     bool build(int M_, int N_, int J_, int L_, int max_order_, int pre_pad_, size_t lds_limit_, KymError* e) {
         M = M_; N = N_; J = J_; L = L_; max_order = max_order_; pre_pad = pre_pad_;
         lds_limit = lds_limit_;
+        if (!(tune.max_pow2 == 16 || tune.max_pow2 == 8 || tune.max_pow2 == 4 || tune.max_pow2 == 2) ||
+            !(tune.node_threads == 0 || tune.node_threads == 256 || tune.node_threads == 512 ||
+              tune.node_threads == 1024) ||
+            tune.col_block < 0 || tune.col_block > 64 || tune.row_block < 0 || tune.row_block > 64) {
+            *e = {KYM_ERR_VALUE, "invalid tuning choice"};
+            return false;
+        }
+        lens = LenTable<T>();
+        lens.max_pow2 = tune.max_pow2;
+        const bool specs = use_specs && tune.max_pow2 == 16;
         if (J < 0 || L < 1 || M < 1 || N < 1) { *e = {KYM_ERR_VALUE, "invalid J, L or shape"}; return false; }
         if (J + 1 > KYM_MAX_RES) { *e = {KYM_ERR_UNSUPPORTED, "J too large"}; return false; }
         if ((1 << J) > M || (1 << J) > N) {
@@ -178,7 +198,10 @@ struct Scat2DPlan {
         std::vector<char> has_children(nodes.size(), 0);
         for (auto& nd : nodes)
             if (nd.parent >= 0) has_children[nd.parent] = 1;
-        auto node_threads = [&](size_t elems) { return elems >= 8192 ? 1024 : elems >= 2048 ? 512 : 256; };
+        auto node_threads = [&](size_t elems) {
+            if (tune.node_threads) return tune.node_threads;
+            return elems >= 8192 ? 1024 : elems >= 2048 ? 512 : 256;
+        };
         // LDS bytes of a node run in one workgroup (leaf not counted for the spectral route)
         auto node_lds = [&](const Node2D& nd) {
             const int r = nd.res;
@@ -296,7 +319,8 @@ struct Scat2DPlan {
         const size_t pass_budget = std::min(lds_limit, (size_t)64 * 1024);
         auto col_blk = [&](int Hh) {
             int cb = (int)(pass_budget / (cs * Hh));
-            return std::max(1, std::min(16, cb));
+            if (tune.col_block) cb = std::min(cb, tune.col_block);
+            return std::max(1, std::min(tune.col_block ? tune.col_block : 16, cb));
         };
         // row passes of FIR nodes also keep the block's rows as reals (stride W + 1)
         // and the leaf's band table along x
@@ -307,7 +331,8 @@ struct Scat2DPlan {
         // FIR row passes hold 8 rows at most, so that four of their workgroups share a CU
         auto row_blk = [&](int r) {
             int rb = (int)((pass_budget - std::min(pass_budget / 2, band_bytes(r))) / row_bytes(W[r]));
-            return std::max(1, std::min(fir ? 8 : 16, rb));
+            const int cap = tune.row_block ? tune.row_block : fir ? 8 : 16;
+            return std::max(1, std::min(cap, rb));
         };
         auto row_lds = [&](int r, int rb) { return (size_t)rb * row_bytes(W[r]) + band_bytes(r); };
         // column pass of a big node's leaf: its row-filtered field plus the band table along y
@@ -340,7 +365,8 @@ struct Scat2DPlan {
                 push(K2_LDS_NODE, ids, 1, 0, node_threads(elems), lds);
                 int nt = 0;
                 const int sid = node_spec_id(H[r], W[r], H[J], W[J], &nt);
-                if (use_specs && sizeof(T) == 4 && sid >= 0 && (n0.fir == 1 || n0.leaf_mode == 0)) {
+                if (specs && tune.node_threads == 0 && sizeof(T) == 4 && sid >= 0 &&
+                    (n0.fir == 1 || n0.leaf_mode == 0)) {
                     launches.back().spec = sid;
                     launches.back().threads = nt;
                 }
@@ -348,7 +374,7 @@ struct Scat2DPlan {
                 const int cb = col_blk(H[r]), rb = row_blk(r);
                 // compile-time passes when the sizes match and every block is full
                 int pnt = 0;
-                const int psid = (use_specs && sizeof(T) == 4 && H[r] % rb == 0 && W[r] % cb == 0)
+                const int psid = (specs && sizeof(T) == 4 && H[r] % rb == 0 && W[r] % cb == 0)
                                      ? pass_spec_id(H[r], W[r], cb, rb, &pnt) : -1;
                 auto pass = [&](int kind, int nblk, int blk, size_t lds) {
                     push(kind, ids, nblk, blk, 256, lds);

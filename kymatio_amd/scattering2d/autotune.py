@@ -1,0 +1,108 @@
+"""Autotuner for Scattering2D plans.
+
+For one (image size, J, L, max_order, batch) it times the fused forward on
+the device under a small grid of plan choices (kym_scat2d_create_tuned):
+the largest power-of-two FFT radix, the thread count of LDS-node workgroups,
+the column / row blocks of the big-node passes, and the LDS budget that
+decides which nodes stay in one workgroup.  The fastest choice is kept in an
+in-process table and, if a path is given, a JSON file keyed by device name.
+Every choice computes the same transform (they differ by FP32 rounding).
+"""
+import json
+import os
+
+import torch
+
+DEFAULT_CACHE = os.path.join(os.path.expanduser('~'), '.cache', 'kymatio_amd', 'autotune2d.json')
+
+# (max_pow2, node_threads, col_block, row_block, lds_limit) - 0 = heuristic
+CANDIDATES = [
+    (16, 0, 0, 0, 0),
+    (8, 0, 0, 0, 0),
+    (16, 256, 0, 0, 0),
+    (16, 512, 0, 0, 0),
+    (16, 1024, 0, 0, 0),
+    (16, 0, 8, 4, 0),
+    (16, 0, 16, 16, 0),
+    (8, 0, 8, 8, 0),
+    (16, 0, 0, 0, 96 * 1024),
+    (16, 0, 0, 0, 64 * 1024),
+]
+
+_TABLE = {}
+
+
+def _key(device, side, J, L, max_order, batch, dtype):
+    name = torch.cuda.get_device_name(device) if device.type == 'cuda' else 'cpu'
+    return '%s|%dx%d|J%d|L%d|o%d|b%d|%s' % (name, side[0], side[1], J, L, max_order, batch, str(dtype))
+
+
+def _load(path):
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _time_forward(module, x, reps):
+    with torch.no_grad():
+        module(x)  # plan, filters, workspace
+        torch.cuda.synchronize(x.device)
+        best = float('inf')
+        for _ in range(reps):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            module(x)
+            e1.record()
+            e1.synchronize()
+            best = min(best, e0.elapsed_time(e1))
+    return best
+
+
+def tune(module, batch, reps=3, candidates=None, cache_path=None, timer=None):
+    """Pick the fastest plan choice for `module` (a ScatteringTorch2D on a
+    HIP device) at this batch size and apply it (module.tune,
+    module.lds_limit).  Returns (choice, {choice: ms}).
+
+    timer(module, x) -> ms replaces the device timing (tests)."""
+    dev = next(module.buffers()).device
+    dtype = next(module.buffers()).dtype
+    side = tuple(module.shape)
+    key = _key(dev, side, module.J, module.L, module.max_order, batch, dtype)
+    cands = [tuple(c) for c in (candidates or CANDIDATES)]
+    choice = _TABLE.get(key)
+    if choice is None and cache_path:
+        hit = _load(cache_path).get(key)
+        if hit is not None and tuple(hit) in cands:
+            choice = tuple(hit)
+    times = {}
+    if choice is None:
+        g = torch.Generator().manual_seed(0)
+        x = torch.rand((batch,) + side, generator=g, dtype=dtype).to(dev)
+        saved = (module.tune, module.lds_limit)
+        for c in cands:
+            module.tune, module.lds_limit = list(c[:4]), c[4]
+            try:
+                times[c] = timer(module, x) if timer else _time_forward(module, x, reps)
+            except (RuntimeError, ValueError, NotImplementedError):
+                continue  # a choice the plan rejects (e.g. LDS budget below one row block)
+            finally:
+                module._plans.clear()
+        module.tune, module.lds_limit = saved
+        if not times:
+            raise RuntimeError('no tuning candidate ran')
+        choice = min(times, key=times.get)
+        _TABLE[key] = choice
+        if cache_path:
+            table = _load(cache_path)
+            table[key] = list(choice)
+            os.makedirs(os.path.dirname(cache_path), exist_ok=True)
+            tmp = cache_path + '.%d.tmp' % os.getpid()
+            with open(tmp, 'w') as f:
+                json.dump(table, f, indent=1, sort_keys=True)
+            os.replace(tmp, cache_path)
+    module.tune, module.lds_limit = list(choice[:4]), choice[4]
+    module._plans.clear()
+    return choice, times

@@ -7,11 +7,16 @@ from ... import _lib
 
 
 class Scat2DPlanHandle:
-    def __init__(self, M, N, J, L, max_order, pre_pad, dtype, lds_limit=0):
+    def __init__(self, M, N, J, L, max_order, pre_pad, dtype, lds_limit=0, tune=None):
+        """tune: None or up to 4 ints {largest power-of-two radix, LDS-node
+        threads, column block, row block} (0 = heuristic), see
+        kym_scat2d_create_tuned."""
         lib = _lib.lib()
         h = ctypes.c_void_p()
-        _lib.check(lib.kym_scat2d_create(M, N, J, L, max_order, pre_pad, _lib.DTYPE_CODE[dtype],
-                                         int(lds_limit), ctypes.byref(h)))
+        self.tune = list(tune) if tune else []
+        t = (ctypes.c_int * max(len(self.tune), 1))(*self.tune)
+        _lib.check(lib.kym_scat2d_create_tuned(M, N, J, L, max_order, pre_pad, _lib.DTYPE_CODE[dtype],
+                                               int(lds_limit), t, len(self.tune), ctypes.byref(h)))
         self._h = h
         self._lib = lib
         info = (ctypes.c_longlong * 12)()

@@ -39,6 +39,8 @@ class ScatteringTorch2D(ScatteringTorch, ScatteringBase):
         self.images_per_chunk = None
         # cap on LDS bytes per workgroup for the plans (0 = device maximum)
         self.lds_limit = 0
+        # plan tuning choices (kym_scat2d_create_tuned; None = heuristic), see scattering2d/autotune.py
+        self.tune = None
 
     # ------------------------------------------------------------ buffers
     def register_filters(self):
@@ -78,12 +80,12 @@ class ScatteringTorch2D(ScatteringTorch, ScatteringBase):
 
     # ------------------------------------------------------------ HIP path
     def _plan(self, device, dtype):
-        key = (device.index, dtype)
+        key = (device.index, dtype, int(self.lds_limit), tuple(self.tune or ()))
         p = self._plans.get(key)
         if p is None:
             with torch.cuda.device(device):
                 p = Scat2DPlanHandle(self.shape[0], self.shape[1], self.J, self.L, self.max_order,
-                                     int(bool(self.pre_pad)), dtype, self.lds_limit)
+                                     int(bool(self.pre_pad)), dtype, self.lds_limit, self.tune)
             self._plans[key] = p
         return p
 

@@ -94,7 +94,7 @@ class Emu:
         return out
 
     def scat2d(self, x, J, shape, filters_packed, L=8, max_order=2, pre_pad=0, lds_limit=160 * 1024,
-               chunk=0):
+               chunk=0, tune=None):
         dt = filters_packed.dtype
         code = 0 if dt == np.float32 else 1
         M, N = shape
@@ -107,8 +107,16 @@ class Emu:
         out = np.zeros((B, info[0], info[1], info[2]), dt)
         xx = np.ascontiguousarray(x, dt)
         filters_packed = self.with_taps(filters_packed, J, shape, L, max_order, pre_pad, lds_limit)
-        rc = self.lib.kym_emu_scat2d_forward(M, N, J, L, max_order, pre_pad, code, lds_limit, xx.ctypes.data,
-                                             filters_packed.ctypes.data, out.ctypes.data, B, chunk, info)
+        if tune:
+            f = self.lib.kym_emu_scat2d_forward_tuned
+            f.argtypes = self.lib.kym_emu_scat2d_forward.argtypes + [ctypes.c_void_p, ctypes.c_int]
+            f.restype = ctypes.c_int
+            t = (ctypes.c_int * len(tune))(*tune)
+            rc = f(M, N, J, L, max_order, pre_pad, code, lds_limit, xx.ctypes.data, filters_packed.ctypes.data,
+                   out.ctypes.data, B, chunk, info, t, len(tune))
+        else:
+            rc = self.lib.kym_emu_scat2d_forward(M, N, J, L, max_order, pre_pad, code, lds_limit, xx.ctypes.data,
+                                                 filters_packed.ctypes.data, out.ctypes.data, B, chunk, info)
         if rc != 0:
             raise RuntimeError(self.lib.kym_emu_last_error().decode())
         return out, {'launches': info[3], 'big_nodes': info[4], 'spec_launches': info[5]}

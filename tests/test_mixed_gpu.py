@@ -1,0 +1,96 @@
+"""Mixed-size batches: one HIP graph replay over per-size plans
+(kym_scat2d_graph_*), checked against the oracle, the per-size module and
+itself across replays."""
+import numpy as np
+import pytest
+import torch
+
+from kymatio_amd import workloads
+from kymatio_amd.scattering2d.mixed import MixedScattering2D, default_J
+from kymatio_amd.torch import Scattering2D
+from oracle import scat2d
+from tests.helpers import RTOL_CHANNEL, channel_rel_err
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device('cuda')
+SIDES = (64, 128, 256, 512)
+
+
+def _images(counts, seed=3):
+    rng = np.random.default_rng(seed)
+    return {s: torch.from_numpy(rng.random((n, s, s), dtype=np.float32)).to(DEV) for s, n in counts.items()}
+
+
+def test_mixed_matches_oracle_per_size():
+    """One image of every size of the mixed workload vs the CPU oracle (J = log2(side) - 3)."""
+    mx = MixedScattering2D(autotune=False)
+    groups = _images({s: 1 for s in SIDES})
+    out = mx.forward_grouped(groups)
+    for s in SIDES:
+        ref = scat2d.Scattering2DTorchCPU(default_J(s), (s, s))(groups[s].cpu()).numpy()
+        got = out[s].cpu().numpy()
+        assert got.shape == ref.shape == (1, {64: 217, 128: 417, 256: 681, 512: 1009}[s], 8, 8)
+        err = channel_rel_err(got, ref)
+        print('side %d: per-channel rel err %.3e' % (s, err))
+        assert err < RTOL_CHANNEL
+
+
+def test_graph_replays_match_eager_with_work_between():
+    mx = MixedScattering2D(autotune=False)
+    counts = {64: 5, 128: 3, 256: 2, 512: 1}
+    groups = _images(counts, seed=4)
+    eager = {}
+    with torch.no_grad():
+        for s in SIDES:
+            eager[s] = Scattering2D(default_J(s), (s, s)).to(DEV)(groups[s])
+    first = mx.forward_grouped(groups)
+    for rep in range(3):
+        # eager work and synchronisation between replays
+        junk = torch.randn(1 << 20, device=DEV)
+        (junk * 2).sum().item()
+        with torch.no_grad():
+            Scattering2D(3, (64, 64)).to(DEV)(groups[64][:1])
+        torch.cuda.synchronize()
+        again = mx.forward_grouped(groups)
+        for s in SIDES:
+            assert torch.equal(again[s], first[s]), (rep, s)
+    for s in SIDES:
+        assert torch.equal(first[s], eager[s]), s
+
+
+def test_list_interface_and_new_counts():
+    mx = MixedScattering2D(autotune=False)
+    rng = np.random.default_rng(5)
+    sides = [64, 256, 64, 128, 512, 128, 64]
+    imgs = [torch.from_numpy(rng.random((s, s), dtype=np.float32)).to(DEV) for s in sides]
+    out = mx(imgs)
+    assert [o.shape[0] for o in out] == [{64: 217, 128: 417, 256: 681, 512: 1009}[s] for s in sides]
+    with torch.no_grad():
+        for i in (0, 1, 4):
+            ref = Scattering2D(default_J(sides[i]), (sides[i], sides[i])).to(DEV)(imgs[i][None])[0]
+            assert torch.equal(out[i], ref)
+    # a different count combination rebuilds the graph
+    out2 = mx(imgs[:3])
+    assert torch.equal(out2[1], out[1])
+    with pytest.raises(RuntimeError):
+        mx([torch.zeros(64, 32, device=DEV)])
+
+
+def test_autotuned_mixed_workload():
+    """The mixed workload (256 images, sides drawn from {64,128,256,512})
+    with autotuned plans agrees with the per-size modules."""
+    imgs = workloads.mixed_size_images(256, seed=6)
+    by = {}
+    for v in imgs:
+        by.setdefault(v.shape[0], []).append(v)
+    groups = {s: torch.from_numpy(np.stack(v)).to(DEV) for s, v in by.items()}
+    assert sum(x.shape[0] for x in groups.values()) == 256
+    mx = MixedScattering2D(autotune=True)
+    out = mx.forward_grouped(groups)
+    assert set(mx.tuning) == set(groups)
+    with torch.no_grad():
+        for s, x in groups.items():
+            ref = Scattering2D(default_J(s), (s, s)).to(DEV)(x)
+            err = channel_rel_err(out[s].cpu().numpy(), ref.cpu().numpy())
+            print('side %d (%d images, tuning %s): rel err vs default plan %.3e' % (s, x.shape[0], mx.tuning[s], err))
+            assert err < RTOL_CHANNEL

@@ -1,0 +1,101 @@
+"""Plan tuning choices (kym_scat2d_create_tuned) and the autotuner.
+
+Every tuning choice computes the same transform: the CPU harness runs the
+tuned plans (other radix factorisations, thread counts, pass blocks, LDS
+budgets) against the reference's outputs.  The autotuner's selection and
+cache logic runs with a substitute timer."""
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from kymatio_amd.scattering2d import autotune
+from kymatio_amd.scattering2d.filter_bank import filter_bank
+from kymatio_amd.scattering2d.frontend.torch_frontend import ScatteringTorch2D
+from kymatio_amd.scattering2d.utils import compute_padding
+from tests.helpers import RTOL_CHANNEL, Emu, channel_rel_err, golden_2d_cases, pack_filters_2d
+
+TUNES = [(8, 0, 0, 0), (4, 0, 0, 0), (2, 0, 0, 0), (16, 256, 0, 0), (16, 1024, 0, 0), (16, 0, 8, 4),
+         (8, 512, 4, 2)]
+
+
+@pytest.fixture(scope='module')
+def emu():
+    return Emu()
+
+
+def _case(tag):
+    for c in golden_2d_cases():
+        if c[0] == tag:
+            return c
+    raise KeyError(tag)
+
+
+@pytest.mark.parametrize('tune', TUNES)
+@pytest.mark.parametrize('tag,lds', [('cfg1', 160 * 1024), ('rect40x24', 160 * 1024), ('cfg1', 12 * 1024)])
+def test_tuned_plans_match_reference(emu, tag, lds, tune):
+    _, x, J, shape, kw, Sx = _case(tag)
+    Mp, Np = compute_padding(shape[0], shape[1], J)
+    f = pack_filters_2d(filter_bank(Mp, Np, J, kw.get('L', 8)))
+    out, _ = emu.scat2d(x, J, shape, f, lds_limit=lds, tune=list(tune), **kw)
+    assert channel_rel_err(out, Sx) < RTOL_CHANNEL
+
+
+@pytest.mark.parametrize('tune', [(8, 0, 8, 4), (16, 0, 4, 2)])
+def test_tuned_big_nodes_match_reference(emu, tune):
+    """J=3 256x256: the order-1 nodes at 272 x 272 run as column / row passes."""
+    _, x, J, shape, kw, Sx = _case('cfg2')
+    Mp, Np = compute_padding(shape[0], shape[1], J)
+    f = pack_filters_2d(filter_bank(Mp, Np, J, 8))
+    out, info = emu.scat2d(x[:1], J, shape, f, tune=list(tune))
+    assert info['big_nodes'] > 0
+    assert channel_rel_err(out, Sx[:1]) < RTOL_CHANNEL
+
+
+def test_invalid_tuning_rejected(emu):
+    Mp, Np = compute_padding(32, 32, 2)
+    f = pack_filters_2d(filter_bank(Mp, Np, 2, 8))
+    with pytest.raises(RuntimeError, match='tuning'):
+        emu.scat2d(np.zeros((1, 32, 32), np.float32), 2, (32, 32), f, tune=[12, 0, 0, 0])
+    with pytest.raises(RuntimeError, match='tuning'):
+        emu.scat2d(np.zeros((1, 32, 32), np.float32), 2, (32, 32), f, tune=[16, 300, 0, 0])
+
+
+def test_autotune_picks_fastest_and_caches(tmp_path):
+    m = ScatteringTorch2D(2, (32, 32))
+    cands = [(16, 0, 0, 0, 0), (8, 0, 0, 0, 0), (16, 256, 0, 0, 0)]
+    fake = {(16, 0, 0, 0): 3.0, (8, 0, 0, 0): 1.0, (16, 256, 0, 0): 2.0}
+    calls = []
+
+    def timer(module, x):
+        calls.append(tuple(module.tune))
+        assert x.shape == (5, 32, 32)
+        return fake[tuple(module.tune)]
+
+    path = str(tmp_path / 'tune.json')
+    choice, times = autotune.tune(m, 5, candidates=cands, cache_path=path, timer=timer)
+    assert choice == (8, 0, 0, 0, 0)
+    assert m.tune == [8, 0, 0, 0] and m.lds_limit == 0
+    assert len(calls) == 3
+    table = json.load(open(path))
+    assert list(table.values()) == [[8, 0, 0, 0, 0]]
+    # second module of the same configuration: cached, no timing
+    autotune._TABLE.clear()
+    m2 = ScatteringTorch2D(2, (32, 32))
+    choice2, times2 = autotune.tune(m2, 5, candidates=cands, cache_path=path, timer=timer)
+    assert choice2 == choice and times2 == {} and len(calls) == 3
+    # a rejected candidate is skipped
+    autotune._TABLE.clear()
+    os.remove(path)
+
+    def picky(module, x):
+        if module.tune[0] == 8:
+            raise RuntimeError('rejected')
+        return fake[tuple(module.tune)]
+
+    m3 = ScatteringTorch2D(2, (32, 32))
+    choice3, _ = autotune.tune(m3, 5, candidates=cands, timer=picky)
+    assert choice3 == (16, 256, 0, 0, 0)
+    autotune._TABLE.clear()
