@@ -85,6 +85,7 @@ def main():
     ap.add_argument('--warmup', type=int, default=5)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--chunk', type=int, default=0, help='images per launch chunk (0 = default)')
+    ap.add_argument('--no-autotune', action='store_true', help='use the heuristic plan choices')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='BYTES',
                     help='only run a plain device-to-device copy of BYTES (calibrates HBM counters)')
     args = ap.parse_args()
@@ -108,6 +109,11 @@ def main():
     dev = torch.device('cuda:0')
     x = torch.from_numpy(workloads.grating_images(BATCH, SIDE, seed=SEED)).to(dev)
     sc = Scattering2D(J, (SIDE, SIDE), L=L).to(dev)
+    tuning = None
+    if not args.no_autotune:
+        # plan choices timed on this device before the timed region (excluded, like plan creation)
+        from kymatio_amd.scattering2d import autotune
+        tuning, _ = autotune.tune(sc, BATCH)
     if args.chunk:
         sc.images_per_chunk = args.chunk
     with torch.no_grad():
@@ -157,7 +163,10 @@ def main():
         'config': {'workload': 'Scattering2D J=3 L=8 max_order=2 on 256x256 float32 images: N(0,1) noise '
                                'plus 3 random gratings; output (64, 217, 32, 32)',
                    'batch': BATCH, 'J': J, 'L': L, 'shape': [SIDE, SIDE], 'images_per_chunk': chunk,
-                   'seed': SEED},
+                   'seed': SEED,
+                   'plan_tuning': None if tuning is None else
+                   dict(zip(('max_pow2_radix', 'node_threads', 'col_block', 'row_block', 'lds_limit',
+                             'images_per_chunk'), tuning))},
         'roofline': {'bound': 'hbm', 'achieved': achieved / 1e9, 'peak': HBM_PEAK / 1e9, 'unit': 'GB/s',
                      'frac': achieved / HBM_PEAK, 'traffic': traffic,
                      'traffic_source': 'profiles/scat2d_j3_256_hbm_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, '

@@ -854,11 +854,18 @@ KYM_HD void body_fwd_cols(const P& fp, const Scat2DArgs<T>& a, int task, cpx<T>*
     X(1, 68, 68, 34, 34, 512)        \
     X(2, 80, 80, 10, 10, 512)        \
     X(3, 40, 40, 10, 10, 256)        \
-    X(4, 20, 20, 10, 10, 256)
+    X(4, 20, 20, 10, 10, 256)        \
+    X(5, 68, 68, 34, 34, 256)        \
+    X(6, 80, 80, 10, 10, 256)        \
+    X(7, 136, 136, 34, 34, 512)
 
-inline int node_spec_id(int H, int W, int HJ, int WJ, int* threads) {
-#define KYM_NS_MATCH(id, h, w, hj, wj, nt) \
-    if (H == h && W == w && HJ == hj && WJ == wj) { *threads = nt; return id; }
+// want_threads = 0: the first (default) specialisation of these sizes.
+inline int node_spec_id(int H, int W, int HJ, int WJ, int want_threads, int* threads) {
+#define KYM_NS_MATCH(id, h, w, hj, wj, nt)                                              \
+    if (H == h && W == w && HJ == hj && WJ == wj && (want_threads == 0 || want_threads == nt)) { \
+        *threads = nt;                                                                  \
+        return id;                                                                      \
+    }
     KYM_NODE_SPECS(KYM_NS_MATCH)
 #undef KYM_NS_MATCH
     return -1;
@@ -879,7 +886,11 @@ KYM_HD void dispatch_node_spec(int id, const Scat2DArgs<T>& a, int task, cpx<T>*
 // Compile-time specialisations X(id, H, W, col block, row block, threads) of
 // the big-node passes (FP32): root rows, inverse columns, rows + modulus,
 // forward columns.
-#define KYM_PASS_SPECS(X) X(0, 272, 272, 16, 8, 256)
+#define KYM_PASS_SPECS(X)         \
+    X(0, 272, 272, 16, 8, 256)    \
+    X(1, 272, 272, 16, 4, 256)    \
+    X(2, 272, 272, 8, 8, 256)     \
+    X(3, 272, 272, 32, 8, 256)
 
 inline int pass_spec_id(int H, int W, int cb, int rb, int* threads) {
 #define KYM_PS_MATCH(id, h, w, c, r, nt) \

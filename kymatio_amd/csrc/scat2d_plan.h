@@ -317,10 +317,11 @@ struct Scat2DPlan {
             max_threads = std::max(max_threads, threads);
         };
         const size_t pass_budget = std::min(lds_limit, (size_t)64 * 1024);
+        // pass blocks: the heuristic keeps a block within 64 KB of LDS; a tuned
+        // block may use up to the LDS limit
         auto col_blk = [&](int Hh) {
-            int cb = (int)(pass_budget / (cs * Hh));
-            if (tune.col_block) cb = std::min(cb, tune.col_block);
-            return std::max(1, std::min(tune.col_block ? tune.col_block : 16, cb));
+            if (tune.col_block) return std::max(1, std::min(tune.col_block, (int)(lds_limit / (cs * Hh))));
+            return std::max(1, std::min(16, (int)(pass_budget / (cs * Hh))));
         };
         // row passes of FIR nodes also keep the block's rows as reals (stride W + 1)
         // and the leaf's band table along x
@@ -330,9 +331,12 @@ struct Scat2DPlan {
         };
         // FIR row passes hold 8 rows at most, so that four of their workgroups share a CU
         auto row_blk = [&](int r) {
+            if (tune.row_block) {
+                const size_t room = lds_limit > band_bytes(r) ? lds_limit - band_bytes(r) : 0;
+                return std::max(1, std::min(tune.row_block, (int)(room / row_bytes(W[r]))));
+            }
             int rb = (int)((pass_budget - std::min(pass_budget / 2, band_bytes(r))) / row_bytes(W[r]));
-            const int cap = tune.row_block ? tune.row_block : fir ? 8 : 16;
-            return std::max(1, std::min(cap, rb));
+            return std::max(1, std::min(fir ? 8 : 16, rb));
         };
         auto row_lds = [&](int r, int rb) { return (size_t)rb * row_bytes(W[r]) + band_bytes(r); };
         // column pass of a big node's leaf: its row-filtered field plus the band table along y
@@ -364,9 +368,8 @@ struct Scat2DPlan {
                 const size_t lds = fir ? node_lds(n0) : (elems + (n0.leaf_mode == 0 ? leaf_elems : 0)) * cs;
                 push(K2_LDS_NODE, ids, 1, 0, node_threads(elems), lds);
                 int nt = 0;
-                const int sid = node_spec_id(H[r], W[r], H[J], W[J], &nt);
-                if (specs && tune.node_threads == 0 && sizeof(T) == 4 && sid >= 0 &&
-                    (n0.fir == 1 || n0.leaf_mode == 0)) {
+                const int sid = node_spec_id(H[r], W[r], H[J], W[J], tune.node_threads, &nt);
+                if (specs && sizeof(T) == 4 && sid >= 0 && (n0.fir == 1 || n0.leaf_mode == 0)) {
                     launches.back().spec = sid;
                     launches.back().threads = nt;
                 }

@@ -15,18 +15,20 @@ import torch
 
 DEFAULT_CACHE = os.path.join(os.path.expanduser('~'), '.cache', 'kymatio_amd', 'autotune2d.json')
 
-# (max_pow2, node_threads, col_block, row_block, lds_limit) - 0 = heuristic
+# (max_pow2, node_threads, col_block, row_block, lds_limit, images_per_chunk) - 0 = heuristic
 CANDIDATES = [
-    (16, 0, 0, 0, 0),
-    (8, 0, 0, 0, 0),
-    (16, 256, 0, 0, 0),
-    (16, 512, 0, 0, 0),
-    (16, 1024, 0, 0, 0),
-    (16, 0, 8, 4, 0),
-    (16, 0, 16, 16, 0),
-    (8, 0, 8, 8, 0),
-    (16, 0, 0, 0, 96 * 1024),
-    (16, 0, 0, 0, 64 * 1024),
+    (16, 0, 0, 0, 0, 0),
+    (8, 0, 0, 0, 0, 0),
+    (16, 256, 0, 0, 0, 0),
+    (16, 512, 0, 0, 0, 0),
+    (16, 1024, 0, 0, 0, 0),
+    (16, 0, 16, 4, 0, 0),
+    (16, 0, 8, 8, 0, 0),
+    (16, 0, 32, 8, 0, 0),
+    (8, 0, 8, 4, 0, 0),
+    (16, 0, 0, 0, 96 * 1024, 0),
+    (16, 0, 0, 0, 0, 64),
+    (16, 0, 0, 0, 0, 16),
 ]
 
 _TABLE = {}
@@ -64,7 +66,7 @@ def _time_forward(module, x, reps):
 def tune(module, batch, reps=3, candidates=None, cache_path=None, timer=None):
     """Pick the fastest plan choice for `module` (a ScatteringTorch2D on a
     HIP device) at this batch size and apply it (module.tune,
-    module.lds_limit).  Returns (choice, {choice: ms}).
+    module.lds_limit, module.images_per_chunk).  Returns (choice, {choice: ms}).
 
     timer(module, x) -> ms replaces the device timing (tests)."""
     dev = next(module.buffers()).device
@@ -81,16 +83,16 @@ def tune(module, batch, reps=3, candidates=None, cache_path=None, timer=None):
     if choice is None:
         g = torch.Generator().manual_seed(0)
         x = torch.rand((batch,) + side, generator=g, dtype=dtype).to(dev)
-        saved = (module.tune, module.lds_limit)
+        saved = (module.tune, module.lds_limit, module.images_per_chunk)
         for c in cands:
-            module.tune, module.lds_limit = list(c[:4]), c[4]
+            _apply(module, c)
             try:
                 times[c] = timer(module, x) if timer else _time_forward(module, x, reps)
             except (RuntimeError, ValueError, NotImplementedError):
                 continue  # a choice the plan rejects (e.g. LDS budget below one row block)
             finally:
                 module._plans.clear()
-        module.tune, module.lds_limit = saved
+        module.tune, module.lds_limit, module.images_per_chunk = saved
         if not times:
             raise RuntimeError('no tuning candidate ran')
         choice = min(times, key=times.get)
@@ -103,6 +105,11 @@ def tune(module, batch, reps=3, candidates=None, cache_path=None, timer=None):
             with open(tmp, 'w') as f:
                 json.dump(table, f, indent=1, sort_keys=True)
             os.replace(tmp, cache_path)
-    module.tune, module.lds_limit = list(choice[:4]), choice[4]
+    _apply(module, choice)
     module._plans.clear()
     return choice, times
+
+
+def _apply(module, c):
+    module.tune, module.lds_limit = list(c[:4]), c[4]
+    module.images_per_chunk = c[5] if len(c) > 5 and c[5] else None

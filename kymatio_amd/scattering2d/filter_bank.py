@@ -9,6 +9,8 @@ complex64 FFT, and the alias sums of the crop accumulated in the same
 (row alias, column alias) order.  The crop is vectorised over pixels instead
 of the reference's per-pixel Python loop.
 """
+import os
+
 import numpy as np
 from scipy.fft import fft2
 
@@ -56,20 +58,47 @@ def periodize_filter_fft(x, res):
     return crop
 
 
+def _psi_filter(M, N, J, L, j, theta):
+    sig = morlet_2d(M, N, 0.8 * 2 ** j, (int(L - L / 2 - 1) - theta) * np.pi / L, 3.0 / 4.0 * np.pi / 2 ** j, 4.0 / L)
+    sig_f = np.real(fft2(sig))
+    return {'levels': [periodize_filter_fft(sig_f, r) for r in range(min(j + 1, max(J - 1, 1)))], 'j': j,
+            'theta': theta}
+
+
+def _phi_filter(M, N, J):
+    phi_f = np.real(fft2(gabor_2d(M, N, 0.8 * 2 ** (J - 1), 0, 0)))
+    return {'levels': [periodize_filter_fft(phi_f, r) for r in range(J)], 'j': J}
+
+
+_CACHE = {}
+_CACHE_MAX = 8
+
+
 def filter_bank(M, N, J, L=8):
     """psi: J*L dicts {'levels', 'j', 'theta'} in (j, theta) order, levels
-    r < min(j+1, max(J-1, 1)); phi: {'levels' (r < J), 'j': J}."""
-    psi = []
-    for j in range(J):
-        for theta in range(L):
-            sig = morlet_2d(M, N, 0.8 * 2 ** j, (int(L - L / 2 - 1) - theta) * np.pi / L,
-                            3.0 / 4.0 * np.pi / 2 ** j, 4.0 / L)
-            sig_f = np.real(fft2(sig))
-            psi.append({'levels': [periodize_filter_fft(sig_f, r) for r in range(min(j + 1, max(J - 1, 1)))],
-                        'j': j, 'theta': theta})
-    phi_f = np.real(fft2(gabor_2d(M, N, 0.8 * 2 ** (J - 1), 0, 0)))
-    phi = {'levels': [periodize_filter_fft(phi_f, r) for r in range(J)], 'j': J}
-    return {'phi': phi, 'psi': psi}
+    r < min(j+1, max(J-1, 1)); phi: {'levels' (r < J), 'j': J}.
+
+    The filters are independent, so they are built on a thread pool (numpy
+    releases the GIL in its array loops; each filter's arithmetic is
+    unchanged), and the result is kept per (M, N, J, L) so that several
+    modules of one configuration build it once.  Callers get fresh copies."""
+    key = (M, N, J, L)
+    fb = _CACHE.get(key)
+    if fb is None:
+        from concurrent.futures import ThreadPoolExecutor
+        jobs = [(j, theta) for j in range(J) for theta in range(L)]
+        workers = max(1, min(len(jobs) + 1, os.cpu_count() or 1, 16))
+        with ThreadPoolExecutor(workers) as ex:
+            fphi = ex.submit(_phi_filter, M, N, J)
+            psi = list(ex.map(lambda jt: _psi_filter(M, N, J, L, *jt), jobs))
+            phi = fphi.result()
+        fb = {'phi': phi, 'psi': psi}
+        if len(_CACHE) >= _CACHE_MAX:
+            _CACHE.pop(next(iter(_CACHE)))
+        _CACHE[key] = fb
+    return {'phi': {'levels': [a.copy() for a in fb['phi']['levels']], 'j': fb['phi']['j']},
+            'psi': [{'levels': [a.copy() for a in p['levels']], 'j': p['j'], 'theta': p['theta']}
+                    for p in fb['psi']]}
 
 
 def leaf_taps(phi_levels, desc):

@@ -109,10 +109,14 @@ class ScatteringTorch2D(ScatteringTorch, ScatteringBase):
         return packed
 
     def _chunk(self, plan, batch):
+        """Images per launch chunk: images_per_chunk if set, else as many as a
+        256 MB workspace holds, balanced so that all chunks are equally full."""
         if self.images_per_chunk:
             return max(1, min(batch, int(self.images_per_chunk)))
         per = plan.ws_bytes_per_image
-        return max(1, min(batch, (256 << 20) // max(per, 1)))
+        cap = max(1, min(batch, (256 << 20) // max(per, 1)))
+        nchunks = -(-batch // cap)
+        return -(-batch // nchunks)
 
     def scattering(self, input):
         if not torch.is_tensor(input):

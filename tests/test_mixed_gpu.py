@@ -9,7 +9,7 @@ from kymatio_amd import workloads
 from kymatio_amd.scattering2d.mixed import MixedScattering2D, default_J
 from kymatio_amd.torch import Scattering2D
 from oracle import scat2d
-from tests.helpers import RTOL_CHANNEL, channel_rel_err
+from tests.helpers import RTOL_CHANNEL, channel_rel_err, golden
 
 pytestmark = pytest.mark.gpu
 DEV = torch.device('cuda')
@@ -21,13 +21,28 @@ def _images(counts, seed=3):
     return {s: torch.from_numpy(rng.random((n, s, s), dtype=np.float32)).to(DEV) for s, n in counts.items()}
 
 
+@pytest.fixture(scope='module')
+def per_size():
+    """Single-size modules (J = log2(side) - 3) for the eager comparisons."""
+    return {s: Scattering2D(default_J(s), (s, s)).to(DEV) for s in SIDES}
+
+
 def test_mixed_matches_oracle_per_size():
-    """One image of every size of the mixed workload vs the CPU oracle (J = log2(side) - 3)."""
+    """One image of every size of the mixed workload vs the oracle: sides 64
+    and 128 run the CPU oracle here, sides 256 and 512 compare with its
+    outputs stored in tests/golden/golden_mixed2d.npz."""
+    G = golden('golden_mixed2d.npz')
+    groups = _images({64: 1, 128: 1})
+    for s in (256, 512):
+        x = np.random.default_rng(int(G['s%d_seed' % s])).random((1, s, s), dtype=np.float32)
+        groups[s] = torch.from_numpy(x).to(DEV)
     mx = MixedScattering2D(autotune=False)
-    groups = _images({s: 1 for s in SIDES})
     out = mx.forward_grouped(groups)
     for s in SIDES:
-        ref = scat2d.Scattering2DTorchCPU(default_J(s), (s, s))(groups[s].cpu()).numpy()
+        if s in (256, 512):
+            ref = G['s%d_Sx' % s]
+        else:
+            ref = scat2d.Scattering2DTorchCPU(default_J(s), (s, s))(groups[s].cpu()).numpy()
         got = out[s].cpu().numpy()
         assert got.shape == ref.shape == (1, {64: 217, 128: 417, 256: 681, 512: 1009}[s], 8, 8)
         err = channel_rel_err(got, ref)
@@ -35,21 +50,21 @@ def test_mixed_matches_oracle_per_size():
         assert err < RTOL_CHANNEL
 
 
-def test_graph_replays_match_eager_with_work_between():
+def test_graph_replays_match_eager_with_work_between(per_size):
     mx = MixedScattering2D(autotune=False)
     counts = {64: 5, 128: 3, 256: 2, 512: 1}
     groups = _images(counts, seed=4)
     eager = {}
     with torch.no_grad():
         for s in SIDES:
-            eager[s] = Scattering2D(default_J(s), (s, s)).to(DEV)(groups[s])
+            eager[s] = per_size[s](groups[s])
     first = mx.forward_grouped(groups)
     for rep in range(3):
         # eager work and synchronisation between replays
         junk = torch.randn(1 << 20, device=DEV)
         (junk * 2).sum().item()
         with torch.no_grad():
-            Scattering2D(3, (64, 64)).to(DEV)(groups[64][:1])
+            per_size[64](groups[64][:1])
         torch.cuda.synchronize()
         again = mx.forward_grouped(groups)
         for s in SIDES:
@@ -58,7 +73,7 @@ def test_graph_replays_match_eager_with_work_between():
         assert torch.equal(first[s], eager[s]), s
 
 
-def test_list_interface_and_new_counts():
+def test_list_interface_and_new_counts(per_size):
     mx = MixedScattering2D(autotune=False)
     rng = np.random.default_rng(5)
     sides = [64, 256, 64, 128, 512, 128, 64]
@@ -67,7 +82,7 @@ def test_list_interface_and_new_counts():
     assert [o.shape[0] for o in out] == [{64: 217, 128: 417, 256: 681, 512: 1009}[s] for s in sides]
     with torch.no_grad():
         for i in (0, 1, 4):
-            ref = Scattering2D(default_J(sides[i]), (sides[i], sides[i])).to(DEV)(imgs[i][None])[0]
+            ref = per_size[sides[i]](imgs[i][None])[0]
             assert torch.equal(out[i], ref)
     # a different count combination rebuilds the graph
     out2 = mx(imgs[:3])
@@ -76,7 +91,7 @@ def test_list_interface_and_new_counts():
         mx([torch.zeros(64, 32, device=DEV)])
 
 
-def test_autotuned_mixed_workload():
+def test_autotuned_mixed_workload(per_size):
     """The mixed workload (256 images, sides drawn from {64,128,256,512})
     with autotuned plans agrees with the per-size modules."""
     imgs = workloads.mixed_size_images(256, seed=6)
@@ -90,7 +105,7 @@ def test_autotuned_mixed_workload():
     assert set(mx.tuning) == set(groups)
     with torch.no_grad():
         for s, x in groups.items():
-            ref = Scattering2D(default_J(s), (s, s)).to(DEV)(x)
+            ref = per_size[s](x)
             err = channel_rel_err(out[s].cpu().numpy(), ref.cpu().numpy())
             print('side %d (%d images, tuning %s): rel err vs default plan %.3e' % (s, x.shape[0], mx.tuning[s], err))
             assert err < RTOL_CHANNEL

@@ -65,7 +65,7 @@ def test_invalid_tuning_rejected(emu):
 
 def test_autotune_picks_fastest_and_caches(tmp_path):
     m = ScatteringTorch2D(2, (32, 32))
-    cands = [(16, 0, 0, 0, 0), (8, 0, 0, 0, 0), (16, 256, 0, 0, 0)]
+    cands = [(16, 0, 0, 0, 0, 0), (8, 0, 0, 0, 0, 0), (16, 256, 0, 0, 0, 8)]
     fake = {(16, 0, 0, 0): 3.0, (8, 0, 0, 0): 1.0, (16, 256, 0, 0): 2.0}
     calls = []
 
@@ -76,11 +76,11 @@ def test_autotune_picks_fastest_and_caches(tmp_path):
 
     path = str(tmp_path / 'tune.json')
     choice, times = autotune.tune(m, 5, candidates=cands, cache_path=path, timer=timer)
-    assert choice == (8, 0, 0, 0, 0)
-    assert m.tune == [8, 0, 0, 0] and m.lds_limit == 0
+    assert choice == (8, 0, 0, 0, 0, 0)
+    assert m.tune == [8, 0, 0, 0] and m.lds_limit == 0 and m.images_per_chunk is None
     assert len(calls) == 3
     table = json.load(open(path))
-    assert list(table.values()) == [[8, 0, 0, 0, 0]]
+    assert list(table.values()) == [[8, 0, 0, 0, 0, 0]]
     # second module of the same configuration: cached, no timing
     autotune._TABLE.clear()
     m2 = ScatteringTorch2D(2, (32, 32))
@@ -97,5 +97,6 @@ def test_autotune_picks_fastest_and_caches(tmp_path):
 
     m3 = ScatteringTorch2D(2, (32, 32))
     choice3, _ = autotune.tune(m3, 5, candidates=cands, timer=picky)
-    assert choice3 == (16, 256, 0, 0, 0)
+    assert choice3 == (16, 256, 0, 0, 0, 8)
+    assert m3.images_per_chunk == 8
     autotune._TABLE.clear()
