@@ -6,6 +6,26 @@
 #pragma once
 #include "fft_core.h"
 
+// Digit reversal of a compile-time length (the map of kym_rev_map, i.e. the
+// rev table): rev(p) = d_{K-1} + R_{K-1} (d_{K-2} + R_{K-2} (... + R_1 d_0)),
+// with p's mixed-radix digits d_q = (p / m_q) mod R_q, m_q = R_0 ... R_{q-1}.
+// Constant divisors, no table load.
+template <int N>
+KYM_HD KYM_INL int kym_rev_s(int p) {
+    constexpr RadList rl = kym_factor_list(N);
+    static_assert(rl.ok, "unsupported static FFT length");
+    int value = 0, scale = 1, m = N;
+#pragma unroll
+    for (int q = rl.nrad - 1; q >= 0; --q) {
+        m /= rl.rad[q] > 1 ? rl.rad[q] : 1;
+        const int d = p / m;
+        p -= d * m;
+        value += d * scale;
+        scale *= rl.rad[q] > 1 ? rl.rad[q] : 1;
+    }
+    return value;
+}
+
 template <int R, bool INV, int N, int LP, int NTR, int ES, int TS, bool TFAST, int NT, class T>
 KYM_HD KYM_INL void dit_pass_s(cpx<T>* buf, const cpx<T>* tw, int tid) {
     constexpr int NB = N / R, L = LP * R, TSTEP = N / L, TOTAL = NB * NTR, ST = LP * ES;

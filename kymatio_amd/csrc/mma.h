@@ -42,9 +42,11 @@ KYM_HD KYM_INL void wave_mma16(int lane, int ksteps, FA fa, FB fb, FD fd) {
 }
 
 // Wave index / wave count of a workgroup body (the CPU harness is one wave).
+// (readfirstlane: the compiler then knows the value, and everything derived
+// from it, is uniform across the wave, so tile loops run on scalar registers.)
 KYM_HD KYM_INL int wave_id(int tid) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return tid >> 6;
+    return __builtin_amdgcn_readfirstlane(tid >> 6);
 #else
     (void)tid;
     return 0;

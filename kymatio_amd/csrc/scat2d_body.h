@@ -142,9 +142,10 @@ KYM_HD KYM_INL cpx<T> gather_periodized_k(const cpx<T>* U, const T* f, int Wp, i
 // E = 4 / K elements per thread and round: their 2 K^2 loads each are all
 // issued before the first use, so the memory latency is paid once per round.
 template <int K, class T>
-KYM_HD KYM_INL void gather_block_k(cpx<T>* dst, int ncols, int c0, const cpx<T>* U, const T* f, int Wp, int H,
-                                   int W, T scale, int tid, int nth) {
+KYM_HD KYM_INL void gather_block_k(cpx<T>* dst, int ncols, int c0, const cpx<T>* U, const T* f, int H, int W,
+                                   T scale, int tid, int nth) {
     constexpr int E = K >= 4 ? 1 : 4 / K;
+    const int Wp = W * K;  // the parent's row length
     const int n = H * ncols;
     int i = tid;
     for (; i + (E - 1) * nth < n; i += E * nth) {
@@ -163,14 +164,15 @@ KYM_HD KYM_INL void gather_block_k(cpx<T>* dst, int ncols, int c0, const cpx<T>*
     }
 }
 
+// Wp = W k is the parent's row length.
 template <class T>
 KYM_HD KYM_INL void gather_block(cpx<T>* dst, int ncols, int c0, const cpx<T>* U, const T* f, int Wp, int H, int W, int k,
                          int tid, int nth) {
     const T scale = T(1) / T(k * k);
     switch (k) {
-        case 1: gather_block_k<1>(dst, ncols, c0, U, f, Wp, H, W, scale, tid, nth); break;
-        case 2: gather_block_k<2>(dst, ncols, c0, U, f, Wp, H, W, scale, tid, nth); break;
-        case 4: gather_block_k<4>(dst, ncols, c0, U, f, Wp, H, W, scale, tid, nth); break;
+        case 1: gather_block_k<1>(dst, ncols, c0, U, f, H, W, scale, tid, nth); break;
+        case 2: gather_block_k<2>(dst, ncols, c0, U, f, H, W, scale, tid, nth); break;
+        case 4: gather_block_k<4>(dst, ncols, c0, U, f, H, W, scale, tid, nth); break;
         default:
             for (int i = tid; i < H * ncols; i += nth) {
                 const int y = i / ncols, c = i - y * ncols;
@@ -220,6 +222,8 @@ struct DynGeom {
     KYM_HD const int* revW() const { return fW->rev; }
     KYM_HD const int* revHJ() const { return fHJ->rev; }
     KYM_HD const int* revWJ() const { return fWJ->rev; }
+    KYM_HD int rh(int p) const { return kym_ldg(fH->rev + p); }
+    KYM_HD int rw(int p) const { return kym_ldg(fW->rev + p); }
     KYM_HD void inv2d(cpx<T>* b, int tid, int nth) const {
         fft_dif<true, MAXP>(b, *fW, Batch{H, 1, W, false}, tid, nth);
         fft_dif<true, MAXP>(b, *fH, Batch{W, W, 1, true}, tid, nth);
@@ -250,6 +254,8 @@ struct StatGeom {
     KYM_HD const int* revW() const { return fW->rev; }
     KYM_HD const int* revHJ() const { return fHJ->rev; }
     KYM_HD const int* revWJ() const { return fWJ->rev; }
+    KYM_HD int rh(int p) const { return kym_rev_s<H>(p); }
+    KYM_HD int rw(int p) const { return kym_rev_s<W>(p); }
     KYM_HD void inv2d(cpx<T>* b, int tid, int) const {
         fft_dif_s<true, W, H, 1, W, false, NT>(b, fW->tw, tid);
         fft_dif_s<true, H, W, W, 1, true, NT>(b, fH->tw, tid);
@@ -305,8 +311,6 @@ KYM_HD void leaf_body(const Scat2DArgs<T>& a, const G& g, const cpx<T>* U, const
 template <class T, class G>
 KYM_HD void stage_modulus(const G& g, cpx<T>* lds, cpx<T>* sv, T inv_n, int tid, int nth) {
     const int n = g.H * g.W, rs = g.W + 1;
-    const int* rH = g.revH();
-    const int* rW = g.revW();
     T* R = reinterpret_cast<T*>(lds);
 #if !defined(__HIP_DEVICE_COMPILE__)
     if (nth == 1) {
@@ -316,7 +320,7 @@ KYM_HD void stage_modulus(const G& g, cpx<T>* lds, cpx<T>* sv, T inv_n, int tid,
             if (sv) sv[s] = v;
             m[s] = kym_sqrt(v.re * v.re + v.im * v.im) * inv_n;
         }
-        for (int s = 0; s < n; ++s) R[rH[s / g.W] * rs + rW[s % g.W]] = m[s];
+        for (int s = 0; s < n; ++s) R[g.rh(s / g.W) * rs + g.rw(s % g.W)] = m[s];
         return;
     }
 #endif
@@ -337,7 +341,7 @@ KYM_HD void stage_modulus(const G& g, cpx<T>* lds, cpx<T>* sv, T inv_n, int tid,
         const int s = tid + k * nth;
         if (s < n) {
             const int py = s / g.W, px = s - py * g.W;
-            R[kym_ldg(rH + py) * rs + kym_ldg(rW + px)] = m[k];
+            R[g.rh(py) * rs + g.rw(px)] = m[k];
         }
     }
     wg_barrier();
@@ -347,13 +351,11 @@ KYM_HD void stage_modulus(const G& g, cpx<T>* lds, cpx<T>* sv, T inv_n, int tid,
 template <class T, class G>
 KYM_HD void restage_complex(const G& g, cpx<T>* lds, int tid, int nth) {
     const int n = g.H * g.W, rs = g.W + 1;
-    const int* rH = g.revH();
-    const int* rW = g.revW();
     const T* R = reinterpret_cast<const T*>(lds);
 #if !defined(__HIP_DEVICE_COMPILE__)
     if (nth == 1) {
         std::vector<T> m(n);
-        for (int s = 0; s < n; ++s) m[s] = R[rH[s / g.W] * rs + rW[s % g.W]];
+        for (int s = 0; s < n; ++s) m[s] = R[g.rh(s / g.W) * rs + g.rw(s % g.W)];
         for (int s = 0; s < n; ++s) lds[s] = {m[s], T(0)};
         return;
     }
@@ -365,7 +367,7 @@ KYM_HD void restage_complex(const G& g, cpx<T>* lds, int tid, int nth) {
         const int s = tid + k * nth;
         if (s < n) {
             const int py = s / g.W, px = s - py * g.W;
-            m[k] = R[kym_ldg(rH + py) * rs + kym_ldg(rW + px)];
+            m[k] = R[g.rh(py) * rs + g.rw(px)];
         }
     }
     wg_barrier();

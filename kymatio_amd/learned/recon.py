@@ -2,10 +2,11 @@
 the hand-written scattering backward (the precedent is
 examples/1d/reconstruct_torch.py:77-129, which descends on ||S(y) - S(x)||).
 
-The loss is the relative scattering error over the batch,
-    r(y) = ||S(y) - S_target||_F / ||S_target||_F,
-minimised with Adam (kym_dec_adam) from an initial signal: N(0, 1) noise for
-plain descent, or the decoder's output for the learned warm start.
+The loss is the relative scattering loss over the batch,
+    L(y) = ||S(y) - S_target||_F^2 / ||S_target||_F^2
+(the squared relative error), minimised with Adam (kym_dec_adam) from an
+initial signal: N(0, 1) noise for plain descent, or the decoder's output for
+the learned warm start.
 """
 import ctypes
 import time
@@ -19,16 +20,17 @@ def _p(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
-def rel_error(scattering, y, S_target):
+def rel_loss(scattering, y, S_target):
+    """L(y) = ||S(y) - S_target||^2 / ||S_target||^2."""
     with torch.no_grad():
-        return ((scattering(y) - S_target).norm() / S_target.norm()).item()
+        return (((scattering(y) - S_target).norm() / S_target.norm()) ** 2).item()
 
 
 def reconstruct(scattering, S_target, y0, tol=1e-3, max_iter=1000, lr=1e-2, time_limit=None, b1=0.9, b2=0.999,
                 eps=1e-8, check_every=1):
-    """Adam on y from y0 until r(y) <= tol (or max_iter / time_limit seconds).
+    """Adam on y from y0 until L(y) <= tol (or max_iter / time_limit seconds).
 
-    Returns (y, history of (iteration, seconds, r), reached)."""
+    Returns (y, history of (iteration, seconds, L), reached)."""
     lib = _lib.lib()
     dev = y0.device
     s = _lib.stream_of(dev)
@@ -46,7 +48,7 @@ def reconstruct(scattering, S_target, y0, tol=1e-3, max_iter=1000, lr=1e-2, time
         loss = (diff * diff).sum() / denom
         loss.backward()
         if it == 1 or it % check_every == 0:
-            r = loss.detach().sqrt().item()
+            r = loss.detach().item()
             hist.append((it - 1, time.perf_counter() - t0, r))
             if r <= tol:
                 reached = True

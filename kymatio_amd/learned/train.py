@@ -34,7 +34,8 @@ def scattering_loss_grad(scattering, y, S_target):
 
 def train(scattering, n_pairs=1_000_000, batch=256, lr=1e-3, seed=1234, log_every=200, time_limit=None,
           decoder=None, lr_final=1e-4, log=print):
-    """Train (or continue training) a decoder; returns (decoder, history)."""
+    """Train (or continue training) a decoder on n_pairs generated pairs;
+    returns (decoder, history of (step, pairs, seconds, batch loss))."""
     dev = next(scattering.buffers()).device
     N = scattering.shape[0]
     x0 = chirps(batch, N, seed, 0, device=dev)
@@ -62,7 +63,37 @@ def train(scattering, n_pairs=1_000_000, batch=256, lr=1e-3, seed=1234, log_ever
             l = loss.item()
             el = time.perf_counter() - t0
             hist.append((step, (step + 1) * batch, el, l))
-            log('step %6d  pairs %8d  %7.1f s  rel scattering loss %.4e' % (step, (step + 1) * batch, el, l ** 0.5))
+            log('step %6d  pairs %8d  %7.1f s  relative scattering loss %.4e' % (step, (step + 1) * batch, el, l))
         if time_limit is not None and time.perf_counter() - t0 > time_limit:
             break
     return dec, hist
+
+
+CHECKPOINT = __import__('os').path.join(__import__('os').path.dirname(__import__('os').path.abspath(__file__)),
+                                        'decoder_scat1d_j6q8_n4096.pt')
+
+
+def main(argv=None):
+    """python -m kymatio_amd.learned.train [--pairs N] [--lr LR] [--out PATH]:
+    trains the warm-start decoder for Scattering1D J=6 Q=8 N=4096 on the GPU."""
+    import argparse
+    from ..torch import Scattering1D
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--pairs', type=int, default=1_000_000)
+    ap.add_argument('--batch', type=int, default=256)
+    ap.add_argument('--lr', type=float, default=1e-3)
+    ap.add_argument('--lr-final', type=float, default=1e-4)
+    ap.add_argument('--seed', type=int, default=1234)
+    ap.add_argument('--out', default=CHECKPOINT)
+    args = ap.parse_args(argv)
+    sc = Scattering1D(6, 4096, 8).to(torch.device('cuda'))
+    dec, hist = train(sc, n_pairs=args.pairs, batch=args.batch, lr=args.lr, lr_final=args.lr_final, seed=args.seed)
+    state = dec.state_dict()
+    state['train_pairs'] = torch.tensor(args.pairs)
+    state['train_final_loss'] = torch.tensor(hist[-1][3])
+    torch.save(state, args.out)
+    print('saved', args.out)
+
+
+if __name__ == '__main__':
+    main()
