@@ -75,12 +75,14 @@ int kym_scat2d_filter_offsets(void* plan, long long* offs);
  * (kymatio/scattering2d/core/scattering2d.py:19-23, 42-46, 69-73), equals a
  * separable real filter-and-subsample of the field behind U, because
  * phi.levels[r] is separable: Re(ifft2(phi.levels[r]))[a][b] = g_r[a] h_r[b].
- * desc (4 + 6 J entries, n its capacity) = {taps_off, taps_len, M_padded,
- * N_padded}, then per r < J: {off_y, lo_y, cnt_y, off_x, lo_x, cnt_x}.  The
- * caller fills packed[off_y + i] = g_r[(lo_y + i) mod H_r] for i < cnt_y and
- * packed[off_x + i] = h_r[(lo_x + i) mod W_r] for i < cnt_x, with
- * H_r = M_padded / 2^r, W_r = N_padded / 2^r; taps_len = 0 (FP64 plans) means
- * no taps.  No counterpart in the reference (derived filter data). */
+ * desc (4 + 10 J entries, n its capacity) = {taps_off, taps_len, M_padded,
+ * N_padded}, then per r < J and axis (y, then x):
+ * {off, lo, cnt, band_off, kmax}.  For the axis of n = M_padded / 2^r (y) or
+ * N_padded / 2^r (x) samples, with c = g_r (y) or h_r (x), the caller fills
+ * packed[off + i] = c[(lo + i) mod n] for i < cnt, and the band table
+ * packed[band_off + 16 k + d] = packed[off + t] with t = (k - 2^(J-r) d) mod n
+ * when t < cnt, else 0, for k < kmax, d < 16.  taps_len = 0 (FP64 plans)
+ * means no taps.  No counterpart in the reference (derived filter data). */
 int kym_scat2d_leaf_taps(void* plan, long long* desc, int n);
 
 /* x: (batch, M, N) real (or padded size with pre_pad); filt: packed filters;

@@ -184,8 +184,10 @@ static int leaf_taps(const Scat2DPlan<T>& p, long long* desc, int n) {
             d.push_back(f.off);
             d.push_back(f.lo);
             d.push_back(f.cnt);
+            d.push_back(f.band_off);
+            d.push_back(f.kmax);
         }
-    if (n < (int)d.size()) return fail(KYM_ERR_VALUE, "leaf taps descriptor needs 4 + 6 J entries");
+    if (n < (int)d.size()) return fail(KYM_ERR_VALUE, "leaf taps descriptor needs 4 + 10 J entries");
     for (size_t i = 0; i < d.size(); ++i) desc[i] = d[i];
     return KYM_OK;
 }

@@ -139,9 +139,12 @@ static int emu_taps2d(int M, int N, int J, int L, int max_order, int pre_pad, lo
     std::vector<long long> d = {p.taps_off, p.taps_len, p.Mp, p.Np};
     for (int r = 0; r < J; ++r)
         for (int ax = 0; ax < 2; ++ax) {
-            d.push_back(p.fir_axes[2 * r + ax].off);
-            d.push_back(p.fir_axes[2 * r + ax].lo);
-            d.push_back(p.fir_axes[2 * r + ax].cnt);
+            const FirAxis& f = p.fir_axes[2 * r + ax];
+            d.push_back(f.off);
+            d.push_back(f.lo);
+            d.push_back(f.cnt);
+            d.push_back(f.band_off);
+            d.push_back(f.kmax);
         }
     if (n < (int)d.size()) {
         g_emu_error = "descriptor too small";

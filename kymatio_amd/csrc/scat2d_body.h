@@ -68,10 +68,14 @@ template <class T>
 inline constexpr bool kym_fir = sizeof(T) == 4;
 
 // Taps of one axis of a leaf filter: coefficient c[i] (at filt + off) for the
-// offset lo + i from the output's centre sample.
+// offset lo + i from the output's centre sample, and the band table of that
+// axis (at filt + band_off, kmax x 16 reals, see fir_band) that the matrix-core
+// passes read.
 struct FirAxis {
     long long off;
     int lo, cnt;
+    long long band_off;
+    int kmax, pad_;
 };
 
 template <class T>
@@ -398,15 +402,12 @@ KYM_HD KYM_INL long long fir_lds_reals(int H, int W, int ow, int kl, int cnt_y, 
 KYM_HD KYM_INL int fir_kmax(int n, int kl, int cnt) { return (15 * kl + cnt < n) ? 15 * kl + cnt : n; }
 
 // band[k][d] = c[(k - kl d) mod n] (zero beyond the taps), k < kmax, d < 16:
-// inside a tile's window the circulant factor depends only on (k, d).
+// inside a tile's window the circulant factor depends only on (k, d).  The
+// frontend stores it next to the taps; passes copy it to LDS.
 template <class T>
-KYM_HD void fir_band(T* band, int kmax, int n, int kl, const T* c, int cnt, int tid, int nth) {
-    for (int i = tid; i < 16 * kmax; i += nth) {
-        const int k = i >> 4, d = i & 15;
-        int t = k - kl * d;  // k < n, so one or two additions of n make it non-negative in practice
-        while (t < 0) t += n;
-        band[i] = t < cnt ? kym_ldg(c + t) : T(0);
-    }
+KYM_HD KYM_INL void fir_band(T* band, const T* filt, const FirAxis& f, int tid, int nth) {
+    const T* src = filt + f.band_off;
+    for (int i = tid; i < 16 * f.kmax; i += nth) band[i] = kym_ldg(src + i);
 }
 
 // Row pass on the matrix cores: for rows p < nrows of R (row stride rs,
@@ -495,8 +496,8 @@ KYM_HD void fir_leaf_lds(const Scat2DArgs<T>& a, const G& g, const Node2D& nd, T
     T* Tb = R + H * rs;
     T* Bx = Tb + H * ts;
     T* Ay = Bx + 16 * kxm;
-    fir_band(Bx, kxm, W, kl, a.filt + fx.off, fx.cnt, tid, nth);
-    fir_band(Ay, kym, H, kl, a.filt + fy.off, fy.cnt, tid, nth);
+    fir_band(Bx, a.filt, fx, tid, nth);
+    fir_band(Ay, a.filt, fy, tid, nth);
     wg_barrier();
     fir_rows_mma(R, rs, H, W, kl, ow, fx, Bx, [&](int p, int x, float v) { Tb[p * ts + x] = v; }, tid, nth);
     wg_barrier();
@@ -649,7 +650,7 @@ KYM_HD void fir_rows_to_slot(const Scat2DArgs<T>& a, const Node2D& nd, const T* 
                              const int* rev, cpx<T>* wsi, T* Bx, int tid, int nth) {
     const int r = nd.res, W = a.W[r], ow = a.ow, kl = nd.leaf_k;
     const FirAxis fx = a.fir[2 * r + 1];
-    fir_band(Bx, fir_kmax(W, kl, fx.cnt), W, kl, a.filt + fx.off, fx.cnt, tid, nth);
+    fir_band(Bx, a.filt, fx, tid, nth);
     wg_barrier();
     T* Tg = reinterpret_cast<T*>(wsi + nd.tslot);
     fir_rows_mma(R, rs, nr, W, kl, ow, fx, Bx, [&](int p, int x, float v) { Tg[(long long)rev[r0 + p] * ow + x] = v; },
@@ -792,7 +793,7 @@ KYM_HD void body_fir_cols(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid
             const int p = i / ow, x = i - p * ow;
             Tb[p * ts + x] = Tg[i];
         }
-        fir_band(Ay, kym, H, kl, a.filt + fy.off, fy.cnt, tid, nth);
+        fir_band(Ay, a.filt, fy, tid, nth);
         wg_barrier();
         fir_cols_mma(Tb, ts, H, kl, oh, ow, fy, Ay, a.out + ((long long)(a.img0 + il) * a.C + nd.leaf_ch) * oh * ow,
                      tid, nth);

@@ -87,7 +87,7 @@ struct Scat2DPlan {
     FirAxis fir_axis(int r, int n, long long off) const {
         const double sigma0 = 0.8 * std::ldexp(1.0, J - 1);
         const double c = r == 0 ? 1e30 : 3.14159265358979 * sigma0 / std::ldexp(1.0, r);
-        FirAxis f = {off, -(n / 2), n};
+        FirAxis f = {off, -(n / 2), n, 0, 0, 0};
         if (c >= 7.0) {
             const int R = (int)std::ceil(7.0 * sigma0 / std::ldexp(1.0, r));
             if (2 * R + 1 < n) { f.lo = -R; f.cnt = 2 * R + 1; }
@@ -155,16 +155,20 @@ struct Scat2DPlan {
         filt_off.push_back(off);
         // leaf taps (FP32 plans)
         const bool fir = kym_fir<T>;
-        fir_axes.assign(2 * (size_t)std::max(J, 1), FirAxis{0, 0, 0});
+        fir_axes.assign(2 * (size_t)std::max(J, 1), FirAxis{0, 0, 0, 0, 0, 0});
         taps_off = off;
         taps_len = 0;
         if (fir)
-            for (int r = 0; r < J; ++r) {
-                fir_axes[2 * r] = fir_axis(r, H[r], taps_off + taps_len);
-                taps_len += fir_axes[2 * r].cnt;
-                fir_axes[2 * r + 1] = fir_axis(r, W[r], taps_off + taps_len);
-                taps_len += fir_axes[2 * r + 1].cnt;
-            }
+            for (int r = 0; r < J; ++r)
+                for (int ax = 0; ax < 2; ++ax) {
+                    const int n = ax == 0 ? H[r] : W[r];
+                    FirAxis& f = fir_axes[2 * r + ax];
+                    f = fir_axis(r, n, taps_off + taps_len);
+                    taps_len += f.cnt;
+                    f.kmax = fir_kmax(n, 1 << (J - r), f.cnt);
+                    f.band_off = taps_off + taps_len;
+                    taps_len += 16ll * f.kmax;
+                }
         // nodes (path order of kymatio/scattering2d/core/scattering2d.py:25-78)
         nodes.clear();
         const int npsi = J * L;
@@ -327,7 +331,7 @@ struct Scat2DPlan {
         // and the leaf's band table along x
         auto row_bytes = [&](int Ww) { return cs * Ww + (fir ? sizeof(T) * (Ww + 1) : 0); };
         auto band_bytes = [&](int r) {
-            return fir ? sizeof(T) * 16 * (size_t)fir_kmax(W[r], 1 << (J - r), fir_axes[2 * r + 1].cnt) : 0;
+            return fir ? sizeof(T) * 16 * (size_t)fir_axes[2 * r + 1].kmax : 0;
         };
         // FIR row passes hold 8 rows at most, so that four of their workgroups share a CU
         auto row_blk = [&](int r) {
@@ -342,7 +346,7 @@ struct Scat2DPlan {
         // column pass of a big node's leaf: its row-filtered field plus the band table along y
         auto fir_cols_lds = [&](int r) {
             return sizeof(T) * ((size_t)H[r] * fir_tstride(W[r], ow) +
-                                16 * (size_t)fir_kmax(H[r], 1 << (J - r), fir_axes[2 * r].cnt));
+                                16 * (size_t)fir_axes[2 * r].kmax);
         };
         auto emit_leaves = [&](const std::vector<int>& ids) {
             std::vector<int> small, big, firc;

@@ -102,15 +102,18 @@ def filter_bank(M, N, J, L=8):
 
 
 def leaf_taps(phi_levels, desc):
-    """Taps of the separable low-pass leaves (kym_scat2d_leaf_taps in
-    include/kymatio_hip.h).
+    """Taps and band tables of the separable low-pass leaves
+    (kym_scat2d_leaf_taps in include/kymatio_hip.h).
 
     phi_levels[r] is the Fourier-domain low-pass filter at resolution r (the
     buffers the spectral leaf multiplies with); its spatial form
     phi_r = Re(ifft2(phi_levels[r])) is separable, phi_r[a][b] = g[a] h[b]
     with g = phi_r[:, 0] / sqrt(phi_r[0, 0]) and h = phi_r[0, :] / sqrt(phi_r[0, 0]).
-    desc = (taps_off, taps_len, M_padded, N_padded, then per r: off_y, lo_y,
-    cnt_y, off_x, lo_x, cnt_x).  Returns the taps_len reals to store at
+    desc = (taps_off, taps_len, M_padded, N_padded, then per r < J and axis
+    (y, then x): off, lo, cnt, band_off, kmax).  Per axis of n samples the
+    taps are c[i] = g[(lo + i) mod n], i < cnt, and the band table is
+    band[k][d] = c[(k - kl d) mod n] (0 where that index is >= cnt),
+    k < kmax, d < 16, kl = 2^(J - r).  Returns the taps_len reals to store at
     taps_off of the packed filter buffer (float64).  Raises RuntimeError if a
     filter is not separable or the plan's tap window drops more than 1e-6 of
     its mass, which does not happen for the filter bank of this module.
@@ -119,7 +122,8 @@ def leaf_taps(phi_levels, desc):
     out = np.zeros(taps_len, np.float64)
     if taps_len == 0:
         return out
-    for r in range((len(desc) - 4) // 6):
+    J = (len(desc) - 4) // 10
+    for r in range(J):
         ph = np.asarray(phi_levels[r], np.float64)
         sp = np.real(np.fft.ifft2(ph))
         c00 = sp[0, 0]
@@ -130,12 +134,16 @@ def leaf_taps(phi_levels, desc):
         resid = np.abs(sp - np.outer(g, h)).max() / np.abs(sp).max()
         if resid > 1e-5:
             raise RuntimeError('low-pass filter %d is not separable (residual %.2e)' % (r, resid))
-        for v, base in ((g, 4 + 6 * r), (h, 4 + 6 * r + 3)):
-            off, lo, cnt = int(desc[base]), int(desc[base + 1]), int(desc[base + 2])
-            idx = (lo + np.arange(cnt)) % v.shape[0]
-            kept = v[idx]
+        kl = 2 ** (J - r)
+        for v, base in ((g, 4 + 10 * r), (h, 4 + 10 * r + 5)):
+            off, lo, cnt, band_off, kmax = [int(t) for t in desc[base:base + 5]]
+            n = v.shape[0]
+            kept = v[(lo + np.arange(cnt)) % n]
             dropped = 1.0 - np.abs(kept).sum() / np.abs(v).sum()
             if dropped > 1e-6:
                 raise RuntimeError('leaf taps of level %d drop %.2e of the filter' % (r, dropped))
             out[off - taps_off:off - taps_off + cnt] = kept
+            t = (np.arange(kmax)[:, None] - kl * np.arange(16)[None, :]) % n
+            band = np.where(t < cnt, kept[np.minimum(t, cnt - 1)], 0.0)
+            out[band_off - taps_off:band_off - taps_off + 16 * kmax] = band.reshape(-1)
     return out

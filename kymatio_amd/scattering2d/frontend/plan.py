@@ -27,9 +27,9 @@ class Scat2DPlanHandle:
         _lib.check(lib.kym_scat2d_filter_offsets(h, offs))
         self.filter_offsets = list(offs)
         J = int(J)
-        desc = (ctypes.c_longlong * (4 + 6 * max(J, 1)))()
+        desc = (ctypes.c_longlong * (4 + 10 * max(J, 1)))()
         _lib.check(lib.kym_scat2d_leaf_taps(h, desc, len(desc)))
-        self.leaf_taps_desc = list(desc)[:4 + 6 * J]
+        self.leaf_taps_desc = list(desc)[:4 + 10 * J]
         self.dtype = dtype
 
     KIND_NAMES = ['lds_node', 'leaf', 'root_rows', 'icol', 'rows_mod', 'fwd_cols', 'leaf_icol', 'leaf_rows',

@@ -74,12 +74,12 @@ class Emu:
         f = self.lib.kym_emu_scat2d_taps
         f.argtypes = [ctypes.c_int] * 7 + [ctypes.c_longlong, ctypes.c_void_p, ctypes.c_int]
         f.restype = ctypes.c_int
-        n = 4 + 6 * max(J, 1)
+        n = 4 + 10 * max(J, 1)
         desc = (ctypes.c_longlong * n)()
         rc = f(shape[0], shape[1], J, L, max_order, pre_pad, code, lds_limit, desc, n)
         if rc != 0:
             raise RuntimeError(self.lib.kym_emu_last_error().decode())
-        desc = list(desc)[:4 + 6 * J]
+        desc = list(desc)[:4 + 10 * J]
         if desc[1] == 0:
             return filters_packed
         Mp, Np = desc[2], desc[3]
