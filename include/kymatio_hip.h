@@ -50,10 +50,11 @@ int kym_scat2d_create(int M, int N, int J, int L, int max_order, int pre_pad, in
                       long long lds_limit, void** plan);
 
 /* kym_scat2d_create with tuning choices (the autotuner's knobs; no
- * counterpart in the reference): tune[0..ntune) = {largest power-of-two FFT
- * radix (16/8/4/2), LDS-node threads (256/512/1024), big-node column block,
- * big-node row block}; 0 = built-in heuristic.  Results are the same for
- * every choice up to FP32 rounding. */
+ * counterpart in the reference): tune[0..ntune), ntune <= 5, = {largest
+ * power-of-two FFT radix (16/8/4/2), LDS-node threads (256/512/1024),
+ * big-node column block, big-node row block, big-node pass threads (a
+ * compiled choice for these sizes and blocks)}; 0 = built-in heuristic.
+ * Results are the same for every choice up to FP32 rounding. */
 int kym_scat2d_create_tuned(int M, int N, int J, int L, int max_order, int pre_pad, int dtype,
                             long long lds_limit, const int* tune, int ntune, void** plan);
 int kym_scat2d_destroy(void* plan);

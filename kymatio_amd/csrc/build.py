@@ -22,7 +22,8 @@ COMMON = ['-O3', '-std=c++17', '-fPIC', '-shared', '-Wno-unused-result',
           '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops']
 
 # (source, extra defines, object name); compiled in parallel, then linked
-PRODUCT_UNITS = [('capi.hip', [], 'capi.o'), ('kspec2d.hip', [], 'kspec2d.o'), ('kinst1d.hip', [], 'kinst1d.o'),
+PRODUCT_UNITS = [('kpass2d.hip', [], 'kpass2d.o'), ('kspec2d.hip', [], 'kspec2d.o'), ('capi.hip', [], 'capi.o'),
+                 ('kinst1d.hip', [], 'kinst1d.o'),
                  ('ops.hip', [], 'ops.o'), ('decoder.hip', [], 'decoder.o')] + [
     ('kinst2d.hip', ['-DKYM_T=%s' % T, '-DKYM_P=%d' % P, '-DKYM_PICK=kym_pick2d_%s_p%d' % (tag, P)],
      'kinst2d_%s_p%d.o' % (tag, P))

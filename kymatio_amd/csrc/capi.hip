@@ -93,9 +93,9 @@ extern "C" int kym_scat2d_create_tuned(int M, int N, int J, int L, int max_order
     if (!out) return fail(KYM_ERR_VALUE, "null plan pointer");
     *out = nullptr;
     if (dtype != KYM_F32 && dtype != KYM_F64) return fail(KYM_ERR_TYPE, "dtype must be float32 or float64");
-    if (ntune < 0 || ntune > 4 || (ntune > 0 && !tune)) return fail(KYM_ERR_VALUE, "tuning array must hold 0-4 values");
+    if (ntune < 0 || ntune > 5 || (ntune > 0 && !tune)) return fail(KYM_ERR_VALUE, "tuning array must hold 0-5 values");
     Scat2DTune t;
-    int* fields[4] = {&t.max_pow2, &t.node_threads, &t.col_block, &t.row_block};
+    int* fields[5] = {&t.max_pow2, &t.node_threads, &t.col_block, &t.row_block, &t.pass_threads};
     for (int i = 0; i < ntune; ++i)
         if (tune[i] != 0) *fields[i] = tune[i];
     auto h = std::make_unique<KymScat2D>();

@@ -1,5 +1,4 @@
-// Compile-time specialised 2D node and big-node pass kernels (FP32).
+// Compile-time specialised 2D LDS-node kernels (FP32).
 #include "kernels2d.h"
 
 Scat2DKernel<float> kym_pick_node_spec_f32(int id) { return pick_node_spec<float>(id); }
-Scat2DKernel<float> kym_pick_pass_spec_f32(int kind, int id) { return pick_pass_spec<float>(kind, id); }

@@ -893,11 +893,19 @@ KYM_HD void dispatch_node_spec(int id, const Scat2DArgs<T>& a, int task, cpx<T>*
     X(0, 272, 272, 16, 8, 256)    \
     X(1, 272, 272, 16, 4, 256)    \
     X(2, 272, 272, 8, 8, 256)     \
-    X(3, 272, 272, 32, 8, 256)
+    X(3, 272, 272, 32, 8, 256)    \
+    X(4, 272, 272, 16, 16, 256)   \
+    X(5, 272, 272, 16, 16, 320)   \
+    X(6, 272, 272, 17, 17, 320)   \
+    X(7, 272, 272, 8, 8, 192)
 
-inline int pass_spec_id(int H, int W, int cb, int rb, int* threads) {
-#define KYM_PS_MATCH(id, h, w, c, r, nt) \
-    if (H == h && W == w && cb == c && rb == r) { *threads = nt; return id; }
+// want_threads = 0: the first specialisation of these sizes and blocks.
+inline int pass_spec_id(int H, int W, int cb, int rb, int want_threads, int* threads) {
+#define KYM_PS_MATCH(id, h, w, c, r, nt)                                                        \
+    if (H == h && W == w && cb == c && rb == r && (want_threads == 0 || want_threads == nt)) { \
+        *threads = nt;                                                                        \
+        return id;                                                                            \
+    }
     KYM_PASS_SPECS(KYM_PS_MATCH)
 #undef KYM_PS_MATCH
     return -1;

@@ -27,6 +27,7 @@ struct Scat2DTune {
     int node_threads = 0;   // threads of LDS-node workgroups: 256, 512 or 1024
     int col_block = 0;      // columns per workgroup of the big-node column passes
     int row_block = 0;      // rows per workgroup of the big-node row passes
+    int pass_threads = 0;   // threads of the big-node pass workgroups (compiled choices only)
 };
 
 template <class T>
@@ -101,7 +102,9 @@ struct Scat2DPlan {
         if (!(tune.max_pow2 == 16 || tune.max_pow2 == 8 || tune.max_pow2 == 4 || tune.max_pow2 == 2) ||
             !(tune.node_threads == 0 || tune.node_threads == 256 || tune.node_threads == 512 ||
               tune.node_threads == 1024) ||
-            tune.col_block < 0 || tune.col_block > 64 || tune.row_block < 0 || tune.row_block > 64) {
+            tune.col_block < 0 || tune.col_block > 64 || tune.row_block < 0 || tune.row_block > 64 ||
+            !(tune.pass_threads == 0 || tune.pass_threads == 256 || (tune.pass_threads % 64 == 0 &&
+                                                                      tune.pass_threads <= 1024))) {
             *e = {KYM_ERR_VALUE, "invalid tuning choice"};
             return false;
         }
@@ -321,6 +324,7 @@ struct Scat2DPlan {
             max_threads = std::max(max_threads, threads);
         };
         const size_t pass_budget = std::min(lds_limit, (size_t)64 * 1024);
+        bool no_pass_spec = false;  // a tuned pass thread count with no compiled kernel
         // pass blocks: the heuristic keeps a block within 64 KB of LDS; a tuned
         // block may use up to the LDS limit
         auto col_blk = [&](int Hh) {
@@ -381,8 +385,10 @@ struct Scat2DPlan {
                 const int cb = col_blk(H[r]), rb = row_blk(r);
                 // compile-time passes when the sizes match and every block is full
                 int pnt = 0;
+                const int want = tune.pass_threads == 256 ? 0 : tune.pass_threads;
                 const int psid = (specs && sizeof(T) == 4 && H[r] % rb == 0 && W[r] % cb == 0)
-                                     ? pass_spec_id(H[r], W[r], cb, rb, &pnt) : -1;
+                                     ? pass_spec_id(H[r], W[r], cb, rb, want, &pnt) : -1;
+                if (tune.pass_threads && tune.pass_threads != 256 && psid < 0) no_pass_spec = true;
                 auto pass = [&](int kind, int nblk, int blk, size_t lds) {
                     push(kind, ids, nblk, blk, 256, lds);
                     if (psid >= 0) { launches.back().spec = psid; launches.back().threads = pnt; }
@@ -405,6 +411,10 @@ struct Scat2DPlan {
                         if (nodes[i].order == order && nodes[i].res == r && nodes[i].big == big) ids.push_back((int)i);
                     emit_nodes(ids, false);
                 }
+        if (no_pass_spec) {
+            *e = {KYM_ERR_VALUE, "invalid tuning choice: no compiled pass for these threads and blocks"};
+            return false;
+        }
         // backward: order 2, order 1, root; then the pad fold
         std::vector<Launch2D> fwd = launches;
         launches.clear();

@@ -17,7 +17,7 @@ from .decoder import ScatteringDecoder1D
 from .recon import reconstruct, rel_loss
 from .train import CHECKPOINT, chirps
 
-LRS = (0.3, 0.1)
+LRS = (0.3, 0.1, 0.03)
 
 
 def load_decoder(path, device):

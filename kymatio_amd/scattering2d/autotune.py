@@ -15,20 +15,25 @@ import torch
 
 DEFAULT_CACHE = os.path.join(os.path.expanduser('~'), '.cache', 'kymatio_amd', 'autotune2d.json')
 
-# (max_pow2, node_threads, col_block, row_block, lds_limit, images_per_chunk) - 0 = heuristic
+# (max_pow2, node_threads, col_block, row_block, pass_threads, lds_limit,
+#  images_per_chunk) - 0 = heuristic
+FIELDS = ('max_pow2_radix', 'node_threads', 'col_block', 'row_block', 'pass_threads', 'lds_limit', 'images_per_chunk')
 CANDIDATES = [
-    (16, 0, 0, 0, 0, 0),
-    (8, 0, 0, 0, 0, 0),
-    (16, 256, 0, 0, 0, 0),
-    (16, 512, 0, 0, 0, 0),
-    (16, 1024, 0, 0, 0, 0),
-    (16, 0, 16, 4, 0, 0),
-    (16, 0, 8, 8, 0, 0),
-    (16, 0, 32, 8, 0, 0),
-    (8, 0, 8, 4, 0, 0),
-    (16, 0, 0, 0, 96 * 1024, 0),
-    (16, 0, 0, 0, 0, 64),
-    (16, 0, 0, 0, 0, 16),
+    (16, 0, 0, 0, 0, 0, 0),
+    (8, 0, 0, 0, 0, 0, 0),
+    (16, 256, 0, 0, 0, 0, 0),
+    (16, 512, 0, 0, 0, 0, 0),
+    (16, 0, 8, 8, 0, 0, 0),
+    (16, 0, 16, 4, 0, 0, 0),
+    (16, 0, 16, 16, 0, 0, 0),
+    (16, 0, 32, 8, 0, 0, 0),
+    (16, 0, 16, 16, 320, 0, 0),
+    (16, 0, 17, 17, 320, 0, 0),
+    (16, 0, 8, 8, 192, 0, 0),
+    (8, 0, 8, 4, 0, 0, 0),
+    (16, 0, 0, 0, 0, 96 * 1024, 0),
+    (16, 0, 8, 8, 0, 0, 64),
+    (16, 0, 8, 8, 0, 0, 16),
 ]
 
 _TABLE = {}
@@ -111,5 +116,5 @@ def tune(module, batch, reps=3, candidates=None, cache_path=None, timer=None):
 
 
 def _apply(module, c):
-    module.tune, module.lds_limit = list(c[:4]), c[4]
-    module.images_per_chunk = c[5] if len(c) > 5 and c[5] else None
+    module.tune, module.lds_limit = list(c[:5]), c[5]
+    module.images_per_chunk = c[6] if len(c) > 6 and c[6] else None

@@ -18,7 +18,7 @@ from kymatio_amd.scattering2d.utils import compute_padding
 from tests.helpers import RTOL_CHANNEL, Emu, channel_rel_err, golden_2d_cases, pack_filters_2d
 
 TUNES = [(8, 0, 0, 0), (4, 0, 0, 0), (2, 0, 0, 0), (16, 256, 0, 0), (16, 1024, 0, 0), (16, 0, 8, 4),
-         (8, 512, 4, 2)]
+         (8, 512, 4, 2), (16, 0, 0, 0, 256)]
 
 
 @pytest.fixture(scope='module')
@@ -43,7 +43,8 @@ def test_tuned_plans_match_reference(emu, tag, lds, tune):
     assert channel_rel_err(out, Sx) < RTOL_CHANNEL
 
 
-@pytest.mark.parametrize('tune', [(8, 0, 8, 4), (16, 0, 4, 2)])
+@pytest.mark.parametrize('tune', [(8, 0, 8, 4), (16, 0, 4, 2), (16, 0, 17, 17, 320), (16, 0, 8, 8, 192),
+                                  (16, 0, 16, 16, 320)])
 def test_tuned_big_nodes_match_reference(emu, tune):
     """J=3 256x256: the order-1 nodes at 272 x 272 run as column / row passes."""
     _, x, J, shape, kw, Sx = _case('cfg2')
@@ -61,12 +62,15 @@ def test_invalid_tuning_rejected(emu):
         emu.scat2d(np.zeros((1, 32, 32), np.float32), 2, (32, 32), f, tune=[12, 0, 0, 0])
     with pytest.raises(RuntimeError, match='tuning'):
         emu.scat2d(np.zeros((1, 32, 32), np.float32), 2, (32, 32), f, tune=[16, 300, 0, 0])
+    f256 = pack_filters_2d(filter_bank(272, 272, 3, 8))
+    with pytest.raises(RuntimeError, match='tuning'):  # no compiled pass of 320 threads and 8 x 8 blocks
+        emu.scat2d(np.zeros((1, 256, 256), np.float32), 3, (256, 256), f256, tune=[16, 0, 8, 8, 320])
 
 
 def test_autotune_picks_fastest_and_caches(tmp_path):
     m = ScatteringTorch2D(2, (32, 32))
-    cands = [(16, 0, 0, 0, 0, 0), (8, 0, 0, 0, 0, 0), (16, 256, 0, 0, 0, 8)]
-    fake = {(16, 0, 0, 0): 3.0, (8, 0, 0, 0): 1.0, (16, 256, 0, 0): 2.0}
+    cands = [(16, 0, 0, 0, 0, 0, 0), (8, 0, 0, 0, 0, 0, 0), (16, 256, 0, 0, 0, 0, 8)]
+    fake = {(16, 0, 0, 0, 0): 3.0, (8, 0, 0, 0, 0): 1.0, (16, 256, 0, 0, 0): 2.0}
     calls = []
 
     def timer(module, x):
@@ -76,11 +80,11 @@ def test_autotune_picks_fastest_and_caches(tmp_path):
 
     path = str(tmp_path / 'tune.json')
     choice, times = autotune.tune(m, 5, candidates=cands, cache_path=path, timer=timer)
-    assert choice == (8, 0, 0, 0, 0, 0)
-    assert m.tune == [8, 0, 0, 0] and m.lds_limit == 0 and m.images_per_chunk is None
+    assert choice == (8, 0, 0, 0, 0, 0, 0)
+    assert m.tune == [8, 0, 0, 0, 0] and m.lds_limit == 0 and m.images_per_chunk is None
     assert len(calls) == 3
     table = json.load(open(path))
-    assert list(table.values()) == [[8, 0, 0, 0, 0, 0]]
+    assert list(table.values()) == [[8, 0, 0, 0, 0, 0, 0]]
     # second module of the same configuration: cached, no timing
     autotune._TABLE.clear()
     m2 = ScatteringTorch2D(2, (32, 32))
@@ -97,6 +101,6 @@ def test_autotune_picks_fastest_and_caches(tmp_path):
 
     m3 = ScatteringTorch2D(2, (32, 32))
     choice3, _ = autotune.tune(m3, 5, candidates=cands, timer=picky)
-    assert choice3 == (16, 256, 0, 0, 0, 8)
+    assert choice3 == (16, 256, 0, 0, 0, 0, 8)
     assert m3.images_per_chunk == 8
     autotune._TABLE.clear()
