@@ -26,7 +26,15 @@ KYM_HD KYM_INL int kym_rev_s(int p) {
     return value;
 }
 
-template <int R, bool INV, int N, int LP, int NTR, int ES, int TS, bool TFAST, int NT, class T>
+// TWL: the twiddle table was copied to LDS (plain loads), else it is read
+// from global memory.
+template <bool TWL, class T>
+KYM_HD KYM_INL cpx<T> tw_at(const cpx<T>* tw, int i) {
+    if constexpr (TWL) return tw[i];
+    else return kym_ldg(tw + i);
+}
+
+template <int R, bool INV, int N, int LP, int NTR, int ES, int TS, bool TFAST, int NT, class T, bool TWL = false>
 KYM_HD KYM_INL void dit_pass_s(cpx<T>* buf, const cpx<T>* tw, int tid) {
     constexpr int NB = N / R, L = LP * R, TSTEP = N / L, TOTAL = NB * NTR, ST = LP * ES;
     for (int f = tid; f < TOTAL; f += NT) {
@@ -42,7 +50,7 @@ KYM_HD KYM_INL void dit_pass_s(cpx<T>* buf, const cpx<T>* tw, int tid) {
             if (j != 0) {
 #pragma unroll
                 for (int m = 1; m < R; ++m) {
-                    const cpx<T> w = kym_ldg(tw + j * m * TSTEP);
+                    const cpx<T> w = tw_at<TWL>(tw, j * m * TSTEP);
                     v[m] = INV ? cmulc(v[m], w) : cmul(v[m], w);
                 }
             }
@@ -53,7 +61,7 @@ KYM_HD KYM_INL void dit_pass_s(cpx<T>* buf, const cpx<T>* tw, int tid) {
     }
 }
 
-template <int R, bool INV, int N, int L, int NTR, int ES, int TS, bool TFAST, int NT, class T>
+template <int R, bool INV, int N, int L, int NTR, int ES, int TS, bool TFAST, int NT, class T, bool TWL = false>
 KYM_HD KYM_INL void dif_pass_s(cpx<T>* buf, const cpx<T>* tw, int tid) {
     constexpr int NB = N / R, LS = L / R, TSTEP = N / L, TOTAL = NB * NTR, ST = LS * ES;
     for (int f = tid; f < TOTAL; f += NT) {
@@ -70,7 +78,7 @@ KYM_HD KYM_INL void dif_pass_s(cpx<T>* buf, const cpx<T>* tw, int tid) {
             if (j != 0) {
 #pragma unroll
                 for (int m = 1; m < R; ++m) {
-                    const cpx<T> w = kym_ldg(tw + j * m * TSTEP);
+                    const cpx<T> w = tw_at<TWL>(tw, j * m * TSTEP);
                     v[m] = INV ? cmulc(v[m], w) : cmul(v[m], w);
                 }
             }
@@ -81,28 +89,30 @@ KYM_HD KYM_INL void dif_pass_s(cpx<T>* buf, const cpx<T>* tw, int tid) {
 }
 
 // Full DIT (digit-reversed in, natural out) over a batch, barrier after each pass.
-template <bool INV, int N, int NTR, int ES, int TS, bool TFAST, int NT, class T, int Q = 0, int LP = 1>
+template <bool INV, int N, int NTR, int ES, int TS, bool TFAST, int NT, bool TWL = false, class T, int Q = 0,
+          int LP = 1>
 KYM_HD KYM_INL void fft_dit_s(cpx<T>* buf, const cpx<T>* tw, int tid) {
     constexpr RadList rl = kym_factor_list(N);
     static_assert(rl.ok, "unsupported static FFT length");
     if constexpr (Q < rl.nrad) {
         constexpr int R = rl.rad[Q];
-        dit_pass_s<R, INV, N, LP, NTR, ES, TS, TFAST, NT, T>(buf, tw, tid);
+        dit_pass_s<R, INV, N, LP, NTR, ES, TS, TFAST, NT, T, TWL>(buf, tw, tid);
         wg_barrier();
-        fft_dit_s<INV, N, NTR, ES, TS, TFAST, NT, T, Q + 1, LP * R>(buf, tw, tid);
+        fft_dit_s<INV, N, NTR, ES, TS, TFAST, NT, TWL, T, Q + 1, LP * R>(buf, tw, tid);
     }
 }
 
 // Full DIF (natural in, digit-reversed out); passes in reverse radix order.
-template <bool INV, int N, int NTR, int ES, int TS, bool TFAST, int NT, class T, int Q = -1, int L = N>
+template <bool INV, int N, int NTR, int ES, int TS, bool TFAST, int NT, bool TWL = false, class T, int Q = -1,
+          int L = N>
 KYM_HD KYM_INL void fft_dif_s(cpx<T>* buf, const cpx<T>* tw, int tid) {
     constexpr RadList rl = kym_factor_list(N);
     static_assert(rl.ok, "unsupported static FFT length");
     constexpr int q = (Q < 0) ? rl.nrad - 1 : Q;
     if constexpr (q >= 0 && L > 1) {
         constexpr int R = rl.rad[q];
-        dif_pass_s<R, INV, N, L, NTR, ES, TS, TFAST, NT, T>(buf, tw, tid);
+        dif_pass_s<R, INV, N, L, NTR, ES, TS, TFAST, NT, T, TWL>(buf, tw, tid);
         wg_barrier();
-        if constexpr (q > 0) fft_dif_s<INV, N, NTR, ES, TS, TFAST, NT, T, q - 1, L / R>(buf, tw, tid);
+        if constexpr (q > 0) fft_dif_s<INV, N, NTR, ES, TS, TFAST, NT, TWL, T, q - 1, L / R>(buf, tw, tid);
     }
 }

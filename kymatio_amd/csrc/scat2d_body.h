@@ -260,13 +260,16 @@ struct StatGeom {
     KYM_HD const int* revWJ() const { return fWJ->rev; }
     KYM_HD int rh(int p) const { return kym_rev_s<H>(p); }
     KYM_HD int rw(int p) const { return kym_rev_s<W>(p); }
+    // twiddle tables of the node's lengths, copied to LDS by the kernel
+    const cpx<T>* twH;
+    const cpx<T>* twW;
     KYM_HD void inv2d(cpx<T>* b, int tid, int) const {
-        fft_dif_s<true, W, H, 1, W, false, NT>(b, fW->tw, tid);
-        fft_dif_s<true, H, W, W, 1, true, NT>(b, fH->tw, tid);
+        fft_dif_s<true, W, H, 1, W, false, NT, true>(b, twW, tid);
+        fft_dif_s<true, H, W, W, 1, true, NT, true>(b, twH, tid);
     }
     KYM_HD void fwd2d(cpx<T>* b, int tid, int) const {
-        fft_dit_s<false, W, H, 1, W, false, NT>(b, fW->tw, tid);
-        fft_dit_s<false, H, W, W, 1, true, NT>(b, fH->tw, tid);
+        fft_dit_s<false, W, H, 1, W, false, NT, true>(b, twW, tid);
+        fft_dit_s<false, H, W, W, 1, true, NT, true>(b, twH, tid);
     }
     KYM_HD void leaf_inv2d(cpx<T>* b, int tid, int) const {
         fft_dif_s<true, WJ, HJ, 1, WJ, false, NT>(b, fWJ->tw, tid);
@@ -584,14 +587,31 @@ KYM_HD void body_lds_node(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid
 
 // The same with compile-time sizes (the plan only selects it when the
 // launch's node and leaf sizes equal the template's).
+// Complex elements at the start of a compile-time kernel's LDS that hold
+// twiddle tables (lengths h and w; one table when equal), kept 16-byte aligned.
+KYM_HD constexpr int kym_tw_elems(int h, int w) { return ((w + (h != w ? h : 0)) + 1) / 2 * 2; }
+
+// Copies the twiddles of lengths fh->n and fw->n to tw (LDS); barrier.
+template <class T>
+KYM_HD KYM_INL void load_twiddles(cpx<T>* tw, const FftLen<T>* fh, const FftLen<T>* fw, int h, int w, int tid,
+                                  int nth) {
+    for (int i = tid; i < w; i += nth) tw[i] = kym_ldg(fw->tw + i);
+    if (h != w)
+        for (int i = tid; i < h; i += nth) tw[w + i] = kym_ldg(fh->tw + i);
+    wg_barrier();
+}
+
 template <class T, int H, int W, int HJ, int WJ, int NT>
 KYM_HD void body_lds_node_s(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid) {
     const int il = task / a.ngroup, gi = task - il * a.ngroup;
     const Node2D& nd = a.nodes[a.group[gi]];
     const int r = nd.res, J = a.J;
-    const StatGeom<T, H, W, HJ, WJ, NT> g{&a.lens[a.lenH[r]], &a.lens[a.lenW[r]], &a.lens[a.lenH[J]],
-                                          &a.lens[a.lenW[J]]};
-    node_body(a, g, nd, il, lds, tid, NT);
+    constexpr int TWE = kym_tw_elems(H, W);
+    const FftLen<T>* fH = &a.lens[a.lenH[r]];
+    const FftLen<T>* fW = &a.lens[a.lenW[r]];
+    load_twiddles(lds, fH, fW, H, W, tid, NT);
+    const StatGeom<T, H, W, HJ, WJ, NT> g{fH, fW, &a.lens[a.lenH[J]], &a.lens[a.lenW[J]], lds + (H != W ? W : 0), lds};
+    node_body(a, g, nd, il, lds + TWE, tid, NT);
 }
 
 // Leaf from a spectrum kept in the workspace.  task -> (image, node).
@@ -626,19 +646,22 @@ struct DynPass {
     }
 };
 
+// twH / twW: the twiddle tables copied to LDS by the kernel.
 template <class T, int H, int W, int CB, int RB, int NT>
 struct StatPass {
-    KYM_HD void col_inv(cpx<T>* b, const FftLen<T>& f, int, int tid, int) const {
-        fft_dif_s<true, H, CB, CB, 1, true, NT>(b, f.tw, tid);
+    const cpx<T>* twH;
+    const cpx<T>* twW;
+    KYM_HD void col_inv(cpx<T>* b, const FftLen<T>&, int, int tid, int) const {
+        fft_dif_s<true, H, CB, CB, 1, true, NT, true>(b, twH, tid);
     }
-    KYM_HD void col_fwd(cpx<T>* b, const FftLen<T>& f, int, int tid, int) const {
-        fft_dit_s<false, H, CB, CB, 1, true, NT>(b, f.tw, tid);
+    KYM_HD void col_fwd(cpx<T>* b, const FftLen<T>&, int, int tid, int) const {
+        fft_dit_s<false, H, CB, CB, 1, true, NT, true>(b, twH, tid);
     }
-    KYM_HD void row_inv(cpx<T>* b, const FftLen<T>& f, int, int tid, int) const {
-        fft_dif_s<true, W, RB, 1, W, false, NT>(b, f.tw, tid);
+    KYM_HD void row_inv(cpx<T>* b, const FftLen<T>&, int, int tid, int) const {
+        fft_dif_s<true, W, RB, 1, W, false, NT, true>(b, twW, tid);
     }
-    KYM_HD void row_fwd(cpx<T>* b, const FftLen<T>& f, int, int tid, int) const {
-        fft_dit_s<false, W, RB, 1, W, false, NT>(b, f.tw, tid);
+    KYM_HD void row_fwd(cpx<T>* b, const FftLen<T>&, int, int tid, int) const {
+        fft_dit_s<false, W, RB, 1, W, false, NT, true>(b, twW, tid);
     }
 };
 
@@ -923,14 +946,29 @@ KYM_HD KYM_INL void dispatch_pass(const P& fp, const Scat2DArgs<T>& a, int task,
     else if constexpr (KIND == K2_FWD_COLS) body_fwd_cols<T>(fp, a, task, lds, tid, nth);
 }
 
+// The plan's FFT length record of length n (one of its H_r or W_r).
+template <class T>
+KYM_HD KYM_INL const FftLen<T>* len_of(const Scat2DArgs<T>& a, int n) {
+    for (int r = 0; r <= a.J; ++r) {
+        if (a.H[r] == n) return &a.lens[a.lenH[r]];
+        if (a.W[r] == n) return &a.lens[a.lenW[r]];
+    }
+    return &a.lens[a.lenH[0]];
+}
+
 // Static pass body: EMU = true runs it with one thread (CPU harness).
 template <class T, int KIND, bool EMU>
 KYM_HD void dispatch_pass_spec(int id, const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid) {
     switch (id) {
-#define KYM_PS_CASE(sid, h, w, c, r, nt)                                                          \
-    case sid:                                                                                     \
-        dispatch_pass<T, KIND>(StatPass<T, h, w, c, r, (EMU ? 1 : nt)>(), a, task, lds, tid, EMU ? 1 : nt); \
-        break;
+#define KYM_PS_CASE(sid, h, w, c, r, nt)                                                           \
+    case sid: {                                                                                    \
+        const FftLen<T>* fh = len_of(a, h);                                                        \
+        const FftLen<T>* fw = len_of(a, w);                                                        \
+        load_twiddles(lds, fh, fw, h, w, tid, EMU ? 1 : nt);                                       \
+        const StatPass<T, h, w, c, r, (EMU ? 1 : nt)> fp{lds + (h != w ? w : 0), lds};             \
+        dispatch_pass<T, KIND>(fp, a, task, lds + kym_tw_elems(h, w), tid, EMU ? 1 : nt);          \
+        break;                                                                                     \
+    }
         KYM_PASS_SPECS(KYM_PS_CASE)
 #undef KYM_PS_CASE
         default: break;

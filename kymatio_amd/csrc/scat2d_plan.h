@@ -377,9 +377,12 @@ struct Scat2DPlan {
                 push(K2_LDS_NODE, ids, 1, 0, node_threads(elems), lds);
                 int nt = 0;
                 const int sid = node_spec_id(H[r], W[r], H[J], W[J], tune.node_threads, &nt);
-                if (specs && sizeof(T) == 4 && sid >= 0 && (n0.fir == 1 || n0.leaf_mode == 0)) {
+                if (specs && sizeof(T) == 4 && sid >= 0 && (n0.fir == 1 || n0.leaf_mode == 0) &&
+                    lds + cs * kym_tw_elems(H[r], W[r]) <= lds_limit) {
                     launches.back().spec = sid;
                     launches.back().threads = nt;
+                    launches.back().lds_bytes = lds + cs * kym_tw_elems(H[r], W[r]);  // + twiddles
+                    max_lds = std::max(max_lds, launches.back().lds_bytes);
                 }
             } else {
                 const int cb = col_blk(H[r]), rb = row_blk(r);
@@ -391,7 +394,13 @@ struct Scat2DPlan {
                 if (tune.pass_threads && tune.pass_threads != 256 && psid < 0) no_pass_spec = true;
                 auto pass = [&](int kind, int nblk, int blk, size_t lds) {
                     push(kind, ids, nblk, blk, 256, lds);
-                    if (psid >= 0) { launches.back().spec = psid; launches.back().threads = pnt; }
+                    const size_t tw = cs * kym_tw_elems(H[r], W[r]);
+                    if (psid >= 0 && lds + tw <= lds_limit) {
+                        launches.back().spec = psid;
+                        launches.back().threads = pnt;
+                        launches.back().lds_bytes = lds + tw;  // + twiddles
+                        max_lds = std::max(max_lds, launches.back().lds_bytes);
+                    }
                 };
                 if (is_root) pass(K2_ROOT_ROWS, H[r] / rb + (H[r] % rb != 0), rb, row_lds(r, rb));
                 else {
