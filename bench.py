@@ -152,7 +152,7 @@ def main():
     achieved = dom['model_bytes_per_launch'] / (dom['mean_us'] * 1e-6)
     per_node_image = measured_traffic_per_node_image()
     traffic = None
-    if per_node_image is not None and dom['kind'] == 'lds_node' and dom['res'] == 1 and dom['nodes'] == 64:
+    if per_node_image is not None and dom['kind'] == 'lds_node' and dom['res'] == 1:
         traffic = per_node_image * dom['nodes'] * BATCH * args.steps / dom['launches']
     total_model = sum(l['model_bytes_per_image'] for l in launches) * BATCH
     gpu_ms = sum(r['total_ms'] for r in per_kernel) / args.steps

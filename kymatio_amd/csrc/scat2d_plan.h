@@ -411,15 +411,24 @@ struct Scat2DPlan {
             }
             emit_leaves(ids);
         };
+        // By resolution: the big nodes of resolution r (order 1, then order 2),
+        // then ONE launch for every LDS node of resolution r, first and second
+        // order together.  A node's parent always has a lower resolution, so it
+        // was written by an earlier launch; merging the orders fills the GPU
+        // with one launch instead of a small one plus a large one.
         emit_nodes({0}, true);
-        for (int order = 1; order <= max_order; ++order)
-            for (int r = 0; r < J; ++r)
-                for (int big = 0; big <= 1; ++big) {
-                    std::vector<int> ids;
-                    for (size_t i = 0; i < nodes.size(); ++i)
-                        if (nodes[i].order == order && nodes[i].res == r && nodes[i].big == big) ids.push_back((int)i);
-                    emit_nodes(ids, false);
-                }
+        for (int r = 0; r < J; ++r) {
+            for (int order = 1; order <= max_order; ++order) {
+                std::vector<int> ids;
+                for (size_t i = 0; i < nodes.size(); ++i)
+                    if (nodes[i].order == order && nodes[i].res == r && nodes[i].big) ids.push_back((int)i);
+                emit_nodes(ids, false);
+            }
+            std::vector<int> ids;
+            for (size_t i = 0; i < nodes.size(); ++i)
+                if (nodes[i].order >= 1 && nodes[i].res == r && !nodes[i].big) ids.push_back((int)i);
+            emit_nodes(ids, false);
+        }
         if (no_pass_spec) {
             *e = {KYM_ERR_VALUE, "invalid tuning choice: no compiled pass for these threads and blocks"};
             return false;
