@@ -325,6 +325,7 @@ struct Scat2DPlan {
         };
         const size_t pass_budget = std::min(lds_limit, (size_t)64 * 1024);
         bool no_pass_spec = false;  // a tuned pass thread count with no compiled kernel
+        std::vector<int> pending_fir;  // big nodes whose leaf column pass is still to be launched
         // pass blocks: the heuristic keeps a block within 64 KB of LDS; a tuned
         // block may use up to the LDS limit
         auto col_blk = [&](int Hh) {
@@ -359,7 +360,7 @@ struct Scat2DPlan {
                 else if (nodes[id].fir == 0 && nodes[id].leaf_mode == 1) small.push_back(id);
                 else if (nodes[id].fir == 0 && nodes[id].leaf_mode == 2) big.push_back(id);
             }
-            if (!firc.empty()) push(K2_FIR_COLS, firc, 1, 0, 256, fir_cols_lds(nodes[firc[0]].res));
+            for (int id : firc) pending_fir.push_back(id);  // flushed per resolution (one launch)
             if (!small.empty()) push(K2_LEAF, small, 1, 0, node_threads(leaf_elems), leaf_elems * cs);
             if (!big.empty()) {
                 const int cb = col_blk(H[J]), rb = std::max(1, std::min(16, (int)(pass_budget / (cs * W[J]))));
@@ -428,6 +429,15 @@ struct Scat2DPlan {
             for (size_t i = 0; i < nodes.size(); ++i)
                 if (nodes[i].order >= 1 && nodes[i].res == r && !nodes[i].big) ids.push_back((int)i);
             emit_nodes(ids, false);
+            // leaf column passes of this resolution's big nodes (the root's included): one launch
+            if (!pending_fir.empty()) {
+                push(K2_FIR_COLS, pending_fir, 1, 0, 256, fir_cols_lds(r));
+                pending_fir.clear();
+            }
+        }
+        if (!pending_fir.empty()) {  // J = 0: the root alone
+            push(K2_FIR_COLS, pending_fir, 1, 0, 256, fir_cols_lds(0));
+            pending_fir.clear();
         }
         if (no_pass_spec) {
             *e = {KYM_ERR_VALUE, "invalid tuning choice: no compiled pass for these threads and blocks"};
