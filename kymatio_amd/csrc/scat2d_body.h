@@ -417,6 +417,12 @@ KYM_HD KYM_INL void fir_band(T* band, const T* filt, const FirAxis& f, int tid, 
 // natural columns q < W) and outputs x < ow,
 //   T[p][x] = sum_q R[p][q] c[(q - kl (x+1) - lo) mod W]
 // handed to out(p, x, value).  Bx = fir_band(.., W, kl, c, cnt).
+// A 16 x 16 tile reads the window q = qs + kk (mod W), kk < len, in k-steps
+// of four (lane l takes kk = 4 s + l / 16): the steps before the window
+// wraps and after it run as strided segments (plain loads at fixed strides),
+// the step that straddles the wrap and a partial last step one at a time.
+// A rows past nrows repeat row nrows - 1 and B columns past ow are read as
+// they are: they only reach outputs that are not stored.
 template <class T, class OUT>
 KYM_HD void fir_rows_mma(const T* R, int rs, int nrows, int W, int kl, int ow, const FirAxis& fx, const T* Bx,
                          OUT out, int tid, int nth) {
@@ -428,29 +434,47 @@ KYM_HD void fir_rows_mma(const T* R, int rs, int nrows, int W, int kl, int ow, c
         const int len = span < W ? span : W;
         int qs = kl * (x0 + 1) + fx.lo;
         if (qs < 0) qs += W;
-        wave_mma16(
-            lane, (len + 3) >> 2,
-            [&](int l, int s) -> float {
-                const int kk = 4 * s + (l >> 4), p = p0 + (l & 15);
-                int q = qs + kk;
-                if (q >= W) q -= W;
-                const float v = R[(p < nrows ? p : nrows - 1) * rs + q];
-                return (kk < len && p < nrows) ? v : 0.f;
-            },
-            [&](int l, int s) -> float {
-                const int kk = 4 * s + (l >> 4), x = x0 + (l & 15);
-                const float v = Bx[(kk < len ? kk : len - 1) * 16 + (l & 15)];
-                return (kk < len && x < ow) ? v : 0.f;
-            },
-            [&](int l, int v, float val) {
-                const int p = p0 + 4 * (l >> 4) + v, x = x0 + (l & 15);
-                if (p < nrows && x < ow) out(p, x, val);
-            });
+        const int nfull = len >> 2;                                // steps with every kk < len
+        const int s_a = (W - qs) >> 2 < nfull ? (W - qs) >> 2 : nfull;  // no lane wraps before s_a
+        const int s_b = (W - qs + 3) >> 2 < nfull ? (W - qs + 3) >> 2 : nfull;  // every lane wrapped from s_b
+        auto row = [&](int l) { const int p = p0 + (l & 15); return (p < nrows ? p : nrows - 1) * rs; };
+        auto general = [&](Mma16Acc& c, int s) {
+            mma16_step(
+                c, lane,
+                [&](int l) -> float {
+                    const int kk = 4 * s + (l >> 4);
+                    int q = qs + kk;
+                    q = q >= W ? q - W : q;
+                    q = q < W ? q : W - 1;
+                    const float v = R[row(l) + q];
+                    return kk < len ? v : 0.f;
+                },
+                [&](int l) -> float {
+                    const int kk = 4 * s + (l >> 4);
+                    const float v = Bx[(kk < len ? kk : len - 1) * 16 + (l & 15)];
+                    return kk < len ? v : 0.f;
+                });
+        };
+        Mma16Acc c;
+        mma16_zero(c);
+        mma16_strided<4, 64>(
+            c, lane, [&](int l) { return R + row(l) + qs + (l >> 4); },
+            [&](int l) { return Bx + (l >> 4) * 16 + (l & 15); }, s_a);
+        for (int s = s_a; s < s_b; ++s) general(c, s);
+        mma16_strided<4, 64>(
+            c, lane, [&](int l) { return R + row(l) + qs - W + 4 * s_b + (l >> 4); },
+            [&](int l) { return Bx + (4 * s_b + (l >> 4)) * 16 + (l & 15); }, nfull - s_b);
+        if (len & 3) general(c, nfull);
+        mma16_store(c, lane, [&](int l, int v, float val) {
+            const int p = p0 + 4 * (l >> 4) + v, x = x0 + (l & 15);
+            if (p < nrows && x < ow) out(p, x, val);
+        });
     }
 }
 
 // Column pass on the matrix cores: S[y][x] = sum_p c[(p - kl (y+1) - lo) mod H] Tb[p][x]
 // for y < oh, x < ow (Tb row stride ts), into o[y * ow + x].  Ay = fir_band(.., H, kl, c, cnt).
+// Same k-step segments as the row pass, with the window along p.
 template <class T>
 KYM_HD void fir_cols_mma(const T* Tb, int ts, int H, int kl, int oh, int ow, const FirAxis& fy, const T* Ay, T* o,
                          int tid, int nth) {
@@ -462,24 +486,41 @@ KYM_HD void fir_cols_mma(const T* Tb, int ts, int H, int kl, int oh, int ow, con
         const int len = span < H ? span : H;
         int ps = kl * (y0 + 1) + fy.lo;
         if (ps < 0) ps += H;
-        wave_mma16(
-            lane, (len + 3) >> 2,
-            [&](int l, int s) -> float {
-                const int kk = 4 * s + (l >> 4), y = y0 + (l & 15);
-                const float v = Ay[(kk < len ? kk : len - 1) * 16 + (l & 15)];
-                return (kk < len && y < oh) ? v : 0.f;
-            },
-            [&](int l, int s) -> float {
-                const int kk = 4 * s + (l >> 4), x = x0 + (l & 15);
-                int p = ps + kk;
-                if (p >= H) p -= H;
-                const float v = Tb[p * ts + (x < ow ? x : ow - 1)];
-                return (kk < len && x < ow) ? v : 0.f;
-            },
-            [&](int l, int v, float val) {
-                const int y = y0 + 4 * (l >> 4) + v, x = x0 + (l & 15);
-                if (y < oh && x < ow) o[y * ow + x] = val;
-            });
+        const int nfull = len >> 2;
+        const int s_a = (H - ps) >> 2 < nfull ? (H - ps) >> 2 : nfull;
+        const int s_b = (H - ps + 3) >> 2 < nfull ? (H - ps + 3) >> 2 : nfull;
+        auto col = [&](int l) { const int x = x0 + (l & 15); return x < ow ? x : ow - 1; };
+        auto general = [&](Mma16Acc& c, int s) {
+            mma16_step(
+                c, lane,
+                [&](int l) -> float {
+                    const int kk = 4 * s + (l >> 4);
+                    const float v = Ay[(kk < len ? kk : len - 1) * 16 + (l & 15)];
+                    return kk < len ? v : 0.f;
+                },
+                [&](int l) -> float {
+                    const int kk = 4 * s + (l >> 4);
+                    int p = ps + kk;
+                    p = p >= H ? p - H : p;
+                    p = p < H ? p : H - 1;
+                    const float v = Tb[p * ts + col(l)];
+                    return kk < len ? v : 0.f;
+                });
+        };
+        Mma16Acc c;
+        mma16_zero(c);
+        mma16_strided<64, 0>(
+            c, lane, [&](int l) { return Ay + (l >> 4) * 16 + (l & 15); },
+            [&](int l) { return Tb + (ps + (l >> 4)) * ts + col(l); }, s_a, 64, 4 * ts);
+        for (int s = s_a; s < s_b; ++s) general(c, s);
+        mma16_strided<64, 0>(
+            c, lane, [&](int l) { return Ay + (4 * s_b + (l >> 4)) * 16 + (l & 15); },
+            [&](int l) { return Tb + (ps - H + 4 * s_b + (l >> 4)) * ts + col(l); }, nfull - s_b, 64, 4 * ts);
+        if (len & 3) general(c, nfull);
+        mma16_store(c, lane, [&](int l, int v, float val) {
+            const int y = y0 + 4 * (l >> 4) + v, x = x0 + (l & 15);
+            if (y < oh && x < ow) o[y * ow + x] = val;
+        });
     }
 }
 
