@@ -217,6 +217,8 @@ KYM_HD KYM_INL T fir_sum(const T* src, int stride, int n, int base, const T* c, 
 template <class T, int MAXP>
 struct DynGeom {
     static constexpr int STAGE = KYM_STAGE_MAX;
+    // modulus / restaging as separate sweeps (stage_modulus, restage_complex)
+    static constexpr bool FUSED_INV = false, FUSED_FWD = false;
     int H, W, HJ, WJ;
     const FftLen<T>* fH;
     const FftLen<T>* fW;
@@ -250,6 +252,12 @@ template <class T, int H_, int W_, int HJ_, int WJ_, int NT>
 struct StatGeom {
     static constexpr int H = H_, W = W_, HJ = HJ_, WJ = WJ_;
     static constexpr int STAGE = (H_ * W_ + NT - 1) / NT;
+    // The modulus rides on the inverse's last pass (inv2d_modulus) and the
+    // forward's first pass reads the real field (fwd2d_real) when those
+    // passes have a butterfly for every thread; otherwise the separate
+    // sweeps spread the work over the whole workgroup and are faster.
+    static constexpr bool FUSED_INV = (H_ / kym_factor_list(H_).rad[0]) * W_ >= NT;
+    static constexpr bool FUSED_FWD = (W_ / kym_factor_list(W_).rad[0]) * H_ >= NT;
     const FftLen<T>* fH;
     const FftLen<T>* fW;
     const FftLen<T>* fHJ;
@@ -269,6 +277,33 @@ struct StatGeom {
     }
     KYM_HD void fwd2d(cpx<T>* b, int tid, int) const {
         fft_dit_s<false, W, H, 1, W, false, NT, true>(b, twW, tid);
+        fft_dit_s<false, H, W, W, 1, true, NT, true>(b, twH, tid);
+    }
+    // Inverse 2D FFT with the modulus in its last pass (columns, radix r_1):
+    // R[y (W + 1) + x] = |X[y][x]| scale in natural order over the buffer;
+    // sv (if set) receives X before the modulus at its storage index.
+    KYM_HD void inv2d_modulus(cpx<T>* b, T* R, T scale, cpx<T>* sv, int tid) const {
+        constexpr int R0 = kym_factor_list(H).rad[0];
+        fft_dif_s<true, W, H, 1, W, false, NT, true>(b, twW, tid);
+        fft_dif_s<true, H, W, W, 1, true, NT, true, T, -1, H, 1>(b, twH, tid);
+        dif_last_pass_s<true, H, W, W, 1, true, NT, T>(
+            b, tid,
+            [&](int t, int bb, int m, cpx<T> v) -> T {
+                if (sv) sv[(bb * R0 + m) * W + t] = v;
+                return kym_sqrt(v.re * v.re + v.im * v.im) * scale;
+            },
+            [&](int t, int bb, int m, T val) {
+                R[(kym_rev_s<H>(bb * R0) + (H / R0) * m) * (W + 1) + kym_rev_s<W>(t)] = val;
+            });
+    }
+    // Forward 2D FFT of the real field R[y (W + 1) + x] (natural order, over
+    // the buffer): the first row pass reads R directly.
+    KYM_HD void fwd2d_real(cpx<T>* b, const T* R, int tid) const {
+        constexpr int R0 = kym_factor_list(W).rad[0];
+        dit_first_pass_real_s<false, W, H, 1, W, false, NT, T>(b, tid, [&](int t, int bb, int m) -> T {
+            return R[kym_rev_s<H>(t) * (W + 1) + kym_rev_s<W>(bb * R0) + (W / R0) * m];
+        });
+        fft_dit_s<false, W, H, 1, W, false, NT, true, T, 1, R0>(b, twW, tid);
         fft_dit_s<false, H, W, W, 1, true, NT, true>(b, twH, tid);
     }
     KYM_HD void leaf_inv2d(cpx<T>* b, int tid, int) const {
@@ -567,7 +602,12 @@ KYM_HD void node_body(const Scat2DArgs<T>& a, const G& g, const Node2D& nd, int 
             }
             wg_barrier();
             fir_leaf_lds(a, g, nd, R, img, tid, nth);
-            restage_complex<T>(g, lds, tid, nth);
+            if constexpr (G::FUSED_FWD) {
+                g.fwd2d_real(lds, R, tid);
+            } else {
+                restage_complex<T>(g, lds, tid, nth);
+                g.fwd2d(lds, tid, nth);
+            }
         } else {
             const int* rH = g.revH();
             const int* rW = g.revW();
@@ -576,23 +616,33 @@ KYM_HD void node_body(const Scat2DArgs<T>& a, const G& g, const Node2D& nd, int 
                 lds[i] = {input_pixel(a, xin, rH[py], rW[px]), T(0)};
             }
             wg_barrier();
+            g.fwd2d(lds, tid, nth);
         }
-        g.fwd2d(lds, tid, nth);
     } else {
         const Node2D& pn = a.nodes[nd.parent];
         gather_block(lds, g.W, 0, wsi + pn.slot, a.filt + a.filt_off[nd.filt], g.W * nd.k, g.H, g.W, nd.k, tid, nth);
         wg_barrier();
-        g.inv2d(lds, tid, nth);
         const T inv_n = T(1) / T(g.H * g.W);
         cpx<T>* sv = (a.save && nd.save >= 0) ? wsi + nd.save : nullptr;
         if constexpr (kym_fir<T>) {
-            stage_modulus<T>(g, lds, sv, inv_n, tid, nth);
-            fir_leaf_lds(a, g, nd, reinterpret_cast<T*>(lds), img, tid, nth);
+            T* R = reinterpret_cast<T*>(lds);
+            if constexpr (G::FUSED_INV) {
+                g.inv2d_modulus(lds, R, inv_n, sv, tid);
+            } else {
+                g.inv2d(lds, tid, nth);
+                stage_modulus<T>(g, lds, sv, inv_n, tid, nth);
+            }
+            fir_leaf_lds(a, g, nd, R, img, tid, nth);
             if (nd.fwd) {
-                restage_complex<T>(g, lds, tid, nth);
-                g.fwd2d(lds, tid, nth);
+                if constexpr (G::FUSED_FWD) {
+                    g.fwd2d_real(lds, R, tid);
+                } else {
+                    restage_complex<T>(g, lds, tid, nth);
+                    g.fwd2d(lds, tid, nth);
+                }
             }
         } else {
+            g.inv2d(lds, tid, nth);
             for (int i = tid; i < g.H * g.W; i += nth) {
                 const cpx<T> v = lds[i];
                 if (sv) sv[i] = v;
@@ -673,6 +723,7 @@ KYM_HD void body_leaf(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, in
 // plan selects it only when every block of the launch is full).
 template <class T, int MAXP>
 struct DynPass {
+    static constexpr bool FUSED = false;
     KYM_HD void col_inv(cpx<T>* b, const FftLen<T>& f, int nc, int tid, int nth) const {
         fft_dif<true, MAXP>(b, f, Batch{nc, nc, 1, true}, tid, nth);
     }
@@ -703,6 +754,28 @@ struct StatPass {
     }
     KYM_HD void row_fwd(cpx<T>* b, const FftLen<T>&, int, int tid, int) const {
         fft_dit_s<false, W, RB, 1, W, false, NT, true>(b, twW, tid);
+    }
+    // Row inverse whose last pass writes the modulus R[t rs + x] = |X| scale
+    // (natural x; R does not overlap the rows) and, if sv is set, X at its
+    // storage index; and the row forward whose first pass reads R.
+    static constexpr bool FUSED = true;
+    KYM_HD void row_inv_modulus(cpx<T>* b, T* R, int rs, T scale, cpx<T>* sv, int tid) const {
+        constexpr int R0 = kym_factor_list(W).rad[0];
+        fft_dif_s<true, W, RB, 1, W, false, NT, true, T, -1, W, 1>(b, twW, tid);
+        dif_last_pass_s<true, W, RB, 1, W, false, NT, T, false>(
+            b, tid,
+            [&](int t, int bb, int m, cpx<T> v) -> T {
+                if (sv) sv[t * W + bb * R0 + m] = v;
+                return kym_sqrt(v.re * v.re + v.im * v.im) * scale;
+            },
+            [&](int t, int bb, int m, T val) { R[t * rs + kym_rev_s<W>(bb * R0) + (W / R0) * m] = val; });
+    }
+    KYM_HD void row_fwd_real(cpx<T>* b, const T* R, int rs, int tid) const {
+        constexpr int R0 = kym_factor_list(W).rad[0];
+        dit_first_pass_real_s<false, W, RB, 1, W, false, NT, T, false>(b, tid, [&](int t, int bb, int m) -> T {
+            return R[t * rs + kym_rev_s<W>(bb * R0) + (W / R0) * m];
+        });
+        fft_dit_s<false, W, RB, 1, W, false, NT, true, T, 1, R0>(b, twW, tid);
     }
 };
 
@@ -818,11 +891,19 @@ KYM_HD void body_rows_mod(const P& fp, const Scat2DArgs<T>& a, int task, cpx<T>*
     cpx<T>* buf = wsi + nd.slot + (long long)r0 * W;
     for (int i = tid; i < nr * W; i += nth) lds[i] = buf[i];
     wg_barrier();
-    fp.row_inv(lds, fW, nr, tid, nth);
     const T inv_n = T(1) / T(H * W);
     cpx<T>* sv = (a.save && nd.save >= 0) ? wsi + nd.save + (long long)r0 * W : nullptr;
     T* R = reinterpret_cast<T*>(lds + nr * W);
     const int rs = W + 1;
+    if constexpr (P::FUSED && kym_fir<T>) {
+        fp.row_inv_modulus(lds, R, rs, inv_n, sv, tid);
+        fir_rows_to_slot(a, nd, R, rs, nr, r0, a.lens[a.lenH[r]].rev, wsi, R + nr * rs, tid, nth);
+        if (!nd.fwd) return;
+        fp.row_fwd_real(lds, R, rs, tid);
+        for (int i = tid; i < nr * W; i += nth) buf[i] = lds[i];
+        return;
+    }
+    fp.row_inv(lds, fW, nr, tid, nth);
     for (int i = tid; i < nr * W; i += nth) {
         const cpx<T> v = lds[i];
         if (sv) sv[i] = v;
