@@ -165,7 +165,7 @@ def main():
                    'batch': BATCH, 'J': J, 'L': L, 'shape': [SIDE, SIDE], 'images_per_chunk': chunk,
                    'seed': SEED,
                    'plan_tuning': None if tuning is None else
-                   dict(zip(autotune.FIELDS, tuning))},
+                   dict(zip(autotune.FIELDS, tuple(tuning) + (0,) * (len(autotune.FIELDS) - len(tuning))))},
         'roofline': {'bound': 'hbm', 'achieved': achieved / 1e9, 'peak': HBM_PEAK / 1e9, 'unit': 'GB/s',
                      'frac': achieved / HBM_PEAK, 'traffic': traffic,
                      'traffic_source': 'profiles/scat2d_j3_256_hbm_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, '

@@ -50,11 +50,16 @@ int kym_scat2d_create(int M, int N, int J, int L, int max_order, int pre_pad, in
                       long long lds_limit, void** plan);
 
 /* kym_scat2d_create with tuning choices (the autotuner's knobs; no
- * counterpart in the reference): tune[0..ntune), ntune <= 5, = {largest
+ * counterpart in the reference): tune[0..ntune), ntune <= 6, = {largest
  * power-of-two FFT radix (16/8/4/2), LDS-node threads (256/512/1024),
  * big-node column block, big-node row block, big-node pass threads (a
- * compiled choice for these sizes and blocks)}; 0 = built-in heuristic.
- * Results are the same for every choice up to FP32 rounding. */
+ * compiled choice for these sizes and blocks), streams (1: the inference
+ * forward runs its chunks one after the other on the caller's stream; 2,
+ * the default: the workspace is split in two halves and chunks alternate
+ * between the caller's stream and a second stream of the plan, joined back
+ * to the caller's stream before kym_scat2d_forward's work ends)}; 0 =
+ * built-in heuristic.  Results are the same for every choice up to FP32
+ * rounding. */
 int kym_scat2d_create_tuned(int M, int N, int J, int L, int max_order, int pre_pad, int dtype,
                             long long lds_limit, const int* tune, int ntune, void** plan);
 int kym_scat2d_destroy(void* plan);

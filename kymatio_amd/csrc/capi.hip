@@ -62,6 +62,27 @@ struct LaunchProfile {
     }
 };
 
+// Second stream of a plan's inference forward: chunks alternate between the
+// caller's stream and `aux`, joined by events (fork before, join after), so
+// one chunk's launches fill the GPU while the other's drain.  The mutex makes
+// each record + wait pair atomic when several host threads share a plan.
+struct ChunkStreams {
+    hipStream_t aux = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    std::mutex mu;
+    int create() {
+        KYM_HIP_TRY(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
+        KYM_HIP_TRY(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+        KYM_HIP_TRY(hipEventCreateWithFlags(&join, hipEventDisableTiming));
+        return KYM_OK;
+    }
+    ~ChunkStreams() {
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
+        if (aux) (void)hipStreamDestroy(aux);
+    }
+};
+
 struct KymScat2D {
     int dtype = KYM_F32;
     int device = 0;
@@ -69,6 +90,7 @@ struct KymScat2D {
     std::unique_ptr<Scat2DPlan<float>> pf;
     std::unique_ptr<Scat2DPlan<double>> pd;
     LaunchProfile prof;
+    ChunkStreams cs;
 };
 
 template <class T>
@@ -93,9 +115,9 @@ extern "C" int kym_scat2d_create_tuned(int M, int N, int J, int L, int max_order
     if (!out) return fail(KYM_ERR_VALUE, "null plan pointer");
     *out = nullptr;
     if (dtype != KYM_F32 && dtype != KYM_F64) return fail(KYM_ERR_TYPE, "dtype must be float32 or float64");
-    if (ntune < 0 || ntune > 5 || (ntune > 0 && !tune)) return fail(KYM_ERR_VALUE, "tuning array must hold 0-5 values");
+    if (ntune < 0 || ntune > 6 || (ntune > 0 && !tune)) return fail(KYM_ERR_VALUE, "tuning array must hold 0-6 values");
     Scat2DTune t;
-    int* fields[5] = {&t.max_pow2, &t.node_threads, &t.col_block, &t.row_block, &t.pass_threads};
+    int* fields[6] = {&t.max_pow2, &t.node_threads, &t.col_block, &t.row_block, &t.pass_threads, &t.streams};
     for (int i = 0; i < ntune; ++i)
         if (tune[i] != 0) *fields[i] = tune[i];
     auto h = std::make_unique<KymScat2D>();
@@ -116,6 +138,7 @@ extern "C" int kym_scat2d_create_tuned(int M, int N, int J, int L, int max_order
         if (!h->pd->build(M, N, J, L, max_order, pre_pad, lim, &e)) return fail(e.code, e.msg);
         rc = upload_and_prepare(*h->pd, &h->dblob);
     }
+    if (rc == KYM_OK) rc = h->cs.create();
     if (rc != KYM_OK) {
         if (h->dblob) (void)hipFree(h->dblob);
         return rc;
@@ -132,14 +155,12 @@ extern "C" int kym_scat2d_create(int M, int N, int J, int L, int max_order, int 
 extern "C" int kym_scat2d_destroy(void* plan) {
     if (!plan) return KYM_OK;
     KymScat2D* h = static_cast<KymScat2D*>(plan);
-    if (h->dblob) {
-        int cur = 0;
-        (void)hipGetDevice(&cur);
-        (void)hipSetDevice(h->device);
-        (void)hipFree(h->dblob);
-        (void)hipSetDevice(cur);
-    }
-    delete h;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(h->device);
+    if (h->dblob) (void)hipFree(h->dblob);
+    delete h;  // the plan's stream and events, on its device
+    (void)hipSetDevice(cur);
     return KYM_OK;
 }
 
@@ -251,34 +272,54 @@ static int chunk_of(const Scat2DPlan<T>& p, int mode, long long batch, long long
 
 // mode 0: inference forward (batch in chunks); 1: training forward (saves
 // u, whole batch in the workspace); 2: backward from the saved workspace.
+// With two streams (inference, tune.streams != 1, at least two images of
+// workspace) the workspace is split in two halves: chunk k runs on stream
+// k mod 2 in half k mod 2, so chunk k + 2 reuses a half only after chunk k
+// on the same stream.  The aux stream waits for the caller's stream before
+// its first chunk, and the caller's stream for the aux stream at the end.
 template <class T>
 static int run2d(const Scat2DPlan<T>& p, int mode, const void* x, const void* filt, void* out, const void* dS,
                  void* gx, long long batch, void* ws, long long ws_bytes, hipStream_t stream, uintptr_t base,
-                 LaunchProfile* prof) {
+                 LaunchProfile* prof, ChunkStreams* cs = nullptr) {
     const long long img_elems = mode == 0 ? p.ws_img : p.ws_img_train;
     long long chunk = 0;
     int rc = chunk_of(p, mode, batch, ws_bytes, &chunk);
     if (rc != KYM_OK) return rc;
     const std::vector<Launch2D>& launches = mode == 2 ? p.bwd_launches : p.launches;
-    for (long long i0 = 0; i0 < batch; i0 += chunk) {
+    const bool two = mode == 0 && cs && cs->aux && p.tune.streams != 1 && chunk >= 2 && batch > chunk / 2;
+    if (two) {
+        chunk /= 2;
+        std::lock_guard<std::mutex> lock(cs->mu);
+        KYM_HIP_TRY(hipEventRecord(cs->fork, stream));
+        KYM_HIP_TRY(hipStreamWaitEvent(cs->aux, cs->fork, 0));
+    }
+    int k = 0;
+    for (long long i0 = 0; i0 < batch; i0 += chunk, ++k) {
         const int nimg = (int)std::min(chunk, batch - i0);
+        const hipStream_t st = (two && (k & 1)) ? cs->aux : stream;
+        void* wsk = static_cast<char*>(ws) + (two ? (k & 1) * chunk * img_elems * (long long)sizeof(cpx<T>) : 0);
         for (const Launch2D& l : launches) {
             Launch2DCall<T> c;
-            rc = launch_call(p, l, mode, x, filt, out, dS, gx, img_elems, ws, i0, nimg, base, &c);
+            rc = launch_call(p, l, mode, x, filt, out, dS, gx, img_elems, wsk, i0, nimg, base, &c);
             if (rc != KYM_OK) return rc;
             hipEvent_t e0 = nullptr, e1 = nullptr;
             if (prof && prof->enabled) {
                 e0 = prof->get();
                 e1 = prof->get();
-                if (e0) KYM_HIP_TRY(hipEventRecord(e0, stream));
+                if (e0) KYM_HIP_TRY(hipEventRecord(e0, st));
             }
-            hipLaunchKernelGGL(c.k, dim3(c.grid), dim3(c.threads), c.lds, stream, c.a);
+            hipLaunchKernelGGL(c.k, dim3(c.grid), dim3(c.threads), c.lds, st, c.a);
             KYM_HIP_TRY(hipGetLastError());
             if (e0 && e1) {
-                KYM_HIP_TRY(hipEventRecord(e1, stream));
+                KYM_HIP_TRY(hipEventRecord(e1, st));
                 prof->recs.push_back({(int)(&l - launches.data()), {e0, e1}});
             }
         }
+    }
+    if (two) {
+        std::lock_guard<std::mutex> lock(cs->mu);
+        KYM_HIP_TRY(hipEventRecord(cs->join, cs->aux));
+        KYM_HIP_TRY(hipStreamWaitEvent(stream, cs->join, 0));
     }
     return KYM_OK;
 }
@@ -294,8 +335,11 @@ extern "C" int kym_scat2d_forward(void* plan, const void* x, const void* filt, v
     KYM_HIP_TRY(hipGetDevice(&cur));
     if (cur != h->device) return fail(KYM_ERR_TYPE, "plan was created for another device");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    if (h->pf) return run2d(*h->pf, 0, x, filt, out, nullptr, nullptr, batch, ws, ws_bytes, s, (uintptr_t)h->dblob, &h->prof);
-    return run2d(*h->pd, 0, x, filt, out, nullptr, nullptr, batch, ws, ws_bytes, s, (uintptr_t)h->dblob, &h->prof);
+    if (h->pf)
+        return run2d(*h->pf, 0, x, filt, out, nullptr, nullptr, batch, ws, ws_bytes, s, (uintptr_t)h->dblob, &h->prof,
+                     &h->cs);
+    return run2d(*h->pd, 0, x, filt, out, nullptr, nullptr, batch, ws, ws_bytes, s, (uintptr_t)h->dblob, &h->prof,
+                 &h->cs);
 }
 
 extern "C" int kym_scat2d_forward_train(void* plan, const void* x, const void* filt, void* out, long long batch,

@@ -35,9 +35,9 @@ static int emu_forward2d(int M, int N, int J, int L, int max_order, int pre_pad,
                          const void* filt, void* out, long long batch, long long chunk, long long* info,
                          const int* tune = nullptr, int ntune = 0) {
     Scat2DPlan<T> p;
-    int* fields[5] = {&p.tune.max_pow2, &p.tune.node_threads, &p.tune.col_block, &p.tune.row_block,
-                      &p.tune.pass_threads};
-    for (int i = 0; i < ntune && i < 5; ++i)
+    int* fields[6] = {&p.tune.max_pow2, &p.tune.node_threads, &p.tune.col_block, &p.tune.row_block,
+                      &p.tune.pass_threads, &p.tune.streams};
+    for (int i = 0; i < ntune && i < 6; ++i)
         if (tune[i] != 0) *fields[i] = tune[i];
     KymError e = {0, ""};
     if (!p.build(M, N, J, L, max_order, pre_pad, (size_t)lds_limit, &e)) {

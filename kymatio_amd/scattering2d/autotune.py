@@ -3,8 +3,9 @@
 For one (image size, J, L, max_order, batch) it times the fused forward on
 the device under a small grid of plan choices (kym_scat2d_create_tuned):
 the largest power-of-two FFT radix, the thread count of LDS-node workgroups,
-the column / row blocks of the big-node passes, and the LDS budget that
-decides which nodes stay in one workgroup.  The fastest choice is kept in an
+the column / row blocks of the big-node passes, the LDS budget that decides
+which nodes stay in one workgroup, the images per chunk and whether chunks
+alternate over two streams.  The fastest choice is kept in an
 in-process table and, if a path is given, a JSON file keyed by device name.
 Every choice computes the same transform (they differ by FP32 rounding).
 """
@@ -16,8 +17,9 @@ import torch
 DEFAULT_CACHE = os.path.join(os.path.expanduser('~'), '.cache', 'kymatio_amd', 'autotune2d.json')
 
 # (max_pow2, node_threads, col_block, row_block, pass_threads, lds_limit,
-#  images_per_chunk) - 0 = heuristic
-FIELDS = ('max_pow2_radix', 'node_threads', 'col_block', 'row_block', 'pass_threads', 'lds_limit', 'images_per_chunk')
+#  images_per_chunk[, streams]) - 0 = heuristic (streams: 2)
+FIELDS = ('max_pow2_radix', 'node_threads', 'col_block', 'row_block', 'pass_threads', 'lds_limit', 'images_per_chunk',
+          'streams')
 CANDIDATES = [
     (16, 0, 0, 0, 0, 0, 0),
     (8, 0, 0, 0, 0, 0, 0),
@@ -34,6 +36,9 @@ CANDIDATES = [
     (16, 0, 0, 0, 0, 96 * 1024, 0),
     (16, 0, 8, 8, 0, 0, 64),
     (16, 0, 8, 8, 0, 0, 16),
+    (16, 0, 8, 8, 0, 0, 0, 1),
+    (16, 0, 16, 16, 0, 0, 0, 1),
+    (16, 0, 8, 8, 0, 0, 64, 1),
 ]
 
 _TABLE = {}
@@ -116,5 +121,6 @@ def tune(module, batch, reps=3, candidates=None, cache_path=None, timer=None):
 
 
 def _apply(module, c):
-    module.tune, module.lds_limit = list(c[:5]), c[5]
+    module.tune = list(c[:5]) + ([c[7]] if len(c) > 7 else [])
+    module.lds_limit = c[5]
     module.images_per_chunk = c[6] if len(c) > 6 and c[6] else None

@@ -89,6 +89,23 @@ def test_full_batch_properties():
     sc.images_per_chunk = 5
     c = sc(x)
     assert torch.equal(a, c)
+    # chunks alternate over the caller's stream and the plan's second stream
+    # (the default) or stay on the caller's stream: the same bits; the second
+    # stream waits for the caller's pending work (the clone) and the caller's
+    # stream for the second stream (the result read right after)
+    sc.images_per_chunk = 8
+    d2 = sc(x.clone() * 1.0)
+    sc.tune = [0, 0, 0, 0, 0, 1]
+    d1 = sc(x)
+    assert torch.equal(a, d1) and torch.equal(a, d2)
+    sc.tune = None
+    side = torch.cuda.Stream(device=DEV)
+    side.wait_stream(torch.cuda.current_stream(DEV))
+    with torch.cuda.stream(side):
+        e = sc(x + 0.0)
+    torch.cuda.current_stream(DEV).wait_stream(side)
+    assert torch.equal(a, e)
+    sc.images_per_chunk = None
     for i in (0, 31, 63):
         assert torch.equal(sc(x[i:i + 1].contiguous())[0], a[i])
     assert torch.isfinite(a).all()
