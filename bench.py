@@ -86,6 +86,8 @@ def main():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--chunk', type=int, default=0, help='images per launch chunk (0 = default)')
     ap.add_argument('--no-autotune', action='store_true', help='use the heuristic plan choices')
+    ap.add_argument('--tune', default='', metavar='V,V,...',
+                    help='fixed plan choice (the autotuner fields, comma separated) instead of autotuning')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='BYTES',
                     help='only run a plain device-to-device copy of BYTES (calibrates HBM counters)')
     args = ap.parse_args()
@@ -110,7 +112,11 @@ def main():
     x = torch.from_numpy(workloads.grating_images(BATCH, SIDE, seed=SEED)).to(dev)
     sc = Scattering2D(J, (SIDE, SIDE), L=L).to(dev)
     tuning = None
-    if not args.no_autotune:
+    if args.tune:
+        from kymatio_amd.scattering2d import autotune
+        tuning = tuple(int(v) for v in args.tune.split(','))
+        autotune._apply(sc, tuning)
+    elif not args.no_autotune:
         # plan choices timed on this device before the timed region (excluded, like plan creation)
         from kymatio_amd.scattering2d import autotune
         tuning, _ = autotune.tune(sc, BATCH)
@@ -137,6 +143,9 @@ def main():
 
     # per launch: model bytes per launch = model bytes per image * images of the launch
     chunk = sc._chunk(plan, BATCH)
+    # images per launch: chunks are split in halves over two streams unless the plan says one
+    streams = 1 if (len(sc.tune or ()) > 5 and sc.tune[5] == 1) or chunk < 2 or BATCH <= chunk // 2 else 2
+    per_launch = chunk // 2 if streams == 2 else chunk
     per_kernel = []
     for i, l in enumerate(launches):
         if cnt[i] == 0:
@@ -163,6 +172,7 @@ def main():
         'config': {'workload': 'Scattering2D J=3 L=8 max_order=2 on 256x256 float32 images: N(0,1) noise '
                                'plus 3 random gratings; output (64, 217, 32, 32)',
                    'batch': BATCH, 'J': J, 'L': L, 'shape': [SIDE, SIDE], 'images_per_chunk': chunk,
+                   'streams': streams, 'images_per_launch': per_launch,
                    'seed': SEED,
                    'plan_tuning': None if tuning is None else
                    dict(zip(autotune.FIELDS, tuple(tuning) + (0,) * (len(autotune.FIELDS) - len(tuning))))},
