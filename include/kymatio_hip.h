@@ -50,16 +50,18 @@ int kym_scat2d_create(int M, int N, int J, int L, int max_order, int pre_pad, in
                       long long lds_limit, void** plan);
 
 /* kym_scat2d_create with tuning choices (the autotuner's knobs; no
- * counterpart in the reference): tune[0..ntune), ntune <= 6, = {largest
+ * counterpart in the reference): tune[0..ntune) = {largest
  * power-of-two FFT radix (16/8/4/2), LDS-node threads (256/512/1024),
  * big-node column block, big-node row block, big-node pass threads (a
  * compiled choice for these sizes and blocks), streams (1: the inference
  * forward runs its chunks one after the other on the caller's stream; 2,
  * the default: the workspace is split in two halves and chunks alternate
  * between the caller's stream and a second stream of the plan, joined back
- * to the caller's stream before kym_scat2d_forward's work ends)}; 0 =
- * built-in heuristic.  Results are the same for every choice up to FP32
- * rounding. */
+ * to the caller's stream before kym_scat2d_forward's work ends), pair (1:
+ * one LDS node per workgroup; 2, the default: sibling nodes of one parent
+ * share a workgroup and one read of the parent's spectrum, where a kernel
+ * for the size is compiled)}; ntune <= 7; 0 = built-in heuristic.  Results
+ * are the same for every choice up to FP32 rounding. */
 int kym_scat2d_create_tuned(int M, int N, int J, int L, int max_order, int pre_pad, int dtype,
                             long long lds_limit, const int* tune, int ntune, void** plan);
 int kym_scat2d_destroy(void* plan);

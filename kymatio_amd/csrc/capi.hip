@@ -115,9 +115,9 @@ extern "C" int kym_scat2d_create_tuned(int M, int N, int J, int L, int max_order
     if (!out) return fail(KYM_ERR_VALUE, "null plan pointer");
     *out = nullptr;
     if (dtype != KYM_F32 && dtype != KYM_F64) return fail(KYM_ERR_TYPE, "dtype must be float32 or float64");
-    if (ntune < 0 || ntune > 6 || (ntune > 0 && !tune)) return fail(KYM_ERR_VALUE, "tuning array must hold 0-6 values");
+    if (ntune < 0 || ntune > 7 || (ntune > 0 && !tune)) return fail(KYM_ERR_VALUE, "tuning array must hold 0-7 values");
     Scat2DTune t;
-    int* fields[6] = {&t.max_pow2, &t.node_threads, &t.col_block, &t.row_block, &t.pass_threads, &t.streams};
+    int* fields[7] = {&t.max_pow2, &t.node_threads, &t.col_block, &t.row_block, &t.pass_threads, &t.streams, &t.pair};
     for (int i = 0; i < ntune; ++i)
         if (tune[i] != 0) *fields[i] = tune[i];
     auto h = std::make_unique<KymScat2D>();
@@ -375,7 +375,7 @@ extern "C" int kym_scat2d_backward(void* plan, const void* dS, const void* filt,
 template <class T>
 static void launch_rec(const Scat2DPlan<T>& p, int i, long long* rec) {
     const Launch2D& l = p.launches[i];
-    rec[0] = l.kind; rec[1] = l.ngroup; rec[2] = l.nblk; rec[3] = l.threads;
+    rec[0] = l.kind; rec[1] = l.pair ? 2 * l.ngroup : l.ngroup; rec[2] = l.nblk; rec[3] = l.threads;
     rec[4] = (long long)l.lds_bytes; rec[5] = l.model_bytes; rec[6] = l.res;
 }
 

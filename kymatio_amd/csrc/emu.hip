@@ -35,9 +35,9 @@ static int emu_forward2d(int M, int N, int J, int L, int max_order, int pre_pad,
                          const void* filt, void* out, long long batch, long long chunk, long long* info,
                          const int* tune = nullptr, int ntune = 0) {
     Scat2DPlan<T> p;
-    int* fields[6] = {&p.tune.max_pow2, &p.tune.node_threads, &p.tune.col_block, &p.tune.row_block,
-                      &p.tune.pass_threads, &p.tune.streams};
-    for (int i = 0; i < ntune && i < 6; ++i)
+    int* fields[7] = {&p.tune.max_pow2, &p.tune.node_threads, &p.tune.col_block, &p.tune.row_block,
+                      &p.tune.pass_threads, &p.tune.streams, &p.tune.pair};
+    for (int i = 0; i < ntune && i < 7; ++i)
         if (tune[i] != 0) *fields[i] = tune[i];
     KymError e = {0, ""};
     if (!p.build(M, N, J, L, max_order, pre_pad, (size_t)lds_limit, &e)) {
@@ -48,11 +48,15 @@ static int emu_forward2d(int M, int N, int J, int L, int max_order, int pre_pad,
     const uintptr_t base = (uintptr_t)p.blob.bytes.data();
     if (info) {
         info[0] = p.C; info[1] = p.oh; info[2] = p.ow; info[3] = (long long)p.launches.size();
-        int nbig = 0, nspec = 0;
+        int nbig = 0, nspec = 0, npair = 0;
         for (auto& nd : p.nodes) nbig += nd.big;
-        for (auto& l : p.launches) nspec += l.spec >= 0;
+        for (auto& l : p.launches) {
+            nspec += l.spec >= 0;
+            npair += l.pair;
+        }
         info[4] = nbig;
         info[5] = nspec;
+        info[6] = npair;
     }
     if (!x) return 0;
     if (chunk <= 0 || chunk > batch) chunk = batch;

@@ -26,7 +26,7 @@ __global__ void __launch_bounds__(NT) k_node2d_spec(Scat2DArgs<T> a) {
     extern __shared__ __attribute__((aligned(16))) char kym_smem2d_ns[];
     const int task = xcd_task(blockIdx.x, gridDim.x);
     if (task >= a.ntask) return;
-    dispatch_node_spec<T, false>(SPEC, a, task, reinterpret_cast<cpx<T>*>(kym_smem2d_ns), threadIdx.x);
+    node_spec_body<T, SPEC, false>(a, task, reinterpret_cast<cpx<T>*>(kym_smem2d_ns), threadIdx.x);
 }
 
 template <class T, int KIND, int SPEC, int NT>
@@ -44,7 +44,7 @@ template <class T>
 Scat2DKernel<T> pick_node_spec(int id) {
     if constexpr (sizeof(T) != 4) return nullptr;  // specialisations are FP32 only
     else switch (id) {
-#define KYM_NS_K(sid, h, w, hj, wj, nt) \
+#define KYM_NS_K(sid, h, w, hj, wj, nt, pr) \
     case sid: return k_node2d_spec<T, sid, nt>;
         KYM_NODE_SPECS(KYM_NS_K)
 #undef KYM_NS_K

@@ -587,6 +587,53 @@ KYM_HD void fir_leaf_lds(const Scat2DArgs<T>& a, const G& g, const Node2D& nd, T
 // A whole node in LDS: root (pad + forward FFT) or order 1/2 (gather +
 // periodise, inverse FFT, modulus, forward FFT); then the optional store of
 // the spectrum for the children and the optional in-kernel leaf.
+// A node of order >= 1 from its gathered spectrum in `lds` (natural
+// order): inverse FFT, modulus, leaf, forward FFT and slot if needed.
+template <class T, class G>
+KYM_HD KYM_INL void node_after_gather(const Scat2DArgs<T>& a, const G& g, const Node2D& nd, int il, cpx<T>* lds, int tid,
+                              int nth) {
+    const int img = a.img0 + il;
+    cpx<T>* wsi = a.ws + il * a.ws_img;
+    {
+        const T inv_n = T(1) / T(g.H * g.W);
+        cpx<T>* sv = (a.save && nd.save >= 0) ? wsi + nd.save : nullptr;
+        if constexpr (kym_fir<T>) {
+            T* R = reinterpret_cast<T*>(lds);
+            if constexpr (G::FUSED_INV) {
+                g.inv2d_modulus(lds, R, inv_n, sv, tid);
+            } else {
+                g.inv2d(lds, tid, nth);
+                stage_modulus<T>(g, lds, sv, inv_n, tid, nth);
+            }
+            fir_leaf_lds(a, g, nd, R, img, tid, nth);
+            if (nd.fwd) {
+                if constexpr (G::FUSED_FWD) {
+                    g.fwd2d_real(lds, R, tid);
+                } else {
+                    restage_complex<T>(g, lds, tid, nth);
+                    g.fwd2d(lds, tid, nth);
+                }
+            }
+        } else {
+            g.inv2d(lds, tid, nth);
+            for (int i = tid; i < g.H * g.W; i += nth) {
+                const cpx<T> v = lds[i];
+                if (sv) sv[i] = v;
+                lds[i] = {kym_sqrt(v.re * v.re + v.im * v.im) * inv_n, T(0)};
+            }
+            wg_barrier();
+            g.fwd2d(lds, tid, nth);
+        }
+    }
+    if (nd.slot >= 0 && nd.fwd) {
+        cpx<T>* dst = wsi + nd.slot;
+        for (int i = tid; i < g.H * g.W; i += nth) dst[i] = lds[i];
+    }
+    if (!kym_fir<T> && nd.leaf_mode == 0)
+        leaf_body(a, g, lds, a.filt + a.filt_off[nd.leaf_filt], nd.leaf_k, nd.leaf_ch, img, lds + g.H * g.W, tid,
+                  nth);
+}
+
 template <class T, class G>
 KYM_HD void node_body(const Scat2DArgs<T>& a, const G& g, const Node2D& nd, int il, cpx<T>* lds, int tid, int nth) {
     const int img = a.img0 + il;
@@ -622,35 +669,8 @@ KYM_HD void node_body(const Scat2DArgs<T>& a, const G& g, const Node2D& nd, int 
         const Node2D& pn = a.nodes[nd.parent];
         gather_block(lds, g.W, 0, wsi + pn.slot, a.filt + a.filt_off[nd.filt], g.W * nd.k, g.H, g.W, nd.k, tid, nth);
         wg_barrier();
-        const T inv_n = T(1) / T(g.H * g.W);
-        cpx<T>* sv = (a.save && nd.save >= 0) ? wsi + nd.save : nullptr;
-        if constexpr (kym_fir<T>) {
-            T* R = reinterpret_cast<T*>(lds);
-            if constexpr (G::FUSED_INV) {
-                g.inv2d_modulus(lds, R, inv_n, sv, tid);
-            } else {
-                g.inv2d(lds, tid, nth);
-                stage_modulus<T>(g, lds, sv, inv_n, tid, nth);
-            }
-            fir_leaf_lds(a, g, nd, R, img, tid, nth);
-            if (nd.fwd) {
-                if constexpr (G::FUSED_FWD) {
-                    g.fwd2d_real(lds, R, tid);
-                } else {
-                    restage_complex<T>(g, lds, tid, nth);
-                    g.fwd2d(lds, tid, nth);
-                }
-            }
-        } else {
-            g.inv2d(lds, tid, nth);
-            for (int i = tid; i < g.H * g.W; i += nth) {
-                const cpx<T> v = lds[i];
-                if (sv) sv[i] = v;
-                lds[i] = {kym_sqrt(v.re * v.re + v.im * v.im) * inv_n, T(0)};
-            }
-            wg_barrier();
-            g.fwd2d(lds, tid, nth);
-        }
+        node_after_gather(a, g, nd, il, lds, tid, nth);
+        return;
     }
     if (nd.slot >= 0 && nd.fwd) {
         cpx<T>* dst = wsi + nd.slot;
@@ -703,6 +723,93 @@ KYM_HD void body_lds_node_s(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int t
     load_twiddles(lds, fH, fW, H, W, tid, NT);
     const StatGeom<T, H, W, HJ, WJ, NT> g{fH, fW, &a.lens[a.lenH[J]], &a.lens[a.lenW[J]], lds + (H != W ? W : 0), lds};
     node_body(a, g, nd, il, lds + TWE, tid, NT);
+}
+
+// Gather of two siblings: da[i] = P_K(U fa)[i] scale and db[i] = P_K(U fb)[i]
+// scale for i < H W (both in LDS).  Each alias value of the parent spectrum
+// U is loaded once for both filters.
+template <int K, int H, int W, int NT, class T>
+KYM_HD KYM_INL void gather_pair_s(cpx<T>* da, cpx<T>* db, const cpx<T>* U, const T* fa, const T* fb, T scale,
+                                  int tid) {
+    constexpr int n = H * W, E = K >= 4 ? 1 : 4 / K, Wp = W * K;
+    for (int i0 = tid; i0 < n; i0 += E * NT) {
+        cpx<T> va[E], vb[E];
+#pragma unroll
+        for (int u = 0; u < E; ++u) {
+            const int i = i0 + u * NT;
+            va[u] = {T(0), T(0)};
+            vb[u] = {T(0), T(0)};
+            if (i < n) {
+                const int y = i / W, x = i - y * W;
+#pragma unroll
+                for (int al = 0; al < K; ++al) {
+                    const int row = (y + al * H) * Wp + x;
+#pragma unroll
+                    for (int b = 0; b < K; ++b) {
+                        const cpx<T> uv = kym_ldg(U + row + b * W);
+                        const T wa = kym_ldg(fa + row + b * W), wb = kym_ldg(fb + row + b * W);
+                        va[u].re += uv.re * wa;
+                        va[u].im += uv.im * wa;
+                        vb[u].re += uv.re * wb;
+                        vb[u].im += uv.im * wb;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < E; ++u) {
+            const int i = i0 + u * NT;
+            if (i < n) {
+                da[i] = va[u] * scale;
+                db[i] = vb[u] * scale;
+            }
+        }
+    }
+}
+
+// Complex elements of the LDS region one FP32 node uses (its complex buffer
+// or, if larger, the leaf's FIR fields; the plan's node_lds).
+template <int H, int W, class T>
+KYM_HD KYM_INL long long node_region_cpx(const Scat2DArgs<T>& a, const Node2D& nd) {
+    const long long reals = fir_lds_reals(H, W, a.ow, nd.leaf_k, a.fir[2 * nd.res].cnt, a.fir[2 * nd.res + 1].cnt);
+    const long long c = (reals + 1) / 2;
+    return c > (long long)H * W ? c : (long long)H * W;
+}
+
+// Two sibling nodes (same parent and resolution; the plan lists them at
+// group[2 gi] and group[2 gi + 1]) in one workgroup.  The gather reads the
+// parent once for both: the first node's spectrum goes to the node region,
+// the second's to a stash after it; the first node runs, then the stash
+// moves into the node region and the second runs.  LDS: twiddles, node
+// region (node_region_cpx), stash (H W).  task -> (image, pair).
+template <class T, int H, int W, int HJ, int WJ, int NT>
+KYM_HD KYM_INL void body_lds_pair_s(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid) {
+    const int il = task / a.ngroup, gi = task - il * a.ngroup;
+    const Node2D& na = a.nodes[a.group[2 * gi]];
+    const Node2D& nb = a.nodes[a.group[2 * gi + 1]];
+    const int r = na.res, J = a.J;
+    constexpr int TWE = kym_tw_elems(H, W);
+    const FftLen<T>* fH = &a.lens[a.lenH[r]];
+    const FftLen<T>* fW = &a.lens[a.lenW[r]];
+    load_twiddles(lds, fH, fW, H, W, tid, NT);
+    const StatGeom<T, H, W, HJ, WJ, NT> g{fH, fW, &a.lens[a.lenH[J]], &a.lens[a.lenW[J]], lds + (H != W ? W : 0), lds};
+    cpx<T>* buf = lds + TWE;
+    cpx<T>* stash = buf + node_region_cpx<H, W>(a, na);
+    const cpx<T>* U = a.ws + il * a.ws_img + a.nodes[na.parent].slot;
+    const T* fa = a.filt + a.filt_off[na.filt];
+    const T* fb = a.filt + a.filt_off[nb.filt];
+    const T scale = T(1) / T(na.k * na.k);
+    switch (na.k) {
+        case 1: gather_pair_s<1, H, W, NT>(buf, stash, U, fa, fb, scale, tid); break;
+        case 2: gather_pair_s<2, H, W, NT>(buf, stash, U, fa, fb, scale, tid); break;
+        default: gather_pair_s<4, H, W, NT>(buf, stash, U, fa, fb, scale, tid); break;  // the plan pairs k <= 4
+    }
+    wg_barrier();
+    node_after_gather(a, g, na, il, buf, tid, NT);
+    wg_barrier();
+    for (int i = tid; i < H * W; i += NT) buf[i] = stash[i];
+    wg_barrier();
+    node_after_gather(a, g, nb, il, buf, tid, NT);
 }
 
 // Leaf from a spectrum kept in the workspace.  task -> (image, node).
@@ -994,37 +1101,56 @@ KYM_HD void body_fwd_cols(const P& fp, const Scat2DArgs<T>& a, int task, cpx<T>*
     }
 }
 
-// Compile-time specialisations X(id, H, W, HJ, WJ, threads) of the LDS node
-// kernel (FP32).  A launch uses one when its sizes match; every other size
-// runs the run-time-sized bodies.
-#define KYM_NODE_SPECS(X)            \
-    X(0, 136, 136, 34, 34, 1024)     \
-    X(1, 68, 68, 34, 34, 512)        \
-    X(2, 80, 80, 10, 10, 512)        \
-    X(3, 40, 40, 10, 10, 256)        \
-    X(4, 20, 20, 10, 10, 256)        \
-    X(5, 68, 68, 34, 34, 256)        \
-    X(6, 80, 80, 10, 10, 256)        \
-    X(7, 136, 136, 34, 34, 512)
+// Compile-time specialisations X(id, H, W, HJ, WJ, threads, pair) of the LDS
+// node kernel (FP32).  A launch uses one when its sizes match; every other
+// size runs the run-time-sized bodies.  pair = 1: sibling pairs per
+// workgroup (body_lds_pair_s; sizes whose LDS holds the second node's
+// spectrum beside the first node).
+#define KYM_NODE_SPECS(X)              \
+    X(0, 136, 136, 34, 34, 1024, 0)    \
+    X(1, 68, 68, 34, 34, 512, 0)       \
+    X(2, 80, 80, 10, 10, 512, 0)       \
+    X(3, 40, 40, 10, 10, 256, 0)       \
+    X(4, 20, 20, 10, 10, 256, 0)       \
+    X(5, 68, 68, 34, 34, 256, 0)       \
+    X(6, 80, 80, 10, 10, 256, 0)       \
+    X(7, 136, 136, 34, 34, 512, 0)     \
+    X(8, 68, 68, 34, 34, 512, 1)       \
+    X(9, 68, 68, 34, 34, 256, 1)
 
 // want_threads = 0: the first (default) specialisation of these sizes.
-inline int node_spec_id(int H, int W, int HJ, int WJ, int want_threads, int* threads) {
-#define KYM_NS_MATCH(id, h, w, hj, wj, nt)                                              \
-    if (H == h && W == w && HJ == hj && WJ == wj && (want_threads == 0 || want_threads == nt)) { \
-        *threads = nt;                                                                  \
-        return id;                                                                      \
+inline int node_spec_id(int H, int W, int HJ, int WJ, int want_threads, int* threads, int want_pair = 0) {
+#define KYM_NS_MATCH(id, h, w, hj, wj, nt, pr)                                                  \
+    if (H == h && W == w && HJ == hj && WJ == wj && (want_threads == 0 || want_threads == nt) && \
+        pr == want_pair) {                                                                      \
+        *threads = nt;                                                                          \
+        return id;                                                                              \
     }
     KYM_NODE_SPECS(KYM_NS_MATCH)
 #undef KYM_NS_MATCH
     return -1;
 }
 
-// EMU = true: one thread per workgroup (CPU harness), otherwise the spec's count.
+// The body of specialisation SPEC (selected at compile time, so a kernel
+// carries only its own body).  EMU = true: one thread per workgroup (CPU
+// harness), otherwise the spec's count.
+template <class T, int SPEC, bool EMU>
+KYM_HD KYM_INL void node_spec_body(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid) {
+#define KYM_NS_BODY(sid, h, w, hj, wj, nt, pr)                                                    \
+    if constexpr (SPEC == sid) {                                                                 \
+        if constexpr (pr) body_lds_pair_s<T, h, w, hj, wj, (EMU ? 1 : nt)>(a, task, lds, tid);    \
+        else body_lds_node_s<T, h, w, hj, wj, (EMU ? 1 : nt)>(a, task, lds, tid);                 \
+    }
+    KYM_NODE_SPECS(KYM_NS_BODY)
+#undef KYM_NS_BODY
+}
+
+// Run-time selection (the CPU harness).
 template <class T, bool EMU>
 KYM_HD void dispatch_node_spec(int id, const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid) {
     switch (id) {
-#define KYM_NS_CASE(sid, h, w, hj, wj, nt) \
-    case sid: body_lds_node_s<T, h, w, hj, wj, (EMU ? 1 : nt)>(a, task, lds, tid); break;
+#define KYM_NS_CASE(sid, h, w, hj, wj, nt, pr) \
+    case sid: node_spec_body<T, sid, EMU>(a, task, lds, tid); break;
         KYM_NODE_SPECS(KYM_NS_CASE)
 #undef KYM_NS_CASE
         default: break;

@@ -18,6 +18,7 @@ struct Launch2D {
     int res;                // resolution of the launch's nodes
     long long model_bytes;  // algorithmic HBM bytes per image (stage-boundary model)
     int spec;               // compile-time specialisation id, -1 = run-time sizes
+    int pair = 0;           // 1: a task is a sibling pair (group lists 2 ngroup node ids)
 };
 
 // Tuning choices of a plan (0 = the built-in heuristic).  Any non-default
@@ -29,6 +30,7 @@ struct Scat2DTune {
     int row_block = 0;      // rows per workgroup of the big-node row passes
     int pass_threads = 0;   // threads of the big-node pass workgroups (compiled choices only)
     int streams = 0;        // inference chunks in flight: 1, or 2 (0 = default 2; see kym_scat2d_forward)
+    int pair = 0;           // LDS nodes: 1 one node per workgroup, 2 sibling pairs where compiled (0 = heuristic: 2)
 };
 
 template <class T>
@@ -106,7 +108,7 @@ struct Scat2DPlan {
             tune.col_block < 0 || tune.col_block > 64 || tune.row_block < 0 || tune.row_block > 64 ||
             !(tune.pass_threads == 0 || tune.pass_threads == 256 || (tune.pass_threads % 64 == 0 &&
                                                                       tune.pass_threads <= 1024)) ||
-            !(tune.streams >= 0 && tune.streams <= 2)) {
+            !(tune.streams >= 0 && tune.streams <= 2) || !(tune.pair >= 0 && tune.pair <= 2)) {
             *e = {KYM_ERR_VALUE, "invalid tuning choice"};
             return false;
         }
@@ -228,8 +230,15 @@ struct Scat2DPlan {
             const size_t elems = (size_t)H[nd.res] * W[nd.res];
             // LDS-resident if the node (and, FP32, its in-LDS leaf) fits; leaf_mode
             // places the spectral leaf and, in training, the leaf's gradient
+            // (run-time-sized kernels stage at most KYM_STAGE_MAX values per thread;
+            // a compile-time kernel for these sizes and threads has no such limit)
+            int spec_threads = 0;
+            const bool spec_fit =
+                specs && sizeof(T) == 4 &&
+                node_spec_id(H[nd.res], W[nd.res], H[J], W[J], node_threads(elems), &spec_threads, 0) >= 0 &&
+                node_lds(nd) + cs * kym_tw_elems(H[nd.res], W[nd.res]) <= lds_limit;
             const bool fits = node_lds(nd) <= lds_limit &&
-                              (!fir || elems <= (size_t)KYM_STAGE_MAX * node_threads(elems));
+                              (!fir || elems <= (size_t)KYM_STAGE_MAX * node_threads(elems) || spec_fit);
             if (fits) { nd.big = 0; nd.leaf_mode = ((elems + leaf_elems) * cs <= lds_limit) ? 0 : 1; }
             else { nd.big = 1; nd.leaf_mode = (leaf_elems * cs <= lds_limit) ? 1 : 2; }
             nd.fir = fir ? (nd.big ? 2 : 1) : 0;
@@ -377,9 +386,35 @@ struct Scat2DPlan {
             const size_t elems = (size_t)H[r] * W[r];
             if (!n0.big) {
                 const size_t lds = fir ? node_lds(n0) : (elems + (n0.leaf_mode == 0 ? leaf_elems : 0)) * cs;
+                // sibling pairs: the nodes sorted by parent, every parent with an even
+                // number of children here, periodisation k <= 4, a compiled pair kernel
+                if (tune.pair != 1 && specs && sizeof(T) == 4 && n0.fir == 1) {
+                    std::vector<int> sib = ids;
+                    std::stable_sort(sib.begin(), sib.end(),
+                                     [&](int u, int v) { return nodes[u].parent < nodes[v].parent; });
+                    bool ok = sib.size() % 2 == 0;
+                    for (size_t i = 0; ok && i < sib.size(); i += 2) {
+                        const Node2D& u = nodes[sib[i]];
+                        const Node2D& v = nodes[sib[i + 1]];
+                        ok = u.order >= 1 && u.parent == v.parent && u.k == v.k && u.k <= 4 && v.fir == 1;
+                    }
+                    int pnt = 0;
+                    const int psid = ok ? node_spec_id(H[r], W[r], H[J], W[J], tune.node_threads, &pnt, 1) : -1;
+                    // twiddles, the node region, the second node's stash
+                    const size_t need = cs * kym_tw_elems(H[r], W[r]) + (lds + cs - 1) / cs * cs + elems * cs;
+                    if (psid >= 0 && need <= lds_limit) {
+                        push(K2_LDS_NODE, sib, 1, 0, pnt, need);
+                        Launch2D& lb = launches.back();
+                        lb.ngroup /= 2;
+                        lb.pair = 1;
+                        lb.spec = psid;
+                        emit_leaves(ids);
+                        return;
+                    }
+                }
                 push(K2_LDS_NODE, ids, 1, 0, node_threads(elems), lds);
                 int nt = 0;
-                const int sid = node_spec_id(H[r], W[r], H[J], W[J], tune.node_threads, &nt);
+                const int sid = node_spec_id(H[r], W[r], H[J], W[J], tune.node_threads, &nt, 0);
                 if (specs && sizeof(T) == 4 && sid >= 0 && (n0.fir == 1 || n0.leaf_mode == 0) &&
                     lds + cs * kym_tw_elems(H[r], W[r]) <= lds_limit) {
                     launches.back().spec = sid;

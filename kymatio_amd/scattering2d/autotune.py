@@ -4,8 +4,8 @@ For one (image size, J, L, max_order, batch) it times the fused forward on
 the device under a small grid of plan choices (kym_scat2d_create_tuned):
 the largest power-of-two FFT radix, the thread count of LDS-node workgroups,
 the column / row blocks of the big-node passes, the LDS budget that decides
-which nodes stay in one workgroup, the images per chunk and whether chunks
-alternate over two streams.  The fastest choice is kept in an
+which nodes stay in one workgroup, the images per chunk, whether chunks
+alternate over two streams and whether sibling nodes share a workgroup.  The fastest choice is kept in an
 in-process table and, if a path is given, a JSON file keyed by device name.
 Every choice computes the same transform (they differ by FP32 rounding).
 """
@@ -17,9 +17,9 @@ import torch
 DEFAULT_CACHE = os.path.join(os.path.expanduser('~'), '.cache', 'kymatio_amd', 'autotune2d.json')
 
 # (max_pow2, node_threads, col_block, row_block, pass_threads, lds_limit,
-#  images_per_chunk[, streams]) - 0 = heuristic (streams: 2)
+#  images_per_chunk[, streams[, pair]]) - 0 = heuristic (streams: 2, pair: 2)
 FIELDS = ('max_pow2_radix', 'node_threads', 'col_block', 'row_block', 'pass_threads', 'lds_limit', 'images_per_chunk',
-          'streams')
+          'streams', 'pair')
 CANDIDATES = [
     (16, 0, 0, 0, 0, 0, 0),
     (8, 0, 0, 0, 0, 0, 0),
@@ -39,6 +39,8 @@ CANDIDATES = [
     (16, 0, 8, 8, 0, 0, 0, 1),
     (16, 0, 16, 16, 0, 0, 0, 1),
     (16, 0, 8, 8, 0, 0, 64, 1),
+    (16, 0, 8, 8, 0, 0, 0, 0, 1),
+    (16, 0, 8, 8, 192, 0, 0, 0, 1),
 ]
 
 _TABLE = {}
@@ -121,6 +123,6 @@ def tune(module, batch, reps=3, candidates=None, cache_path=None, timer=None):
 
 
 def _apply(module, c):
-    module.tune = list(c[:5]) + ([c[7]] if len(c) > 7 else [])
+    module.tune = list(c[:5]) + list(c[7:9])
     module.lds_limit = c[5]
     module.images_per_chunk = c[6] if len(c) > 6 and c[6] else None

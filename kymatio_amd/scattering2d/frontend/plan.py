@@ -8,8 +8,8 @@ from ... import _lib
 
 class Scat2DPlanHandle:
     def __init__(self, M, N, J, L, max_order, pre_pad, dtype, lds_limit=0, tune=None):
-        """tune: None or up to 6 ints {largest power-of-two radix, LDS-node
-        threads, column block, row block, pass threads, streams} (0 =
+        """tune: None or up to 7 ints {largest power-of-two radix, LDS-node
+        threads, column block, row block, pass threads, streams, pair} (0 =
         heuristic), see kym_scat2d_create_tuned."""
         lib = _lib.lib()
         h = ctypes.c_void_p()

@@ -44,15 +44,19 @@ def test_tuned_plans_match_reference(emu, tag, lds, tune):
 
 
 @pytest.mark.parametrize('tune', [(8, 0, 8, 4), (16, 0, 4, 2), (16, 0, 17, 17, 320), (16, 0, 8, 8, 192),
-                                  (16, 0, 16, 16, 320)])
+                                  (16, 0, 16, 16, 320), (16, 0, 0, 0, 0, 0, 1), (16, 512, 8, 8, 0, 1, 2)])
 def test_tuned_big_nodes_match_reference(emu, tune):
-    """J=3 256x256: the order-1 nodes at 272 x 272 run as column / row passes."""
+    """J=3 256x256: the order-1 nodes at 272 x 272 run as column / row passes
+    (and, with pair = 2, the 136 x 136 and 68 x 68 nodes as sibling pairs)."""
     _, x, J, shape, kw, Sx = _case('cfg2')
     Mp, Np = compute_padding(shape[0], shape[1], J)
     f = pack_filters_2d(filter_bank(Mp, Np, J, 8))
-    out, info = emu.scat2d(x[:1], J, shape, f, tune=list(tune))
+    out, info = emu.scat2d(x[:2], J, shape, f, tune=list(tune), chunk=2)
     assert info['big_nodes'] > 0
-    assert channel_rel_err(out, Sx[:1]) < RTOL_CHANNEL
+    # sibling pairs (the default where a pair kernel is compiled: 68x68 with the
+    # compile-time kernels, i.e. radix 16)
+    assert info['pair_launches'] == (1 if tune[0] == 16 and not (len(tune) > 6 and tune[6] == 1) else 0)
+    assert channel_rel_err(out, Sx[:2]) < RTOL_CHANNEL
 
 
 def test_invalid_tuning_rejected(emu):
