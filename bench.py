@@ -78,6 +78,154 @@ def cpu_baseline(budget_s=15.0, max_images=2048):
                       'restatement of the reference torch backend, %.1f s, %s' % (n, t_total, model)}
 
 
+def _line(metric, value, unit, steps, warmup, ms, dtype, config, model_bytes_per_step, kernel, cpu):
+    """One JSON line of the other workloads: the roofline is the whole step's
+    stage-boundary model bytes over its wall time (one graph replay or one
+    forward / forward+backward call; no per-launch events there)."""
+    achieved = model_bytes_per_step / (ms * 1e-3)
+    return {'metric': metric, 'value': value, 'unit': unit, 'n_gpus': 1, 'steps': steps, 'warmup': warmup,
+            'ms_per_step': ms, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': dtype,
+            'data': 'synthetic', 'config': config,
+            'roofline': {'bound': 'hbm', 'achieved': achieved / 1e9, 'peak': HBM_PEAK / 1e9, 'unit': 'GB/s',
+                         'frac': achieved / HBM_PEAK, 'traffic': None, 'kernel': kernel,
+                         'model_bytes_per_step': model_bytes_per_step},
+            'cpu_baseline': cpu}
+
+
+def _timed(fn, steps, warmup):
+    for _ in range(max(1, warmup)):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def _cores():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def bench_mixed(args, n_images=256, seed=5):
+    """256 square images, side uniform over {64, 128, 256, 512}, pixels
+    U[0,1), J = log2(side) - 3 (8x8 outputs): one HIP-graph replay per step."""
+    from kymatio_amd import workloads
+    from kymatio_amd.scattering2d.mixed import MixedScattering2D, default_J
+    dev = torch.device('cuda:0')
+    imgs = workloads.mixed_size_images(n_images, seed=seed)
+    groups = {}
+    for im in imgs:
+        groups.setdefault(im.shape[0], []).append(im)
+    groups = {s: torch.from_numpy(np.stack(v)).to(dev) for s, v in sorted(groups.items())}
+    mx = MixedScattering2D(device=dev, autotune=not args.no_autotune)
+    with torch.no_grad():
+        dt = _timed(lambda: mx.forward_grouped(groups), args.steps, args.warmup)
+    model = 0
+    for side, x in groups.items():
+        plan = mx.module(side)._plan(dev, torch.float32)
+        model += x.shape[0] * sum(l['model_bytes_per_image'] for l in plan.launches())
+    cpu = None
+    if not args.no_cpu_baseline:
+        # oracle per size on a bounded sample, weighted by this batch's size counts
+        from oracle import scat2d
+        torch.set_num_threads(_cores())
+        per = {}
+        for side, x in groups.items():
+            sc = scat2d.Scattering2DTorchCPU(default_J(side), (side, side))
+            xs = x[:2].cpu()
+            with torch.no_grad():
+                sc(xs[:1])
+                t0 = time.perf_counter()
+                sc(xs)
+                per[side] = (time.perf_counter() - t0) / xs.shape[0]
+        t_batch = sum(per[s] * x.shape[0] for s, x in groups.items())
+        cpu = {'value': n_images / t_batch, 'unit': 'images/s', 'cores': _cores(), 'kind': 'port',
+               'sample': '2 images of each side (%s), oracle torch CPU FP32, time per image weighted by the batch '
+                         'counts %s' % (sorted(per), {s: int(x.shape[0]) for s, x in groups.items()})}
+    return _line('images/s MixedScattering2D sides {64,128,256,512} J=log2(side)-3 L=8 fp32', n_images / dt,
+                 'images/s', args.steps, args.warmup, dt * 1e3, 'f32',
+                 {'workload': 'mixed-size batch of %d images (sides uniform over 64/128/256/512, pixels U[0,1), '
+                              'seed %d), one graph replay per step' % (n_images, seed),
+                  'counts': {int(s): int(x.shape[0]) for s, x in groups.items()}},
+                 model, 'graph replay (all sizes)', cpu)
+
+
+def bench_scat1d(args, batch=256, seed=3):
+    """Scattering1D J=6 Q=8 on 256 chirp signals of length 4096."""
+    from kymatio_amd import workloads
+    from kymatio_amd.torch import Scattering1D
+    dev = torch.device('cuda:0')
+    J, Q, N = 6, 8, 4096
+    x = torch.from_numpy(workloads.chirp_signals(batch, N, seed=seed)).to(dev)
+    sc = Scattering1D(J, N, Q).to(dev)
+    with torch.no_grad():
+        out = sc(x)
+        dt = _timed(lambda: sc(x), args.steps, args.warmup)
+    model_per_signal = 10913280  # stage-boundary model at J=6 Q=8 N=4096 (write + read of every stage tensor)
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import scat1d
+        torch.set_num_threads(_cores())
+        ref = scat1d.Scattering1DTorchCPU(J, N, Q)
+        xs = x[:64].cpu()
+        with torch.no_grad():
+            ref(xs[:4])
+            t0 = time.perf_counter()
+            ref(xs)
+            t = time.perf_counter() - t0
+        cpu = {'value': xs.shape[0] / t, 'unit': 'signals/s', 'cores': _cores(), 'kind': 'port',
+               'sample': '64 of the same chirp signals, oracle torch CPU FP32 (%.1f s)' % t}
+    return _line('signals/s Scattering1D J=6 Q=8 N=4096 fp32', batch / dt, 'signals/s', args.steps, args.warmup,
+                 dt * 1e3, 'f32', {'workload': 'Scattering1D J=6 Q=8 on %d chirp signals of length %d (seed %d); '
+                                               'output %s' % (batch, N, seed, tuple(out.shape)), 'batch': batch},
+                 model_per_signal * batch, 'whole forward', cpu)
+
+
+def bench_grad2d(args, batch=32, seed=4):
+    """Scattering2D J=4 L=8 on 32 images of 128x128: forward with the saved
+    fields plus the hand-written backward of sum((S(x) - S(y))^2)."""
+    from kymatio_amd import workloads
+    from kymatio_amd.torch import Scattering2D
+    dev = torch.device('cuda:0')
+    x = torch.from_numpy(workloads.normal_images(batch, 128, seed=seed)).to(dev)
+    y = torch.from_numpy(workloads.normal_images(batch, 128, seed=seed + 1)).to(dev)
+    sc = Scattering2D(4, (128, 128)).to(dev)
+    with torch.no_grad():
+        sy = sc(y)
+
+    def step():
+        xr = x.clone().requires_grad_(True)
+        loss = ((sc(xr) - sy) ** 2).sum()
+        loss.backward()
+        return xr.grad
+
+    dt = _timed(step, args.steps, args.warmup)
+    model_per_image = 87259776  # forward + backward stage-boundary model at J=4 128x128
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import scat2d
+        torch.set_num_threads(_cores())
+        ref = scat2d.Scattering2DTorchCPU(4, (128, 128))
+        xs, ys = x[:8].cpu(), y[:8].cpu()
+        with torch.no_grad():
+            rys = ref(ys)
+        t0 = time.perf_counter()
+        xr = xs.clone().requires_grad_(True)
+        ((ref(xr) - rys) ** 2).sum().backward()
+        t = time.perf_counter() - t0
+        cpu = {'value': xs.shape[0] / t, 'unit': 'images/s', 'cores': _cores(), 'kind': 'port',
+               'sample': '8 of the same images, oracle torch CPU FP32 forward + autograd backward (%.1f s)' % t}
+    return _line('images/s Scattering2D J=4 L=8 128x128 fp32 forward+backward', batch / dt, 'images/s', args.steps,
+                 args.warmup, dt * 1e3, 'f32',
+                 {'workload': 'sum((S(x) - S(y))^2) and its gradient in x, J=4 L=8, %d N(0,1) images of 128x128 '
+                              '(seed %d)' % (batch, seed), 'batch': batch},
+                 model_per_image * batch, 'forward with saved fields + backward', cpu)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -90,6 +238,9 @@ def main():
                     help='fixed plan choice (the autotuner fields, comma separated) instead of autotuning')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='BYTES',
                     help='only run a plain device-to-device copy of BYTES (calibrates HBM counters)')
+    ap.add_argument('--workload', default='scat2d', choices=('scat2d', 'mixed', 'scat1d', 'grad2d'),
+                    help='scat2d: the headline (default); mixed / scat1d / grad2d: the other parts of the path '
+                         '(one JSON line each, same keys)')
     args = ap.parse_args()
     if args.copy_calibration:
         n = args.copy_calibration
@@ -104,6 +255,10 @@ def main():
         print('bench.py measures one GPU only (the scattering path shards over the batch with no '
               'exchange; see DESIGN.md). Run with --gpus 1.', file=sys.stderr)
         sys.exit(2)
+    if args.workload != 'scat2d':
+        res = {'mixed': bench_mixed, 'scat1d': bench_scat1d, 'grad2d': bench_grad2d}[args.workload](args)
+        print(json.dumps(res))
+        return
 
     from kymatio_amd import workloads
     from kymatio_amd.torch import Scattering2D
