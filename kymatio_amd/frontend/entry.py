@@ -1,12 +1,13 @@
 """Frontend dispatch by class swap, as kymatio/frontend/entry.py:6-47 does.
 
-Only the 'torch' frontend exists in this package (the MI355X path); it is
-also the default.  Other frontend names raise the reference's RuntimeError.
+The 'torch' frontend (the default), 'numpy' (arrays staged through the
+device) and 'sklearn' (transformers over the numpy frontend) exist; other
+frontend names raise the reference's RuntimeError.
 """
 import importlib
 import logging
 
-_SUFFIX = {'torch': 'Torch'}
+_SUFFIX = {'torch': 'Torch', 'numpy': 'NumPy', 'sklearn': 'Transformer'}
 _KNOWN = ['torch', 'numpy', 'tensorflow', 'jax', 'keras', 'sklearn']
 
 
@@ -19,7 +20,7 @@ class ScatteringEntry:
             raise RuntimeError('The frontend \'%s\" is not valid. Must be one of \'%s\', or \'%s\'.'
                                % (frontend, '\', \''.join(_KNOWN[:-1]), _KNOWN[-1]))
         if frontend not in _SUFFIX:
-            raise RuntimeError('The frontend \'%s\' is not provided by kymatio_amd (torch only).' % frontend)
+            raise RuntimeError('The frontend \'%s\' is not provided by kymatio_amd (torch, numpy, sklearn).' % frontend)
         module = importlib.import_module('kymatio_amd.' + self.class_name + '.frontend.' + frontend + '_frontend')
         cls = getattr(module, self.__class__.__name__.replace('Entry', _SUFFIX[frontend]))
         self.__class__ = cls

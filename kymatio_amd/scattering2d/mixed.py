@@ -72,7 +72,11 @@ class MixedScattering2D:
                 self.tuning[side] = choice
             plan = m._plan(self.device, torch.float32)
             filt = m._packed_filters(plan, self.device, torch.float32)
-            chunk = max(1, min(n, self.ws_budget // max(plan.ws_bytes_per_image, 1)))
+            # the autotuned images per chunk, else as many as ws_budget holds
+            if m.images_per_chunk:
+                chunk = m._chunk(plan, n)
+            else:
+                chunk = max(1, min(n, self.ws_budget // max(plan.ws_bytes_per_image, 1)))
             x = torch.zeros(n, side, side, device=self.device)
             out = torch.zeros(n, plan.C, plan.oh, plan.ow, device=self.device)
             ws = torch.empty(chunk * plan.ws_bytes_per_image, dtype=torch.uint8, device=self.device)
