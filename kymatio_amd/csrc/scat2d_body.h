@@ -1159,7 +1159,8 @@ KYM_HD void dispatch_node_spec(int id, const Scat2DArgs<T>& a, int task, cpx<T>*
 
 // Compile-time specialisations X(id, H, W, col block, row block, threads) of
 // the big-node passes (FP32): root rows, inverse columns, rows + modulus,
-// forward columns.
+// forward columns.  272: 256x256 at J=3; 160 / 320 / 640: 128x128 at J=4,
+// 256x256 at J=5 and 512x512 at J=6 (the mixed-size batch).
 #define KYM_PASS_SPECS(X)         \
     X(0, 272, 272, 16, 8, 256)    \
     X(1, 272, 272, 16, 4, 256)    \
@@ -1168,7 +1169,10 @@ KYM_HD void dispatch_node_spec(int id, const Scat2DArgs<T>& a, int task, cpx<T>*
     X(4, 272, 272, 16, 16, 256)   \
     X(5, 272, 272, 16, 16, 320)   \
     X(6, 272, 272, 17, 17, 320)   \
-    X(7, 272, 272, 8, 8, 192)
+    X(7, 272, 272, 8, 8, 192)     \
+    X(8, 160, 160, 8, 8, 256)     \
+    X(9, 320, 320, 8, 8, 256)     \
+    X(10, 640, 640, 8, 8, 256)
 
 // want_threads = 0: the first specialisation of these sizes and blocks.
 inline int pass_spec_id(int H, int W, int cb, int rb, int want_threads, int* threads) {

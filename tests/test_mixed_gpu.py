@@ -50,6 +50,23 @@ def test_mixed_matches_oracle_per_size():
         assert err < RTOL_CHANNEL
 
 
+@pytest.mark.parametrize('tune', [[16, 0, 8, 8], [16, 0, 8, 8, 0, 1]])
+def test_compiled_passes_large_sizes(tune):
+    """Sides 256 (J=5) and 512 (J=6) with 8 x 8 pass blocks: the 160 / 320 /
+    640 big-node passes run their compile-time kernels; vs the oracle's
+    outputs in golden_mixed2d.npz."""
+    G = golden('golden_mixed2d.npz')
+    for s in (256, 512):
+        J = int(G['s%d_J' % s])
+        x = np.random.default_rng(int(G['s%d_seed' % s])).random((1, s, s), dtype=np.float32)
+        sc = Scattering2D(J, (s, s)).to(DEV)
+        sc.tune = list(tune)
+        got = sc(torch.from_numpy(x).to(DEV)).cpu().numpy()
+        err = channel_rel_err(got, G['s%d_Sx' % s])
+        print('side %d tune %s: per-channel rel err %.3e' % (s, tune, err))
+        assert err < RTOL_CHANNEL
+
+
 def test_graph_replays_match_eager_with_work_between(per_size):
     mx = MixedScattering2D(autotune=False)
     counts = {64: 5, 128: 3, 256: 2, 512: 1}

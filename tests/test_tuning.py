@@ -110,3 +110,19 @@ def test_autotune_picks_fastest_and_caches(tmp_path):
     assert choice3 == (16, 256, 0, 0, 0, 0, 8)
     assert m3.images_per_chunk == 8
     autotune._TABLE.clear()
+
+
+@pytest.mark.parametrize('side', [256])
+def test_compiled_passes_at_other_sizes(emu, side):
+    """256x256 at J=5 (P = 320): the 320 x 320 and 160 x 160 big nodes run
+    the compile-time passes (blocks 8 x 8) -- against the oracle's outputs in
+    golden_mixed2d.npz (written by tests/golden/make_mixed_golden.py)."""
+    from tests.helpers import golden
+    G = golden('golden_mixed2d.npz')
+    J = int(G['s%d_J' % side])
+    x = np.random.default_rng(int(G['s%d_seed' % side])).random((1, side, side), dtype=np.float32)
+    Mp, Np = compute_padding(side, side, J)
+    f = pack_filters_2d(filter_bank(Mp, Np, J, 8))
+    out, info = emu.scat2d(x, J, (side, side), f, tune=[16, 0, 8, 8])
+    assert info['spec_launches'] >= 10  # the big-node passes included
+    assert channel_rel_err(out, G['s%d_Sx' % side]) < RTOL_CHANNEL
