@@ -83,4 +83,26 @@ class TorchBackend1D(TorchBackend):
         return torch.sum(x, axis=-2, keepdims=True)
 
 
+    # ---- frequency-axis layout ops of time-frequency scattering
+    # (kymatio/scattering1d/backend/torch_backend.py:150-223); the axis n1
+    # sits before time, tensors end in a real/imag (or singleton) axis.
+    @classmethod
+    def pad_frequency(cls, x, padding):
+        """Zeros appended to the n1 axis (one-sided: towards lower frequencies)."""
+        return torch.nn.functional.pad(x, (0, 0, 0, padding), mode='constant', value=0)
+
+    @classmethod
+    def swap_time_frequency(cls, x):
+        """(batch, a, b, 1|2) -> (batch, b, a, 1|2), contiguous."""
+        return torch.transpose(x, dim0=-2, dim1=-3).contiguous()
+
+    @staticmethod
+    def unpad_frequency(x, n1_max, n1_stride):
+        return x[:, :, :1 + (n1_max // n1_stride), :]
+
+    @staticmethod
+    def split_frequency_axis(x):
+        return torch.split(x, 1, dim=-3)
+
+
 backend = TorchBackend1D

@@ -83,6 +83,19 @@ def anden_generator(J, Q, sigma0, r_psi, **unused):
         yield xi, sigma_min
 
 
+def spin(filterbank_fn, filterbank_kwargs):
+    """Filter bank with both signs of every center frequency
+    (kymatio/scattering1d/filter_bank.py:314-319): the generator's (xi, sigma)
+    pairs, then the same pairs with -xi; for frequency scattering, where the
+    sign of xi is the direction (spin) of the wavelet along log-frequency."""
+    def spinned(J, Q, **kwargs):
+        pairs = list(filterbank_fn(J, Q, **kwargs))
+        yield from pairs
+        for xi, sigma in pairs:
+            yield -xi, sigma
+    return spinned, filterbank_kwargs
+
+
 def scattering_filter_factory(N, J, Q, T, filterbank, _reduction=np.sum):
     fn, kw = filterbank
     max_j, prev_J = 0, 0

@@ -1,0 +1,243 @@
+"""TimeFrequencyScattering (joint time-frequency scattering, JTFS) as a
+torch.nn.Module on the HIP path.
+
+Drop-in for kymatio.torch.TimeFrequencyScattering
+(kymatio/scattering1d/frontend/torch_frontend.py:83-150 over
+TimeFrequencyScatteringBase, kymatio/scattering1d/frontend/base_frontend.py:
+524-760): same constructor (J, J_fr, Q, shape, T, stride, Q_fr, F,
+stride_fr, out_type, format), the same frequency padding and filters, the
+same paths and layouts for format 'time' / 'joint' and out_type 'array' /
+'list' / 'dict', and meta().  The transform runs the JTFS generator of
+kymatio_amd/scattering1d/core/timefrequency_scattering.py over the 1D HIP
+backend (every FFT, filter product, periodisation and modulus is a HIP
+kernel on the GPU tensors).
+"""
+import math
+import numbers
+
+import numpy as np
+import torch
+
+from ...frontend.base_frontend import ScatteringBase
+from ..core.timefrequency_scattering import jtfs_average_and_format, joint_timefrequency_scattering
+from ..filter_bank import anden_generator, scattering_filter_factory, spin
+from ..utils import parse_T
+from .torch_frontend import ScatteringTorch1D
+
+
+class _DryBackend:
+    """Every op returns None: runs the path generators for meta()."""
+    __getattr__ = lambda self, attr: (lambda *args, **kwargs: None)
+
+
+class TimeFrequencyScatteringTorch1D(ScatteringTorch1D):
+    def __init__(self, J, J_fr, Q, shape, T=None, stride=None, Q_fr=1, F=None, stride_fr=None, out_type='array',
+                 format='time', backend='torch'):
+        self.J_fr, self.Q_fr, self._F = J_fr, Q_fr, F
+        self.oversampling_fr = 0
+        self._stride_fr = stride_fr
+        self.format = format
+        ScatteringTorch1D.__init__(self, J, shape, Q, T, stride, max_order=2, oversampling=None, out_type=out_type,
+                                   backend=backend)
+        self._fr_dev = {}
+
+    # ------------------------------------------------------------ setup
+    def build(self):
+        """base_frontend.py:539-568: the time filters, then the padded length
+        of the n1 axis (a multiple of every frequential subsampling)."""
+        ScatteringTorch1D.build(self)
+        ScatteringTorch1D.create_filters(self)
+        if np.any(np.array(self.Q_fr) < 1):
+            raise ValueError('Q_fr must be >= 1, got {}'.format(self.Q_fr))
+        if isinstance(self.Q_fr, int):
+            self.Q_fr = (self.Q_fr,)
+        elif isinstance(self.Q_fr, tuple):
+            if len(self.Q_fr) != 1:
+                raise NotImplementedError('Q_fr must be an integer or 1-tuple. Time-frequency scattering beyond '
+                                          'order 2 is not implemented.')
+        else:
+            raise ValueError('Q_fr must be an integer or 1-tuple.')
+        min_to_pad_fr = 8 * min(self.F, 2 ** self.J_fr)
+        N_input_fr = (self.J + 1) * self.Q[0]
+        K_fr = max(self.J_fr - self.oversampling_fr, 0)
+        self._N_padded_fr = (N_input_fr + min_to_pad_fr) // (2 ** K_fr) * (2 ** K_fr)
+
+    def create_filters(self):
+        """base_frontend.py:570-579: phi of width F, then the width-2^J_fr
+        low-pass and the spinned wavelets along n1."""
+        phi0_fr, = scattering_filter_factory(self._N_padded_fr, self.J_fr, (), self.F, self.filterbank_fr,
+                                             _reduction=np.sum)
+        phi1_fr, psis_fr = scattering_filter_factory(self._N_padded_fr, self.J_fr, self.Q_fr, 2 ** self.J_fr,
+                                                     self.filterbank_fr, _reduction=np.sum)
+        self.filters_fr = (phi0_fr, [phi1_fr] + psis_fr)
+        assert all(abs(p['xi']) < 0.5 / (2 ** p['j']) for p in psis_fr)
+
+    def register_filters(self):
+        """Time filters tensor0.., then the frequency filters (phi, then each
+        psi's levels), as the reference's buffers."""
+        n = ScatteringTorch1D.register_filters(self)
+        for f in [self.filters_fr[0]] + self.filters_fr[1]:
+            for lv in range(len(f['levels'])):
+                f['levels'][lv] = torch.from_numpy(f['levels'][lv]).float().view(-1, 1)
+                self.register_buffer('tensor' + str(n), f['levels'][lv])
+                n += 1
+        self._n_filters = n
+        return n
+
+    # ------------------------------------------------------------ properties (base_frontend.py:700-760)
+    @property
+    def average_fr(self):
+        return parse_T(self._F, self.J_fr, (self.J + 1) * self.Q[0], T_alias='F')[1]
+
+    @property
+    def F(self):
+        return parse_T(self._F, self.J_fr, (self.J + 1) * self.Q[0], T_alias='F')[0]
+
+    @property
+    def filterbank_fr(self):
+        return spin(anden_generator, {'alpha': self.alpha, 'r_psi': self.r_psi, 'sigma0': self.sigma0})
+
+    @property
+    def log2_F(self):
+        return int(math.floor(math.log2(self.F)))
+
+    @property
+    def log2_stride_fr(self):
+        if self._stride_fr is None:
+            return self.log2_F
+        if not isinstance(self._stride_fr, numbers.Integral):
+            raise ValueError('stride_fr must be integer. Got: {}'.format(self._stride_fr))
+        l2 = math.log2(self._stride_fr)
+        if math.floor(l2) != math.ceil(l2):
+            raise ValueError('stride_fr must be a power of two. Got: {}'.format(self._stride_fr))
+        if self.average_fr in [False, 'global']:
+            raise ValueError('stride_fr={} is incompatible with F={}.'.format(self._stride, self._F))
+        return int(l2)
+
+    @property
+    def stride_fr(self):
+        return 2 ** self.log2_stride_fr
+
+    def _check_runtime_args(self):
+        ScatteringTorch1D._check_runtime_args(self)
+        if self.format == 'joint' and not self.average_fr and self.out_type == 'array':
+            raise ValueError("Cannot convert to format='joint' with out_type='array' and F=0. Either set "
+                             "format='time', out_type='dict', or out_type='list'.")
+        if self.oversampling_fr < 0:
+            raise ValueError('oversampling_fr must be nonnegative. Got: {}'.format(self.oversampling_fr))
+        if not isinstance(self.oversampling_fr, numbers.Integral):
+            raise ValueError('oversampling_fr must be integer. Got: {}'.format(self.oversampling_fr))
+        if self.format not in ['time', 'joint']:
+            raise ValueError("format must be 'time' or 'joint'. Got: {}".format(self.format))
+
+    # ------------------------------------------------------------ transform
+    def _device_filters(self, device, dtype):
+        """The filter dicts with their levels as device tensors (from the
+        module's buffers, so .to() / .double() carry over)."""
+        bufs = self._filter_buffers()
+        key = (str(device), dtype)
+        stamp = tuple((b.data_ptr(), b._version) for b in bufs)
+        hit = self._fr_dev.get(key)
+        if hit is not None and hit[0] == stamp:
+            return hit[1]
+        it = iter(b.to(device=device, dtype=dtype) for b in bufs)
+
+        def take(f):
+            return {**f, 'levels': [next(it) for _ in f['levels']]}
+
+        phi, psi1, psi2 = take(self.phi_f), [take(p) for p in self.psi1_f], [take(p) for p in self.psi2_f]
+        fr = (take(self.filters_fr[0]), [take(p) for p in self.filters_fr[1]])
+        out = ([phi, psi1, psi2], fr)
+        self._fr_dev[key] = (stamp, out)
+        return out
+
+    def scattering(self, x):
+        """base_frontend.py:581-648 over the HIP backend."""
+        self._check_runtime_args()
+        if not torch.is_tensor(x):
+            raise TypeError('The input should be a PyTorch Tensor.')
+        if len(x.shape) < 1:
+            raise ValueError('Input tensor x should have at least one axis, got {}'.format(len(x.shape)))
+        if x.shape[-1] != self.shape[0]:
+            raise RuntimeError('Tensor must be of length %i.' % self.shape[0])
+        bufs = self._filter_buffers()
+        if x.dtype is not bufs[0].dtype:
+            raise TypeError('Input and filter must be of the same dtype.')
+        if not x.is_cuda:
+            raise TypeError('TimeFrequencyScattering runs on the GPU (HIP); got a CPU tensor.')
+        B = self.backend
+        batch_shape, signal_shape = x.shape[:-1], x.shape[-1:]
+        x = B.reshape_input(x.contiguous(), signal_shape)
+        U_0 = B.pad(x, pad_left=self.pad_left, pad_right=self.pad_right)
+        filters, filters_fr = self._device_filters(x.device, x.dtype)
+        U_gen = joint_timefrequency_scattering(U_0, B, filters, self.log2_stride, self.average == 'local',
+                                               filters_fr, self.log2_stride_fr, self.average_fr == 'local')
+        S_gen = jtfs_average_and_format(U_gen, B, filters[0], self.log2_stride, self.average, filters_fr[0],
+                                        self.log2_stride_fr, self.average_fr, self.out_type, self.format)
+        path = next(S_gen)
+        if self.average != 'global':
+            res = self.log2_stride if self.average else 0
+            path['coef'] = B.unpad(path['coef'], self.ind_start[res], self.ind_end[res])
+        path['coef'] = B.reshape_output(path['coef'], batch_shape, n_kept_dims=1)
+        S = [path]
+        for path in S_gen:
+            if self.average != 'global':
+                res = max(path['j'][-1], 0) if (not self.average and len(path['n']) > 1) else max(self.log2_stride, 0)
+                path['coef'] = B.unpad(path['coef'], self.ind_start[res], self.ind_end[res])
+            path['coef'] = B.reshape_output(path['coef'], batch_shape, n_kept_dims=1 + (self.format == 'joint'))
+            S.append(path)
+        if self.format == 'joint' and self.out_type == 'array':
+            return B.stack([p['coef'] for p in S[1:]], dim=-3)
+        if self.format == 'time' and self.out_type == 'array':
+            return B.stack([p['coef'] for p in S], dim=-2)
+        if self.out_type == 'dict':
+            return {p['n']: p['coef'] for p in S}
+        return S
+
+    def meta(self):
+        """base_frontend.py:650-698."""
+        filters = [self.phi_f, self.psi1_f, self.psi2_f]
+        U_gen = joint_timefrequency_scattering(None, _DryBackend(), filters, self.log2_stride,
+                                               self.average == 'local', self.filters_fr, self.log2_stride_fr,
+                                               self.average_fr == 'local')
+        S_gen = jtfs_average_and_format(U_gen, _DryBackend(), self.phi_f, self.log2_stride, self.average,
+                                        self.filters_fr[0], self.log2_stride_fr, self.average_fr, self.out_type,
+                                        self.format)
+        S = sorted(list(S_gen), key=lambda p: (len(p['n']), p['n']))
+        meta = dict(key=[p['n'] for p in S], n=[], n_fr=[], order=[])
+        for p in S:
+            if len(p['n']) == 0:
+                if not (self.format == 'joint' and self.out_type == 'array'):
+                    meta['n'].append([np.nan, np.nan])
+                    meta['n_fr'].append(np.nan)
+                    meta['order'].append(0)
+                continue
+            if len(p['n']) == 1:
+                meta['n'].append([range(0, p['n1_max'], p['n1_stride']), np.nan])
+            elif len(p['n']) == 2 and self.format == 'joint':
+                meta['n'].append([range(0, p['n1_max'], p['n1_stride']), p['n'][0]])
+            elif len(p['n']) == 2 and self.format == 'time':
+                meta['n'].append([p['n'][0], np.nan])
+            elif len(p['n']) == 3 and self.format == 'time':
+                meta['n'].append(p['n'][:2])
+            meta['n_fr'].append(p['n_fr'][0])
+            meta['order'].append(len(p['n']) - (self.format == 'time'))
+        meta['n'] = np.array(meta['n'], dtype=object)
+        meta['n_fr'] = np.array(meta['n_fr'])
+        meta['order'] = np.array(meta['order'])
+        for key in ['xi', 'sigma', 'j']:
+            meta[key] = np.zeros((meta['n_fr'].shape[0], 2)) * np.nan
+            for order, bank in enumerate(filters[1:]):
+                for n, psi in enumerate(bank):
+                    meta[key][meta['n'][:, order] == n, order] = psi[key]
+            meta[key + '_fr'] = meta['n_fr'] * np.nan
+            for n_fr, psi_fr in enumerate(self.filters_fr[1]):
+                meta[key + '_fr'][meta['n_fr'] == n_fr] = psi_fr[key]
+        meta['spin'] = np.sign(meta['xi_fr'])
+        return meta
+
+    def output_size(self, detail=False):
+        raise NotImplementedError('output_size is not defined for TimeFrequencyScattering; use meta().')
+
+
+__all__ = ['TimeFrequencyScatteringTorch1D']
