@@ -226,6 +226,26 @@ def bench_grad2d(args, batch=32, seed=4):
                  model_per_image * batch, 'forward with saved fields + backward', cpu)
 
 
+def bench_jtfs(args, batch=64, seed=6):
+    """TimeFrequencyScattering J=6 J_fr=2 Q=8 on 64 chirps of length 4096
+    (the path runs op by op over the 1D HIP kernels; no byte model and no CPU
+    restatement exist for it, so roofline and cpu_baseline are null)."""
+    from kymatio_amd import workloads
+    from kymatio_amd.torch import TimeFrequencyScattering
+    dev = torch.device('cuda:0')
+    x = torch.from_numpy(workloads.chirp_signals(batch, 4096, seed=seed)).to(dev)
+    sc = TimeFrequencyScattering(J=6, J_fr=2, Q=8, shape=4096).to(dev)
+    with torch.no_grad():
+        out = sc(x)
+        dt = _timed(lambda: sc(x), args.steps, args.warmup)
+    return {'metric': 'signals/s TimeFrequencyScattering J=6 J_fr=2 Q=8 N=4096 fp32', 'value': batch / dt,
+            'unit': 'signals/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': dt * 1e3,
+            'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic',
+            'config': {'workload': 'JTFS J=6 J_fr=2 Q=8 on %d chirp signals of length 4096 (seed %d); output %s'
+                                   % (batch, seed, tuple(out.shape)), 'batch': batch},
+            'roofline': None, 'cpu_baseline': None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -238,7 +258,7 @@ def main():
                     help='fixed plan choice (the autotuner fields, comma separated) instead of autotuning')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='BYTES',
                     help='only run a plain device-to-device copy of BYTES (calibrates HBM counters)')
-    ap.add_argument('--workload', default='scat2d', choices=('scat2d', 'mixed', 'scat1d', 'grad2d'),
+    ap.add_argument('--workload', default='scat2d', choices=('scat2d', 'mixed', 'scat1d', 'grad2d', 'jtfs'),
                     help='scat2d: the headline (default); mixed / scat1d / grad2d: the other parts of the path '
                          '(one JSON line each, same keys)')
     args = ap.parse_args()
@@ -256,7 +276,8 @@ def main():
               'exchange; see DESIGN.md). Run with --gpus 1.', file=sys.stderr)
         sys.exit(2)
     if args.workload != 'scat2d':
-        res = {'mixed': bench_mixed, 'scat1d': bench_scat1d, 'grad2d': bench_grad2d}[args.workload](args)
+        res = {'mixed': bench_mixed, 'scat1d': bench_scat1d, 'grad2d': bench_grad2d,
+               'jtfs': bench_jtfs}[args.workload](args)
         print(json.dumps(res))
         return
 

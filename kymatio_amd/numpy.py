@@ -1,6 +1,7 @@
 """kymatio_amd.numpy: the scattering transforms for NumPy arrays on MI355X
 (drop-in names of kymatio/numpy.py:1-27; arrays staged through the device)."""
 from .scattering1d.frontend.numpy_frontend import ScatteringNumPy1D as Scattering1D
+from .scattering1d.frontend.numpy_frontend import TimeFrequencyScatteringNumPy as TimeFrequencyScattering
 from .scattering2d.frontend.numpy_frontend import ScatteringNumPy2D as Scattering2D
 
 Scattering1D.__module__ = 'kymatio_amd.numpy'
@@ -8,4 +9,7 @@ Scattering1D.__name__ = 'Scattering1D'
 Scattering2D.__module__ = 'kymatio_amd.numpy'
 Scattering2D.__name__ = 'Scattering2D'
 
-__all__ = ['Scattering1D', 'Scattering2D']
+TimeFrequencyScattering.__module__ = 'kymatio_amd.numpy'
+TimeFrequencyScattering.__name__ = 'TimeFrequencyScattering'
+
+__all__ = ['Scattering1D', 'Scattering2D', 'TimeFrequencyScattering']

@@ -7,6 +7,7 @@ the arrays are staged through the device
 import torch
 
 from ...frontend.numpy_frontend import ScatteringNumPy
+from .jtfs_frontend import TimeFrequencyScatteringTorch1D
 from .torch_frontend import ScatteringTorch1D
 
 
@@ -49,4 +50,33 @@ class ScatteringNumPy1D(ScatteringNumPy):
         return self._stage_out(S)
 
 
-__all__ = ['ScatteringNumPy1D']
+class TimeFrequencyScatteringNumPy(ScatteringNumPy1D):
+    """TimeFrequencyScattering for NumPy arrays
+    (kymatio/scattering1d/frontend/numpy_frontend.py:18-34); meta() from the
+    CPU-side module, the transform staged through the device."""
+
+    def __init__(self, J, J_fr, Q, shape, T=None, stride=None, Q_fr=1, F=None, stride_fr=None, out_type='array',
+                 format='time', backend='numpy', device=None):
+        ScatteringNumPy.__init__(self)
+        if isinstance(backend, str):
+            if not backend.startswith('numpy'):
+                raise ImportError('The backend ' + backend + ' is not supported by the numpy frontend.')
+        elif not str(getattr(backend, 'name', '')).startswith('numpy'):
+            raise ImportError('The backend ' + str(getattr(backend, 'name', backend)) + ' is not supported by the '
+                              'numpy frontend.')
+        self.J, self.J_fr, self.Q, self.shape, self.T, self.stride = J, J_fr, Q, shape, T, stride
+        self.Q_fr, self.F, self.stride_fr, self.out_type, self.format = Q_fr, F, stride_fr, out_type, format
+        self.backend, self.device = backend, device
+        self._torch_modules = {}
+        self._spec = self._make_torch_module()
+
+    def _make_torch_module(self):
+        return TimeFrequencyScatteringTorch1D(self.J, self.J_fr, self.Q, self.shape, T=self.T, stride=self.stride,
+                                              Q_fr=self.Q_fr, F=self.F, stride_fr=self.stride_fr,
+                                              out_type=self.out_type, format=self.format)
+
+    def output_size(self, detail=False):
+        raise NotImplementedError('output_size is not defined for TimeFrequencyScattering; use meta().')
+
+
+__all__ = ['ScatteringNumPy1D', 'TimeFrequencyScatteringNumPy']

@@ -96,3 +96,16 @@ def test_batch_shapes_and_float64():
         d = S.double()(x.double())
     assert d.dtype == torch.float64
     assert (d.float() - a).abs().max() <= 1e-5 * a.abs().max()
+
+
+@pytest.mark.gpu
+def test_numpy_and_sklearn_frontends():
+    """test_timefrequency_scattering.py:497-560 (numpy / sklearn)."""
+    from kymatio_amd.numpy import TimeFrequencyScattering as NumPyJTFS
+    from kymatio_amd.sklearn import TimeFrequencyScattering as SkJTFS
+    x = G['time_x']
+    Sx = NumPyJTFS(**_kw('time'))(x)
+    assert isinstance(Sx, np.ndarray)
+    _close(Sx, G['time_Sx'])
+    t = SkJTFS(**_kw('time')).transform(x)
+    assert np.allclose(t, Sx.reshape(x.shape[0], -1))
