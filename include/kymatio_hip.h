@@ -225,6 +225,15 @@ int kym_op_unpad2d(const void* x, void* y, long long batch, int H, int W, int dt
  * (status 3 if it is then null); may be null otherwise. */
 int kym_op_fft(const void* x, void* y, void* work, long long batch, int H, int W, int kind, int dtype,
                long long lds_limit, void* stream);
+/* modulus_rotation (kymatio/scattering3d/backend/torch_backend.py:97-124):
+ * y = sqrt(prev^2 + |x|^2) elementwise over n complex x; prev may be null
+ * (the first m of a solid-harmonic order l). */
+int kym_op_modulus_rotation(const void* x, const void* prev, void* y, long long n, int dtype, void* stream);
+/* compute_integrals (kymatio/scattering3d/backend/torch_backend.py:126-151):
+ * out[b, q] = sum over the n elements of x[b] of x^powers[q], q < npow <= 8
+ * (accumulated in FP64, stored in the dtype). */
+int kym_op_integrals(const void* x, void* out, long long batch, long long n, const double* powers, int npow, int dtype,
+                     void* stream);
 
 /* ------------------------------------------------------ learned part ----
  * Kernels of the warm-start decoder (kymatio_amd/learned).  They have no

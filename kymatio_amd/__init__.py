@@ -7,6 +7,7 @@ kernels for gfx950 through the C ABI in include/kymatio_hip.h.
 """
 from .scattering1d import ScatteringEntry1D as Scattering1D
 from .scattering2d import ScatteringEntry2D as Scattering2D
+from .scattering3d import HarmonicScatteringEntry3D as HarmonicScattering3D
 
 __version__ = '0.1.0'
-__all__ = ['Scattering1D', 'Scattering2D']
+__all__ = ['Scattering1D', 'Scattering2D', 'HarmonicScattering3D']

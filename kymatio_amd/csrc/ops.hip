@@ -68,6 +68,48 @@ __global__ void k_modulus(const cpx<T>* x, T* y, long long n) {
     }
 }
 
+// Rotation-covariant modulus accumulated over m (3D solid harmonics):
+// y[i] = sqrt(prev[i]^2 + |x[i]|^2), prev absent for the first m.
+template <class T>
+__global__ void k_modulus_rot(const cpx<T>* x, const T* prev, T* y, long long n) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const cpx<T> v = x[i];
+        const T t = v.re * v.re + v.im * v.im;
+        const T p = prev ? prev[i] : T(0);
+        y[i] = kym_sqrt(p * p + t);
+    }
+}
+
+// out[b * npow + q] = sum_i x[b, i]^pw[q]: one workgroup per b, FP64
+// partial sums reduced through LDS.
+#define KYM_MAX_POWERS 8
+struct PowerList {
+    double p[KYM_MAX_POWERS];
+    int n;
+};
+
+template <class T>
+__global__ void __launch_bounds__(256) k_integrals(const T* x, T* out, long long n, PowerList pw) {
+    __shared__ double part[KYM_MAX_POWERS][256];
+    const long long b = blockIdx.x;
+    const T* xb = x + b * n;
+    double acc[KYM_MAX_POWERS];
+    for (int q = 0; q < KYM_MAX_POWERS; ++q) acc[q] = 0.0;
+    for (long long i = threadIdx.x; i < n; i += blockDim.x) {
+        const double v = (double)xb[i];
+        for (int q = 0; q < pw.n; ++q) acc[q] += pow(v, pw.p[q]);
+    }
+    for (int q = 0; q < pw.n; ++q) part[q][threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int q = 0; q < pw.n; ++q) part[q][threadIdx.x] += part[q][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int q = 0; q < pw.n; ++q) out[b * pw.n + q] = (T)part[q][0];
+}
+
 // out[b, i, j] = mean over a, c < k of x[b, i + a H/k, j + c W/k]   (W = 1 for 1D)
 template <class T>
 __global__ void k_subsample(const cpx<T>* x, cpx<T>* y, long long batch, int H, int W, int kh, int kw) {
@@ -246,7 +288,47 @@ static int launch_unpad2d(const void* x, void* y, long long batch, int H, int W,
     return KYM_OK;
 }
 
+template <class T>
+static int launch_modulus_rot(const void* x, const void* prev, void* y, long long n, hipStream_t s) {
+    hipLaunchKernelGGL(k_modulus_rot<T>, dim3(grid_for(n, 256)), dim3(256), 0, s, (const cpx<T>*)x, (const T*)prev,
+                       (T*)y, n);
+    OPS_HIP_TRY(hipGetLastError());
+    return KYM_OK;
+}
+
+template <class T>
+static int launch_integrals(const void* x, void* out, long long batch, long long n, const PowerList& pw,
+                            hipStream_t s) {
+    hipLaunchKernelGGL(k_integrals<T>, dim3((unsigned)batch), dim3(256), 0, s, (const T*)x, (T*)out, n, pw);
+    OPS_HIP_TRY(hipGetLastError());
+    return KYM_OK;
+}
+
 static int bad_dtype() { return kym_ops_fail(KYM_ERR_TYPE, "dtype must be float32 or float64"); }
+
+extern "C" int kym_op_modulus_rotation(const void* x, const void* prev, void* y, long long n, int dtype,
+                                       void* stream) {
+    if (n <= 0) return KYM_OK;
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == KYM_F32) return launch_modulus_rot<float>(x, prev, y, n, s);
+    if (dtype == KYM_F64) return launch_modulus_rot<double>(x, prev, y, n, s);
+    return bad_dtype();
+}
+
+extern "C" int kym_op_integrals(const void* x, void* out, long long batch, long long n, const double* powers,
+                                int npow, int dtype, void* stream) {
+    if (npow < 1 || npow > KYM_MAX_POWERS || !powers)
+        return kym_ops_fail(KYM_ERR_VALUE, "between 1 and 8 integral powers are supported");
+    if (batch <= 0) return KYM_OK;
+    if (batch > 0x7fffffffll) return kym_ops_fail(KYM_ERR_VALUE, "batch too large");
+    PowerList pw{};
+    pw.n = npow;
+    for (int q = 0; q < npow; ++q) pw.p[q] = powers[q];
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == KYM_F32) return launch_integrals<float>(x, out, batch, n, pw, s);
+    if (dtype == KYM_F64) return launch_integrals<double>(x, out, batch, n, pw, s);
+    return bad_dtype();
+}
 
 extern "C" int kym_op_pad2d(const void* x, void* y, long long batch, int M, int N, int top, int bottom, int left,
                             int right, int dtype, void* stream) {
