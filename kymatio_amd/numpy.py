@@ -3,6 +3,7 @@
 from .scattering1d.frontend.numpy_frontend import ScatteringNumPy1D as Scattering1D
 from .scattering1d.frontend.numpy_frontend import TimeFrequencyScatteringNumPy as TimeFrequencyScattering
 from .scattering2d.frontend.numpy_frontend import ScatteringNumPy2D as Scattering2D
+from .scattering3d.frontend.numpy_frontend import HarmonicScatteringNumPy3D as HarmonicScattering3D
 
 Scattering1D.__module__ = 'kymatio_amd.numpy'
 Scattering1D.__name__ = 'Scattering1D'
@@ -12,4 +13,7 @@ Scattering2D.__name__ = 'Scattering2D'
 TimeFrequencyScattering.__module__ = 'kymatio_amd.numpy'
 TimeFrequencyScattering.__name__ = 'TimeFrequencyScattering'
 
-__all__ = ['Scattering1D', 'Scattering2D', 'TimeFrequencyScattering']
+HarmonicScattering3D.__module__ = 'kymatio_amd.numpy'
+HarmonicScattering3D.__name__ = 'HarmonicScattering3D'
+
+__all__ = ['Scattering1D', 'Scattering2D', 'TimeFrequencyScattering', 'HarmonicScattering3D']

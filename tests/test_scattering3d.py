@@ -140,3 +140,19 @@ def test_batch_shape_agnostic_and_rotation_flag():
     a = HarmonicScattering3D(J=2, shape=(16, 16, 16), rotation_covariant=False).to(DEV)(x)
     b = HarmonicScattering3D(J=2, shape=(16, 16, 16)).to(DEV)(x)
     assert a.shape == b.shape and torch.allclose(a[:, :, 0], b[:, :, 0])  # l = 0 has the single m = 0
+
+
+@pytest.mark.gpu
+def test_numpy_and_sklearn_3d():
+    from kymatio_amd.numpy import HarmonicScattering3D as NumPy3D
+    from kymatio_amd.sklearn import HarmonicScattering3D as Sk3D
+    d = golden('test_data_3d.npz')
+    x = d['x']
+    S = NumPy3D(J=int(d['J']), shape=x.shape[1:], L=int(d['L']), integral_powers=d['integral_powers'])
+    Sx = S(x)
+    assert isinstance(Sx, np.ndarray)
+    ref = d['Sx'][:, len(d['integral_powers']):]
+    assert relative_difference(ref, Sx.reshape(x.shape[0], -1)) < 1e-6
+    t = Sk3D(J=int(d['J']), shape=x.shape[1:], L=int(d['L']),
+             integral_powers=d['integral_powers']).transform(x.reshape(x.shape[0], -1))
+    assert np.allclose(t, Sx.reshape(x.shape[0], -1))
