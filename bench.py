@@ -319,9 +319,11 @@ def main():
 
     # per launch: model bytes per launch = model bytes per image * images of the launch
     chunk = sc._chunk(plan, BATCH)
-    # images per launch: chunks are split in halves over two streams unless the plan says one
-    streams = 1 if (len(sc.tune or ()) > 5 and sc.tune[5] == 1) or chunk < 2 or BATCH <= chunk // 2 else 2
-    per_launch = chunk // 2 if streams == 2 else chunk
+    # images per launch: chunks are split over the plan's streams (tune field 5, 0 = 2) while each gets one
+    streams = (sc.tune[5] if len(sc.tune or ()) > 5 else 0) or 2
+    while streams > 1 and (chunk < streams or BATCH <= chunk // streams * (streams - 1)):
+        streams -= 1
+    per_launch = chunk // streams
     per_kernel = []
     for i, l in enumerate(launches):
         if cnt[i] == 0:

@@ -54,10 +54,10 @@ int kym_scat2d_create(int M, int N, int J, int L, int max_order, int pre_pad, in
  * power-of-two FFT radix (16/8/4/2), LDS-node threads (256/512/1024),
  * big-node column block, big-node row block, big-node pass threads (a
  * compiled choice for these sizes and blocks), streams (1: the inference
- * forward runs its chunks one after the other on the caller's stream; 2,
- * the default: the workspace is split in two halves and chunks alternate
- * between the caller's stream and a second stream of the plan, joined back
- * to the caller's stream before kym_scat2d_forward's work ends), pair (1:
+ * forward runs its chunks one after the other on the caller's stream; S in
+ * 2..4, default 2: the workspace is split in S parts and chunks rotate over
+ * the caller's stream and S - 1 streams of the plan, joined back to the
+ * caller's stream before kym_scat2d_forward's work ends), pair (1:
  * one LDS node per workgroup; 2, the default: sibling nodes of one parent
  * share a workgroup and one read of the parent's spectrum, where a kernel
  * for the size is compiled)}; ntune <= 7; 0 = built-in heuristic.  Results

@@ -62,24 +62,31 @@ struct LaunchProfile {
     }
 };
 
-// Second stream of a plan's inference forward: chunks alternate between the
-// caller's stream and `aux`, joined by events (fork before, join after), so
-// one chunk's launches fill the GPU while the other's drain.  The mutex makes
-// each record + wait pair atomic when several host threads share a plan.
+// Extra streams of a plan's inference forward: chunks rotate over the
+// caller's stream and up to KYM_AUX_STREAMS streams of the plan, forked from
+// and joined back to the caller's stream by events, so one chunk's launches
+// fill the GPU while another's drain.  The mutex makes each record + wait
+// sequence atomic when several host threads share a plan.
+#define KYM_AUX_STREAMS 3
 struct ChunkStreams {
-    hipStream_t aux = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
+    hipStream_t aux[KYM_AUX_STREAMS] = {};
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[KYM_AUX_STREAMS] = {};
     std::mutex mu;
     int create() {
-        KYM_HIP_TRY(hipStreamCreateWithFlags(&aux, hipStreamNonBlocking));
         KYM_HIP_TRY(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
-        KYM_HIP_TRY(hipEventCreateWithFlags(&join, hipEventDisableTiming));
+        for (int i = 0; i < KYM_AUX_STREAMS; ++i) {
+            KYM_HIP_TRY(hipStreamCreateWithFlags(&aux[i], hipStreamNonBlocking));
+            KYM_HIP_TRY(hipEventCreateWithFlags(&join[i], hipEventDisableTiming));
+        }
         return KYM_OK;
     }
     ~ChunkStreams() {
         if (fork) (void)hipEventDestroy(fork);
-        if (join) (void)hipEventDestroy(join);
-        if (aux) (void)hipStreamDestroy(aux);
+        for (int i = 0; i < KYM_AUX_STREAMS; ++i) {
+            if (join[i]) (void)hipEventDestroy(join[i]);
+            if (aux[i]) (void)hipStreamDestroy(aux[i]);
+        }
     }
 };
 
@@ -272,11 +279,11 @@ static int chunk_of(const Scat2DPlan<T>& p, int mode, long long batch, long long
 
 // mode 0: inference forward (batch in chunks); 1: training forward (saves
 // u, whole batch in the workspace); 2: backward from the saved workspace.
-// With two streams (inference, tune.streams != 1, at least two images of
-// workspace) the workspace is split in two halves: chunk k runs on stream
-// k mod 2 in half k mod 2, so chunk k + 2 reuses a half only after chunk k
-// on the same stream.  The aux stream waits for the caller's stream before
-// its first chunk, and the caller's stream for the aux stream at the end.
+// With S streams (inference; tune.streams, 0 = 2; at least S images of
+// workspace) the workspace is split in S parts: chunk k runs on stream
+// k mod S in part k mod S, so chunk k + S reuses a part only after chunk k on
+// the same stream.  The extra streams wait for the caller's stream before
+// their first chunk, and the caller's stream waits for them at the end.
 template <class T>
 static int run2d(const Scat2DPlan<T>& p, int mode, const void* x, const void* filt, void* out, const void* dS,
                  void* gx, long long batch, void* ws, long long ws_bytes, hipStream_t stream, uintptr_t base,
@@ -286,18 +293,21 @@ static int run2d(const Scat2DPlan<T>& p, int mode, const void* x, const void* fi
     int rc = chunk_of(p, mode, batch, ws_bytes, &chunk);
     if (rc != KYM_OK) return rc;
     const std::vector<Launch2D>& launches = mode == 2 ? p.bwd_launches : p.launches;
-    const bool two = mode == 0 && cs && cs->aux && p.tune.streams != 1 && chunk >= 2 && batch > chunk / 2;
-    if (two) {
-        chunk /= 2;
+    int ns = p.tune.streams == 0 ? 2 : p.tune.streams;
+    if (mode != 0 || !cs || !cs->aux[0]) ns = 1;
+    while (ns > 1 && (chunk < ns || batch <= chunk / ns * (ns - 1))) --ns;  // every stream gets a chunk
+    if (ns > 1) {
+        chunk /= ns;
         std::lock_guard<std::mutex> lock(cs->mu);
         KYM_HIP_TRY(hipEventRecord(cs->fork, stream));
-        KYM_HIP_TRY(hipStreamWaitEvent(cs->aux, cs->fork, 0));
+        for (int i = 0; i < ns - 1; ++i) KYM_HIP_TRY(hipStreamWaitEvent(cs->aux[i], cs->fork, 0));
     }
     int k = 0;
     for (long long i0 = 0; i0 < batch; i0 += chunk, ++k) {
         const int nimg = (int)std::min(chunk, batch - i0);
-        const hipStream_t st = (two && (k & 1)) ? cs->aux : stream;
-        void* wsk = static_cast<char*>(ws) + (two ? (k & 1) * chunk * img_elems * (long long)sizeof(cpx<T>) : 0);
+        const int si = k % ns;
+        const hipStream_t st = si ? cs->aux[si - 1] : stream;
+        void* wsk = static_cast<char*>(ws) + si * chunk * img_elems * (long long)sizeof(cpx<T>);
         for (const Launch2D& l : launches) {
             Launch2DCall<T> c;
             rc = launch_call(p, l, mode, x, filt, out, dS, gx, img_elems, wsk, i0, nimg, base, &c);
@@ -316,10 +326,12 @@ static int run2d(const Scat2DPlan<T>& p, int mode, const void* x, const void* fi
             }
         }
     }
-    if (two) {
+    if (ns > 1) {
         std::lock_guard<std::mutex> lock(cs->mu);
-        KYM_HIP_TRY(hipEventRecord(cs->join, cs->aux));
-        KYM_HIP_TRY(hipStreamWaitEvent(stream, cs->join, 0));
+        for (int i = 0; i < ns - 1; ++i) {
+            KYM_HIP_TRY(hipEventRecord(cs->join[i], cs->aux[i]));
+            KYM_HIP_TRY(hipStreamWaitEvent(stream, cs->join[i], 0));
+        }
     }
     return KYM_OK;
 }

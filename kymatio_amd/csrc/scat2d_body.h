@@ -168,6 +168,113 @@ KYM_HD KYM_INL void gather_block_k(cpx<T>* dst, int ncols, int c0, const cpx<T>*
     }
 }
 
+// Gathers of a whole spectrum (ncols == W, c0 == 0) by mirror pairs.  Every
+// parent is the spectrum of a real field (the padded input, or a modulus), so
+// U[-u] = conj(U[u]) on the parent's K H x K W grid, and for 0 < y, 0 < x the
+// child's points (y, x) and (H - y, W - x) sum the same K^2 parent values:
+// the mirror's alias (a, b) is conj(U[y + (K-1-a) H][x + (K-1-b) W]).  One
+// load of each parent value then serves both points; the filter is read at
+// both.  Pair index p (H and W even): rows 0 < y < H/2 with 0 < x < W, then
+// row H/2 with 0 < x <= W/2 (x = W/2 pairs with itself); the remaining
+// pairs (row 0 with 0 <= x <= W/2, column 0 with 0 < y <= H/2) gather both
+// points directly.  Results equal the plain gather's up to rounding.
+KYM_HD KYM_INL constexpr int herm_pairs(int H, int W) { return (H / 2 - 1) * (W - 1) + W + 1 + H / 2; }
+
+// Point of pair p; returns false for the pairs gathered directly.
+KYM_HD KYM_INL bool herm_point(int p, int H, int W, int* y, int* x) {
+    const int na = (H / 2 - 1) * (W - 1);
+    if (p < na) {
+        *y = 1 + p / (W - 1);
+        *x = 1 + p - (*y - 1) * (W - 1);
+        return true;
+    }
+    p -= na;
+    if (p < W / 2) {
+        *y = H / 2;
+        *x = 1 + p;
+        return true;
+    }
+    p -= W / 2;
+    if (p <= W / 2) {
+        *y = 0;
+        *x = p;
+    } else {
+        *y = p - W / 2;
+        *x = 0;
+    }
+    return false;
+}
+
+// For NF filters fs[i] and 0 < y, 0 < x: d[i] += sum u fs[i] at (y, x) and
+// m[i] += sum conj(u') fs[i] at the mirror (H - y, W - x).  Alias rows a and
+// K - 1 - a are taken together (the mirror of one reads the other), so 2 K
+// parent values are live at a time.
+template <int K, int NF, class T>
+KYM_HD KYM_INL void herm_gather(cpx<T>* d, cpx<T>* m, const cpx<T>* U, const T* const* fs, int H, int W, int y,
+                                int x) {
+    const int Wp = W * K, ym = H - y, xm = W - x;
+#pragma unroll
+    for (int a0 = 0; a0 < (K + 1) / 2; ++a0) {
+        const int a1 = K - 1 - a0;
+        cpx<T> u0[K], u1[K];
+#pragma unroll
+        for (int b = 0; b < K; ++b) {
+            u0[b] = kym_ldg(U + (y + a0 * H) * Wp + x + b * W);
+            u1[b] = a1 != a0 ? kym_ldg(U + (y + a1 * H) * Wp + x + b * W) : u0[b];
+        }
+#pragma unroll
+        for (int i = 0; i < NF; ++i) {
+            const T* f = fs[i];
+#pragma unroll
+            for (int b = 0; b < K; ++b) {
+#pragma unroll
+                for (int r = 0; r < (a1 != a0 ? 2 : 1); ++r) {
+                    const int al = r ? a1 : a0;
+                    const cpx<T> v = r ? u1[b] : u0[b];             // alias (al, b) of (y, x)
+                    const cpx<T> w = r ? u0[K - 1 - b] : u1[K - 1 - b];  // alias (al, b) of the mirror, conjugated
+                    const T fd = kym_ldg(f + (y + al * H) * Wp + x + b * W);
+                    const T fm = kym_ldg(f + (ym + al * H) * Wp + xm + b * W);
+                    d[i].re += v.re * fd;
+                    d[i].im += v.im * fd;
+                    m[i].re += w.re * fm;
+                    m[i].im -= w.im * fm;
+                }
+            }
+        }
+    }
+}
+
+// dst[y W + x] = scale P_K(U f)[y][x] for the whole H x W child.
+template <int K, class T>
+KYM_HD KYM_INL void gather_herm_k(cpx<T>* dst, const cpx<T>* U, const T* f, int H, int W, T scale, int tid,
+                                  int nth) {
+    constexpr int E = K >= 4 ? 1 : 2;  // pairs per thread and round
+    const int np = herm_pairs(H, W), Wp = W * K;
+    const T* fs[1] = {f};
+    for (int p0 = tid; p0 < np; p0 += E * nth) {
+        cpx<T> d[E], m[E];
+        int y[E], x[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            d[e] = m[e] = {T(0), T(0)};
+            const bool h = herm_point(p0 + e * nth < np ? p0 + e * nth : p0, H, W, &y[e], &x[e]);
+            if (h) {
+                herm_gather<K, 1>(&d[e], &m[e], U, fs, H, W, y[e], x[e]);
+            } else {
+                d[e] = gather_periodized_k<K>(U, f, Wp, H, W, y[e], x[e]);
+                m[e] = gather_periodized_k<K>(U, f, Wp, H, W, y[e] == 0 ? 0 : H - y[e], x[e] == 0 ? 0 : W - x[e]);
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            if (p0 + e * nth >= np) continue;
+            const int ym = y[e] == 0 ? 0 : H - y[e], xm = x[e] == 0 ? 0 : W - x[e];
+            dst[y[e] * W + x[e]] = d[e] * scale;
+            if (ym != y[e] || xm != x[e]) dst[ym * W + xm] = m[e] * scale;
+        }
+    }
+}
+
 // Wp = W k is the parent's row length.
 template <class T>
 KYM_HD KYM_INL void gather_block(cpx<T>* dst, int ncols, int c0, const cpx<T>* U, const T* f, int Wp, int H, int W, int k,
@@ -183,6 +290,22 @@ KYM_HD KYM_INL void gather_block(cpx<T>* dst, int ncols, int c0, const cpx<T>* U
                 dst[i] = gather_periodized(U, f, Wp, H, W, k, y, c0 + c) * scale;
             }
     }
+}
+
+// The whole H x W child spectrum (a node in LDS): by mirror pairs where the
+// sizes are even and K is 1, 2 or 4, else gather_block.
+template <class T>
+KYM_HD KYM_INL void gather_whole(cpx<T>* dst, const cpx<T>* U, const T* f, int H, int W, int k, int tid, int nth) {
+    const T scale = T(1) / T(k * k);
+    if (H % 2 == 0 && W % 2 == 0 && H >= 2 && W >= 2) {
+        switch (k) {
+            case 1: gather_herm_k<1>(dst, U, f, H, W, scale, tid, nth); return;
+            case 2: gather_herm_k<2>(dst, U, f, H, W, scale, tid, nth); return;
+            case 4: gather_herm_k<4>(dst, U, f, H, W, scale, tid, nth); return;
+            default: break;
+        }
+    }
+    gather_block(dst, W, 0, U, f, W * k, H, W, k, tid, nth);
 }
 
 template <class T>
@@ -667,7 +790,7 @@ KYM_HD void node_body(const Scat2DArgs<T>& a, const G& g, const Node2D& nd, int 
         }
     } else {
         const Node2D& pn = a.nodes[nd.parent];
-        gather_block(lds, g.W, 0, wsi + pn.slot, a.filt + a.filt_off[nd.filt], g.W * nd.k, g.H, g.W, nd.k, tid, nth);
+        gather_whole(lds, wsi + pn.slot, a.filt + a.filt_off[nd.filt], g.H, g.W, nd.k, tid, nth);
         wg_barrier();
         node_after_gather(a, g, nd, il, lds, tid, nth);
         return;
@@ -727,42 +850,45 @@ KYM_HD void body_lds_node_s(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int t
 
 // Gather of two siblings: da[i] = P_K(U fa)[i] scale and db[i] = P_K(U fb)[i]
 // scale for i < H W (both in LDS).  Each alias value of the parent spectrum
-// U is loaded once for both filters.
+// U is loaded once for both filters and, by mirror pairs (gather_herm_k),
+// for both points i and -i.
 template <int K, int H, int W, int NT, class T>
 KYM_HD KYM_INL void gather_pair_s(cpx<T>* da, cpx<T>* db, const cpx<T>* U, const T* fa, const T* fb, T scale,
                                   int tid) {
-    constexpr int n = H * W, E = K >= 4 ? 1 : 4 / K, Wp = W * K;
-    for (int i0 = tid; i0 < n; i0 += E * NT) {
-        cpx<T> va[E], vb[E];
+    static_assert(H % 2 == 0 && W % 2 == 0, "mirror pairs need even sizes");
+    constexpr int np = herm_pairs(H, W), Wp = W * K;
+    for (int p = tid; p < np; p += NT) {
+        int y, x;
+        const bool h = herm_point(p, H, W, &y, &x);
+        const int ym = y == 0 ? 0 : H - y, xm = x == 0 ? 0 : W - x;
+        cpx<T> d[2] = {{T(0), T(0)}, {T(0), T(0)}}, m[2] = {{T(0), T(0)}, {T(0), T(0)}};
+        if (h) {
+            const T* fs[2] = {fa, fb};
+            herm_gather<K, 2>(d, m, U, fs, H, W, y, x);
+        } else {
+            for (int s = 0; s < 2; ++s) {
+                const int yy = s ? ym : y, xx = s ? xm : x;
+                cpx<T> va = {T(0), T(0)}, vb = {T(0), T(0)};
 #pragma unroll
-        for (int u = 0; u < E; ++u) {
-            const int i = i0 + u * NT;
-            va[u] = {T(0), T(0)};
-            vb[u] = {T(0), T(0)};
-            if (i < n) {
-                const int y = i / W, x = i - y * W;
-#pragma unroll
-                for (int al = 0; al < K; ++al) {
-                    const int row = (y + al * H) * Wp + x;
+                for (int al = 0; al < K; ++al)
 #pragma unroll
                     for (int b = 0; b < K; ++b) {
-                        const cpx<T> uv = kym_ldg(U + row + b * W);
-                        const T wa = kym_ldg(fa + row + b * W), wb = kym_ldg(fb + row + b * W);
-                        va[u].re += uv.re * wa;
-                        va[u].im += uv.im * wa;
-                        vb[u].re += uv.re * wb;
-                        vb[u].im += uv.im * wb;
+                        const int idx = (yy + al * H) * Wp + xx + b * W;
+                        const cpx<T> uv = kym_ldg(U + idx);
+                        const T wa = kym_ldg(fa + idx), wb = kym_ldg(fb + idx);
+                        va.re += uv.re * wa;
+                        va.im += uv.im * wa;
+                        vb.re += uv.re * wb;
+                        vb.im += uv.im * wb;
                     }
-                }
+                if (s) { m[0] = va; m[1] = vb; } else { d[0] = va; d[1] = vb; }
             }
         }
-#pragma unroll
-        for (int u = 0; u < E; ++u) {
-            const int i = i0 + u * NT;
-            if (i < n) {
-                da[i] = va[u] * scale;
-                db[i] = vb[u] * scale;
-            }
+        da[y * W + x] = d[0] * scale;
+        db[y * W + x] = d[1] * scale;
+        if (ym != y || xm != x) {
+            da[ym * W + xm] = m[0] * scale;
+            db[ym * W + xm] = m[1] * scale;
         }
     }
 }

@@ -29,7 +29,7 @@ struct Scat2DTune {
     int col_block = 0;      // columns per workgroup of the big-node column passes
     int row_block = 0;      // rows per workgroup of the big-node row passes
     int pass_threads = 0;   // threads of the big-node pass workgroups (compiled choices only)
-    int streams = 0;        // inference chunks in flight: 1, or 2 (0 = default 2; see kym_scat2d_forward)
+    int streams = 0;        // inference chunks in flight: 1 to 4 (0 = default 2; see kym_scat2d_forward)
     int pair = 0;           // LDS nodes: 1 one node per workgroup, 2 sibling pairs where compiled (0 = heuristic: 2)
 };
 
@@ -108,7 +108,7 @@ struct Scat2DPlan {
             tune.col_block < 0 || tune.col_block > 64 || tune.row_block < 0 || tune.row_block > 64 ||
             !(tune.pass_threads == 0 || tune.pass_threads == 256 || (tune.pass_threads % 64 == 0 &&
                                                                       tune.pass_threads <= 1024)) ||
-            !(tune.streams >= 0 && tune.streams <= 2) || !(tune.pair >= 0 && tune.pair <= 2)) {
+            !(tune.streams >= 0 && tune.streams <= 4) || !(tune.pair >= 0 && tune.pair <= 2)) {
             *e = {KYM_ERR_VALUE, "invalid tuning choice"};
             return false;
         }

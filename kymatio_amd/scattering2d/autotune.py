@@ -5,7 +5,7 @@ the device under a small grid of plan choices (kym_scat2d_create_tuned):
 the largest power-of-two FFT radix, the thread count of LDS-node workgroups,
 the column / row blocks of the big-node passes, the LDS budget that decides
 which nodes stay in one workgroup, the images per chunk, whether chunks
-alternate over two streams and whether sibling nodes share a workgroup.  The fastest choice is kept in an
+rotate over one to four streams and whether sibling nodes share a workgroup.  The fastest choice is kept in an
 in-process table and, if a path is given, a JSON file keyed by device name.
 Every choice computes the same transform (they differ by FP32 rounding).
 """
@@ -39,6 +39,8 @@ CANDIDATES = [
     (16, 0, 8, 8, 0, 0, 0, 1),
     (16, 0, 16, 16, 0, 0, 0, 1),
     (16, 0, 8, 8, 0, 0, 64, 1),
+    (16, 0, 8, 8, 0, 0, 0, 3),
+    (16, 0, 8, 8, 0, 0, 0, 4),
     (16, 0, 8, 8, 0, 0, 0, 0, 1),
     (16, 0, 8, 8, 192, 0, 0, 0, 1),
 ]

@@ -67,7 +67,7 @@ def test_invalid_tuning_rejected(emu):
     with pytest.raises(RuntimeError, match='tuning'):
         emu.scat2d(np.zeros((1, 32, 32), np.float32), 2, (32, 32), f, tune=[16, 300, 0, 0])
     with pytest.raises(RuntimeError, match='tuning'):
-        emu.scat2d(np.zeros((1, 32, 32), np.float32), 2, (32, 32), f, tune=[16, 0, 0, 0, 0, 3])
+        emu.scat2d(np.zeros((1, 32, 32), np.float32), 2, (32, 32), f, tune=[16, 0, 0, 0, 0, 5])
     f256 = pack_filters_2d(filter_bank(272, 272, 3, 8))
     with pytest.raises(RuntimeError, match='tuning'):  # no compiled pass of 320 threads and 8 x 8 blocks
         emu.scat2d(np.zeros((1, 256, 256), np.float32), 3, (256, 256), f256, tune=[16, 0, 8, 8, 320])
