@@ -367,9 +367,30 @@ def main():
         'kernels': [{k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()}
                     for r in sorted(per_kernel, key=lambda r: -r['total_ms'])[:8]],
     }
+    res['roofline']['measured_copy_GBps'] = copy_bandwidth(dev)
     if not args.no_cpu_baseline:
         res['cpu_baseline'] = cpu_baseline()
     print(json.dumps(res))
+
+
+def copy_bandwidth(dev, nbytes=1 << 30, reps=5):
+    """Best device-to-device copy rate (bytes read + written per second, GB/s)
+    of a plain nbytes copy on dev: the attainable HBM rate beside the 8 TB/s
+    peak.  Measured after the timed region."""
+    src = torch.empty(nbytes, dtype=torch.uint8, device=dev).fill_(1)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    best = float('inf')
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dst.copy_(src)
+        e1.record()
+        e1.synchronize()
+        best = min(best, e0.elapsed_time(e1) * 1e-3)
+    del src, dst
+    return 2 * nbytes / best / 1e9
 
 
 if __name__ == '__main__':

@@ -176,9 +176,18 @@ int kym_scat1d_info(void* plan, long long* info);
 int kym_scat1d_layout(void* plan, long long* filt_offs, long long* path_len);
 
 /* x: (batch, N) real; out: (batch, out_row) with path c at
- * [sum(path_len[:c]), ...); ws: workspace (>= 1 signal). */
+ * [sum(path_len[:c]), ...); ws: workspace (>= 1 signal).  Chunks of the
+ * batch rotate over the caller's stream and the plan's streams as in
+ * kym_scat2d_forward (kym_scat1d_set_streams); the work ends on `stream`. */
 int kym_scat1d_forward(void* plan, const void* x, const void* filt, void* out, long long batch, void* ws,
                        long long ws_bytes, void* stream);
+
+/* Inference chunks in flight for kym_scat1d_forward (no counterpart in the
+ * reference; a scheduling choice): 1 = one after the other on the caller's
+ * stream, S in 2..4 = the workspace split in S parts and chunks rotating over
+ * the caller's stream and S - 1 streams of the plan; 0 = default 2.  Results
+ * do not depend on it. */
+int kym_scat1d_set_streams(void* plan, int streams);
 
 /* Differentiation of the 1D path, as kym_scat2d_forward_train /
  * kym_scat2d_backward: the training forward keeps every node's pre-modulus

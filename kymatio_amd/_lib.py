@@ -46,6 +46,7 @@ _SIGNATURES = {
     'kym_scat1d_info': [_c_vp, ctypes.POINTER(_c_ll)],
     'kym_scat1d_layout': [_c_vp, ctypes.POINTER(_c_ll), ctypes.POINTER(_c_ll)],
     'kym_scat1d_forward': [_c_vp, _c_vp, _c_vp, _c_vp, _c_ll, _c_vp, _c_ll, _c_vp],
+    'kym_scat1d_set_streams': [_c_vp, _c_i],
     'kym_scat1d_forward_train': [_c_vp, _c_vp, _c_vp, _c_vp, _c_ll, _c_vp, _c_ll, _c_vp],
     'kym_scat1d_backward': [_c_vp, _c_vp, _c_vp, _c_vp, _c_ll, _c_vp, _c_ll, _c_vp],
     'kym_op_pad2d': [_c_vp, _c_vp, _c_ll] + [_c_i] * 7 + [_c_vp],

@@ -90,6 +90,36 @@ struct ChunkStreams {
     }
 };
 
+// Streams a chunked inference call uses: `want` (0 = 2) reduced until every
+// stream gets a chunk; the chunk then shrinks to its share of the workspace.
+static int chunk_streams(const ChunkStreams* cs, int want, int mode, long long batch, long long* chunk) {
+    int ns = want == 0 ? 2 : want;
+    if (mode != 0 || !cs || !cs->aux[0]) ns = 1;
+    while (ns > 1 && (*chunk < ns || batch <= *chunk / ns * (ns - 1))) --ns;
+    if (ns > 1) *chunk /= ns;
+    return ns;
+}
+
+// Fork the extra streams of a chunked call from the caller's stream (begin)
+// or join them back into it (end).
+static int chunk_fork(ChunkStreams* cs, int ns, hipStream_t stream) {
+    if (ns <= 1) return KYM_OK;
+    std::lock_guard<std::mutex> lock(cs->mu);
+    KYM_HIP_TRY(hipEventRecord(cs->fork, stream));
+    for (int i = 0; i < ns - 1; ++i) KYM_HIP_TRY(hipStreamWaitEvent(cs->aux[i], cs->fork, 0));
+    return KYM_OK;
+}
+
+static int chunk_join(ChunkStreams* cs, int ns, hipStream_t stream) {
+    if (ns <= 1) return KYM_OK;
+    std::lock_guard<std::mutex> lock(cs->mu);
+    for (int i = 0; i < ns - 1; ++i) {
+        KYM_HIP_TRY(hipEventRecord(cs->join[i], cs->aux[i]));
+        KYM_HIP_TRY(hipStreamWaitEvent(stream, cs->join[i], 0));
+    }
+    return KYM_OK;
+}
+
 struct KymScat2D {
     int dtype = KYM_F32;
     int device = 0;
@@ -293,15 +323,9 @@ static int run2d(const Scat2DPlan<T>& p, int mode, const void* x, const void* fi
     int rc = chunk_of(p, mode, batch, ws_bytes, &chunk);
     if (rc != KYM_OK) return rc;
     const std::vector<Launch2D>& launches = mode == 2 ? p.bwd_launches : p.launches;
-    int ns = p.tune.streams == 0 ? 2 : p.tune.streams;
-    if (mode != 0 || !cs || !cs->aux[0]) ns = 1;
-    while (ns > 1 && (chunk < ns || batch <= chunk / ns * (ns - 1))) --ns;  // every stream gets a chunk
-    if (ns > 1) {
-        chunk /= ns;
-        std::lock_guard<std::mutex> lock(cs->mu);
-        KYM_HIP_TRY(hipEventRecord(cs->fork, stream));
-        for (int i = 0; i < ns - 1; ++i) KYM_HIP_TRY(hipStreamWaitEvent(cs->aux[i], cs->fork, 0));
-    }
+    const int ns = chunk_streams(cs, p.tune.streams, mode, batch, &chunk);
+    rc = chunk_fork(cs, ns, stream);
+    if (rc != KYM_OK) return rc;
     int k = 0;
     for (long long i0 = 0; i0 < batch; i0 += chunk, ++k) {
         const int nimg = (int)std::min(chunk, batch - i0);
@@ -326,14 +350,7 @@ static int run2d(const Scat2DPlan<T>& p, int mode, const void* x, const void* fi
             }
         }
     }
-    if (ns > 1) {
-        std::lock_guard<std::mutex> lock(cs->mu);
-        for (int i = 0; i < ns - 1; ++i) {
-            KYM_HIP_TRY(hipEventRecord(cs->join[i], cs->aux[i]));
-            KYM_HIP_TRY(hipStreamWaitEvent(stream, cs->join[i], 0));
-        }
-    }
-    return KYM_OK;
+    return chunk_join(cs, ns, stream);
 }
 
 extern "C" int kym_scat2d_forward(void* plan, const void* x, const void* filt, void* out, long long batch,
@@ -523,6 +540,8 @@ struct KymScat1D {
     std::unique_ptr<Scat1DPlan<float>> pf;
     std::unique_ptr<Scat1DPlan<double>> pd;
     LaunchProfile prof;
+    ChunkStreams cs;
+    int streams = 0;  // inference chunks in flight (kym_scat1d_set_streams; 0 = 2)
 };
 
 template <class T>
@@ -566,6 +585,7 @@ extern "C" int kym_scat1d_create(int N, int N_pad, int pad_left, int log2_stride
         if (!h->pd->build(d, lim, &e)) return fail(e.code, e.msg);
         rc = upload1d(*h->pd, &h->dblob);
     }
+    if (rc == KYM_OK) rc = h->cs.create();
     if (rc != KYM_OK) {
         if (h->dblob) (void)hipFree(h->dblob);
         return rc;
@@ -577,14 +597,19 @@ extern "C" int kym_scat1d_create(int N, int N_pad, int pad_left, int log2_stride
 extern "C" int kym_scat1d_destroy(void* plan) {
     if (!plan) return KYM_OK;
     KymScat1D* h = static_cast<KymScat1D*>(plan);
-    if (h->dblob) {
-        int cur = 0;
-        (void)hipGetDevice(&cur);
-        (void)hipSetDevice(h->device);
-        (void)hipFree(h->dblob);
-        (void)hipSetDevice(cur);
-    }
-    delete h;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(h->device);
+    if (h->dblob) (void)hipFree(h->dblob);
+    delete h;  // the plan's streams and events, on its device
+    (void)hipSetDevice(cur);
+    return KYM_OK;
+}
+
+extern "C" int kym_scat1d_set_streams(void* plan, int streams) {
+    if (!plan) return fail(KYM_ERR_VALUE, "null plan");
+    if (streams < 0 || streams > KYM_AUX_STREAMS + 1) return fail(KYM_ERR_VALUE, "streams must be 0-4");
+    static_cast<KymScat1D*>(plan)->streams = streams;
     return KYM_OK;
 }
 
@@ -623,7 +648,7 @@ extern "C" int kym_scat1d_layout(void* plan, long long* filt_offs, long long* pa
 template <class T>
 static int run1d(const Scat1DPlan<T>& p, int mode, const void* x, const void* filt, void* out, const void* dS,
                  void* gx, long long batch, void* ws, long long ws_bytes, hipStream_t stream, uintptr_t base,
-                 LaunchProfile* prof) {
+                 LaunchProfile* prof, ChunkStreams* cs = nullptr, int streams = 1) {
     const long long sig_elems = mode == 0 ? p.ws_sig : p.ws_sig_train;
     const long long per = sig_elems * (long long)sizeof(cpx<T>);
     long long chunk = ws_bytes / per;
@@ -631,13 +656,20 @@ static int run1d(const Scat1DPlan<T>& p, int mode, const void* x, const void* fi
     if (chunk > batch) chunk = batch;
     if (mode != 0 && chunk < batch) return fail(KYM_ERR_VALUE, "training workspace must hold the whole batch");
     const std::vector<Launch1D>& launches = mode == 2 ? p.bwd_launches : p.launches;
-    for (long long i0 = 0; i0 < batch; i0 += chunk) {
+    // inference chunks rotate over the plan's streams as in run2d
+    const int ns = chunk_streams(cs, streams, mode, batch, &chunk);
+    int rc = chunk_fork(cs, ns, stream);
+    if (rc != KYM_OK) return rc;
+    int k = 0;
+    for (long long i0 = 0; i0 < batch; i0 += chunk, ++k) {
         const int nimg = (int)std::min(chunk, batch - i0);
+        const int si = k % ns;
+        const hipStream_t st = si ? cs->aux[si - 1] : stream;
         for (const Launch1D& l : launches) {
             Scat1DArgs<T> a = p.args(l, base);
             a.x = static_cast<const T*>(x);
             a.filt = static_cast<const T*>(filt);
-            a.ws = static_cast<cpx<T>*>(ws);
+            a.ws = reinterpret_cast<cpx<T>*>(static_cast<char*>(ws) + si * chunk * per);
             a.ws_sig = sig_elems;
             a.out = static_cast<T*>(out);
             a.dS = static_cast<const T*>(dS);
@@ -652,18 +684,17 @@ static int run1d(const Scat1DPlan<T>& p, int mode, const void* x, const void* fi
             if (prof && prof->enabled) {
                 e0 = prof->get();
                 e1 = prof->get();
-                if (e0) KYM_HIP_TRY(hipEventRecord(e0, stream));
+                if (e0) KYM_HIP_TRY(hipEventRecord(e0, st));
             }
-            hipLaunchKernelGGL(pick_scat1d<T>(l.threads, l.kind), dim3(grid), dim3(l.threads), l.lds_bytes, stream,
-                               a);
+            hipLaunchKernelGGL(pick_scat1d<T>(l.threads, l.kind), dim3(grid), dim3(l.threads), l.lds_bytes, st, a);
             KYM_HIP_TRY(hipGetLastError());
             if (e0 && e1) {
-                KYM_HIP_TRY(hipEventRecord(e1, stream));
+                KYM_HIP_TRY(hipEventRecord(e1, st));
                 prof->recs.push_back({(int)(&l - launches.data()), {e0, e1}});
             }
         }
     }
-    return KYM_OK;
+    return chunk_join(cs, ns, stream);
 }
 
 extern "C" int kym_scat1d_forward(void* plan, const void* x, const void* filt, void* out, long long batch, void* ws,
@@ -677,8 +708,11 @@ extern "C" int kym_scat1d_forward(void* plan, const void* x, const void* filt, v
     KYM_HIP_TRY(hipGetDevice(&cur));
     if (cur != h->device) return fail(KYM_ERR_TYPE, "plan was created for another device");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    if (h->pf) return run1d(*h->pf, 0, x, filt, out, nullptr, nullptr, batch, ws, ws_bytes, s, (uintptr_t)h->dblob, &h->prof);
-    return run1d(*h->pd, 0, x, filt, out, nullptr, nullptr, batch, ws, ws_bytes, s, (uintptr_t)h->dblob, &h->prof);
+    if (h->pf)
+        return run1d(*h->pf, 0, x, filt, out, nullptr, nullptr, batch, ws, ws_bytes, s, (uintptr_t)h->dblob, &h->prof,
+                     &h->cs, h->streams);
+    return run1d(*h->pd, 0, x, filt, out, nullptr, nullptr, batch, ws, ws_bytes, s, (uintptr_t)h->dblob, &h->prof,
+                 &h->cs, h->streams);
 }
 
 extern "C" int kym_scat1d_forward_train(void* plan, const void* x, const void* filt, void* out, long long batch,

@@ -54,6 +54,10 @@ class Scat1DPlanHandle:
         _lib.check(lib.kym_scat1d_layout(h, fo, pl))
         self.filter_offsets, self.path_len = list(fo), list(pl)
 
+    def set_streams(self, streams):
+        """Inference chunks in flight (0 = default 2; kym_scat1d_set_streams)."""
+        _lib.check(self._lib.kym_scat1d_set_streams(self._h, int(streams)))
+
     def forward(self, x, filt, out, batch, ws):
         _lib.check(self._lib.kym_scat1d_forward(
             self._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(filt.data_ptr()),
@@ -97,6 +101,7 @@ class ScatteringTorch1D(ScatteringTorch, ScatteringBase):
         self._packed = {}
         self.signals_per_chunk = None
         self.lds_limit = 0
+        self.streams = 0  # inference chunks in flight on the device (0 = the library's default, 2)
 
     # ------------------------------------------------------------ setup
     def build(self):
@@ -264,6 +269,8 @@ class ScatteringTorch1D(ScatteringTorch, ScatteringBase):
         if p is None:
             with torch.cuda.device(device):
                 p = Scat1DPlanHandle(self, self.max_order, dtype, self.lds_limit)
+                if self.streams:
+                    p.set_streams(self.streams)
             self._plans[key] = p
         return p
 

@@ -52,6 +52,14 @@ def test_workload_batch_properties():
     assert torch.equal(a, sc(x))
     for i in (0, 255):
         assert torch.equal(sc(x[i:i + 1].contiguous())[0], a[i])
+    # chunks rotate over 1-4 streams (the plan's own streams joined back to
+    # the caller's): the same bits, also with many chunks per stream
+    for streams, per_chunk in ((1, None), (3, None), (4, 40), (2, 40)):
+        sc.streams, sc.signals_per_chunk = streams, per_chunk
+        sc._plans.clear()
+        assert torch.equal(sc(x * 1.0), a), (streams, per_chunk)
+    sc.streams, sc.signals_per_chunk = 0, None
+    sc._plans.clear()
     ref = scat1d.Scattering1DTorchCPU(6, 4096, 8)(x[:4].cpu())
     assert channel_rel_err(a[:4].cpu().numpy(), ref.numpy()) <= RTOL_CHANNEL
 

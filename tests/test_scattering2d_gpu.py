@@ -97,7 +97,14 @@ def test_full_batch_properties():
     d2 = sc(x.clone() * 1.0)
     sc.tune = [0, 0, 0, 0, 0, 1]
     d1 = sc(x)
-    assert torch.equal(a, d1) and torch.equal(a, d2)
+    sc.tune = [0, 0, 0, 0, 0, 4]
+    sc._plans.clear()
+    d4 = sc(x * 1.0)
+    sc.tune = [0, 0, 0, 0, 0, 3]
+    sc._plans.clear()
+    d3 = sc(x * 1.0)
+    assert torch.equal(a, d1) and torch.equal(a, d2) and torch.equal(a, d3) and torch.equal(a, d4)
+    sc._plans.clear()
     sc.tune = None
     side = torch.cuda.Stream(device=DEV)
     side.wait_stream(torch.cuda.current_stream(DEV))
