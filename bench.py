@@ -8,7 +8,10 @@ before the timed region).  Prints one JSON line:
   value            images/s over the K timed steps
   roofline         dominant kernel: algorithmic bytes per launch (stage-boundary
                    byte model) / its mean launch time from HIP events recorded
-                   around every launch during the timed steps, vs 8 TB/s
+                   around every launch during the timed steps, vs 8 TB/s;
+                   .isolated: the same launch timed after the timed region
+                   with every chunk on one stream (nothing beside it);
+                   .measured_copy_GBps: a plain 1 GiB device copy's rate
   pipeline_roofline the same model over the whole forward
   cpu_baseline     the CPU oracle (op-for-op restatement of the reference's torch
                    CPU path) on a bounded sample of the same workload
@@ -319,11 +322,16 @@ def main():
 
     # per launch: model bytes per launch = model bytes per image * images of the launch
     chunk = sc._chunk(plan, BATCH)
-    # images per launch: chunks are split over the plan's streams (tune field 5, 0 = 2) while each gets one
+    # images per launch: chunks are split over the plan's streams (tune field 5, 0 = 2) while each gets
+    # one, the chunk count rounded up to a multiple of the streams (as the library's chunk_streams)
     streams = (sc.tune[5] if len(sc.tune or ()) > 5 else 0) or 2
     while streams > 1 and (chunk < streams or BATCH <= chunk // streams * (streams - 1)):
         streams -= 1
-    per_launch = chunk // streams
+    per_launch = chunk
+    if streams > 1:
+        n = -(-BATCH // (chunk // streams))
+        n = -(-n // streams) * streams
+        per_launch = -(-BATCH // n)
     per_kernel = []
     for i, l in enumerate(launches):
         if cnt[i] == 0:
@@ -337,6 +345,8 @@ def main():
                            'gbps': bytes_per_launch / (mean_ms * 1e-3) / 1e9})
     dom = max(per_kernel, key=lambda r: r['total_ms'])
     achieved = dom['model_bytes_per_launch'] / (dom['mean_us'] * 1e-6)
+    isolated = isolated_launch(sc, x, dev, dom['launch'], per_launch)
+    isolated['frac'] = dom['model_bytes_per_launch'] / (isolated['mean_launch_us'] * 1e-6) / HBM_PEAK
     per_node_image = measured_traffic_per_node_image()
     traffic = None
     if per_node_image is not None and dom['kind'] == 'lds_node' and dom['res'] == 1:
@@ -359,7 +369,8 @@ def main():
                      'traffic_source': 'profiles/scat2d_j3_256_hbm_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, '
                                        'copy-calibrated), bytes per launch' if traffic is not None else None,
                      'kernel': '%s (res %d, %d nodes/image)' % (dom['kind'], dom['res'], dom['nodes']),
-                     'mean_launch_us': dom['mean_us'], 'model_bytes_per_launch': dom['model_bytes_per_launch']},
+                     'mean_launch_us': dom['mean_us'], 'model_bytes_per_launch': dom['model_bytes_per_launch'],
+                     'isolated': isolated},
         'pipeline_roofline': {'model_bytes_per_image': total_model / BATCH,
                               'achieved_GBps': total_model / (elapsed / args.steps) / 1e9,
                               'frac': total_model / (elapsed / args.steps) / HBM_PEAK,
@@ -371,6 +382,31 @@ def main():
     if not args.no_cpu_baseline:
         res['cpu_baseline'] = cpu_baseline()
     print(json.dumps(res))
+
+
+def isolated_launch(sc, x, dev, index, per_launch, steps=5):
+    """Mean duration of launch `index` with the same images per launch but
+    every chunk on one stream, so no other launch shares the GPU with it
+    (measured after the timed region; the timed run overlaps two streams)."""
+    saved = (sc.tune, sc.images_per_chunk)
+    tune = list(sc.tune or [])
+    tune += [0] * (6 - len(tune))
+    tune[5] = 1
+    sc.tune, sc.images_per_chunk = tune, per_launch
+    try:
+        with torch.no_grad():
+            sc(x)
+            plan = sc._plan(dev, torch.float32)
+            torch.cuda.synchronize()
+            plan.profile(True)
+            for _ in range(steps):
+                sc(x)
+            torch.cuda.synchronize()
+            ms, cnt = plan.profile_read()
+            plan.profile(False)
+    finally:
+        sc.tune, sc.images_per_chunk = saved
+    return {'mean_launch_us': 1e3 * ms[index] / cnt[index], 'streams': 1, 'images_per_launch': per_launch}
 
 
 def copy_bandwidth(dev, nbytes=1 << 30, reps=5):

@@ -91,12 +91,20 @@ struct ChunkStreams {
 };
 
 // Streams a chunked inference call uses: `want` (0 = 2) reduced until every
-// stream gets a chunk; the chunk then shrinks to its share of the workspace.
+// stream gets a chunk.  The chunk then fits one stream's share of the
+// workspace, with the chunk count a multiple of the stream count and the
+// chunks as even as that allows (256 over 3 streams: 6 chunks of <= 43, not
+// 85, 85, 85, 1).
 static int chunk_streams(const ChunkStreams* cs, int want, int mode, long long batch, long long* chunk) {
     int ns = want == 0 ? 2 : want;
     if (mode != 0 || !cs || !cs->aux[0]) ns = 1;
     while (ns > 1 && (*chunk < ns || batch <= *chunk / ns * (ns - 1))) --ns;
-    if (ns > 1) *chunk /= ns;
+    if (ns > 1) {
+        const long long part = *chunk / ns;
+        long long n = (batch + part - 1) / part;
+        n = (n + ns - 1) / ns * ns;
+        *chunk = (batch + n - 1) / n;
+    }
     return ns;
 }
 
