@@ -206,32 +206,43 @@ KYM_HD KYM_INL bool herm_point(int p, int H, int W, int* y, int* x) {
 }
 
 // For NF filters fs[i] and 0 < y, 0 < x: d[i] += sum u fs[i] at (y, x) and
-// m[i] += sum conj(u') fs[i] at the mirror (H - y, W - x).  Alias rows a and
-// K - 1 - a are taken together (the mirror of one reads the other), so 2 K
-// parent values are live at a time.
+// m[i] += sum conj(u') fs[i] at the mirror (H - y, W - x).  The mirror of
+// alias (a, b) reads the point's alias (K-1-a, K-1-b), so alias rows
+// {a, K-1-a} and column chunks {c, C-1-c} (CH = min(K, 4) columns each) are
+// taken together: at most 16 parent values are live at a time.
 template <int K, int NF, class T>
-KYM_HD KYM_INL void herm_gather(cpx<T>* d, cpx<T>* m, const cpx<T>* U, const T* const* fs, int H, int W, int y,
-                                int x) {
-    const int Wp = W * K, ym = H - y, xm = W - x;
+KYM_HD KYM_INL void herm_block(cpx<T>* d, cpx<T>* m, const cpx<T>* U, const T* const* fs, int H, int W, int y, int x,
+                               int a0, int c0) {
+    constexpr int CH = K < 4 ? K : 4, NC = K / CH;
+    const int Wp = W * K, ym = H - y, xm = W - x, a1 = K - 1 - a0, c1 = NC - 1 - c0;
+    const bool two_a = a1 != a0, two_c = c1 != c0;
+    cpx<T> u[2][2][CH];
 #pragma unroll
-    for (int a0 = 0; a0 < (K + 1) / 2; ++a0) {
-        const int a1 = K - 1 - a0;
-        cpx<T> u0[K], u1[K];
+    for (int r = 0; r < 2; ++r)
 #pragma unroll
-        for (int b = 0; b < K; ++b) {
-            u0[b] = kym_ldg(U + (y + a0 * H) * Wp + x + b * W);
-            u1[b] = a1 != a0 ? kym_ldg(U + (y + a1 * H) * Wp + x + b * W) : u0[b];
-        }
+        for (int q = 0; q < 2; ++q)
 #pragma unroll
-        for (int i = 0; i < NF; ++i) {
-            const T* f = fs[i];
+            for (int e = 0; e < CH; ++e) {
+                if ((r && !two_a) || (q && !two_c)) {
+                    u[r][q][e] = u[r && two_a][q && two_c][e];
+                } else {
+                    const int al = r ? a1 : a0, b = (q ? c1 : c0) * CH + e;
+                    u[r][q][e] = kym_ldg(U + (y + al * H) * Wp + x + b * W);
+                }
+            }
 #pragma unroll
-            for (int b = 0; b < K; ++b) {
+    for (int i = 0; i < NF; ++i) {
+        const T* f = fs[i];
 #pragma unroll
-                for (int r = 0; r < (a1 != a0 ? 2 : 1); ++r) {
-                    const int al = r ? a1 : a0;
-                    const cpx<T> v = r ? u1[b] : u0[b];             // alias (al, b) of (y, x)
-                    const cpx<T> w = r ? u0[K - 1 - b] : u1[K - 1 - b];  // alias (al, b) of the mirror, conjugated
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int q = 0; q < 2; ++q)
+#pragma unroll
+                for (int e = 0; e < CH; ++e) {
+                    if ((r && !two_a) || (q && !two_c)) continue;
+                    const int al = r ? a1 : a0, b = (q ? c1 : c0) * CH + e;
+                    const cpx<T> v = u[r][q][e];                   // alias (al, b) of (y, x)
+                    const cpx<T> w = u[1 - r][1 - q][CH - 1 - e];  // alias (al, b) of the mirror, conjugated
                     const T fd = kym_ldg(f + (y + al * H) * Wp + x + b * W);
                     const T fm = kym_ldg(f + (ym + al * H) * Wp + xm + b * W);
                     d[i].re += v.re * fd;
@@ -239,8 +250,19 @@ KYM_HD KYM_INL void herm_gather(cpx<T>* d, cpx<T>* m, const cpx<T>* U, const T* 
                     m[i].re += w.re * fm;
                     m[i].im -= w.im * fm;
                 }
-            }
-        }
+    }
+}
+
+template <int K, int NF, class T>
+KYM_HD KYM_INL void herm_gather(cpx<T>* d, cpx<T>* m, const cpx<T>* U, const T* const* fs, int H, int W, int y,
+                                int x) {
+    constexpr int NC = K < 4 ? 1 : K / 4;
+    if constexpr (K <= 4) {
+#pragma unroll
+        for (int a0 = 0; a0 < (K + 1) / 2; ++a0) herm_block<K, NF>(d, m, U, fs, H, W, y, x, a0, 0);
+    } else {  // rolled loops over the row and chunk pairs: the body is already 32 alias terms
+        for (int a0 = 0; a0 < K / 2; ++a0)
+            for (int c0 = 0; c0 < NC / 2; ++c0) herm_block<K, NF>(d, m, U, fs, H, W, y, x, a0, c0);
     }
 }
 
@@ -261,8 +283,14 @@ KYM_HD KYM_INL void gather_herm_k(cpx<T>* dst, const cpx<T>* U, const T* f, int 
             if (h) {
                 herm_gather<K, 1>(&d[e], &m[e], U, fs, H, W, y[e], x[e]);
             } else {
-                d[e] = gather_periodized_k<K>(U, f, Wp, H, W, y[e], x[e]);
-                m[e] = gather_periodized_k<K>(U, f, Wp, H, W, y[e] == 0 ? 0 : H - y[e], x[e] == 0 ? 0 : W - x[e]);
+                const int ym = y[e] == 0 ? 0 : H - y[e], xm = x[e] == 0 ? 0 : W - x[e];
+                if constexpr (K <= 4) {
+                    d[e] = gather_periodized_k<K>(U, f, Wp, H, W, y[e], x[e]);
+                    m[e] = gather_periodized_k<K>(U, f, Wp, H, W, ym, xm);
+                } else {  // rolled: K^2 terms
+                    d[e] = gather_periodized(U, f, Wp, H, W, K, y[e], x[e]);
+                    m[e] = gather_periodized(U, f, Wp, H, W, K, ym, xm);
+                }
             }
         }
 #pragma unroll
@@ -293,7 +321,7 @@ KYM_HD KYM_INL void gather_block(cpx<T>* dst, int ncols, int c0, const cpx<T>* U
 }
 
 // The whole H x W child spectrum (a node in LDS): by mirror pairs where the
-// sizes are even and K is 1, 2 or 4, else gather_block.
+// sizes are even and K is a power of two up to 32, else gather_block.
 template <class T>
 KYM_HD KYM_INL void gather_whole(cpx<T>* dst, const cpx<T>* U, const T* f, int H, int W, int k, int tid, int nth) {
     const T scale = T(1) / T(k * k);
@@ -302,6 +330,9 @@ KYM_HD KYM_INL void gather_whole(cpx<T>* dst, const cpx<T>* U, const T* f, int H
             case 1: gather_herm_k<1>(dst, U, f, H, W, scale, tid, nth); return;
             case 2: gather_herm_k<2>(dst, U, f, H, W, scale, tid, nth); return;
             case 4: gather_herm_k<4>(dst, U, f, H, W, scale, tid, nth); return;
+            case 8: gather_herm_k<8>(dst, U, f, H, W, scale, tid, nth); return;
+            case 16: gather_herm_k<16>(dst, U, f, H, W, scale, tid, nth); return;
+            case 32: gather_herm_k<32>(dst, U, f, H, W, scale, tid, nth); return;
             default: break;
         }
     }
