@@ -27,11 +27,29 @@ KYM_HD KYM_INL cpx<T> grad_u_elem(const Scat2DArgs<T>& a, const Node2D& nd, cons
     const T ph = a.filt[a.filt_off[nd.leaf_filt] + i];
     const cpx<T> gy = GY[(y % HJ) * WJ + (x % WJ)];
     cpx<T> acc = {gy.re * ph, gy.im * ph};
-    for (int c = nd.child_begin; c < nd.child_end; ++c) {
+    auto term = [&](int c, T* fv, cpx<T>* gx) {
         const Node2D& cn = a.nodes[c];
         const int Hc = a.H[cn.res], Wc = a.W[cn.res];
-        const T fv = a.filt[a.filt_off[cn.filt] + i] / T(cn.k * cn.k);
-        const cpx<T> gx = wsi[cn.gsave + (long long)(y % Hc) * Wc + (x % Wc)];
+        *fv = kym_ldg(a.filt + a.filt_off[cn.filt] + i) / T(cn.k * cn.k);
+        *gx = wsi[cn.gsave + (long long)(y % Hc) * Wc + (x % Wc)];
+    };
+    // children four at a time, their loads issued before the sums
+    int c = nd.child_begin;
+    for (; c + 4 <= nd.child_end; c += 4) {
+        T fv[4];
+        cpx<T> gx[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) term(c + u, &fv[u], &gx[u]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            acc.re += gx[u].re * fv[u];
+            acc.im += gx[u].im * fv[u];
+        }
+    }
+    for (; c < nd.child_end; ++c) {
+        T fv;
+        cpx<T> gx;
+        term(c, &fv, &gx);
         acc.re += gx.re * fv;
         acc.im += gx.im * fv;
     }
