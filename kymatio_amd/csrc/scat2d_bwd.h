@@ -33,15 +33,15 @@ KYM_HD KYM_INL cpx<T> grad_u_elem(const Scat2DArgs<T>& a, const Node2D& nd, cons
         *fv = kym_ldg(a.filt + a.filt_off[cn.filt] + i) / T(cn.k * cn.k);
         *gx = wsi[cn.gsave + (long long)(y % Hc) * Wc + (x % Wc)];
     };
-    // children four at a time, their loads issued before the sums
+    // children eight at a time, their loads issued before the sums
     int c = nd.child_begin;
-    for (; c + 4 <= nd.child_end; c += 4) {
-        T fv[4];
-        cpx<T> gx[4];
+    for (; c + 8 <= nd.child_end; c += 8) {
+        T fv[8];
+        cpx<T> gx[8];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) term(c + u, &fv[u], &gx[u]);
+        for (int u = 0; u < 8; ++u) term(c + u, &fv[u], &gx[u]);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < 8; ++u) {
             acc.re += gx[u].re * fv[u];
             acc.im += gx[u].im * fv[u];
         }
