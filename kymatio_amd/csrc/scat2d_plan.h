@@ -423,7 +423,15 @@ struct Scat2DPlan {
                     max_lds = std::max(max_lds, launches.back().lds_bytes);
                 }
             } else {
-                const int cb = col_blk(H[r]), rb = row_blk(r);
+                int cb = col_blk(H[r]);
+                const int rb = row_blk(r);
+                // the heuristic's column block yields to 8 columns where only that
+                // block has compile-time passes for these sizes (160, 320, 640)
+                if (!tune.col_block && specs && sizeof(T) == 4 && cb > 8 && W[r] % 8 == 0 && H[r] % rb == 0) {
+                    int t0 = 0;
+                    if (pass_spec_id(H[r], W[r], cb, rb, 0, &t0) < 0 && pass_spec_id(H[r], W[r], 8, rb, 0, &t0) >= 0)
+                        cb = 8;
+                }
                 // compile-time passes when the sizes match and every block is full
                 int pnt = 0;
                 const int want = tune.pass_threads == 256 ? 0 : tune.pass_threads;
