@@ -57,6 +57,9 @@ static int emu_forward2d(int M, int N, int J, int L, int max_order, int pre_pad,
         info[4] = nbig;
         info[5] = nspec;
         info[6] = npair;
+        int nbspec = 0;
+        for (auto& l : p.bwd_launches) nbspec += l.spec >= 0;
+        info[7] = nbspec;
     }
     if (!x) return 0;
     if (chunk <= 0 || chunk > batch) chunk = batch;
@@ -113,7 +116,8 @@ static int emu_train2d(int M, int N, int J, int L, int max_order, int pre_pad, l
             a.img0 = 0;
             a.ntask = (int)(batch * l.ngroup * l.nblk);
             for (int t = 0; t < a.ntask; ++t) {
-                if (l.kind >= K2B_NODE) dispatch_bwd2d<T, 31>(l.kind, a, t, lds.data(), 0, 1);
+                if (l.kind == K2B_NODE && l.spec >= 0) dispatch_bwd_node_spec<T, true>(l.spec, a, t, lds.data(), 0);
+                else if (l.kind >= K2B_NODE) dispatch_bwd2d<T, 31>(l.kind, a, t, lds.data(), 0, 1);
                 else if (l.spec >= 0) emu_spec(l, a, t, lds.data());
                 else dispatch_body2d<T, 31>(l.kind, a, t, lds.data(), 0, 1);
             }

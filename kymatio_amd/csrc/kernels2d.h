@@ -37,8 +37,28 @@ __global__ void __launch_bounds__(NT, 2) k_pass2d_spec(Scat2DArgs<T> a) {
     dispatch_pass_spec<T, KIND, false>(SPEC, a, task, reinterpret_cast<cpx<T>*>(kym_smem2d_ps), threadIdx.x);
 }
 
+template <class T, int SPEC, int NT>
+__global__ void __launch_bounds__(NT) k_bwd_node2d_spec(Scat2DArgs<T> a) {
+    extern __shared__ __attribute__((aligned(16))) char kym_smem2d_bns[];
+    const int task = xcd_task(blockIdx.x, gridDim.x);
+    if (task >= a.ntask) return;
+    bwd_node_spec_body<T, SPEC, false>(a, task, reinterpret_cast<cpx<T>*>(kym_smem2d_bns), threadIdx.x);
+}
+
 template <class T>
 using Scat2DKernel = void (*)(Scat2DArgs<T>);
+
+template <class T>
+Scat2DKernel<T> pick_bwd_node_spec(int id) {
+    if constexpr (sizeof(T) != 4) return nullptr;  // specialisations are FP32 only
+    else switch (id) {
+#define KYM_BNS_K(sid, h, w, hj, wj, nt, pr) \
+    case sid: return k_bwd_node2d_spec<T, sid, nt>;
+        KYM_NODE_SPECS(KYM_BNS_K)
+#undef KYM_BNS_K
+        default: return nullptr;
+    }
+}
 
 template <class T>
 Scat2DKernel<T> pick_node_spec(int id) {

@@ -14,6 +14,7 @@ Scat2DKernel<double> kym_pick2d_f64_p7(int kind, int threads);
 Scat2DKernel<double> kym_pick2d_f64_p17(int kind, int threads);
 Scat2DKernel<double> kym_pick2d_f64_p31(int kind, int threads);
 Scat2DKernel<float> kym_pick_node_spec_f32(int id);
+Scat2DKernel<float> kym_pick_bwd_node_spec_f32(int id);
 Scat2DKernel<float> kym_pick_pass_spec_f32(int kind, int id);
 Scat1DKernel<float> kym_pick1d_f32(int threads, int kind);
 Scat1DKernel<double> kym_pick1d_f64(int threads, int kind);
@@ -24,6 +25,7 @@ template <class T>
 inline Scat2DKernel<T> pick_scat2d(int kind, int threads, int pclass, int spec = -1) {
     if constexpr (sizeof(T) == 4) {
         if (spec >= 0 && kind == K2_LDS_NODE) return kym_pick_node_spec_f32(spec);
+        if (spec >= 0 && kind == K2B_NODE) return kym_pick_bwd_node_spec_f32(spec);
         if (spec >= 0) return kym_pick_pass_spec_f32(kind, spec);
         if (pclass <= 7) return kym_pick2d_f32_p7(kind, threads);
         if (pclass <= 17) return kym_pick2d_f32_p17(kind, threads);

@@ -127,6 +127,43 @@ KYM_HD void body_bwd_node(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid
     bwd_node_body(a, dyn_geom<T, MAXP>(a, nd.res), nd, il, lds, tid, nth);
 }
 
+// A node's backward with compile-time sizes (FP32): twiddles of both lengths
+// in LDS first, the node's buffers after them.  task -> (image, node).
+template <class T, int H, int W, int HJ, int WJ, int NT>
+KYM_HD void body_bwd_node_s(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid) {
+    const int il = task / a.ngroup, gi = task - il * a.ngroup;
+    const Node2D& nd = a.nodes[a.group[gi]];
+    const int r = nd.res, J = a.J;
+    constexpr int TWE = kym_tw_elems(H, W);
+    const FftLen<T>* fH = &a.lens[a.lenH[r]];
+    const FftLen<T>* fW = &a.lens[a.lenW[r]];
+    load_twiddles(lds, fH, fW, H, W, tid, NT);
+    const StatGeom<T, H, W, HJ, WJ, NT> g{fH, fW, &a.lens[a.lenH[J]], &a.lens[a.lenW[J]], lds + (H != W ? W : 0), lds};
+    bwd_node_body(a, g, nd, il, lds + TWE, tid, NT);
+}
+
+// The backward body of node specialisation SPEC (its sizes and threads; the
+// pair flag does not apply: the backward runs one node per workgroup).
+template <class T, int SPEC, bool EMU>
+KYM_HD KYM_INL void bwd_node_spec_body(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid) {
+#define KYM_BNS_BODY(sid, h, w, hj, wj, nt, pr) \
+    if constexpr (SPEC == sid) body_bwd_node_s<T, h, w, hj, wj, (EMU ? 1 : nt)>(a, task, lds, tid);
+    KYM_NODE_SPECS(KYM_BNS_BODY)
+#undef KYM_BNS_BODY
+}
+
+// Run-time selection (the CPU harness).
+template <class T, bool EMU>
+KYM_HD void dispatch_bwd_node_spec(int id, const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid) {
+    switch (id) {
+#define KYM_BNS_CASE(sid, h, w, hj, wj, nt, pr) \
+    case sid: bwd_node_spec_body<T, sid, EMU>(a, task, lds, tid); break;
+        KYM_NODE_SPECS(KYM_BNS_CASE)
+#undef KYM_BNS_CASE
+        default: break;
+    }
+}
+
 // Leaf gradient of a node whose leaf is not formed in its node kernel.
 template <class T, int MAXP>
 KYM_HD void body_bwd_leafg(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {

@@ -503,6 +503,16 @@ struct Scat2DPlan {
             if (!n0.big) {
                 const size_t lds = (elems + (n0.leaf_mode == 0 ? leaf_elems : 0)) * cs;
                 push(K2B_NODE, ids, 1, 0, node_threads(elems), lds);
+                // compile-time sizes as in the forward (one node per workgroup)
+                int nt = 0;
+                const int sid = node_spec_id(H[r], W[r], H[J], W[J], tune.node_threads, &nt, 0);
+                const size_t tw = cs * kym_tw_elems(H[r], W[r]);
+                if (specs && sizeof(T) == 4 && sid >= 0 && lds + tw <= lds_limit) {
+                    launches.back().spec = sid;
+                    launches.back().threads = nt;
+                    launches.back().lds_bytes = lds + tw;
+                    max_lds = std::max(max_lds, launches.back().lds_bytes);
+                }
             } else {
                 const int cb = col_blk(H[r]), rb = std::max(1, std::min(16, (int)(pass_budget / (cs * W[r]))));
                 push(K2B_GCOLS, ids, (W[r] + cb - 1) / cb, cb, 256, (size_t)H[r] * cb * cs);

@@ -119,7 +119,8 @@ class Emu:
                                                  filters_packed.ctypes.data, out.ctypes.data, B, chunk, info)
         if rc != 0:
             raise RuntimeError(self.lib.kym_emu_last_error().decode())
-        return out, {'launches': info[3], 'big_nodes': info[4], 'spec_launches': info[5], 'pair_launches': info[6]}
+        return out, {'launches': info[3], 'big_nodes': info[4], 'spec_launches': info[5], 'pair_launches': info[6],
+                     'bwd_spec_launches': info[7]}
 
 
 def emu_scat1d(emu, module, x, dtype=np.float32, dS=None, nbwd=1):
