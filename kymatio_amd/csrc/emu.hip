@@ -117,6 +117,12 @@ static int emu_train2d(int M, int N, int J, int L, int max_order, int pre_pad, l
             a.ntask = (int)(batch * l.ngroup * l.nblk);
             for (int t = 0; t < a.ntask; ++t) {
                 if (l.kind == K2B_NODE && l.spec >= 0) dispatch_bwd_node_spec<T, true>(l.spec, a, t, lds.data(), 0);
+                else if (l.kind == K2B_GCOLS && l.spec >= 0)
+                    dispatch_bwd_pass_spec<T, K2B_GCOLS, true>(l.spec, a, t, lds.data(), 0);
+                else if (l.kind == K2B_GROWS && l.spec >= 0)
+                    dispatch_bwd_pass_spec<T, K2B_GROWS, true>(l.spec, a, t, lds.data(), 0);
+                else if (l.kind == K2B_GFCOLS && l.spec >= 0)
+                    dispatch_bwd_pass_spec<T, K2B_GFCOLS, true>(l.spec, a, t, lds.data(), 0);
                 else if (l.kind >= K2B_NODE) dispatch_bwd2d<T, 31>(l.kind, a, t, lds.data(), 0, 1);
                 else if (l.spec >= 0) emu_spec(l, a, t, lds.data());
                 else dispatch_body2d<T, 31>(l.kind, a, t, lds.data(), 0, 1);

@@ -34,7 +34,10 @@ __global__ void __launch_bounds__(NT, 2) k_pass2d_spec(Scat2DArgs<T> a) {
     extern __shared__ __attribute__((aligned(16))) char kym_smem2d_ps[];
     const int task = xcd_task(blockIdx.x, gridDim.x);
     if (task >= a.ntask) return;
-    dispatch_pass_spec<T, KIND, false>(SPEC, a, task, reinterpret_cast<cpx<T>*>(kym_smem2d_ps), threadIdx.x);
+    if constexpr (KIND >= K2B_NODE)
+        dispatch_bwd_pass_spec<T, KIND, false>(SPEC, a, task, reinterpret_cast<cpx<T>*>(kym_smem2d_ps), threadIdx.x);
+    else
+        dispatch_pass_spec<T, KIND, false>(SPEC, a, task, reinterpret_cast<cpx<T>*>(kym_smem2d_ps), threadIdx.x);
 }
 
 template <class T, int SPEC, int NT>
@@ -91,6 +94,9 @@ Scat2DKernel<T> pick_pass_spec(int kind, int id) {
         case K2_ICOL: return pick_pass_spec_k<T, K2_ICOL>(id);
         case K2_ROWS_MOD: return pick_pass_spec_k<T, K2_ROWS_MOD>(id);
         case K2_FWD_COLS: return pick_pass_spec_k<T, K2_FWD_COLS>(id);
+        case K2B_GCOLS: return pick_pass_spec_k<T, K2B_GCOLS>(id);
+        case K2B_GROWS: return pick_pass_spec_k<T, K2B_GROWS>(id);
+        case K2B_GFCOLS: return pick_pass_spec_k<T, K2B_GFCOLS>(id);
         default: return nullptr;
     }
 }

@@ -176,8 +176,8 @@ KYM_HD void body_bwd_leafg(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int ti
 }
 
 // Big nodes: dL/dU by column blocks, inverse DIF along y -> U slot (rows digit-reversed).
-template <class T, int MAXP>
-KYM_HD void body_bwd_gcols(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
+template <class T, class P>
+KYM_HD void body_bwd_gcols(const P& fp, const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
     const int per = a.ngroup * a.nblk;
     const int il = task / per, rem = task - il * per;
     const int gi = rem / a.nblk, cb = rem - gi * a.nblk;
@@ -190,7 +190,7 @@ KYM_HD void body_bwd_gcols(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int ti
         lds[i] = grad_u_elem(a, nd, wsi, wsi + nd.gslot, W, a.H[J], a.W[J], y, c0 + c);
     }
     wg_barrier();
-    fft_dif<true, MAXP>(lds, a.lens[a.lenH[r]], Batch{nc, nc, 1, true}, tid, nth);
+    fp.col_inv(lds, a.lens[a.lenH[r]], nc, tid, nth);
     cpx<T>* dst = wsi + nd.slot;
     for (int i = tid; i < H * nc; i += nth) {
         const int p = i / nc, c = i - (i / nc) * nc;
@@ -200,8 +200,8 @@ KYM_HD void body_bwd_gcols(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int ti
 
 // Big nodes: rows of the U slot, inverse DIF along x, then modulus adjoint and
 // forward DIT along x into the gsave slot (root: dL/dpad(x) into gpad).
-template <class T, int MAXP>
-KYM_HD void body_bwd_grows(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
+template <class T, class P>
+KYM_HD void body_bwd_grows(const P& fp, const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
     const int per = a.ngroup * a.nblk;
     const int il = task / per, rem = task - il * per;
     const int gi = rem / a.nblk, rb = rem - gi * a.nblk;
@@ -214,7 +214,7 @@ KYM_HD void body_bwd_grows(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int ti
     const cpx<T>* src = wsi + nd.slot + (long long)r0 * W;
     for (int i = tid; i < nr * W; i += nth) lds[i] = src[i];
     wg_barrier();
-    fft_dif<true, MAXP>(lds, fW, Batch{nr, 1, W, false}, tid, nth);
+    fp.row_inv(lds, fW, nr, tid, nth);
     if (nd.order == 0) {
         T* gp = reinterpret_cast<T*>(wsi + nd.gpad);
         for (int i = tid; i < nr * W; i += nth) {
@@ -231,14 +231,14 @@ KYM_HD void body_bwd_grows(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int ti
         lds[i] = mag > T(0) ? cpx<T>{u.re * (gm / mag), u.im * (gm / mag)} : cpx<T>{T(0), T(0)};
     }
     wg_barrier();
-    fft_dit<false, MAXP>(lds, fW, Batch{nr, 1, W, false}, tid, nth);
+    fp.row_fwd(lds, fW, nr, tid, nth);
     cpx<T>* gs = wsi + nd.gsave + (long long)r0 * W;
     for (int i = tid; i < nr * W; i += nth) gs[i] = lds[i];
 }
 
 // Big nodes: forward DIT along y of the gsave slot -> natural dL/dX (scaled 1/n).
-template <class T, int MAXP>
-KYM_HD void body_bwd_gfcols(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
+template <class T, class P>
+KYM_HD void body_bwd_gfcols(const P& fp, const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
     const int per = a.ngroup * a.nblk;
     const int il = task / per, rem = task - il * per;
     const int gi = rem / a.nblk, cb = rem - gi * a.nblk;
@@ -251,7 +251,7 @@ KYM_HD void body_bwd_gfcols(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int t
         lds[i] = buf[(long long)y * W + c];
     }
     wg_barrier();
-    fft_dit<false, MAXP>(lds, a.lens[a.lenH[r]], Batch{nc, nc, 1, true}, tid, nth);
+    fp.col_fwd(lds, a.lens[a.lenH[r]], nc, tid, nth);
     const T inv_n = T(1) / T(H * W);
     for (int i = tid; i < H * nc; i += nth) {
         const int y = i / nc, c = i - y * nc;
@@ -310,10 +310,33 @@ KYM_HD KYM_INL void dispatch_bwd2d(int kind, const Scat2DArgs<T>& a, int task, c
     switch (kind) {
         case K2B_NODE: body_bwd_node<T, MAXP>(a, task, lds, tid, nth); break;
         case K2B_LEAFG: body_bwd_leafg<T, MAXP>(a, task, lds, tid, nth); break;
-        case K2B_GCOLS: body_bwd_gcols<T, MAXP>(a, task, lds, tid, nth); break;
-        case K2B_GROWS: body_bwd_grows<T, MAXP>(a, task, lds, tid, nth); break;
-        case K2B_GFCOLS: body_bwd_gfcols<T, MAXP>(a, task, lds, tid, nth); break;
+        case K2B_GCOLS: body_bwd_gcols<T>(DynPass<T, MAXP>(), a, task, lds, tid, nth); break;
+        case K2B_GROWS: body_bwd_grows<T>(DynPass<T, MAXP>(), a, task, lds, tid, nth); break;
+        case K2B_GFCOLS: body_bwd_gfcols<T>(DynPass<T, MAXP>(), a, task, lds, tid, nth); break;
         case K2B_FOLD: body_bwd_fold<T, MAXP>(a, task, lds, tid, nth); break;
+        default: break;
+    }
+}
+
+// Backward big-node passes with compile-time sizes, blocks and threads (the
+// forward's pass specialisations); EMU = true runs one thread (CPU harness).
+template <class T, int KIND, bool EMU>
+KYM_HD void dispatch_bwd_pass_spec(int id, const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid) {
+    switch (id) {
+#define KYM_BPS_CASE(sid, h, w, c, r, nt)                                                          \
+    case sid: {                                                                                    \
+        const FftLen<T>* fh = len_of(a, h);                                                        \
+        const FftLen<T>* fw = len_of(a, w);                                                        \
+        load_twiddles(lds, fh, fw, h, w, tid, EMU ? 1 : nt);                                       \
+        const StatPass<T, h, w, c, r, (EMU ? 1 : nt)> fp{lds + (h != w ? w : 0), lds};             \
+        cpx<T>* b = lds + kym_tw_elems(h, w);                                                      \
+        if constexpr (KIND == K2B_GCOLS) body_bwd_gcols<T>(fp, a, task, b, tid, EMU ? 1 : nt);     \
+        else if constexpr (KIND == K2B_GROWS) body_bwd_grows<T>(fp, a, task, b, tid, EMU ? 1 : nt); \
+        else if constexpr (KIND == K2B_GFCOLS) body_bwd_gfcols<T>(fp, a, task, b, tid, EMU ? 1 : nt); \
+        break;                                                                                     \
+    }
+        KYM_PASS_SPECS(KYM_BPS_CASE)
+#undef KYM_BPS_CASE
         default: break;
     }
 }

@@ -514,10 +514,31 @@ struct Scat2DPlan {
                     max_lds = std::max(max_lds, launches.back().lds_bytes);
                 }
             } else {
-                const int cb = col_blk(H[r]), rb = std::max(1, std::min(16, (int)(pass_budget / (cs * W[r]))));
-                push(K2B_GCOLS, ids, (W[r] + cb - 1) / cb, cb, 256, (size_t)H[r] * cb * cs);
-                push(K2B_GROWS, ids, (H[r] + rb - 1) / rb, rb, 256, (size_t)rb * W[r] * cs);
-                if (n0.order > 0) push(K2B_GFCOLS, ids, (W[r] + cb - 1) / cb, cb, 256, (size_t)H[r] * cb * cs);
+                int cb = col_blk(H[r]), rb = std::max(1, std::min(16, (int)(pass_budget / (cs * W[r]))));
+                // compile-time passes for these sizes and blocks; untuned blocks
+                // take 8 x 8 where only that shape is compiled
+                int pnt = 0, psid = -1;
+                if (specs && sizeof(T) == 4) {
+                    const int want = tune.pass_threads == 256 ? 0 : tune.pass_threads;
+                    auto spec_for = [&](int c, int rr) {
+                        return (H[r] % rr == 0 && W[r] % c == 0) ? pass_spec_id(H[r], W[r], c, rr, want, &pnt) : -1;
+                    };
+                    psid = spec_for(cb, rb);
+                    if (psid < 0 && !tune.col_block && !tune.row_block && (psid = spec_for(8, 8)) >= 0) cb = rb = 8;
+                }
+                auto bpass = [&](int kind, int nblk, int blk, size_t lds) {
+                    push(kind, ids, nblk, blk, 256, lds);
+                    const size_t tw = cs * kym_tw_elems(H[r], W[r]);
+                    if (psid >= 0 && lds + tw <= lds_limit) {
+                        launches.back().spec = psid;
+                        launches.back().threads = pnt;
+                        launches.back().lds_bytes = lds + tw;
+                        max_lds = std::max(max_lds, launches.back().lds_bytes);
+                    }
+                };
+                bpass(K2B_GCOLS, (W[r] + cb - 1) / cb, cb, (size_t)H[r] * cb * cs);
+                bpass(K2B_GROWS, (H[r] + rb - 1) / rb, rb, (size_t)rb * W[r] * cs);
+                if (n0.order > 0) bpass(K2B_GFCOLS, (W[r] + cb - 1) / cb, cb, (size_t)H[r] * cb * cs);
             }
         };
         for (int order = max_order; order >= 1; --order)
