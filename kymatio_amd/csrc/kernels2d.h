@@ -34,10 +34,9 @@ __global__ void __launch_bounds__(NT, 2) k_pass2d_spec(Scat2DArgs<T> a) {
     extern __shared__ __attribute__((aligned(16))) char kym_smem2d_ps[];
     const int task = xcd_task(blockIdx.x, gridDim.x);
     if (task >= a.ntask) return;
-    if constexpr (KIND >= K2B_NODE)
-        dispatch_bwd_pass_spec<T, KIND, false>(SPEC, a, task, reinterpret_cast<cpx<T>*>(kym_smem2d_ps), threadIdx.x);
-    else
-        dispatch_pass_spec<T, KIND, false>(SPEC, a, task, reinterpret_cast<cpx<T>*>(kym_smem2d_ps), threadIdx.x);
+    cpx<T>* lds = reinterpret_cast<cpx<T>*>(kym_smem2d_ps);
+    if constexpr (KIND >= K2B_NODE) bwd_pass_spec_body<T, KIND, SPEC, false>(a, task, lds, threadIdx.x);
+    else pass_spec_body<T, KIND, SPEC, false>(a, task, lds, threadIdx.x);
 }
 
 template <class T, int SPEC, int NT>

@@ -384,19 +384,19 @@ struct DynGeom {
     KYM_HD const int* revWJ() const { return fWJ->rev; }
     KYM_HD int rh(int p) const { return kym_ldg(fH->rev + p); }
     KYM_HD int rw(int p) const { return kym_ldg(fW->rev + p); }
-    KYM_HD void inv2d(cpx<T>* b, int tid, int nth) const {
+    KYM_HD KYM_INL void inv2d(cpx<T>* b, int tid, int nth) const {
         fft_dif<true, MAXP>(b, *fW, Batch{H, 1, W, false}, tid, nth);
         fft_dif<true, MAXP>(b, *fH, Batch{W, W, 1, true}, tid, nth);
     }
-    KYM_HD void fwd2d(cpx<T>* b, int tid, int nth) const {
+    KYM_HD KYM_INL void fwd2d(cpx<T>* b, int tid, int nth) const {
         fft_dit<false, MAXP>(b, *fW, Batch{H, 1, W, false}, tid, nth);
         fft_dit<false, MAXP>(b, *fH, Batch{W, W, 1, true}, tid, nth);
     }
-    KYM_HD void leaf_inv2d(cpx<T>* b, int tid, int nth) const {
+    KYM_HD KYM_INL void leaf_inv2d(cpx<T>* b, int tid, int nth) const {
         fft_dif<true, MAXP>(b, *fWJ, Batch{HJ, 1, WJ, false}, tid, nth);
         fft_dif<true, MAXP>(b, *fHJ, Batch{WJ, WJ, 1, true}, tid, nth);
     }
-    KYM_HD void leaf_fwd2d(cpx<T>* b, int tid, int nth) const {
+    KYM_HD KYM_INL void leaf_fwd2d(cpx<T>* b, int tid, int nth) const {
         fft_dit<false, MAXP>(b, *fWJ, Batch{HJ, 1, WJ, false}, tid, nth);
         fft_dit<false, MAXP>(b, *fHJ, Batch{WJ, WJ, 1, true}, tid, nth);
     }
@@ -425,18 +425,18 @@ struct StatGeom {
     // twiddle tables of the node's lengths, copied to LDS by the kernel
     const cpx<T>* twH;
     const cpx<T>* twW;
-    KYM_HD void inv2d(cpx<T>* b, int tid, int) const {
+    KYM_HD KYM_INL void inv2d(cpx<T>* b, int tid, int) const {
         fft_dif_s<true, W, H, 1, W, false, NT, true>(b, twW, tid);
         fft_dif_s<true, H, W, W, 1, true, NT, true>(b, twH, tid);
     }
-    KYM_HD void fwd2d(cpx<T>* b, int tid, int) const {
+    KYM_HD KYM_INL void fwd2d(cpx<T>* b, int tid, int) const {
         fft_dit_s<false, W, H, 1, W, false, NT, true>(b, twW, tid);
         fft_dit_s<false, H, W, W, 1, true, NT, true>(b, twH, tid);
     }
     // Inverse 2D FFT with the modulus in its last pass (columns, radix r_1):
     // R[y (W + 1) + x] = |X[y][x]| scale in natural order over the buffer;
     // sv (if set) receives X before the modulus at its storage index.
-    KYM_HD void inv2d_modulus(cpx<T>* b, T* R, T scale, cpx<T>* sv, int tid) const {
+    KYM_HD KYM_INL void inv2d_modulus(cpx<T>* b, T* R, T scale, cpx<T>* sv, int tid) const {
         constexpr int R0 = kym_factor_list(H).rad[0];
         fft_dif_s<true, W, H, 1, W, false, NT, true>(b, twW, tid);
         fft_dif_s<true, H, W, W, 1, true, NT, true, T, -1, H, 1>(b, twH, tid);
@@ -452,7 +452,7 @@ struct StatGeom {
     }
     // Forward 2D FFT of the real field R[y (W + 1) + x] (natural order, over
     // the buffer): the first row pass reads R directly.
-    KYM_HD void fwd2d_real(cpx<T>* b, const T* R, int tid) const {
+    KYM_HD KYM_INL void fwd2d_real(cpx<T>* b, const T* R, int tid) const {
         constexpr int R0 = kym_factor_list(W).rad[0];
         dit_first_pass_real_s<false, W, H, 1, W, false, NT, T>(b, tid, [&](int t, int bb, int m) -> T {
             return R[kym_rev_s<H>(t) * (W + 1) + kym_rev_s<W>(bb * R0) + (W / R0) * m];
@@ -460,11 +460,11 @@ struct StatGeom {
         fft_dit_s<false, W, H, 1, W, false, NT, true, T, 1, R0>(b, twW, tid);
         fft_dit_s<false, H, W, W, 1, true, NT, true>(b, twH, tid);
     }
-    KYM_HD void leaf_inv2d(cpx<T>* b, int tid, int) const {
+    KYM_HD KYM_INL void leaf_inv2d(cpx<T>* b, int tid, int) const {
         fft_dif_s<true, WJ, HJ, 1, WJ, false, NT>(b, fWJ->tw, tid);
         fft_dif_s<true, HJ, WJ, WJ, 1, true, NT>(b, fHJ->tw, tid);
     }
-    KYM_HD void leaf_fwd2d(cpx<T>* b, int tid, int) const {
+    KYM_HD KYM_INL void leaf_fwd2d(cpx<T>* b, int tid, int) const {
         fft_dit_s<false, WJ, HJ, 1, WJ, false, NT>(b, fWJ->tw, tid);
         fft_dit_s<false, HJ, WJ, WJ, 1, true, NT>(b, fHJ->tw, tid);
     }
@@ -988,16 +988,16 @@ KYM_HD void body_leaf(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, in
 template <class T, int MAXP>
 struct DynPass {
     static constexpr bool FUSED = false;
-    KYM_HD void col_inv(cpx<T>* b, const FftLen<T>& f, int nc, int tid, int nth) const {
+    KYM_HD KYM_INL void col_inv(cpx<T>* b, const FftLen<T>& f, int nc, int tid, int nth) const {
         fft_dif<true, MAXP>(b, f, Batch{nc, nc, 1, true}, tid, nth);
     }
-    KYM_HD void col_fwd(cpx<T>* b, const FftLen<T>& f, int nc, int tid, int nth) const {
+    KYM_HD KYM_INL void col_fwd(cpx<T>* b, const FftLen<T>& f, int nc, int tid, int nth) const {
         fft_dit<false, MAXP>(b, f, Batch{nc, nc, 1, true}, tid, nth);
     }
-    KYM_HD void row_inv(cpx<T>* b, const FftLen<T>& f, int nr, int tid, int nth) const {
+    KYM_HD KYM_INL void row_inv(cpx<T>* b, const FftLen<T>& f, int nr, int tid, int nth) const {
         fft_dif<true, MAXP>(b, f, Batch{nr, 1, f.n, false}, tid, nth);
     }
-    KYM_HD void row_fwd(cpx<T>* b, const FftLen<T>& f, int nr, int tid, int nth) const {
+    KYM_HD KYM_INL void row_fwd(cpx<T>* b, const FftLen<T>& f, int nr, int tid, int nth) const {
         fft_dit<false, MAXP>(b, f, Batch{nr, 1, f.n, false}, tid, nth);
     }
 };
@@ -1007,23 +1007,23 @@ template <class T, int H, int W, int CB, int RB, int NT>
 struct StatPass {
     const cpx<T>* twH;
     const cpx<T>* twW;
-    KYM_HD void col_inv(cpx<T>* b, const FftLen<T>&, int, int tid, int) const {
+    KYM_HD KYM_INL void col_inv(cpx<T>* b, const FftLen<T>&, int, int tid, int) const {
         fft_dif_s<true, H, CB, CB, 1, true, NT, true>(b, twH, tid);
     }
-    KYM_HD void col_fwd(cpx<T>* b, const FftLen<T>&, int, int tid, int) const {
+    KYM_HD KYM_INL void col_fwd(cpx<T>* b, const FftLen<T>&, int, int tid, int) const {
         fft_dit_s<false, H, CB, CB, 1, true, NT, true>(b, twH, tid);
     }
-    KYM_HD void row_inv(cpx<T>* b, const FftLen<T>&, int, int tid, int) const {
+    KYM_HD KYM_INL void row_inv(cpx<T>* b, const FftLen<T>&, int, int tid, int) const {
         fft_dif_s<true, W, RB, 1, W, false, NT, true>(b, twW, tid);
     }
-    KYM_HD void row_fwd(cpx<T>* b, const FftLen<T>&, int, int tid, int) const {
+    KYM_HD KYM_INL void row_fwd(cpx<T>* b, const FftLen<T>&, int, int tid, int) const {
         fft_dit_s<false, W, RB, 1, W, false, NT, true>(b, twW, tid);
     }
     // Row inverse whose last pass writes the modulus R[t rs + x] = |X| scale
     // (natural x; R does not overlap the rows) and, if sv is set, X at its
     // storage index; and the row forward whose first pass reads R.
     static constexpr bool FUSED = true;
-    KYM_HD void row_inv_modulus(cpx<T>* b, T* R, int rs, T scale, cpx<T>* sv, int tid) const {
+    KYM_HD KYM_INL void row_inv_modulus(cpx<T>* b, T* R, int rs, T scale, cpx<T>* sv, int tid) const {
         constexpr int R0 = kym_factor_list(W).rad[0];
         fft_dif_s<true, W, RB, 1, W, false, NT, true, T, -1, W, 1>(b, twW, tid);
         dif_last_pass_s<true, W, RB, 1, W, false, NT, T, false>(
@@ -1034,7 +1034,7 @@ struct StatPass {
             },
             [&](int t, int bb, int m, T val) { R[t * rs + kym_rev_s<W>(bb * R0) + (W / R0) * m] = val; });
     }
-    KYM_HD void row_fwd_real(cpx<T>* b, const T* R, int rs, int tid) const {
+    KYM_HD KYM_INL void row_fwd_real(cpx<T>* b, const T* R, int rs, int tid) const {
         constexpr int R0 = kym_factor_list(W).rad[0];
         dit_first_pass_real_s<false, W, RB, 1, W, false, NT, T, false>(b, tid, [&](int t, int bb, int m) -> T {
             return R[t * rs + kym_rev_s<W>(bb * R0) + (W / R0) * m];
@@ -1365,19 +1365,29 @@ KYM_HD KYM_INL const FftLen<T>* len_of(const Scat2DArgs<T>& a, int n) {
     return &a.lens[a.lenH[0]];
 }
 
-// Static pass body: EMU = true runs it with one thread (CPU harness).
+// Static pass body of specialisation SPEC, selected at compile time so a
+// kernel carries only its own body.  EMU = true runs it with one thread (CPU
+// harness).
+template <class T, int KIND, int SPEC, bool EMU>
+KYM_HD KYM_INL void pass_spec_body(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid) {
+#define KYM_PS_BODY(sid, h, w, c, r, nt)                                                       \
+    if constexpr (SPEC == sid) {                                                               \
+        const FftLen<T>* fh = len_of(a, h);                                                    \
+        const FftLen<T>* fw = len_of(a, w);                                                    \
+        load_twiddles(lds, fh, fw, h, w, tid, EMU ? 1 : nt);                                   \
+        const StatPass<T, h, w, c, r, (EMU ? 1 : nt)> fp{lds + (h != w ? w : 0), lds};         \
+        dispatch_pass<T, KIND>(fp, a, task, lds + kym_tw_elems(h, w), tid, EMU ? 1 : nt);      \
+    }
+    KYM_PASS_SPECS(KYM_PS_BODY)
+#undef KYM_PS_BODY
+}
+
+// Run-time selection (the CPU harness).
 template <class T, int KIND, bool EMU>
 KYM_HD void dispatch_pass_spec(int id, const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid) {
     switch (id) {
-#define KYM_PS_CASE(sid, h, w, c, r, nt)                                                           \
-    case sid: {                                                                                    \
-        const FftLen<T>* fh = len_of(a, h);                                                        \
-        const FftLen<T>* fw = len_of(a, w);                                                        \
-        load_twiddles(lds, fh, fw, h, w, tid, EMU ? 1 : nt);                                       \
-        const StatPass<T, h, w, c, r, (EMU ? 1 : nt)> fp{lds + (h != w ? w : 0), lds};             \
-        dispatch_pass<T, KIND>(fp, a, task, lds + kym_tw_elems(h, w), tid, EMU ? 1 : nt);          \
-        break;                                                                                     \
-    }
+#define KYM_PS_CASE(sid, h, w, c, r, nt) \
+    case sid: pass_spec_body<T, KIND, sid, EMU>(a, task, lds, tid); break;
         KYM_PASS_SPECS(KYM_PS_CASE)
 #undef KYM_PS_CASE
         default: break;

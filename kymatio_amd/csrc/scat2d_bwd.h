@@ -319,22 +319,32 @@ KYM_HD KYM_INL void dispatch_bwd2d(int kind, const Scat2DArgs<T>& a, int task, c
 }
 
 // Backward big-node passes with compile-time sizes, blocks and threads (the
-// forward's pass specialisations); EMU = true runs one thread (CPU harness).
+// forward's pass specialisations), selected at compile time; EMU = true runs
+// one thread (CPU harness).
+template <class T, int KIND, int SPEC, bool EMU>
+KYM_HD KYM_INL void bwd_pass_spec_body(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid) {
+#define KYM_BPS_BODY(sid, h, w, c, r, nt)                                                            \
+    if constexpr (SPEC == sid) {                                                                     \
+        constexpr int NTE = EMU ? 1 : nt;                                                            \
+        const FftLen<T>* fh = len_of(a, h);                                                          \
+        const FftLen<T>* fw = len_of(a, w);                                                          \
+        load_twiddles(lds, fh, fw, h, w, tid, NTE);                                                  \
+        const StatPass<T, h, w, c, r, NTE> fp{lds + (h != w ? w : 0), lds};                          \
+        cpx<T>* b = lds + kym_tw_elems(h, w);                                                        \
+        if constexpr (KIND == K2B_GCOLS) body_bwd_gcols<T>(fp, a, task, b, tid, NTE);                \
+        else if constexpr (KIND == K2B_GROWS) body_bwd_grows<T>(fp, a, task, b, tid, NTE);           \
+        else if constexpr (KIND == K2B_GFCOLS) body_bwd_gfcols<T>(fp, a, task, b, tid, NTE);         \
+    }
+    KYM_PASS_SPECS(KYM_BPS_BODY)
+#undef KYM_BPS_BODY
+}
+
+// Run-time selection (the CPU harness).
 template <class T, int KIND, bool EMU>
 KYM_HD void dispatch_bwd_pass_spec(int id, const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid) {
     switch (id) {
-#define KYM_BPS_CASE(sid, h, w, c, r, nt)                                                          \
-    case sid: {                                                                                    \
-        const FftLen<T>* fh = len_of(a, h);                                                        \
-        const FftLen<T>* fw = len_of(a, w);                                                        \
-        load_twiddles(lds, fh, fw, h, w, tid, EMU ? 1 : nt);                                       \
-        const StatPass<T, h, w, c, r, (EMU ? 1 : nt)> fp{lds + (h != w ? w : 0), lds};             \
-        cpx<T>* b = lds + kym_tw_elems(h, w);                                                      \
-        if constexpr (KIND == K2B_GCOLS) body_bwd_gcols<T>(fp, a, task, b, tid, EMU ? 1 : nt);     \
-        else if constexpr (KIND == K2B_GROWS) body_bwd_grows<T>(fp, a, task, b, tid, EMU ? 1 : nt); \
-        else if constexpr (KIND == K2B_GFCOLS) body_bwd_gfcols<T>(fp, a, task, b, tid, EMU ? 1 : nt); \
-        break;                                                                                     \
-    }
+#define KYM_BPS_CASE(sid, h, w, c, r, nt) \
+    case sid: bwd_pass_spec_body<T, KIND, sid, EMU>(a, task, lds, tid); break;
         KYM_PASS_SPECS(KYM_BPS_CASE)
 #undef KYM_BPS_CASE
         default: break;
