@@ -561,7 +561,7 @@ static int upload1d(Scat1DPlan<T>& p, void** dblob) {
     for (const std::vector<Launch1D>* ls : {&p.launches, &p.bwd_launches})
         for (const Launch1D& l : *ls)
             if (l.lds_bytes > 65536)
-                KYM_HIP_TRY(hipFuncSetAttribute((const void*)pick_scat1d<T>(l.threads, l.kind),
+                KYM_HIP_TRY(hipFuncSetAttribute((const void*)pick_scat1d<T>(l.threads, l.kind, l.spec),
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.lds_bytes));
     return KYM_OK;
 }
@@ -694,7 +694,7 @@ static int run1d(const Scat1DPlan<T>& p, int mode, const void* x, const void* fi
                 e1 = prof->get();
                 if (e0) KYM_HIP_TRY(hipEventRecord(e0, st));
             }
-            hipLaunchKernelGGL(pick_scat1d<T>(l.threads, l.kind), dim3(grid), dim3(l.threads), l.lds_bytes, st, a);
+            hipLaunchKernelGGL(pick_scat1d<T>(l.threads, l.kind, l.spec), dim3(grid), dim3(l.threads), l.lds_bytes, st, a);
             KYM_HIP_TRY(hipGetLastError());
             if (e0 && e1) {
                 KYM_HIP_TRY(hipEventRecord(e1, st));

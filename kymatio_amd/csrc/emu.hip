@@ -213,7 +213,12 @@ extern "C" int kym_emu_scat1d_forward(int N, int N_pad, int pad_left, int log2_s
         }
         p.blob.relocate((uintptr_t)p.blob.bytes.data());
         const uintptr_t base = (uintptr_t)p.blob.bytes.data();
-        if (info) { info[0] = p.C; info[1] = p.out_row; info[2] = (long long)p.filt_off.size() - 1; info[3] = p.filt_off.back(); }
+        if (info) {
+            info[0] = p.C; info[1] = p.out_row; info[2] = (long long)p.filt_off.size() - 1; info[3] = p.filt_off.back();
+            int nspec = 0;
+            for (const Launch1D& l : p.launches) nspec += l.spec >= 0;
+            info[4] = nspec;
+        }
         if (!x) return 0;
         const bool train = dS != nullptr;
         const long long stride = train ? p.ws_sig_train : p.ws_sig;
@@ -233,7 +238,8 @@ extern "C" int kym_emu_scat1d_forward(int N, int N_pad, int pad_left, int log2_s
                 a.img0 = 0;
                 a.ntask = (int)(batch * l.ngroup);
                 for (int t = 0; t < a.ntask; ++t) {
-                    if (l.kind == 0) body_node1d<T, 31>(a, t, lds.data(), 0, 1);
+                    if (l.kind == 0 && l.spec >= 0) dispatch_node1d_spec<T, true>(l.spec, a, t, lds.data(), 0);
+                    else if (l.kind == 0) body_node1d<T, 31>(a, t, lds.data(), 0, 1);
                     else if (l.kind == 1) body_bwd_node1d<T, 31>(a, t, lds.data(), 0, 1);
                     else body_bwd_fold1d<T>(a, t, 0, 1);
                 }

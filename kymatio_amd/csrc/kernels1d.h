@@ -14,8 +14,28 @@ __global__ void __launch_bounds__(NT) k_scat1d(Scat1DArgs<T> a) {
     else body_bwd_fold1d<T>(a, task, threadIdx.x, NT);
 }
 
+template <class T, int SPEC, int NT>
+__global__ void __launch_bounds__(NT) k_node1d_spec(Scat1DArgs<T> a) {
+    extern __shared__ __attribute__((aligned(16))) char kym_smem1d_ns[];
+    const int task = xcd_task(blockIdx.x, gridDim.x);
+    if (task >= a.ntask) return;
+    node1d_spec_body<T, SPEC, false>(a, task, reinterpret_cast<cpx<T>*>(kym_smem1d_ns), threadIdx.x);
+}
+
 template <class T>
 using Scat1DKernel = void (*)(Scat1DArgs<T>);
+
+template <class T>
+Scat1DKernel<T> pick_node1d_spec(int id) {
+    if constexpr (sizeof(T) != 4) return nullptr;  // specialisations are FP32 only
+    else switch (id) {
+#define KYM_N1_K(sid, nn, nt) \
+    case sid: return k_node1d_spec<T, sid, nt>;
+        KYM_NODE1D_SPECS(KYM_N1_K)
+#undef KYM_N1_K
+        default: return nullptr;
+    }
+}
 
 template <class T, int KIND>
 Scat1DKernel<T> pick_scat1d_k(int threads) {

@@ -17,6 +17,7 @@ Scat2DKernel<float> kym_pick_node_spec_f32(int id);
 Scat2DKernel<float> kym_pick_bwd_node_spec_f32(int id);
 Scat2DKernel<float> kym_pick_pass_spec_f32(int kind, int id);
 Scat1DKernel<float> kym_pick1d_f32(int threads, int kind);
+Scat1DKernel<float> kym_pick_node1d_spec_f32(int id);
 Scat1DKernel<double> kym_pick1d_f64(int threads, int kind);
 
 // pclass: 7, 17 or 31 (kym_prime_class of the plan's largest odd prime);
@@ -37,8 +38,13 @@ inline Scat2DKernel<T> pick_scat2d(int kind, int threads, int pclass, int spec =
     }
 }
 
+// spec >= 0 (FP32, forward nodes) selects a compile-time specialisation.
 template <class T>
-inline Scat1DKernel<T> pick_scat1d(int threads, int kind = 0) {
-    if constexpr (sizeof(T) == 4) return kym_pick1d_f32(threads, kind);
-    else return kym_pick1d_f64(threads, kind);
+inline Scat1DKernel<T> pick_scat1d(int threads, int kind = 0, int spec = -1) {
+    if constexpr (sizeof(T) == 4) {
+        if (spec >= 0 && kind == 0) return kym_pick_node1d_spec_f32(spec);
+        return kym_pick1d_f32(threads, kind);
+    } else {
+        return kym_pick1d_f64(threads, kind);
+    }
 }

@@ -98,8 +98,30 @@ KYM_HD void leaf1d(const Scat1DArgs<T>& a, const Node1D& nd, const cpx<T>* U, in
     wg_barrier();
 }
 
+// FFT of a node's length: run-time (Dyn1D) or compile-time (Stat1D, length
+// N and NT threads; the same factorisation and digit-reversed order).
 template <class T, int MAXP>
-KYM_HD void body_node1d(const Scat1DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
+struct Dyn1D {
+    KYM_HD KYM_INL void inv(cpx<T>* b, const FftLen<T>& fl, int tid, int nth) const {
+        fft_dif<true, MAXP>(b, fl, Batch{1, 1, fl.n, false}, tid, nth);
+    }
+    KYM_HD KYM_INL void fwd(cpx<T>* b, const FftLen<T>& fl, int tid, int nth) const {
+        fft_dit<false, MAXP>(b, fl, Batch{1, 1, fl.n, false}, tid, nth);
+    }
+};
+
+template <class T, int N, int NT>
+struct Stat1D {
+    KYM_HD KYM_INL void inv(cpx<T>* b, const FftLen<T>& fl, int tid, int) const {
+        fft_dif_s<true, N, 1, 1, N, false, NT>(b, fl.tw, tid);
+    }
+    KYM_HD KYM_INL void fwd(cpx<T>* b, const FftLen<T>& fl, int tid, int) const {
+        fft_dit_s<false, N, 1, 1, N, false, NT>(b, fl.tw, tid);
+    }
+};
+
+template <class T, int MAXP, class F>
+KYM_HD KYM_INL void node1d_body(const F& ff, const Scat1DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
     const int il = task / a.ngroup, gi = task - il * a.ngroup;
     const int img = a.img0 + il;
     const Node1D& nd = a.nodes[a.group[gi]];
@@ -116,7 +138,7 @@ KYM_HD void body_node1d(const Scat1DArgs<T>& a, int task, cpx<T>* lds, int tid, 
             for (int t = s + tid; t < e; t += nth) o[t - s] = xin[reflect_index(t - a.pad_left, a.N)];
         }
         wg_barrier();
-        fft_dit<false, MAXP>(lds, fl, Batch{1, 1, n, false}, tid, nth);
+        ff.fwd(lds, fl, tid, nth);
     } else {
         const Node1D& pn = a.nodes[nd.parent];
         const cpx<T>* U = wsi + pn.slot;
@@ -134,7 +156,7 @@ KYM_HD void body_node1d(const Scat1DArgs<T>& a, int task, cpx<T>* lds, int tid, 
             lds[i] = acc * inv_k;
         }
         wg_barrier();
-        fft_dif<true, MAXP>(lds, fl, Batch{1, 1, n, false}, tid, nth);
+        ff.inv(lds, fl, tid, nth);
         const T inv_n = T(1) / T(n);
         const int s = a.ind_start[nd.unpad_res], e = a.ind_end[nd.unpad_res];
         cpx<T>* sv = (a.save && nd.save >= 0) ? wsi + nd.save : nullptr;
@@ -149,13 +171,59 @@ KYM_HD void body_node1d(const Scat1DArgs<T>& a, int task, cpx<T>* lds, int tid, 
             }
         }
         wg_barrier();
-        if (nd.need_fwd) fft_dit<false, MAXP>(lds, fl, Batch{1, 1, n, false}, tid, nth);
+        if (nd.need_fwd) ff.fwd(lds, fl, tid, nth);
     }
     if (nd.slot >= 0) {
         cpx<T>* dst = wsi + nd.slot;
         for (int i = tid; i < n; i += nth) dst[i] = lds[i];
     }
     if (a.mode != AVG1_NONE) leaf1d<T, MAXP>(a, nd, lds, n, img, lds + n, tid, nth);
+}
+
+template <class T, int MAXP>
+KYM_HD void body_node1d(const Scat1DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
+    node1d_body<T, MAXP>(Dyn1D<T, MAXP>(), a, task, lds, tid, nth);
+}
+
+// Compile-time specialisations X(id, N, threads) of the forward node (FP32):
+// a launch uses one when its nodes' length matches (threads follow the
+// plan's rule max(64, min(1024, N / 16))).
+#define KYM_NODE1D_SPECS(X) \
+    X(0, 16384, 1024)       \
+    X(1, 8192, 512)         \
+    X(2, 4096, 256)         \
+    X(3, 2048, 128)         \
+    X(4, 1024, 64)          \
+    X(5, 512, 64)           \
+    X(6, 256, 64)           \
+    X(7, 128, 64)
+
+inline int node1d_spec_id(int n, int threads) {
+#define KYM_N1_MATCH(id, nn, nt) \
+    if (n == nn && threads == nt) return id;
+    KYM_NODE1D_SPECS(KYM_N1_MATCH)
+#undef KYM_N1_MATCH
+    return -1;
+}
+
+// Body of specialisation SPEC; EMU = true: one thread (CPU harness).
+template <class T, int SPEC, bool EMU>
+KYM_HD KYM_INL void node1d_spec_body(const Scat1DArgs<T>& a, int task, cpx<T>* lds, int tid) {
+#define KYM_N1_BODY(sid, nn, nt) \
+    if constexpr (SPEC == sid) node1d_body<T, 7>(Stat1D<T, nn, (EMU ? 1 : nt)>(), a, task, lds, tid, EMU ? 1 : nt);
+    KYM_NODE1D_SPECS(KYM_N1_BODY)
+#undef KYM_N1_BODY
+}
+
+template <class T, bool EMU>
+KYM_HD void dispatch_node1d_spec(int id, const Scat1DArgs<T>& a, int task, cpx<T>* lds, int tid) {
+    switch (id) {
+#define KYM_N1_CASE(sid, nn, nt) \
+    case sid: node1d_spec_body<T, sid, EMU>(a, task, lds, tid); break;
+        KYM_NODE1D_SPECS(KYM_N1_CASE)
+#undef KYM_N1_CASE
+        default: break;
+    }
 }
 
 // ------------------------------------------------------------------ backward

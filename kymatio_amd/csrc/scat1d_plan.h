@@ -13,6 +13,7 @@ struct Launch1D {
     int ngroup, threads, res;
     size_t lds_bytes;
     long long model_bytes;  // algorithmic bytes per signal (stage-boundary model)
+    int spec = -1;          // compile-time node specialisation (FP32 forward), or -1
 };
 
 struct Scat1DDesc {
@@ -222,6 +223,7 @@ struct Scat1DPlan {
                 mb += (nd.order == 0 ? 4ll * N + 16ll * N_pad : 40ll * n[nd.res]) + leaf_model;
             }
             l.model_bytes = mb;
+            if (kind == 0 && sizeof(T) == 4) l.spec = node1d_spec_id(len0, l.threads);
             launches.push_back(l);
             max_lds = std::max(max_lds, l.lds_bytes);
         };

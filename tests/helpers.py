@@ -144,11 +144,12 @@ def emu_scat1d(emu, module, x, dtype=np.float32, dS=None, nbwd=1):
             len(p2), i32([p['j'] for p in p2]), i32([len(p['levels']) for p in p2]),
             len(m.phi_f['levels']), i32([m.ind_start[i] for i in range(nind)]),
             i32([m.ind_end[i] for i in range(nind)]), nind, 0 if dtype == np.float32 else 1]
-    info = (ctypes.c_longlong * 4)()
+    info = (ctypes.c_longlong * 5)()
     B = x.shape[0]
     rc = f(*args, None, None, None, B, info, None, None, 0)
     if rc != 0:
         raise RuntimeError(emu.lib.kym_emu_last_error().decode())
+    emu.last_spec_launches1d = info[4]  # forward launches on compile-time node kernels
     out = np.zeros((B, info[1]), dtype)
     xx = np.ascontiguousarray(x, dtype)
     if dS is None:
