@@ -145,15 +145,33 @@ KYM_HD KYM_INL void node1d_body(const F& ff, const Scat1DArgs<T>& a, int task, c
         const T* f = a.filt + a.filt_off[nd.filt];
         const int k = nd.k;
         const T inv_k = T(1) / T(k);
-        for (int i = tid; i < n; i += nth) {
-            cpx<T> acc = {T(0), T(0)};
-            for (int q = 0; q < k; ++q) {
-                const cpx<T> u = U[i + q * n];
-                const T fv = f[i + q * n];
-                acc.re += u.re * fv;
-                acc.im += u.im * fv;
+        // mirror pairs: the parent is the spectrum of a real signal, so point
+        // n - i reads the conjugates of point i's aliases in reverse order
+        // (alias q of n - i is conj(U[i + (k-1-q) n])); one load serves both.
+        // i = 0 and i = n/2 are their own mirrors.
+        const int half = n / 2;
+        for (int i = tid; i <= half; i += nth) {
+            const int im = i == 0 ? 0 : n - i;
+            cpx<T> d = {T(0), T(0)}, m = {T(0), T(0)};
+            for (int q = 0; q < (k + 1) / 2; ++q) {
+                const int q2 = k - 1 - q;
+                const cpx<T> u = kym_ldg(U + i + q * n);
+                const cpx<T> u2 = q2 != q ? kym_ldg(U + i + q2 * n) : u;
+                d.re += u.re * kym_ldg(f + i + q * n);
+                d.im += u.im * kym_ldg(f + i + q * n);
+                const T fm = kym_ldg(f + im + q * n);  // alias q of the mirror: conj(u2)
+                m.re += u2.re * fm;
+                m.im -= u2.im * fm;
+                if (q2 != q) {
+                    const T fd2 = kym_ldg(f + i + q2 * n), fm2 = kym_ldg(f + im + q2 * n);
+                    d.re += u2.re * fd2;
+                    d.im += u2.im * fd2;
+                    m.re += u.re * fm2;
+                    m.im -= u.im * fm2;
+                }
             }
-            lds[i] = acc * inv_k;
+            lds[i] = d * inv_k;
+            if (im != i) lds[im] = m * inv_k;
         }
         wg_barrier();
         ff.inv(lds, fl, tid, nth);
