@@ -153,19 +153,24 @@ KYM_HD KYM_INL void node1d_body(const F& ff, const Scat1DArgs<T>& a, int task, c
         for (int i = tid; i <= half; i += nth) {
             const int im = i == 0 ? 0 : n - i;
             cpx<T> d = {T(0), T(0)}, m = {T(0), T(0)};
-            for (int q = 0; q < (k + 1) / 2; ++q) {
-                const int q2 = k - 1 - q;
-                const cpx<T> u = kym_ldg(U + i + q * n);
-                const cpx<T> u2 = q2 != q ? kym_ldg(U + i + q2 * n) : u;
-                d.re += u.re * kym_ldg(f + i + q * n);
-                d.im += u.im * kym_ldg(f + i + q * n);
-                const T fm = kym_ldg(f + im + q * n);  // alias q of the mirror: conj(u2)
-                m.re += u2.re * fm;
-                m.im -= u2.im * fm;
-                if (q2 != q) {
-                    const T fd2 = kym_ldg(f + i + q2 * n), fm2 = kym_ldg(f + im + q2 * n);
+            if (k == 1) {
+                const cpx<T> u = kym_ldg(U + i);
+                const T fd = kym_ldg(f + i), fm = kym_ldg(f + im);
+                d = {u.re * fd, u.im * fd};
+                m = {u.re * fm, -u.im * fm};
+            } else {  // k is a power of two: alias pairs (q, k-1-q), several in flight
+#pragma unroll 4
+                for (int q = 0; q < k / 2; ++q) {
+                    const int q2 = k - 1 - q;
+                    const cpx<T> u = kym_ldg(U + i + q * n), u2 = kym_ldg(U + i + q2 * n);
+                    const T fd = kym_ldg(f + i + q * n), fd2 = kym_ldg(f + i + q2 * n);
+                    const T fm = kym_ldg(f + im + q * n), fm2 = kym_ldg(f + im + q2 * n);
+                    d.re += u.re * fd;
+                    d.im += u.im * fd;
                     d.re += u2.re * fd2;
                     d.im += u2.im * fd2;
+                    m.re += u2.re * fm;  // alias q of the mirror is conj(u2)
+                    m.im -= u2.im * fm;
                     m.re += u.re * fm2;
                     m.im -= u.im * fm2;
                 }
