@@ -62,6 +62,22 @@ KYM_HD KYM_INL cpx<T> kym_ldg(const cpx<T>* p) {
 #endif
 }
 
+// for i < n: put(i, get(i)), with E values per thread loaded before any of
+// them is stored, so E memory requests are in flight at once (a plain copy
+// loop into LDS waits for each load before its store).
+template <int E, class GET, class PUT>
+KYM_HD KYM_INL void copy_batched(int n, GET get, PUT put, int tid, int nth) {
+    int i = tid;
+    for (; i + (E - 1) * nth < n; i += E * nth) {
+        decltype(get(0)) v[E];
+#pragma unroll
+        for (int u = 0; u < E; ++u) v[u] = get(i + u * nth);
+#pragma unroll
+        for (int u = 0; u < E; ++u) put(i + u * nth, v[u]);
+    }
+    for (; i < n; i += nth) put(i, get(i));
+}
+
 // Workgroup barrier; a no-op when a body runs on the host with one thread.
 KYM_HD KYM_INL void wg_barrier() {
 #if defined(__HIP_DEVICE_COMPILE__)

@@ -76,9 +76,10 @@ KYM_HD void leaf1d(const Scat1DArgs<T>& a, const Node1D& nd, const cpx<T>* U, in
     const T inv_k = T(1) / T(kk);
     for (int i = tid; i < nl; i += nth) {
         cpx<T> acc = {T(0), T(0)};
-        for (int q = 0; q < kk; ++q) {
+#pragma unroll 8
+        for (int q = 0; q < kk; ++q) {  // unrolled: several filter loads in flight
             const cpx<T> u = U[i + q * nl];
-            const T f = phi[i + q * nl];
+            const T f = kym_ldg(phi + i + q * nl);
             acc.re += u.re * f;
             acc.im += u.im * f;
         }
@@ -131,7 +132,9 @@ KYM_HD KYM_INL void node1d_body(const F& ff, const Scat1DArgs<T>& a, int task, c
     T* o = a.out + (long long)img * a.out_row + nd.out_off;
     if (nd.order == 0) {
         const T* xin = a.x + (long long)img * a.N;
-        for (int p = tid; p < n; p += nth) lds[p] = {xin[reflect_index(fl.rev[p] - a.pad_left, a.N)], T(0)};
+        copy_batched<8>(
+            n, [&](int p) { return kym_ldg(xin + reflect_index(fl.rev[p] - a.pad_left, a.N)); },
+            [&](int p, T v) { lds[p] = {v, T(0)}; }, tid, nth);
         if (a.mode == AVG1_NONE) {
             // S_0 = U_0 unpadded = the input itself
             const int s = a.ind_start[0], e = a.ind_end[0];
@@ -290,11 +293,12 @@ KYM_HD void body_bwd_node1d(const Scat1DArgs<T>& a, int task, cpx<T>* lds, int t
             const cpx<T> gy = B[i % nl];
             acc = {gy.re * phi[i] * sc_leaf, gy.im * phi[i] * sc_leaf};
         }
-        for (int c = nd.child_begin; c < nd.child_end; ++c) {
+#pragma unroll 4
+        for (int c = nd.child_begin; c < nd.child_end; ++c) {  // unrolled: several loads in flight
             const Node1D& cn = a.nodes[c];
             const int nc = a.n[cn.res];
-            const T fv = a.filt[a.filt_off[cn.filt] + i] / T(cn.k);
-            const cpx<T> gx = wsi[cn.gsave + (i % nc)];
+            const T fv = kym_ldg(a.filt + a.filt_off[cn.filt] + i) / T(cn.k);
+            const cpx<T> gx = kym_ldg(wsi + cn.gsave + (i % nc));
             acc.re += gx.re * fv;
             acc.im += gx.im * fv;
         }

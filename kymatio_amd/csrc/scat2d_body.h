@@ -208,7 +208,7 @@ KYM_HD KYM_INL bool herm_point(int p, int H, int W, int* y, int* x) {
 // For NF filters fs[i] and 0 < y, 0 < x: d[i] += sum u fs[i] at (y, x) and
 // m[i] += sum conj(u') fs[i] at the mirror (H - y, W - x).  The mirror of
 // alias (a, b) reads the point's alias (K-1-a, K-1-b), so alias rows
-// {a, K-1-a} and column chunks {c, C-1-c} (CH = min(K, 4) columns each) are
+// {a, K-1-a} and column chunks {c, NC-1-c} (CH = min(K, 4) columns each) are
 // taken together: at most 16 parent values are live at a time.
 template <int K, int NF, class T>
 KYM_HD KYM_INL void herm_block(cpx<T>* d, cpx<T>* m, const cpx<T>* U, const T* const* fs, int H, int W, int y, int x,
@@ -797,10 +797,17 @@ KYM_HD void node_body(const Scat2DArgs<T>& a, const G& g, const Node2D& nd, int 
         if constexpr (kym_fir<T>) {
             T* R = reinterpret_cast<T*>(lds);
             const int rs = g.W + 1;
-            for (int i = tid; i < g.H * g.W; i += nth) {
-                const int y = i / g.W, x = i - y * g.W;
-                R[y * rs + x] = input_pixel(a, xin, y, x);
-            }
+            copy_batched<4>(
+                g.H * g.W,
+                [&](int i) {
+                    const int y = i / g.W, x = i - y * g.W;
+                    return input_pixel(a, xin, y, x);
+                },
+                [&](int i, T v) {
+                    const int y = i / g.W, x = i - y * g.W;
+                    R[y * rs + x] = v;
+                },
+                tid, nth);
             wg_barrier();
             fir_leaf_lds(a, g, nd, R, img, tid, nth);
             if constexpr (G::FUSED_FWD) {
@@ -1075,10 +1082,17 @@ KYM_HD void body_root_rows(const P& fp, const Scat2DArgs<T>& a, int task, cpx<T>
     if constexpr (kym_fir<T>) {
         T* R = reinterpret_cast<T*>(lds + nr * W);
         const int rs = W + 1;
-        for (int i = tid; i < nr * W; i += nth) {
-            const int rr = i / W, x = i - rr * W;
-            R[rr * rs + x] = input_pixel(a, xin, fH.rev[r0 + rr], x);
-        }
+        copy_batched<8>(
+            nr * W,
+            [&](int i) {
+                const int rr = i / W, x = i - rr * W;
+                return input_pixel(a, xin, fH.rev[r0 + rr], x);
+            },
+            [&](int i, T v) {
+                const int rr = i / W, x = i - rr * W;
+                R[rr * rs + x] = v;
+            },
+            tid, nth);
         wg_barrier();
         for (int i = tid; i < nr * W; i += nth) {
             const int rr = i / W, px = i - rr * W;
@@ -1086,10 +1100,13 @@ KYM_HD void body_root_rows(const P& fp, const Scat2DArgs<T>& a, int task, cpx<T>
         }
         fir_rows_to_slot(a, nd, R, rs, nr, r0, fH.rev, a.ws + il * a.ws_img, R + nr * rs, tid, nth);
     } else {
-        for (int i = tid; i < nr * W; i += nth) {
-            const int rr = i / W, px = i - rr * W;
-            lds[i] = {input_pixel(a, xin, fH.rev[r0 + rr], fW.rev[px]), T(0)};
-        }
+        copy_batched<8>(
+            nr * W,
+            [&](int i) {
+                const int rr = i / W, px = i - rr * W;
+                return input_pixel(a, xin, fH.rev[r0 + rr], fW.rev[px]);
+            },
+            [&](int i, T v) { lds[i] = {v, T(0)}; }, tid, nth);
     }
     wg_barrier();
     fp.row_fwd(lds, fW, nr, tid, nth);
@@ -1153,7 +1170,7 @@ KYM_HD void body_rows_mod(const P& fp, const Scat2DArgs<T>& a, int task, cpx<T>*
     const int r0 = rb * a.blk, nr = (H - r0 < a.blk) ? H - r0 : a.blk;
     cpx<T>* wsi = a.ws + il * a.ws_img;
     cpx<T>* buf = wsi + nd.slot + (long long)r0 * W;
-    for (int i = tid; i < nr * W; i += nth) lds[i] = buf[i];
+    copy_batched<8>(nr * W, [&](int i) { return kym_ldg(buf + i); }, [&](int i, cpx<T> v) { lds[i] = v; }, tid, nth);
     wg_barrier();
     const T inv_n = T(1) / T(H * W);
     cpx<T>* sv = (a.save && nd.save >= 0) ? wsi + nd.save + (long long)r0 * W : nullptr;
@@ -1198,10 +1215,13 @@ KYM_HD void body_fir_cols(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid
         T* Tb = reinterpret_cast<T*>(lds);
         T* Ay = Tb + H * ts;
         const T* Tg = reinterpret_cast<const T*>(a.ws + il * a.ws_img + nd.tslot);
-        for (int i = tid; i < H * ow; i += nth) {
-            const int p = i / ow, x = i - p * ow;
-            Tb[p * ts + x] = Tg[i];
-        }
+        copy_batched<8>(
+            H * ow, [&](int i) { return kym_ldg(Tg + i); },
+            [&](int i, T v) {
+                const int p = i / ow, x = i - p * ow;
+                Tb[p * ts + x] = v;
+            },
+            tid, nth);
         fir_band(Ay, a.filt, fy, tid, nth);
         wg_barrier();
         fir_cols_mma(Tb, ts, H, kl, oh, ow, fy, Ay, a.out + ((long long)(a.img0 + il) * a.C + nd.leaf_ch) * oh * ow,
@@ -1246,10 +1266,13 @@ KYM_HD void body_fwd_cols(const P& fp, const Scat2DArgs<T>& a, int task, cpx<T>*
     const FftLen<T>& fH = a.lens[a.lenH[r]];
     const int c0 = cb * a.blk, nc = (W - c0 < a.blk) ? W - c0 : a.blk;
     cpx<T>* buf = a.ws + il * a.ws_img + nd.slot + c0;
-    for (int i = tid; i < H * nc; i += nth) {
-        const int y = i / nc, c = i - y * nc;
-        lds[i] = buf[(long long)y * W + c];
-    }
+    copy_batched<8>(
+        H * nc,
+        [&](int i) {
+            const int y = i / nc, c = i - y * nc;
+            return kym_ldg(buf + (long long)y * W + c);
+        },
+        [&](int i, cpx<T> v) { lds[i] = v; }, tid, nth);
     wg_barrier();
     fp.col_fwd(lds, fH, nc, tid, nth);
     for (int i = tid; i < H * nc; i += nth) {

@@ -107,12 +107,14 @@ KYM_HD void bwd_node_body(const Scat2DArgs<T>& a, const G& g, const Node2D& nd, 
         return;
     }
     const cpx<T>* sv = wsi + nd.save;
-    for (int p = tid; p < n; p += nth) {
-        const cpx<T> u = sv[p];
-        const T mag = kym_sqrt(u.re * u.re + u.im * u.im);
-        const T gm = lds[p].re;
-        lds[p] = mag > T(0) ? cpx<T>{u.re * (gm / mag), u.im * (gm / mag)} : cpx<T>{T(0), T(0)};
-    }
+    copy_batched<8>(
+        n, [&](int p) { return kym_ldg(sv + p); },
+        [&](int p, cpx<T> u) {
+            const T mag = kym_sqrt(u.re * u.re + u.im * u.im);
+            const T gm = lds[p].re;
+            lds[p] = mag > T(0) ? cpx<T>{u.re * (gm / mag), u.im * (gm / mag)} : cpx<T>{T(0), T(0)};
+        },
+        tid, nth);
     wg_barrier();
     g.fwd2d(lds, tid, nth);
     const T inv_n = T(1) / T(n);
@@ -212,7 +214,7 @@ KYM_HD void body_bwd_grows(const P& fp, const Scat2DArgs<T>& a, int task, cpx<T>
     const int r0 = rb * a.blk, nr = (H - r0 < a.blk) ? H - r0 : a.blk;
     cpx<T>* wsi = a.ws + il * a.ws_img;
     const cpx<T>* src = wsi + nd.slot + (long long)r0 * W;
-    for (int i = tid; i < nr * W; i += nth) lds[i] = src[i];
+    copy_batched<8>(nr * W, [&](int i) { return kym_ldg(src + i); }, [&](int i, cpx<T> v) { lds[i] = v; }, tid, nth);
     wg_barrier();
     fp.row_inv(lds, fW, nr, tid, nth);
     if (nd.order == 0) {
@@ -224,12 +226,14 @@ KYM_HD void body_bwd_grows(const P& fp, const Scat2DArgs<T>& a, int task, cpx<T>
         return;
     }
     const cpx<T>* sv = wsi + nd.save + (long long)r0 * W;
-    for (int i = tid; i < nr * W; i += nth) {
-        const cpx<T> u = sv[i];
-        const T mag = kym_sqrt(u.re * u.re + u.im * u.im);
-        const T gm = lds[i].re;
-        lds[i] = mag > T(0) ? cpx<T>{u.re * (gm / mag), u.im * (gm / mag)} : cpx<T>{T(0), T(0)};
-    }
+    copy_batched<8>(
+        nr * W, [&](int i) { return kym_ldg(sv + i); },
+        [&](int i, cpx<T> u) {
+            const T mag = kym_sqrt(u.re * u.re + u.im * u.im);
+            const T gm = lds[i].re;
+            lds[i] = mag > T(0) ? cpx<T>{u.re * (gm / mag), u.im * (gm / mag)} : cpx<T>{T(0), T(0)};
+        },
+        tid, nth);
     wg_barrier();
     fp.row_fwd(lds, fW, nr, tid, nth);
     cpx<T>* gs = wsi + nd.gsave + (long long)r0 * W;
@@ -246,10 +250,13 @@ KYM_HD void body_bwd_gfcols(const P& fp, const Scat2DArgs<T>& a, int task, cpx<T
     const int r = nd.res, H = a.H[r], W = a.W[r];
     const int c0 = cb * a.blk, nc = (W - c0 < a.blk) ? W - c0 : a.blk;
     cpx<T>* buf = a.ws + il * a.ws_img + nd.gsave + c0;
-    for (int i = tid; i < H * nc; i += nth) {
-        const int y = i / nc, c = i - y * nc;
-        lds[i] = buf[(long long)y * W + c];
-    }
+    copy_batched<8>(
+        H * nc,
+        [&](int i) {
+            const int y = i / nc, c = i - y * nc;
+            return kym_ldg(buf + (long long)y * W + c);
+        },
+        [&](int i, cpx<T> v) { lds[i] = v; }, tid, nth);
     wg_barrier();
     fp.col_fwd(lds, a.lens[a.lenH[r]], nc, tid, nth);
     const T inv_n = T(1) / T(H * W);
