@@ -143,12 +143,12 @@ KYM_HD KYM_INL cpx<T> gather_periodized_k(const cpx<T>* U, const T* f, int Wp, i
 }
 
 // dst[y * ncols + c] = scale * P_k(U * f)[y][c0 + c] for y < H, c < ncols.
-// E = 4 / K elements per thread and round: their 2 K^2 loads each are all
+// E = 8 / K elements per thread and round: their 2 K^2 loads each are all
 // issued before the first use, so the memory latency is paid once per round.
 template <int K, class T>
 KYM_HD KYM_INL void gather_block_k(cpx<T>* dst, int ncols, int c0, const cpx<T>* U, const T* f, int H, int W,
                                    T scale, int tid, int nth) {
-    constexpr int E = K >= 4 ? 1 : 4 / K;
+    constexpr int E = K >= 8 ? 1 : 8 / K;
     const int Wp = W * K;  // the parent's row length
     const int n = H * ncols;
     int i = tid;
@@ -599,7 +599,7 @@ KYM_HD KYM_INL int fir_kmax(int n, int kl, int cnt) { return (15 * kl + cnt < n)
 template <class T>
 KYM_HD KYM_INL void fir_band(T* band, const T* filt, const FirAxis& f, int tid, int nth) {
     const T* src = filt + f.band_off;
-    for (int i = tid; i < 16 * f.kmax; i += nth) band[i] = kym_ldg(src + i);
+    copy_batched<8>(16 * f.kmax, [&](int i) { return kym_ldg(src + i); }, [&](int i, T v) { band[i] = v; }, tid, nth);
 }
 
 // Row pass on the matrix cores: for rows p < nrows of R (row stride rs,
