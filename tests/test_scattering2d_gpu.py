@@ -89,10 +89,10 @@ def test_full_batch_properties():
     sc.images_per_chunk = 5
     c = sc(x)
     assert torch.equal(a, c)
-    # chunks alternate over the caller's stream and the plan's second stream
-    # (the default) or stay on the caller's stream: the same bits; the second
-    # stream waits for the caller's pending work (the clone) and the caller's
-    # stream for the second stream (the result read right after)
+    # chunks rotate over the caller's stream and the plan's streams (two by
+    # default, up to four) or stay on the caller's stream: the same bits; the
+    # plan's streams wait for the caller's pending work (the clone) and the
+    # caller's stream for them (the result read right after)
     sc.images_per_chunk = 8
     d2 = sc(x.clone() * 1.0)
     sc.tune = [0, 0, 0, 0, 0, 1]
