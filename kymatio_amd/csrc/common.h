@@ -78,6 +78,22 @@ KYM_HD KYM_INL void copy_batched(int n, GET get, PUT put, int tid, int nth) {
     for (; i < n; i += nth) put(i, get(i));
 }
 
+// The same with the last batch partial (predicated) instead of a serial
+// tail: every element's load is issued with its batch.  Faster in the
+// backward passes, slower in the forward ones (measured), so both exist.
+template <int E, class GET, class PUT>
+KYM_HD KYM_INL void copy_batched_pred(int n, GET get, PUT put, int tid, int nth) {
+    for (int i0 = tid; i0 < n; i0 += E * nth) {
+        decltype(get(0)) v[E];
+#pragma unroll
+        for (int u = 0; u < E; ++u)
+            if (i0 + u * nth < n) v[u] = get(i0 + u * nth);
+#pragma unroll
+        for (int u = 0; u < E; ++u)
+            if (i0 + u * nth < n) put(i0 + u * nth, v[u]);
+    }
+}
+
 // Workgroup barrier; a no-op when a body runs on the host with one thread.
 KYM_HD KYM_INL void wg_barrier() {
 #if defined(__HIP_DEVICE_COMPILE__)

@@ -107,7 +107,7 @@ KYM_HD void bwd_node_body(const Scat2DArgs<T>& a, const G& g, const Node2D& nd, 
         return;
     }
     const cpx<T>* sv = wsi + nd.save;
-    copy_batched<8>(
+    copy_batched_pred<8>(
         n, [&](int p) { return kym_ldg(sv + p); },
         [&](int p, cpx<T> u) {
             const T mag = kym_sqrt(u.re * u.re + u.im * u.im);
@@ -214,7 +214,7 @@ KYM_HD void body_bwd_grows(const P& fp, const Scat2DArgs<T>& a, int task, cpx<T>
     const int r0 = rb * a.blk, nr = (H - r0 < a.blk) ? H - r0 : a.blk;
     cpx<T>* wsi = a.ws + il * a.ws_img;
     const cpx<T>* src = wsi + nd.slot + (long long)r0 * W;
-    copy_batched<8>(nr * W, [&](int i) { return kym_ldg(src + i); }, [&](int i, cpx<T> v) { lds[i] = v; }, tid, nth);
+    copy_batched_pred<8>(nr * W, [&](int i) { return kym_ldg(src + i); }, [&](int i, cpx<T> v) { lds[i] = v; }, tid, nth);
     wg_barrier();
     fp.row_inv(lds, fW, nr, tid, nth);
     if (nd.order == 0) {
@@ -226,7 +226,7 @@ KYM_HD void body_bwd_grows(const P& fp, const Scat2DArgs<T>& a, int task, cpx<T>
         return;
     }
     const cpx<T>* sv = wsi + nd.save + (long long)r0 * W;
-    copy_batched<8>(
+    copy_batched_pred<8>(
         nr * W, [&](int i) { return kym_ldg(sv + i); },
         [&](int i, cpx<T> u) {
             const T mag = kym_sqrt(u.re * u.re + u.im * u.im);
@@ -250,7 +250,7 @@ KYM_HD void body_bwd_gfcols(const P& fp, const Scat2DArgs<T>& a, int task, cpx<T
     const int r = nd.res, H = a.H[r], W = a.W[r];
     const int c0 = cb * a.blk, nc = (W - c0 < a.blk) ? W - c0 : a.blk;
     cpx<T>* buf = a.ws + il * a.ws_img + nd.gsave + c0;
-    copy_batched<8>(
+    copy_batched_pred<8>(
         H * nc,
         [&](int i) {
             const int y = i / nc, c = i - y * nc;
