@@ -18,42 +18,56 @@
 #pragma once
 #include "scat2d_body.h"
 
-// dL/dU at (y, x) of a node of H x W: leaf term (GY: HJ x WJ, already scaled
-// by 1 / (n_J kl^2)) plus the children's dL/dX (scaled here by 1 / kc^2).
-template <class T>
-KYM_HD KYM_INL cpx<T> grad_u_elem(const Scat2DArgs<T>& a, const Node2D& nd, const cpx<T>* wsi, const cpx<T>* GY,
-                                  int W, int HJ, int WJ, int y, int x) {
-    const long long i = (long long)y * W + x;
-    const T ph = a.filt[a.filt_off[nd.leaf_filt] + i];
-    const cpx<T> gy = GY[(y % HJ) * WJ + (x % WJ)];
-    cpx<T> acc = {gy.re * ph, gy.im * ph};
-    auto term = [&](int c, T* fv, cpx<T>* gx) {
-        const Node2D& cn = a.nodes[c];
-        const int Hc = a.H[cn.res], Wc = a.W[cn.res];
-        *fv = kym_ldg(a.filt + a.filt_off[cn.filt] + i) / T(cn.k * cn.k);
-        *gx = wsi[cn.gsave + (long long)(y % Hc) * Wc + (x % Wc)];
-    };
-    // children eight at a time, their loads issued before the sums
+// dL/dU at E points (ys[e], xs[e]) of a node of H x W: the leaf term (GY:
+// HJ x WJ, already scaled by 1 / (n_J kl^2)) plus the children's dL/dX
+// (scaled here by 1 / kc^2), summed in that order.  Each child's loads for
+// all E points are issued together, four children at a time.
+template <int E, class T>
+KYM_HD KYM_INL void grad_u_elems(const Scat2DArgs<T>& a, const Node2D& nd, const cpx<T>* wsi, const cpx<T>* GY, int W,
+                                 int HJ, int WJ, const int* ys, const int* xs, cpx<T>* acc) {
+    long long idx[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        idx[e] = (long long)ys[e] * W + xs[e];
+        const T ph = kym_ldg(a.filt + a.filt_off[nd.leaf_filt] + idx[e]);
+        const cpx<T> gy = GY[(ys[e] % HJ) * WJ + (xs[e] % WJ)];
+        acc[e] = {gy.re * ph, gy.im * ph};
+    }
     int c = nd.child_begin;
-    for (; c + 8 <= nd.child_end; c += 8) {
-        T fv[8];
-        cpx<T> gx[8];
+    for (; c + 4 <= nd.child_end; c += 4) {  // four children x E elements of loads in flight
+        T fv[4][E];
+        cpx<T> gx[4][E];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) term(c + u, &fv[u], &gx[u]);
+        for (int u = 0; u < 4; ++u) {
+            const Node2D& cn = a.nodes[c + u];
+            const int Hc = a.H[cn.res], Wc = a.W[cn.res];
+            const T* f = a.filt + a.filt_off[cn.filt];
+            const T inv = T(1) / T(cn.k * cn.k);
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            acc.re += gx[u].re * fv[u];
-            acc.im += gx[u].im * fv[u];
+            for (int e = 0; e < E; ++e) {
+                fv[u][e] = kym_ldg(f + idx[e]) * inv;
+                gx[u][e] = kym_ldg(wsi + cn.gsave + (long long)(ys[e] % Hc) * Wc + (xs[e] % Wc));
+            }
         }
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                acc[e].re += gx[u][e].re * fv[u][e];
+                acc[e].im += gx[u][e].im * fv[u][e];
+            }
     }
     for (; c < nd.child_end; ++c) {
-        T fv;
-        cpx<T> gx;
-        term(c, &fv, &gx);
-        acc.re += gx.re * fv;
-        acc.im += gx.im * fv;
+        const Node2D& cn = a.nodes[c];
+        const int Hc = a.H[cn.res], Wc = a.W[cn.res];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const T fv = kym_ldg(a.filt + a.filt_off[cn.filt] + idx[e]) / T(cn.k * cn.k);
+            const cpx<T> gx = kym_ldg(wsi + cn.gsave + (long long)(ys[e] % Hc) * Wc + (xs[e] % Wc));
+            acc[e].re += gx.re * fv;
+            acc[e].im += gx.im * fv;
+        }
     }
-    return acc;
 }
 
 // Leaf gradient into B (HJ x WJ, natural), scaled by 1 / (n_J kl^2).
@@ -90,9 +104,20 @@ KYM_HD void bwd_node_body(const Scat2DArgs<T>& a, const G& g, const Node2D& nd, 
     } else {
         GY = wsi + nd.gslot;
     }
-    for (int i = tid; i < n; i += nth) {
-        const int y = i / g.W, x = i - (i / g.W) * g.W;
-        lds[i] = grad_u_elem(a, nd, wsi, GY, g.W, g.HJ, g.WJ, y, x);
+    constexpr int E = 2;  // elements per thread and round
+    for (int i0 = tid; i0 < n; i0 += E * nth) {
+        int ys[E], xs[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int i = i0 + e * nth < n ? i0 + e * nth : i0;
+            ys[e] = i / g.W;
+            xs[e] = i - ys[e] * g.W;
+        }
+        cpx<T> gu[E];
+        grad_u_elems<E>(a, nd, wsi, GY, g.W, g.HJ, g.WJ, ys, xs, gu);
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+            if (i0 + e * nth < n) lds[i0 + e * nth] = gu[e];
     }
     wg_barrier();
     g.inv2d(lds, tid, nth);  // unnormalised: Re() is dL/dm (or dL/dpad(x) at the root)
@@ -187,9 +212,20 @@ KYM_HD void body_bwd_gcols(const P& fp, const Scat2DArgs<T>& a, int task, cpx<T>
     const int r = nd.res, H = a.H[r], W = a.W[r], J = a.J;
     const int c0 = cb * a.blk, nc = (W - c0 < a.blk) ? W - c0 : a.blk;
     cpx<T>* wsi = a.ws + il * a.ws_img;
-    for (int i = tid; i < H * nc; i += nth) {
-        const int y = i / nc, c = i - (i / nc) * nc;
-        lds[i] = grad_u_elem(a, nd, wsi, wsi + nd.gslot, W, a.H[J], a.W[J], y, c0 + c);
+    constexpr int E = 2;  // elements per thread and round
+    for (int i0 = tid; i0 < H * nc; i0 += E * nth) {
+        int ys[E], xs[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int i = i0 + e * nth < H * nc ? i0 + e * nth : i0;
+            ys[e] = i / nc;
+            xs[e] = c0 + i - ys[e] * nc;
+        }
+        cpx<T> g[E];
+        grad_u_elems<E>(a, nd, wsi, wsi + nd.gslot, W, a.H[J], a.W[J], ys, xs, g);
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+            if (i0 + e * nth < H * nc) lds[i0 + e * nth] = g[e];
     }
     wg_barrier();
     fp.col_inv(lds, a.lens[a.lenH[r]], nc, tid, nth);
