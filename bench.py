@@ -276,7 +276,8 @@ def main():
         return
     if args.gpus != 1:
         print('bench.py measures one GPU only (the scattering path shards over the batch with no '
-              'exchange; see DESIGN.md). Run with --gpus 1.', file=sys.stderr)
+              'exchange between GPUs, so one GPU\'s throughput is the per-GPU figure). Run with --gpus 1.',
+              file=sys.stderr)
         sys.exit(2)
     if args.workload != 'scat2d':
         res = {'mixed': bench_mixed, 'scat1d': bench_scat1d, 'grad2d': bench_grad2d,
