@@ -238,6 +238,20 @@ int kym_op_fft(const void* x, void* y, void* work, long long batch, int H, int W
  * y = sqrt(prev^2 + |x|^2) elementwise over n complex x; prev may be null
  * (the first m of a solid-harmonic order l). */
 int kym_op_modulus_rotation(const void* x, const void* prev, void* y, long long n, int dtype, void* stream);
+/* All m of one (l, j) of the 3D solid-harmonic loop at once
+ * (kymatio/scattering3d/core/scattering3d.py:31-35, 51-55, whose m loop calls
+ * cdgmm3d, ifft and modulus_rotation once per m).
+ * harmonic_products: out (batch, nm, n) complex = U (batch, n) complex times
+ *   F (nm, n) complex, the cdgmm3d(U, F[m]) of every m.
+ * modulus_rotation_sum: y[b * ystride + i] for (batch, nm, n) complex x, the
+ *   result of the reference's nm chained modulus_rotation calls
+ *   (kymatio/scattering3d/backend/torch_backend.py:97-124):
+ *   p = sqrt(p^2 + |x_m|^2) from p = 0; nm = 1 is the plain modulus
+ *   (rotation_covariant=False, scattering3d.py:36-39). */
+int kym_op_harmonic_products(const void* U, const void* F, void* out, long long batch, int nm, long long n, int dtype,
+                             void* stream);
+int kym_op_modulus_rotation_sum(const void* x, void* y, long long batch, int nm, long long n, long long ystride,
+                                int dtype, void* stream);
 /* compute_integrals (kymatio/scattering3d/backend/torch_backend.py:126-151):
  * out[b, q] = sum over the n elements of x[b] of x^powers[q], q < npow <= 8
  * (accumulated in FP64, stored in the dtype). */

@@ -58,6 +58,8 @@ _SIGNATURES = {
     'kym_op_fft': [_c_vp, _c_vp, _c_vp, _c_ll, _c_i, _c_i, _c_i, _c_i, _c_ll, _c_vp],
     'kym_op_modulus_rotation': [_c_vp, _c_vp, _c_vp, _c_ll, _c_i, _c_vp],
     'kym_op_integrals': [_c_vp, _c_vp, _c_ll, _c_ll, _c_vp, _c_i, _c_i, _c_vp],
+    'kym_op_harmonic_products': [_c_vp, _c_vp, _c_vp, _c_ll, _c_i, _c_ll, _c_i, _c_vp],
+    'kym_op_modulus_rotation_sum': [_c_vp, _c_vp, _c_ll, _c_i, _c_ll, _c_ll, _c_i, _c_vp],
     'kym_dec_conv_fwd': [_c_vp] * 5 + [_c_i] * 8 + [_c_vp],
     'kym_dec_conv_bwd_weight': [_c_vp] * 4 + [_c_i] * 8 + [_c_vp],
     'kym_dec_up_relu_adjoint': [_c_vp] * 3 + [_c_i] * 5 + [_c_vp],

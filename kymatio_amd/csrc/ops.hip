@@ -80,6 +80,39 @@ __global__ void k_modulus_rot(const cpx<T>* x, const T* prev, T* y, long long n)
     }
 }
 
+// Solid-harmonic products for every m of one (l, j) at once:
+// out[b, m, i] = U[b, i] * F[m, i], the reference's cdgmm3d(U, F[m]) for all
+// m (same complex product as k_cdgmm, U the left operand).
+template <class T>
+__global__ void k_harmonic_products(const cpx<T>* U, const cpx<T>* F, cpx<T>* out, long long batch, int nm,
+                                    long long n) {
+    const long long total = batch * nm * n;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total; i += (long long)gridDim.x * blockDim.x) {
+        const long long bm = i / n, e = i - bm * n;
+        const long long b = bm / nm, m = bm - b * nm;
+        out[i] = cmul(U[b * n + e], F[m * n + e]);
+    }
+}
+
+// Rotation-covariant modulus of all m at once: for each (b, i) the running
+// value p = sqrt(p^2 + |x[b, m, i]|^2) over m = 0..nm-1 (p = 0 before m = 0),
+// the order of the reference's modulus_rotation calls; y[b * ystride + i].
+template <class T>
+__global__ void k_modulus_rot_sum(const cpx<T>* x, T* y, long long batch, int nm, long long n, long long ystride) {
+    const long long total = batch * n;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total; i += (long long)gridDim.x * blockDim.x) {
+        const long long b = i / n, e = i - b * n;
+        const cpx<T>* xb = x + b * nm * n + e;
+        T p = T(0);
+        for (int m = 0; m < nm; ++m) {
+            const cpx<T> v = xb[m * n];
+            const T t = v.re * v.re + v.im * v.im;
+            p = kym_sqrt(p * p + t);
+        }
+        y[b * ystride + e] = p;
+    }
+}
+
 // out[b * npow + q] = sum_i x[b, i]^pw[q]: one workgroup per b, FP64
 // partial sums reduced through LDS.
 #define KYM_MAX_POWERS 8
@@ -304,6 +337,24 @@ static int launch_integrals(const void* x, void* out, long long batch, long long
     return KYM_OK;
 }
 
+template <class T>
+static int launch_harmonic_products(const void* U, const void* F, void* out, long long batch, int nm, long long n,
+                                    hipStream_t s) {
+    hipLaunchKernelGGL(k_harmonic_products<T>, dim3(grid_for(batch * nm * n, 256)), dim3(256), 0, s,
+                       (const cpx<T>*)U, (const cpx<T>*)F, (cpx<T>*)out, batch, nm, n);
+    OPS_HIP_TRY(hipGetLastError());
+    return KYM_OK;
+}
+
+template <class T>
+static int launch_modulus_rot_sum(const void* x, void* y, long long batch, int nm, long long n, long long ystride,
+                                  hipStream_t s) {
+    hipLaunchKernelGGL(k_modulus_rot_sum<T>, dim3(grid_for(batch * n, 256)), dim3(256), 0, s, (const cpx<T>*)x,
+                       (T*)y, batch, nm, n, ystride);
+    OPS_HIP_TRY(hipGetLastError());
+    return KYM_OK;
+}
+
 static int bad_dtype() { return kym_ops_fail(KYM_ERR_TYPE, "dtype must be float32 or float64"); }
 
 extern "C" int kym_op_modulus_rotation(const void* x, const void* prev, void* y, long long n, int dtype,
@@ -312,6 +363,29 @@ extern "C" int kym_op_modulus_rotation(const void* x, const void* prev, void* y,
     hipStream_t s = (hipStream_t)stream;
     if (dtype == KYM_F32) return launch_modulus_rot<float>(x, prev, y, n, s);
     if (dtype == KYM_F64) return launch_modulus_rot<double>(x, prev, y, n, s);
+    return bad_dtype();
+}
+
+extern "C" int kym_op_harmonic_products(const void* U, const void* F, void* out, long long batch, int nm,
+                                        long long n, int dtype, void* stream) {
+    if (nm < 1) return kym_ops_fail(KYM_ERR_VALUE, "at least one filter per (l, j)");
+    if (batch <= 0 || n <= 0) return KYM_OK;
+    if (!U || !F || !out) return kym_ops_fail(KYM_ERR_VALUE, "null device pointer");
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == KYM_F32) return launch_harmonic_products<float>(U, F, out, batch, nm, n, s);
+    if (dtype == KYM_F64) return launch_harmonic_products<double>(U, F, out, batch, nm, n, s);
+    return bad_dtype();
+}
+
+extern "C" int kym_op_modulus_rotation_sum(const void* x, void* y, long long batch, int nm, long long n,
+                                           long long ystride, int dtype, void* stream) {
+    if (nm < 1) return kym_ops_fail(KYM_ERR_VALUE, "at least one filter per (l, j)");
+    if (ystride < n) return kym_ops_fail(KYM_ERR_VALUE, "output stride smaller than the volume");
+    if (batch <= 0 || n <= 0) return KYM_OK;
+    if (!x || !y) return kym_ops_fail(KYM_ERR_VALUE, "null device pointer");
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == KYM_F32) return launch_modulus_rot_sum<float>(x, y, batch, nm, n, ystride, s);
+    if (dtype == KYM_F64) return launch_modulus_rot_sum<double>(x, y, batch, nm, n, ystride, s);
     return bad_dtype();
 }
 

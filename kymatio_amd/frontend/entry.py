@@ -1,8 +1,10 @@
 """Frontend dispatch by class swap, as kymatio/frontend/entry.py:6-47 does.
 
-The 'torch' frontend (the default), 'numpy' (arrays staged through the
-device) and 'sklearn' (transformers over the numpy frontend) exist; other
-frontend names raise the reference's RuntimeError.
+Without a frontend= argument the NumPy frontend is chosen, as the
+reference's entry classes do (kymatio/frontend/entry.py:20-21).  The 'numpy'
+(arrays staged through the device), 'torch' and 'sklearn' (transformers over
+the numpy frontend) frontends exist; other frontend names raise the
+reference's RuntimeError.
 """
 import importlib
 import logging
@@ -15,7 +17,7 @@ class ScatteringEntry:
     def __init__(self, *args, **kwargs):
         self.name = kwargs.pop('name')
         self.class_name = kwargs.pop('class_name')
-        frontend = kwargs.pop('frontend', 'torch').lower()
+        frontend = kwargs.pop('frontend', 'numpy').lower()
         if frontend not in _KNOWN:
             raise RuntimeError('The frontend \'%s\" is not valid. Must be one of \'%s\', or \'%s\'.'
                                % (frontend, '\', \''.join(_KNOWN[:-1]), _KNOWN[-1]))

@@ -1,15 +1,31 @@
-"""HarmonicScattering3D as a torch.nn.Module on the HIP ops
-(kymatio/scattering3d/frontend/torch_frontend.py:7-80 over
-kymatio/scattering3d/frontend/base_frontend.py:4-36): same constructor,
-buffers (tensor0.. the solid harmonic filters per l, then
-tensor_gaussian_filter), checks, messages and output layout
-(batch, paths, L + 1, P)."""
+"""HarmonicScattering3D as a torch.nn.Module over batched HIP launches.
+
+Interface of kymatio/scattering3d/frontend/torch_frontend.py:7-80 (with
+kymatio/scattering3d/frontend/base_frontend.py:4-36): the constructor
+arguments, the buffer names (tensor{l}: the order-l wavelets as
+(J + 1, 2l + 1, M, N, O, 2) real/imaginary pairs; tensor_gaussian_filter:
+the Gaussians, imaginary part zero), the checks with their messages and
+the output layout batch_shape + (paths, L + 1, P).  The transform itself is
+kymatio_amd/scattering3d/core/harmonic.py, three launches per (l, j) over
+every m and image.
+"""
+import numpy as np
 import torch
 
 from ...frontend.base_frontend import ScatteringBase
 from ...frontend.torch_frontend import ScatteringTorch
-from ..core.scattering3d import scattering3d
+from ..core.harmonic import harmonic_scattering
 from ..filter_bank import gaussian_filter_bank, solid_harmonic_filter_bank
+
+_METHODS = ('integral',)
+
+
+def _as_pairs(a, keep_imag=True):
+    """complex ndarray -> float32 tensor of (re, im) pairs in a last axis."""
+    c = np.asarray(a)
+    re = torch.from_numpy(np.ascontiguousarray(c.real, dtype=np.float32))
+    im = torch.from_numpy(np.ascontiguousarray(c.imag, dtype=np.float32)) if keep_imag else torch.zeros_like(re)
+    return torch.stack((re, im), dim=-1)
 
 
 class HarmonicScatteringTorch3D(ScatteringTorch, ScatteringBase):
@@ -24,44 +40,45 @@ class HarmonicScatteringTorch3D(ScatteringTorch, ScatteringBase):
     def build(self):
         self._instantiate_backend('kymatio_amd.scattering3d.backend.')
         self.M, self.N, self.O = self.shape
-        self.filters = solid_harmonic_filter_bank(self.M, self.N, self.O, self.J, self.L, self.sigma_0)
-        self.gaussian_filters = gaussian_filter_bank(self.M, self.N, self.O, self.J + 1, self.sigma_0)
-        self.register_filters()
+        bank = solid_harmonic_filter_bank(self.M, self.N, self.O, self.J, self.L, self.sigma_0)
+        for l, psi_l in enumerate(bank):
+            self.register_buffer('tensor%d' % l, _as_pairs(psi_l))
+        self.register_buffer('tensor_gaussian_filter',
+                             _as_pairs(gaussian_filter_bank(self.M, self.N, self.O, self.J + 1, self.sigma_0),
+                                       keep_imag=False))
 
     def register_filters(self):
-        for k in range(len(self.filters)):
-            f = torch.zeros(self.filters[k].shape + (2,))
-            f[..., 0] = torch.from_numpy(self.filters[k].real).reshape(self.filters[k].shape)
-            f[..., 1] = torch.from_numpy(self.filters[k].imag).reshape(self.filters[k].shape)
-            self.filters[k] = f
-            self.register_buffer('tensor' + str(k), f)
-        g = torch.zeros(self.gaussian_filters.shape + (2,))
-        g[..., 0] = torch.from_numpy(self.gaussian_filters.real)
-        self.gaussian_filters = g
-        self.register_buffer('tensor_gaussian_filter', g)
+        """The buffers are registered by build()."""
 
-    def scattering(self, input_array):
-        if not torch.is_tensor(input_array):
+    @property
+    def filters(self):
+        """Per l, the (J + 1, 2l + 1, M, N, O, 2) wavelet buffer on the module's device."""
+        return [getattr(self, 'tensor%d' % l) for l in range(self.L + 1)]
+
+    @property
+    def gaussian_filters(self):
+        return self.tensor_gaussian_filter
+
+    def _check(self, x):
+        if not torch.is_tensor(x):
             raise TypeError('The input should be a torch.cuda.FloatTensor, a torch.FloatTensor or a '
                             'torch.DoubleTensor.')
-        if input_array.dim() < 3:
+        if x.dim() < 3:
             raise RuntimeError('Input tensor must have at least three dimensions.')
-        if input_array.shape[-1] != self.O or input_array.shape[-2] != self.N or input_array.shape[-3] != self.M:
+        if tuple(x.shape[-3:]) != (self.M, self.N, self.O):
             raise RuntimeError('Tensor must be of spatial size (%i, %i, %i).' % (self.M, self.N, self.O))
-        if not input_array.is_cuda:
+        if self.method not in _METHODS:
+            raise ValueError('method must be in {}'.format(list(_METHODS)))
+        if not x.is_cuda:
             raise TypeError('HarmonicScattering3D runs on the GPU (HIP); got a CPU tensor.')
-        x = input_array.contiguous()
-        batch_shape, signal_shape = x.shape[:-3], x.shape[-3:]
-        x = x.reshape((-1,) + signal_shape + (1,))
-        bufs = dict(self.named_buffers())
-        for k in range(len(self.filters)):
-            self.filters[k] = bufs['tensor' + str(k)]
-        if self.method not in ['integral']:
-            raise ValueError('method must be in {}'.format(['integral']))
-        averaging = lambda U: self.backend.compute_integrals(U, self.integral_powers)  # noqa: E731
-        S = scattering3d(x, filters=self.filters, rotation_covariant=self.rotation_covariant, L=self.L, J=self.J,
-                         max_order=self.max_order, backend=self.backend, averaging=averaging)
-        return S.reshape(batch_shape + S.shape[1:])
+
+    def scattering(self, input_array):
+        self._check(input_array)
+        lead = input_array.shape[:-3]
+        volumes = input_array.contiguous().reshape((-1, self.M, self.N, self.O, 1))
+        S = harmonic_scattering(volumes, self.filters, self.J, self.L, self.max_order, self.rotation_covariant,
+                                self.integral_powers, self.backend)
+        return S.reshape(tuple(lead) + tuple(S.shape[1:]))
 
 
 __all__ = ['HarmonicScatteringTorch3D']

@@ -36,6 +36,16 @@ def test_frontend_entry():
     assert isinstance(kymatio_amd.Scattering1D(3, 64, 2, frontend='torch'), Scattering1D)
 
 
+def test_frontend_entry_defaults_to_numpy():
+    """kymatio/frontend/entry.py:20-21: no frontend= argument means NumPy."""
+    from kymatio_amd.numpy import HarmonicScattering3D as NumPy3D
+    from kymatio_amd.numpy import Scattering1D as NumPy1D
+    from kymatio_amd.numpy import Scattering2D as NumPy2D
+    assert type(kymatio_amd.Scattering2D(2, shape=(10, 10))) is NumPy2D
+    assert type(kymatio_amd.Scattering1D(3, 64, 2)) is NumPy1D
+    assert type(kymatio_amd.HarmonicScattering3D(1, (8, 8, 8))) is NumPy3D
+
+
 def test_smallest_dimension():
     with pytest.raises(RuntimeError, match='smallest dimension'):
         Scattering2D(10, shape=(10, 10))
