@@ -100,6 +100,23 @@ int kym_scat2d_leaf_taps(void* plan, long long* desc, int n);
 int kym_scat2d_forward(void* plan, const void* x, const void* filt, void* out, long long batch,
                        void* ws, long long ws_bytes, void* stream);
 
+/* kym_scat2d_forward without a workspace argument and without allocating:
+ * no counterpart in the reference; it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  The workspace is a static device array of
+ * KYM_NOALLOC_SCRATCH_BYTES, one copy per GPU, held until the process ends;
+ * the batch runs in chunks of KYM_NOALLOC_SCRATCH_BYTES /
+ * workspace_bytes_per_image images (info[7]).  Status 3 if one image needs
+ * more than KYM_NOALLOC_SCRATCH_BYTES.  Every call on a GPU shares that GPU's
+ * array, so calls that overlap in time on one GPU (other host threads, other
+ * streams) race; calls ordered on one stream do not.  No part of the array
+ * needs to start at zero (every element a kernel reads was written earlier in
+ * the same call).  Results equal kym_scat2d_forward's with a workspace of
+ * KYM_NOALLOC_SCRATCH_BYTES. */
+#define KYM_NOALLOC_SCRATCH_BYTES (64ll << 20)
+int kym_scat2d_forward_noalloc(void* plan, const void* x, const void* filt, void* out, long long batch,
+                               void* stream);
+
 /* Differentiation (replaces torch autograd through the reference's ops:
  * ModulusStable.backward, kymatio/backend/torch_backend.py:63-96, and the
  * autograd of fft2/ifft2, mean, the real filter product, ReflectionPad2d and

@@ -35,6 +35,7 @@ _SIGNATURES = {
     'kym_scat2d_graph_launch': [_c_vp, _c_vp],
     'kym_scat2d_graph_destroy': [_c_vp],
     'kym_scat2d_forward': [_c_vp, _c_vp, _c_vp, _c_vp, _c_ll, _c_vp, _c_ll, _c_vp],
+    'kym_scat2d_forward_noalloc': [_c_vp, _c_vp, _c_vp, _c_vp, _c_ll, _c_vp],
     'kym_scat2d_forward_train': [_c_vp, _c_vp, _c_vp, _c_vp, _c_ll, _c_vp, _c_ll, _c_vp],
     'kym_scat2d_backward': [_c_vp, _c_vp, _c_vp, _c_vp, _c_ll, _c_vp, _c_ll, _c_vp],
     'kym_scat2d_launch_info': [_c_vp, _c_i, ctypes.POINTER(_c_ll)],

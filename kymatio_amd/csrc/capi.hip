@@ -379,6 +379,59 @@ extern "C" int kym_scat2d_forward(void* plan, const void* x, const void* filt, v
                  &h->cs);
 }
 
+// ------------------------------------------------- 2D forward, no allocation
+// Workspace of kym_scat2d_forward_noalloc: one file-scope device array, a
+// copy per GPU, held until the process ends.  Host code never passes the
+// array's name to a kernel (on the host it is only a stand-in for the symbol):
+// the device address of each GPU's copy comes from hipGetSymbolAddress on the
+// first call with that GPU current, and that address is what the kernels get.
+#define KYM_NOALLOC_MAX_DEVICES 64
+static __device__ __align__(256) unsigned char scratch[KYM_NOALLOC_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[KYM_NOALLOC_MAX_DEVICES] = {};
+
+static int scratch_of_device(int dev, void** ptr) {
+    if (dev < 0 || dev >= KYM_NOALLOC_MAX_DEVICES) return fail(KYM_ERR_VALUE, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        KYM_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(KYM_ERR_HIP, "hipGetSymbolAddress returned a null scratch address");
+        g_scratch_ptr[dev] = p;
+    }
+    *ptr = g_scratch_ptr[dev];
+    return KYM_OK;
+}
+
+extern "C" int kym_scat2d_forward_noalloc(void* plan, const void* x, const void* filt, void* out, long long batch,
+                                          void* stream) {
+    if (!plan) return fail(KYM_ERR_VALUE, "null plan");
+    if (batch < 0) return fail(KYM_ERR_VALUE, "negative batch");
+    if (!x || !filt || !out) return fail(KYM_ERR_VALUE, "null device pointer");
+    KymScat2D* h = static_cast<KymScat2D*>(plan);
+    int cur = 0;
+    KYM_HIP_TRY(hipGetDevice(&cur));
+    if (cur != h->device) return fail(KYM_ERR_TYPE, "plan was created for another device");
+    const long long per_image = h->pf ? h->pf->ws_img * (long long)sizeof(cpx<float>)
+                                      : h->pd->ws_img * (long long)sizeof(cpx<double>);
+    if (per_image > KYM_NOALLOC_SCRATCH_BYTES)
+        return fail(KYM_ERR_VALUE, "kym_scat2d_forward_noalloc: one image needs " + std::to_string(per_image) +
+                                       " bytes of workspace, more than the " +
+                                       std::to_string((long long)KYM_NOALLOC_SCRATCH_BYTES) +
+                                       "-byte static scratch; use kym_scat2d_forward with a workspace");
+    void* ws = nullptr;
+    int rc = scratch_of_device(cur, &ws);
+    if (rc != KYM_OK) return rc;
+    if (batch == 0) return KYM_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    // no per-launch profiling here: its events would be created on demand
+    if (h->pf)
+        return run2d(*h->pf, 0, x, filt, out, nullptr, nullptr, batch, ws, KYM_NOALLOC_SCRATCH_BYTES, s,
+                     (uintptr_t)h->dblob, nullptr, &h->cs);
+    return run2d(*h->pd, 0, x, filt, out, nullptr, nullptr, batch, ws, KYM_NOALLOC_SCRATCH_BYTES, s,
+                 (uintptr_t)h->dblob, nullptr, &h->cs);
+}
+
 extern "C" int kym_scat2d_forward_train(void* plan, const void* x, const void* filt, void* out, long long batch,
                                         void* ws, long long ws_bytes, void* stream) {
     if (!plan) return fail(KYM_ERR_VALUE, "null plan");
