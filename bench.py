@@ -249,6 +249,46 @@ def bench_jtfs(args, batch=64, seed=6):
             'roofline': None, 'cpu_baseline': None}
 
 
+def bench_recon(args, batch=256, seed=99, tol=1e-3, time_limit=20.0):
+    """Reconstruction from scattering coefficients, Scattering1D J=6 Q=8
+    N=4096, 256 unseen chirp signals: wall time until the relative
+    scattering error ||S(y) - S(x)|| / ||S(x)|| of the batch (un-squared)
+    reaches tol, from the trained decoder's output (decoder forward included)
+    and from N(0, 1) noise, by the same descent through the hand-written
+    backward for both starts, each stopped at time_limit seconds: per-signal
+    L-BFGS (the fastest method measured here for either start) and Adam with
+    step 0.1.  value is the warm start's L-BFGS time to tol (null if it did
+    not get there); the line carries each run's time to the intermediate
+    levels and the plain / warm ratios."""
+    from kymatio_amd.learned.evaluate import compare, load_decoder
+    from kymatio_amd.learned.train import CHECKPOINT
+    from kymatio_amd.torch import Scattering1D
+    dev = torch.device('cuda:0')
+    sc = Scattering1D(6, 4096, 8).to(dev)
+    dec = load_decoder(CHECKPOINT, dev)
+    runs = {}
+    for method, lr in (('lbfgs', None), ('adam', 0.1)):
+        res = compare(sc, dec, n=batch, seed=seed, tol=tol, method=method, lr=lr, time_limit=time_limit)
+        for k in ('warm', 'plain'):
+            res[k].pop('curve', None)
+        runs[method] = {k: res[k] for k in ('warm', 'plain', 'speedup', 'speedup_by_level')}
+        runs[method]['speedup_at_least'] = res.get('speedup_at_least')
+        start_rel, t_dec = res['start_rel'], res['decoder_forward_s']
+    w = runs['lbfgs']['warm']
+    return {'metric': 'recon time to 1e-3: seconds until ||S(y)-S(x)||/||S(x)|| <= 1e-3, learned warm start, '
+                      'Scattering1D J=6 Q=8 N=4096, batch 256',
+            'value': w['seconds'] if w['reached'] else None, 'unit': 's', 'n_gpus': 1, 'steps': 1, 'warmup': 0,
+            'ms_per_step': None, 'higher_is_better': False, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32',
+            'data': 'synthetic',
+            'config': {'workload': '%d unseen chirp signals (seed %d) of length 4096: 1-8 linear chirps + 0.05 '
+                                   'N(0,1) noise; descent through the HIP Scattering1D backward, %g s limit per '
+                                   'start and method' % (batch, seed, time_limit),
+                       'batch': batch, 'tol': tol, 'loss_definition': 'relative scattering error of the batch, '
+                                                                      '||S(y)-S(x)||_F / ||S(x)||_F (un-squared)'},
+            'recon': {'start_rel': start_rel, 'decoder_forward_s': t_dec, 'runs': runs},
+            'roofline': None, 'cpu_baseline': None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -261,7 +301,7 @@ def main():
                     help='fixed plan choice (the autotuner fields, comma separated) instead of autotuning')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='BYTES',
                     help='only run a plain device-to-device copy of BYTES (calibrates HBM counters)')
-    ap.add_argument('--workload', default='scat2d', choices=('scat2d', 'mixed', 'scat1d', 'grad2d', 'jtfs'),
+    ap.add_argument('--workload', default='scat2d', choices=('scat2d', 'mixed', 'scat1d', 'grad2d', 'jtfs', 'recon'),
                     help='scat2d: the headline (default); mixed / scat1d / grad2d: the other parts of the path '
                          '(one JSON line each, same keys)')
     args = ap.parse_args()
@@ -281,7 +321,7 @@ def main():
         sys.exit(2)
     if args.workload != 'scat2d':
         res = {'mixed': bench_mixed, 'scat1d': bench_scat1d, 'grad2d': bench_grad2d,
-               'jtfs': bench_jtfs}[args.workload](args)
+               'jtfs': bench_jtfs, 'recon': bench_recon}[args.workload](args)
         print(json.dumps(res))
         return
 
@@ -380,9 +420,47 @@ def main():
                     for r in sorted(per_kernel, key=lambda r: -r['total_ms'])[:8]],
     }
     res['roofline']['measured_copy_GBps'] = copy_bandwidth(dev)
+    res['parity'] = parity_block(dev)
     if not args.no_cpu_baseline:
         res['cpu_baseline'] = cpu_baseline()
     print(json.dumps(res))
+
+
+def parity_block(dev):
+    """Parity of the headline path, measured after the timed region: the FP32
+    HIP forward on the four reference-written 256x256 J=3 images
+    (tests/golden: the reference's torch-backend outputs) and, per output
+    order, against the FP64 HIP plan (spectral leaves; tested against the FP64
+    NumPy restatement to 1e-12)."""
+    from kymatio_amd.torch import Scattering2D
+    gdir = os.path.join(ROOT, 'tests', 'golden')
+    try:
+        a = np.load(os.path.join(gdir, 'golden_scat2d.npz'))
+        b = np.load(os.path.join(gdir, 'golden_headline2d.npz'))
+    except OSError:
+        return None
+    x = np.concatenate([a['cfg2_x'], b['x']])
+    ref = np.concatenate([a['cfg2_Sx'], b['Sx']]).astype(np.float64)
+    with torch.no_grad():
+        out = Scattering2D(J, (SIDE, SIDE), L=L).to(dev)(torch.from_numpy(x).to(dev)).cpu().double().numpy()
+        out64 = Scattering2D(J, (SIDE, SIDE), L=L).double().to(dev)(
+            torch.from_numpy(x.astype(np.float64)).to(dev)).cpu().numpy()
+
+    def chan_err(o, r):
+        o, r = o.reshape(-1, o.shape[-2] * o.shape[-1]), r.reshape(-1, r.shape[-2] * r.shape[-1])
+        scale = np.abs(r).max(axis=1)
+        scale[scale == 0] = 1.0
+        return float((np.abs(o - r).max(axis=1) / scale).max())
+
+    n1 = J * L
+    orders = np.array([0] + [1] * n1 + [2] * (out.shape[1] - 1 - n1))
+    return {'images': int(x.shape[0]),
+            'reference_per_channel_max_rel': chan_err(out, ref),
+            'reference_max_rel_of_peak': float(np.abs(out - ref).max() / np.abs(ref).max()),
+            'reference_allclose_1e-5_1e-8_fraction': float(np.isclose(out, ref, rtol=1e-5, atol=1e-8).mean()),
+            'fp64_per_channel_max_rel_by_order': [chan_err(out[:, orders == o], out64[:, orders == o])
+                                                  for o in (0, 1, 2)],
+            'bar': 'per output channel max|ours - ref| <= 1e-5 max|ref|'}
 
 
 def isolated_launch(sc, x, dev, index, per_launch, steps=5):

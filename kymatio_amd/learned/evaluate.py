@@ -21,6 +21,17 @@ from .recon import reconstruct, rel_error
 from .train import CHECKPOINT, chirps
 
 LOSS_DEFINITION = '||S(y)-S(x)||_F / ||S(x)||_F over the batch (un-squared)'
+LEVELS = (1e-1, 5e-2, 3e-2, 2e-2, 1e-2, 5e-3, 2e-3, 1e-3)
+
+
+def time_to_levels(hist, extra_s=0.0, levels=LEVELS):
+    """Seconds (and iterations) at which a descent history first reached
+    each relative error level; None where it did not."""
+    out = {}
+    for lev in levels:
+        hit = next((h for h in hist if h[2] <= lev), None)
+        out['%g' % lev] = None if hit is None else {'seconds': hit[1] + extra_s, 'iterations': hit[0]}
+    return out
 
 
 def load_decoder(path, device):
@@ -36,7 +47,8 @@ def _run(scattering, S, y0, tol, method, lr, time_limit, max_iter, extra_s=0.0):
                               time_limit=time_limit)
     it, t, r = hist[-1]
     return {'method': method, 'lr': lr, 'reached': bool(ok), 'iterations': it, 'seconds': t + extra_s,
-            'final_rel': r, 'curve': [(h[0], round(h[1], 4), h[2]) for h in hist[::max(1, len(hist) // 40)]]}
+            'final_rel': r, 'time_to': time_to_levels(hist, extra_s),
+            'curve': [(h[0], round(h[1], 4), h[2]) for h in hist[::max(1, len(hist) // 20)]]}
 
 
 def compare(scattering, decoder, n=256, seed=99, tol=1e-3, method='lbfgs', lr=None, time_limit=30.0,
@@ -60,6 +72,8 @@ def compare(scattering, decoder, n=256, seed=99, tol=1e-3, method='lbfgs', lr=No
     out['warm'] = _run(scattering, S, y0, tol, method, lr, time_limit, max_iter, t_dec)
     out['plain'] = _run(scattering, S, noise, tol, method, lr, time_limit, max_iter)
     w, p = out['warm'], out['plain']
+    out['speedup_by_level'] = {k: (p['time_to'][k]['seconds'] / w['time_to'][k]['seconds']
+                                   if w['time_to'][k] and p['time_to'][k] else None) for k in w['time_to']}
     if w['reached'] and p['reached']:
         out['speedup'] = p['seconds'] / w['seconds']
     elif w['reached']:

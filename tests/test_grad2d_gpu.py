@@ -55,7 +55,7 @@ def test_finite_differences_fp64_patches():
             assert abs(fd.item() - an.item()) <= 1e-6 * max(abs(an.item()), 1e-12) + 1e-9
 
 
-@pytest.mark.parametrize('dtype,tol', [(np.float64, 1e-10), (np.float32, 1e-3)])
+@pytest.mark.parametrize('dtype,tol', [(np.float64, 1e-10), (np.float32, 1e-5)])
 def test_matches_reference_autograd_golden(dtype, tol):
     """d/dx sum((S(x)-S(y))^2) at J=4 128x128, as torch autograd through the
     reference computed it (tests/golden/make_golden.py)."""
@@ -73,6 +73,33 @@ def test_matches_reference_autograd_golden(dtype, tol):
     print('loss %.10e ref %.10e, grad max rel err %.3e' % (loss.item(), float(G['grad_' + tag + '_loss']), rel))
     assert abs(loss.item() - float(G['grad_' + tag + '_loss'])) <= tol * abs(float(G['grad_' + tag + '_loss']))
     assert rel <= tol
+
+
+def test_fp32_backward_matches_fp64_oracle_autograd():
+    """FP32 hand-written backward at J=4 128x128 (the compile-time node and
+    160x160 pass kernels, many threads per workgroup) against torch autograd
+    through the FP64 oracle, for a loss linear in S with random weights
+    (no cancellation between S(x) and S(y) to amplify rounding): max error
+    <= 1e-5 of the gradient's peak, on two images."""
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal((2, 128, 128))
+    ref_sc = scat2d.Scattering2DTorchCPU(4, (128, 128), dtype=torch.float64)
+    xr = torch.from_numpy(x).requires_grad_()
+    S = ref_sc(xr)
+    w = rng.standard_normal(tuple(S.shape))
+    (S * torch.from_numpy(w)).sum().backward()
+    g_ref = xr.grad.numpy()
+    sc = Scattering2D(4, (128, 128)).to(DEV)
+    xt = torch.from_numpy(x.astype(np.float32)).to(DEV).requires_grad_()
+    St = sc(xt)
+    (St * torch.from_numpy(w.astype(np.float32)).to(DEV)).sum().backward()
+    got = xt.grad.cpu().double().numpy()
+    s_err = (St.detach().cpu().double().numpy() - S.detach().numpy())
+    rel_s = np.abs(s_err).max() / np.abs(S.detach().numpy()).max()
+    rel_g = np.abs(got - g_ref).max() / np.abs(g_ref).max()
+    print('FP32 vs FP64 oracle at J=4 128x128: S max rel %.3e, dL/dx max rel %.3e' % (rel_s, rel_g))
+    assert rel_s <= 1e-5
+    assert rel_g <= 1e-5
 
 
 def test_matches_oracle_autograd_multipass():

@@ -66,6 +66,25 @@ def test_headline_error_by_order_vs_fp64():
         assert e <= RTOL_CHANNEL
 
 
+def test_headline_more_reference_images():
+    """Three more reference-written images of the 256x256 J=3 workload
+    (tests/golden/make_headline_golden.py), in one batch with the golden
+    cfg2 image: per-channel bar, and the fraction of elements within the
+    reference's own torch.allclose(rtol=1e-5, atol=1e-8)."""
+    G = golden('golden_headline2d.npz')
+    C = golden('golden_scat2d.npz')
+    x = np.concatenate([C['cfg2_x'], G['x']])
+    ref = np.concatenate([C['cfg2_Sx'], G['Sx']])
+    sc = Scattering2D(3, (256, 256)).to(DEV)
+    out = sc(torch.from_numpy(x).to(DEV)).cpu().numpy()
+    err = channel_rel_err(out, ref)
+    close = np.isclose(out, ref, rtol=1e-5, atol=1e-8)
+    print('4 reference images: per-channel rel err %.3e, elements within allclose(1e-5, 1e-8): %.6f'
+          % (err, close.mean()))
+    assert err <= RTOL_CHANNEL
+    assert close.mean() >= 0.99
+
+
 def test_fp64_gpu_vs_numpy():
     G = golden('golden_scat2d.npz')
     x = G['cfg4_x'].astype(np.float64)
