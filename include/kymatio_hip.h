@@ -149,6 +149,16 @@ int kym_scat2d_launch_info(void* plan, int index, long long* rec);
 int kym_scat2d_graph_create(void* const* plans, int nplans, const void* const* x, const void* const* filt,
                             void* const* out, const long long* batch, void* const* ws, const long long* ws_bytes,
                             void** graph);
+/* The same graph laid out for at most max_batch[i] images per plan, whose
+ * kernels read the actual image count of plan i from dev_counts[i] (a device
+ * array of nplans ints) at every replay: counts 0..max_batch[i] replay the
+ * one graph without rebuilding it (write the counts, e.g. with a copy on the
+ * replay's stream, before graph_launch; a count above max_batch[i] is
+ * clamped to it).  Images i < count of plan i are read from x[i] and written
+ * to out[i]; chunks of the layout past the count exit at once. */
+int kym_scat2d_graph_create_counted(void* const* plans, int nplans, const void* const* x, const void* const* filt,
+                                    void* const* out, const long long* max_batch, void* const* ws,
+                                    const long long* ws_bytes, const int* dev_counts, void** graph);
 int kym_scat2d_graph_launch(void* graph, void* stream);
 int kym_scat2d_graph_destroy(void* graph);
 

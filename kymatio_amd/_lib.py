@@ -32,6 +32,9 @@ _SIGNATURES = {
     'kym_scat2d_graph_create': [ctypes.POINTER(_c_vp), _c_i] + [ctypes.POINTER(_c_vp)] * 3
                                + [ctypes.POINTER(_c_ll), ctypes.POINTER(_c_vp), ctypes.POINTER(_c_ll),
                                   ctypes.POINTER(_c_vp)],
+    'kym_scat2d_graph_create_counted': [ctypes.POINTER(_c_vp), _c_i] + [ctypes.POINTER(_c_vp)] * 3
+                                       + [ctypes.POINTER(_c_ll), ctypes.POINTER(_c_vp), ctypes.POINTER(_c_ll), _c_vp,
+                                          ctypes.POINTER(_c_vp)],
     'kym_scat2d_graph_launch': [_c_vp, _c_vp],
     'kym_scat2d_graph_destroy': [_c_vp],
     'kym_scat2d_forward': [_c_vp, _c_vp, _c_vp, _c_vp, _c_ll, _c_vp, _c_ll, _c_vp],

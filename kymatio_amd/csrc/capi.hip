@@ -515,12 +515,16 @@ struct KymScat2DGraph {
     }
 };
 
+// dev_count (counted graphs): the chain is laid out for `batch` images (the
+// maximum) and its kernels read the actual count from *dev_count at replay.
 template <class T>
 static int add_plan_chain(hipGraph_t g, const Scat2DPlan<T>& p, const void* x, const void* filt, void* out,
-                          long long batch, void* ws, long long ws_bytes, uintptr_t base) {
+                          long long batch, void* ws, long long ws_bytes, uintptr_t base,
+                          const int* dev_count = nullptr) {
     long long chunk = 0;
     int rc = chunk_of(p, 0, batch, ws_bytes, &chunk);
     if (rc != KYM_OK) return rc;
+    if (dev_count && batch > 0x7fffffffll) return fail(KYM_ERR_VALUE, "maximum batch too large");
     hipGraphNode_t prev = nullptr;
     for (long long i0 = 0; i0 < batch; i0 += chunk) {
         const int nimg = (int)std::min(chunk, batch - i0);
@@ -528,6 +532,8 @@ static int add_plan_chain(hipGraph_t g, const Scat2DPlan<T>& p, const void* x, c
             Launch2DCall<T> c;
             rc = launch_call(p, l, 0, x, filt, out, nullptr, nullptr, p.ws_img, ws, i0, nimg, base, &c);
             if (rc != KYM_OK) return rc;
+            c.a.dev_count = dev_count;
+            c.a.max_count = (int)batch;
             void* args[1] = {&c.a};
             hipKernelNodeParams kp = {};
             kp.func = reinterpret_cast<void*>(c.k);
@@ -565,6 +571,35 @@ extern "C" int kym_scat2d_graph_create(void* const* plans, int nplans, const voi
                                               (uintptr_t)h->dblob)
                              : add_plan_chain(G->g, *h->pd, x[i], filt[i], out[i], batch[i], ws[i], ws_bytes[i],
                                               (uintptr_t)h->dblob);
+        if (rc != KYM_OK) return rc;
+    }
+    KYM_HIP_TRY(hipGraphInstantiate(&G->exec, G->g, nullptr, nullptr, 0));
+    *graph = G.release();
+    return KYM_OK;
+}
+
+extern "C" int kym_scat2d_graph_create_counted(void* const* plans, int nplans, const void* const* x,
+                                               const void* const* filt, void* const* out, const long long* max_batch,
+                                               void* const* ws, const long long* ws_bytes, const int* dev_counts,
+                                               void** graph) {
+    if (!graph) return fail(KYM_ERR_VALUE, "null graph pointer");
+    *graph = nullptr;
+    if (nplans < 1 || !plans || !x || !filt || !out || !max_batch || !ws || !ws_bytes || !dev_counts)
+        return fail(KYM_ERR_VALUE, "null argument");
+    auto G = std::make_unique<KymScat2DGraph>();
+    KYM_HIP_TRY(hipGetDevice(&G->device));
+    KYM_HIP_TRY(hipGraphCreate(&G->g, 0));
+    for (int i = 0; i < nplans; ++i) {
+        KymScat2D* h = static_cast<KymScat2D*>(plans[i]);
+        if (!h) return fail(KYM_ERR_VALUE, "null plan");
+        if (h->device != G->device) return fail(KYM_ERR_TYPE, "plan was created for another device");
+        if (max_batch[i] < 0) return fail(KYM_ERR_VALUE, "negative batch");
+        if (max_batch[i] == 0) continue;
+        if (!x[i] || !filt[i] || !out[i] || !ws[i]) return fail(KYM_ERR_VALUE, "null device pointer");
+        const int rc = h->pf ? add_plan_chain(G->g, *h->pf, x[i], filt[i], out[i], max_batch[i], ws[i], ws_bytes[i],
+                                              (uintptr_t)h->dblob, dev_counts + i)
+                             : add_plan_chain(G->g, *h->pd, x[i], filt[i], out[i], max_batch[i], ws[i], ws_bytes[i],
+                                              (uintptr_t)h->dblob, dev_counts + i);
         if (rc != KYM_OK) return rc;
     }
     KYM_HIP_TRY(hipGraphInstantiate(&G->exec, G->g, nullptr, nullptr, 0));

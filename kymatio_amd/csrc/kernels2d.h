@@ -15,7 +15,7 @@ template <class T, int KIND, int NT, int MAXP>
 __global__ void __launch_bounds__(NT, 2) k_scat2d(Scat2DArgs<T> a) {
     extern __shared__ __attribute__((aligned(16))) char kym_smem2d[];
     const int task = xcd_task(blockIdx.x, gridDim.x);
-    if (task >= a.ntask) return;
+    if (task >= live_tasks(a)) return;
     cpx<T>* lds = reinterpret_cast<cpx<T>*>(kym_smem2d);
     if constexpr (KIND >= K2B_NODE) dispatch_bwd2d<T, MAXP>(KIND, a, task, lds, threadIdx.x, NT);
     else dispatch_body2d<T, MAXP>(KIND, a, task, lds, threadIdx.x, NT);
@@ -25,7 +25,7 @@ template <class T, int SPEC, int NT>
 __global__ void __launch_bounds__(NT) k_node2d_spec(Scat2DArgs<T> a) {
     extern __shared__ __attribute__((aligned(16))) char kym_smem2d_ns[];
     const int task = xcd_task(blockIdx.x, gridDim.x);
-    if (task >= a.ntask) return;
+    if (task >= live_tasks(a)) return;
     node_spec_body<T, SPEC, false>(a, task, reinterpret_cast<cpx<T>*>(kym_smem2d_ns), threadIdx.x);
 }
 
@@ -33,7 +33,7 @@ template <class T, int KIND, int SPEC, int NT>
 __global__ void __launch_bounds__(NT, 2) k_pass2d_spec(Scat2DArgs<T> a) {
     extern __shared__ __attribute__((aligned(16))) char kym_smem2d_ps[];
     const int task = xcd_task(blockIdx.x, gridDim.x);
-    if (task >= a.ntask) return;
+    if (task >= live_tasks(a)) return;
     cpx<T>* lds = reinterpret_cast<cpx<T>*>(kym_smem2d_ps);
     if constexpr (KIND >= K2B_NODE) bwd_pass_spec_body<T, KIND, SPEC, false>(a, task, lds, threadIdx.x);
     else pass_spec_body<T, KIND, SPEC, false>(a, task, lds, threadIdx.x);

@@ -104,7 +104,28 @@ struct Scat2DArgs {
     int save;        // forward: store u into each node's save slot
     const T* dS;     // backward: dL/dS, same layout as out
     T* gx;           // backward: dL/dx, same layout as x
+    // counted graphs: when dev_count is set, the image count n = *dev_count
+    // is read at run time (clamped to max_count) and this chunk, which starts
+    // at image img0 and was launched for up to ntask / (ngroup nblk) images,
+    // runs only images img0 .. n - 1 of it (live_tasks)
+    const int* dev_count;
+    int max_count;
 };
+
+// Tasks of this launch that have work: ntask, or with a run-time image count
+// the tasks of the chunk's images below that count (the rest exit at once).
+// `a` is not modified: kernels pass their argument on by reference.
+template <class T>
+KYM_HD KYM_INL int live_tasks(const Scat2DArgs<T>& a) {
+    if (!a.dev_count) return a.ntask;
+    int n = *a.dev_count;
+    n = n < 0 ? 0 : (n > a.max_count ? a.max_count : n);
+    const int per = a.ngroup * a.nblk;
+    const int cap = a.ntask / per;
+    int live = n - a.img0;
+    live = live < 0 ? 0 : (live > cap ? cap : live);
+    return live * per;
+}
 
 // ------------------------------------------------------------------ helpers
 template <class T>

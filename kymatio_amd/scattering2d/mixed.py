@@ -7,9 +7,15 @@ J = log2(side) - 3 so every size gives the same 8 x 8 output grid) and runs a
 whole mixed batch as ONE replay of a HIP graph (kym_scat2d_graph_*): each
 size's launch chain is an independent branch, so the sizes share the GPU
 instead of running one after another.  Plans are autotuned per (size, count)
-on first use (scattering2d/autotune.py).  The graph is bound to fixed device
-buffers, built once per combination of per-size counts; inputs are copied
-into them and outputs copied out on every call.
+on first use (scattering2d/autotune.py).
+
+The graph is laid out for a maximum image count per size
+(kym_scat2d_graph_create_counted) and its kernels read the actual counts
+from a small device array at every replay, so batches with other counts
+replay the same graph: it is rebuilt only when a size is new or its count
+exceeds the maximum (which then grows).  The graph is bound to fixed device
+buffers of the maximum sizes; inputs are copied into them and outputs copied
+out on every call.
 """
 import ctypes
 import math
@@ -36,8 +42,10 @@ class MixedScattering2D:
         self.tune_cache = tune_cache
         self.ws_budget = ws_budget
         self._mods = {}
-        self._state = None  # (signature, graph handle, per-size buffers)
+        self._state = None  # (per-size maximum counts, graph handle, per-size buffers)
         self.tuning = {}    # side -> chosen plan parameters
+        self.builds = 0     # graphs built so far (a replay with new counts within the maxima builds none)
+        self._counts = None
 
     def module(self, side):
         """The ScatteringTorch2D used for square images of this side."""
@@ -59,13 +67,18 @@ class MixedScattering2D:
         except Exception:
             pass
 
-    def _build(self, counts):
+    @staticmethod
+    def _headroom(n):
+        """Maximum count the graph is laid out for when a size first needs n."""
+        return max(n + 16, -(-5 * n // 4))
+
+    def _build(self, maxima):
         self._release()
         lib = _lib.lib()
-        sides = sorted(counts)
+        sides = sorted(maxima)
         bufs = []
         for side in sides:
-            n = counts[side]
+            n = maxima[side]
             m = self.module(side)
             if self.autotune:
                 choice, _ = _autotune.tune(m, n, cache_path=self.tune_cache)
@@ -91,10 +104,13 @@ class MixedScattering2D:
         wss = arr([b['ws'].data_ptr() for b in bufs])
         batch = (ctypes.c_longlong * k)(*[b['n'] for b in bufs])
         wsb = (ctypes.c_longlong * k)(*[b['ws'].numel() for b in bufs])
+        self._counts = torch.zeros(k, dtype=torch.int32, device=self.device)
         g = ctypes.c_void_p()
         with torch.cuda.device(self.device):
-            _lib.check(lib.kym_scat2d_graph_create(plans, k, xs, fs, outs, batch, wss, wsb, ctypes.byref(g)))
-        self._state = (tuple((b['side'], b['n']) for b in bufs), g, bufs)
+            _lib.check(lib.kym_scat2d_graph_create_counted(plans, k, xs, fs, outs, batch, wss, wsb,
+                                                           P(self._counts.data_ptr()), ctypes.byref(g)))
+        self._state = ({b['side']: b['n'] for b in bufs}, g, bufs)
+        self.builds += 1
 
     def forward_grouped(self, groups):
         """groups: {side: (n, side, side) float32 tensor on the device} ->
@@ -108,16 +124,24 @@ class MixedScattering2D:
             _lib.require_device(x)
             if x.shape[0]:
                 counts[side] = x.shape[0]
-        sig = tuple(sorted(counts.items()))
-        if not sig:
+        if not counts:
             return {side: None for side in groups}
-        if self._state is None or self._state[0] != sig:
-            self._build(counts)
+        maxima = dict(self._state[0]) if self._state is not None else {}
+        if any(s not in maxima or n > maxima[s] for s, n in counts.items()):
+            for s, n in counts.items():
+                if s not in maxima or n > maxima[s]:
+                    maxima[s] = self._headroom(n)
+            self._build(maxima)
         _, g, bufs = self._state
+        live = []
         for b in bufs:
-            b['x'].copy_(groups[b['side']])
+            n = counts.get(b['side'], 0)
+            live.append(n)
+            if n:
+                b['x'][:n].copy_(groups[b['side']])
+        self._counts.copy_(torch.tensor(live, dtype=torch.int32), non_blocking=False)
         _lib.check(_lib.lib().kym_scat2d_graph_launch(g, _lib.stream_of(self.device)))
-        return {b['side']: b['out'].clone() for b in bufs}
+        return {b['side']: b['out'][:n].clone() for b, n in zip(bufs, live) if n}
 
     def __call__(self, images):
         """images: list of square (side, side) tensors (sides may differ) ->

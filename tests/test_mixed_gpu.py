@@ -28,13 +28,13 @@ def per_size():
 
 
 def test_mixed_matches_oracle_per_size():
-    """One image of every size of the mixed workload vs the oracle: sides 64
-    and 128 run the CPU oracle here, sides 256 and 512 compare with its
+    """Four images of every size of the mixed workload vs the oracle: sides
+    64 and 128 run the CPU oracle here, sides 256 and 512 compare with its
     outputs stored in tests/golden/golden_mixed2d.npz."""
     G = golden('golden_mixed2d.npz')
-    groups = _images({64: 1, 128: 1})
+    groups = _images({64: 4, 128: 4})
     for s in (256, 512):
-        x = np.random.default_rng(int(G['s%d_seed' % s])).random((1, s, s), dtype=np.float32)
+        x = np.random.default_rng(int(G['s%d_seed' % s])).random((4, s, s), dtype=np.float32)
         groups[s] = torch.from_numpy(x).to(DEV)
     mx = MixedScattering2D(autotune=False)
     out = mx.forward_grouped(groups)
@@ -42,9 +42,10 @@ def test_mixed_matches_oracle_per_size():
         if s in (256, 512):
             ref = G['s%d_Sx' % s]
         else:
-            ref = scat2d.Scattering2DTorchCPU(default_J(s), (s, s))(groups[s].cpu()).numpy()
+            with torch.no_grad():
+                ref = scat2d.Scattering2DTorchCPU(default_J(s), (s, s))(groups[s].cpu()).numpy()
         got = out[s].cpu().numpy()
-        assert got.shape == ref.shape == (1, {64: 217, 128: 417, 256: 681, 512: 1009}[s], 8, 8)
+        assert got.shape == ref.shape == (4, {64: 217, 128: 417, 256: 681, 512: 1009}[s], 8, 8)
         err = channel_rel_err(got, ref)
         print('side %d: per-channel rel err %.3e' % (s, err))
         assert err < RTOL_CHANNEL
@@ -58,7 +59,7 @@ def test_compiled_passes_large_sizes(tune):
     G = golden('golden_mixed2d.npz')
     for s in (256, 512):
         J = int(G['s%d_J' % s])
-        x = np.random.default_rng(int(G['s%d_seed' % s])).random((1, s, s), dtype=np.float32)
+        x = np.random.default_rng(int(G['s%d_seed' % s])).random((4, s, s), dtype=np.float32)
         sc = Scattering2D(J, (s, s)).to(DEV)
         sc.tune = list(tune)
         got = sc(torch.from_numpy(x).to(DEV)).cpu().numpy()
@@ -101,11 +102,36 @@ def test_list_interface_and_new_counts(per_size):
         for i in (0, 1, 4):
             ref = per_size[sides[i]](imgs[i][None])[0]
             assert torch.equal(out[i], ref)
-    # a different count combination rebuilds the graph
+    # fewer images of each size replay the same graph (no rebuild)
+    builds = mx.builds
     out2 = mx(imgs[:3])
-    assert torch.equal(out2[1], out[1])
+    assert mx.builds == builds
+    assert torch.equal(out2[1], out[1]) and torch.equal(out2[0], out[0])
     with pytest.raises(RuntimeError):
         mx([torch.zeros(64, 32, device=DEV)])
+
+
+def test_new_counts_replay_without_rebuild(per_size):
+    """Batches whose per-size counts differ (within the graph's maxima, and
+    with a size absent) replay one graph; each result equals the per-size
+    modules' bit for bit.  A count above a maximum rebuilds once."""
+    mx = MixedScattering2D(autotune=False)
+    batches = [{64: 6, 128: 5, 256: 3, 512: 2}, {64: 2, 128: 7, 256: 1, 512: 3}, {64: 9, 256: 4},
+               {64: 1, 128: 1, 256: 1, 512: 1}]
+    for i, counts in enumerate(batches):
+        groups = _images(counts, seed=20 + i)
+        out = mx.forward_grouped(groups)
+        assert mx.builds == 1, (i, mx.builds)
+        assert set(out) == set(counts)
+        with torch.no_grad():
+            for s, x in groups.items():
+                assert torch.equal(out[s], per_size[s](x)), (i, s)
+    grown = {64: 40, 512: 1}
+    groups = _images(grown, seed=30)
+    out = mx.forward_grouped(groups)
+    assert mx.builds == 2
+    with torch.no_grad():
+        assert torch.equal(out[64], per_size[64](groups[64]))
 
 
 def test_autotuned_mixed_workload(per_size):

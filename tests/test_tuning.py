@@ -125,4 +125,4 @@ def test_compiled_passes_at_other_sizes(emu, side):
     f = pack_filters_2d(filter_bank(Mp, Np, J, 8))
     out, info = emu.scat2d(x, J, (side, side), f, tune=[16, 0, 8, 8])
     assert info['spec_launches'] >= 10  # the big-node passes included
-    assert channel_rel_err(out, G['s%d_Sx' % side]) < RTOL_CHANNEL
+    assert channel_rel_err(out, G['s%d_Sx' % side][:1]) < RTOL_CHANNEL
