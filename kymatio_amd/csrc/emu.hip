@@ -261,6 +261,13 @@ static int emu_op_fft(const void* x, void* y, void* work, long long batch, int H
     if (!lt.emit(b, &lens_off, &err)) { g_emu_error = err; return KYM_ERR_UNSUPPORTED; }
     b.relocate((uintptr_t)b.bytes.data());
     const FftLen<T>* lens = reinterpret_cast<const FftLen<T>*>(b.bytes.data() + lens_off);
+    if (const long long m = fftop_multi_count(H, W, batch)) {
+        const long long nblk = (batch + m - 1) / m;
+        FftOpArgs o = {x, y, batch, H, W, kind, 0, (int)m, (int)nblk};
+        std::vector<cpx<T>> lds((size_t)(m * H) + 1);
+        for (long long blk = 0; blk < nblk; ++blk) fftop_multi_body<T>(o, lens, lH, blk, lds.data(), 0, 1);
+        return 0;
+    }
     FftOpStep steps[2];
     int nsteps = 0;
     const char* e = nullptr;

@@ -3,6 +3,7 @@
 // backend/torch_backend.py, kymatio/backend/torch_backend.py) as single HIP
 // kernels, for the plugin route and for op-level tests.  The fused pipelines
 // in capi.hip do not use these.
+#include <algorithm>
 #include <string>
 
 #include "../../include/kymatio_hip.h"
@@ -182,6 +183,12 @@ __global__ void __launch_bounds__(256) k_fft2_lds(FftOpArgs o, const FftLen<T>* 
 }
 
 template <class T>
+__global__ void __launch_bounds__(256) k_fft1_multi(FftOpArgs o, const FftLen<T>* lens, int lH) {
+    extern __shared__ __attribute__((aligned(16))) char kym_smem_op1[];
+    fftop_multi_body<T>(o, lens, lH, blockIdx.x, reinterpret_cast<cpx<T>*>(kym_smem_op1), threadIdx.x, blockDim.x);
+}
+
+template <class T>
 __global__ void __launch_bounds__(256) k_fft2_pass(FftOpArgs o, FftPassIO io, const FftLen<T>* lens, int lH, int lW,
                                                    int axis) {
     extern __shared__ __attribute__((aligned(16))) char kym_smem_op2[];
@@ -246,6 +253,16 @@ static int fft_op(const void* x, void* y, void* work, long long batch, int H, in
     const FftLen<T>* lens = reinterpret_cast<const FftLen<T>*>((char*)t.dev + t.lens_off);
     size_t lim = op_lds_limit();
     if (lds_cap > 0 && (size_t)lds_cap < lim) lim = (size_t)lds_cap;
+    // short 1D transforms: several per workgroup, about 4096 points each
+    if (const long long m = fftop_multi_count(H, W, batch)) {
+        const long long nblk = (batch + m - 1) / m;
+        if (nblk > 0x7fffffffll) return kym_ops_fail(KYM_ERR_VALUE, "batch too large");
+        FftOpArgs o = {x, y, batch, H, W, kind, 0, (int)m, (int)nblk};
+        hipLaunchKernelGGL(k_fft1_multi<T>, dim3((unsigned)nblk), dim3(256), (size_t)(m * H) * sizeof(cpx<T>), s, o,
+                           lens, t.lH);
+        OPS_HIP_TRY(hipGetLastError());
+        return KYM_OK;
+    }
     FftOpStep steps[2];
     int nsteps = 0;
     const char* err = nullptr;

@@ -62,6 +62,52 @@ KYM_HD void fftop_lds_body(const FftOpArgs& o, const FftLen<T>* lens, int lH, in
     }
 }
 
+// Transforms per workgroup of the multi-transform route for (H, W) over a
+// batch: about 4096 points per workgroup for 1D lengths up to 1024; 0 where
+// the one-transform-per-workgroup route applies.
+inline long long fftop_multi_count(int H, int W, long long batch) {
+    if (W != 1 || H > 1024 || H < 1 || batch < 1) return 0;
+    const long long per = 4096 / H;
+    return per < batch ? per : batch;
+}
+
+// Short 1D transforms (W = 1), several per workgroup: transforms
+// b0 .. b0 + m - 1 (m = o.nrow, fewer at the end of the batch) sit one after
+// the other in LDS and run as one batch of m rows, so a workgroup keeps its
+// threads busy on lengths far below its thread count (the frequency axis of
+// time-frequency scattering: tens of points, batches of 10^5).
+template <class T>
+KYM_HD void fftop_multi_body(const FftOpArgs& o, const FftLen<T>* lens, int lH, long long blk, cpx<T>* lds, int tid,
+                             int nth) {
+    const FftLen<T>& f = lens[lH];
+    const int n = o.H;
+    const long long b0 = blk * o.nrow;
+    const int m = (int)((o.batch - b0 < o.nrow) ? o.batch - b0 : o.nrow);
+    const bool inv = o.kind == 1 || o.kind == 3;
+    const int total = m * n;
+    for (int q = tid; q < total; q += nth) {
+        const int t = q / n, p = q - t * n;
+        // forward (DIT) wants digit-reversed storage; inverse (DIF) natural
+        const long long src = (b0 + t) * n + (inv ? p : f.rev[p]);
+        if (o.kind == 2) lds[q] = {static_cast<const T*>(o.x)[src], T(0)};
+        else lds[q] = static_cast<const cpx<T>*>(o.x)[src];
+    }
+    wg_barrier();
+    if (inv) {
+        fft_dif<true, 31>(lds, f, Batch{m, 1, n, false}, tid, nth);
+        const T inv_n = T(1) / T(n);
+        for (int q = tid; q < total; q += nth) {
+            const int t = q / n, p = q - t * n;
+            const long long dst = (b0 + t) * n + f.rev[p];
+            if (o.kind == 3) static_cast<T*>(o.y)[dst] = lds[q].re * inv_n;
+            else static_cast<cpx<T>*>(o.y)[dst] = lds[q] * inv_n;
+        }
+    } else {
+        fft_dit<false, 31>(lds, f, Batch{m, 1, n, false}, tid, nth);
+        for (int q = tid; q < total; q += nth) static_cast<cpx<T>*>(o.y)[b0 * n + q] = lds[q];
+    }
+}
+
 // Pass kernels for transforms that do not fit: axis 1 = rows (along W),
 // axis 0 = columns (along H).  Each task reads whole lines of `src` into LDS,
 // transforms them and writes them to `dst` (complex unless dst_real; src

@@ -1,6 +1,8 @@
 """1D HIP backend namespace (plugin name 'torch', the default): op names and
 checks of kymatio/scattering1d/backend/torch_backend.py:18-148, each op one
 HIP kernel on GPU tensors.  The frontend runs the fused pipeline."""
+import ctypes
+
 import torch
 
 from ... import _lib
@@ -78,9 +80,23 @@ class TorchBackend1D(TorchBackend):
 
     @classmethod
     def average_global(cls, x):
+        """Sum over time, keeping the axis ((..., T, 1) -> (..., 1, 1)), as
+        torch.sum(x, -2, keepdims=True) of the reference; one HIP kernel
+        (kym_op_integrals with the power 1, FP64 accumulation)."""
         cls.contiguous_check(x)
         cls.real_check(x)
-        return torch.sum(x, axis=-2, keepdims=True)
+        _gpu(x)
+        T = x.shape[-2]
+        rows = x.numel() // max(T, 1)
+        out = torch.empty(x.shape[:-2] + (1, 1), dtype=x.dtype, device=x.device)
+        if rows and T:
+            one = (ctypes.c_double * 1)(1.0)
+            with torch.cuda.device(x.device):
+                _lib.check(_lib.lib().kym_op_integrals(_ptr(x), _ptr(out), rows, T, one, 1, _dt(x),
+                                                        _lib.stream_of(x.device)))
+        elif rows:
+            out.zero_()
+        return out
 
 
     # ---- frequency-axis layout ops of time-frequency scattering

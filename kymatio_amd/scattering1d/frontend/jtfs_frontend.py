@@ -19,6 +19,7 @@ import numpy as np
 import torch
 
 from ...frontend.base_frontend import ScatteringBase
+from ...graphs import GraphCache
 from ..core.timefrequency_scattering import jtfs_average_and_format, joint_timefrequency_scattering
 from ..filter_bank import anden_generator, scattering_filter_factory, spin
 from ..utils import parse_T
@@ -40,6 +41,10 @@ class TimeFrequencyScatteringTorch1D(ScatteringTorch1D):
         ScatteringTorch1D.__init__(self, J, shape, Q, T, stride, max_order=2, oversampling=None, out_type=out_type,
                                    backend=backend)
         self._fr_dev = {}
+        # inference batches replay one captured HIP graph per input signature
+        # (the transform is a few thousand small launches); False runs eagerly
+        self.graphs = True
+        self._graph_cache = GraphCache()
 
     # ------------------------------------------------------------ setup
     def build(self):
@@ -165,6 +170,14 @@ class TimeFrequencyScatteringTorch1D(ScatteringTorch1D):
             raise TypeError('Input and filter must be of the same dtype.')
         if not x.is_cuda:
             raise TypeError('TimeFrequencyScattering runs on the GPU (HIP); got a CPU tensor.')
+        if self.graphs and not (torch.is_grad_enabled() and x.requires_grad):
+            key = (tuple(x.shape), x.dtype, str(x.device), self.out_type, self.format,
+                   tuple((b.data_ptr(), b._version) for b in bufs))
+            return self._graph_cache(key, self._transform, x.contiguous())
+        return self._transform(x)
+
+    def _transform(self, x):
+        """The transform of a checked input: JTFS generator over the HIP ops."""
         B = self.backend
         batch_shape, signal_shape = x.shape[:-1], x.shape[-1:]
         x = B.reshape_input(x.contiguous(), signal_shape)

@@ -67,6 +67,23 @@ def test_op_fft(emu, H, W, cap, kind):
     assert np.abs(got - ref).max() <= 1e-12 * max(np.abs(ref).max(), 1)
 
 
+@pytest.mark.parametrize('n,B', [(88, 100), (1024, 9), (5, 3000), (136, 31)])
+@pytest.mark.parametrize('kind', [0, 1, 2, 3])
+def test_op_fft_1d_several_per_workgroup(emu, n, B, kind):
+    """Short 1D transforms run several per workgroup (the last workgroup
+    partly filled): every kind against numpy FP64."""
+    rng = np.random.default_rng(n + B)
+    x = rng.standard_normal((B, n, 1)) + (0 if kind == 2 else 1j) * rng.standard_normal((B, n, 1))
+    got = _run(emu, x, n, 1, kind, 160 * 1024)
+    if kind in (0, 2):
+        ref = np.fft.fft(x, axis=1)
+    elif kind == 1:
+        ref = np.fft.ifft(x, axis=1)
+    else:
+        ref = np.fft.ifft(x, axis=1).real
+    assert np.abs(got - ref).max() <= 1e-12 * max(np.abs(ref).max(), 1)
+
+
 def test_real_inverse_two_pass_needs_work(emu):
     x = np.ones((1, 68, 72)) + 0j
     with pytest.raises(RuntimeError, match='work buffer'):

@@ -109,3 +109,36 @@ def test_numpy_and_sklearn_frontends():
     _close(Sx, G['time_Sx'])
     t = SkJTFS(**_kw('time')).transform(x)
     assert np.allclose(t, Sx.reshape(x.shape[0], -1))
+
+
+def _flat(Sx, B):
+    if isinstance(Sx, dict):
+        return torch.cat([v.reshape(B, -1) for v in Sx.values()], dim=1)
+    if isinstance(Sx, list):
+        return torch.cat([p['coef'].reshape(B, -1) for p in Sx], dim=1)
+    return Sx.reshape(B, -1)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('tag', TAGS)
+def test_graph_replays_equal_eager(tag):
+    """The captured-graph route (default for inference) against the eager
+    launches, bit for bit, over several replays with other GPU work and
+    synchronisations in between, and with a second input."""
+    S = TimeFrequencyScattering(**_kw(tag)).cuda()
+    x = torch.from_numpy(G[tag + '_x']).cuda()
+    B = x.shape[0]
+    with torch.no_grad():
+        S.graphs = False
+        eager = _flat(S(x), B)
+        eager2 = _flat(S(x * 0.5 + 1.0), B)
+        S.graphs = True
+        for rep in range(3):
+            junk = torch.randn(1 << 20, device='cuda')
+            (junk * 3).sum().item()
+            torch.cuda.synchronize()
+            got = _flat(S(x), B)
+            assert torch.equal(got, eager), (tag, rep)
+            got2 = _flat(S(x * 0.5 + 1.0), B)
+            assert torch.equal(got2, eager2), (tag, rep)
+    assert S._graph_cache.captures == 1
