@@ -261,6 +261,17 @@ int kym_op_unpad2d(const void* x, void* y, long long batch, int H, int W, int dt
  * (status 3 if it is then null); may be null otherwise. */
 int kym_op_fft(const void* x, void* y, void* work, long long batch, int H, int W, int kind, int dtype,
                long long lds_limit, void* stream);
+/* The filter-periodise-invert step of the 1D and time-frequency paths in one
+ * kernel: y[b] = ifft_n(subsample_fourier(cdgmm(x[b], f), k)), n = N / k,
+ * for x (batch, N) complex and a real filter f (N) -- the reference's
+ * ifft(subsample_fourier(cdgmm(...))) and irfft(...) of
+ * kymatio/scattering1d/backend/torch_backend.py:18-48, 108-141 with
+ * kymatio/backend/torch_backend.py:147-219, as used by
+ * kymatio/scattering1d/core/timefrequency_scattering.py.  mode 0: y (batch,
+ * n) complex; 1: its real part (irfft); 2: its modulus, y (batch, n) real.
+ * Status 2 if k does not divide N. */
+int kym_op_filter_ifft(const void* x, const void* f, void* y, long long batch, long long N, int k, int mode, int dtype,
+                       void* stream);
 /* modulus_rotation (kymatio/scattering3d/backend/torch_backend.py:97-124):
  * y = sqrt(prev^2 + |x|^2) elementwise over n complex x; prev may be null
  * (the first m of a solid-harmonic order l). */

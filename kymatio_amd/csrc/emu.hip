@@ -286,6 +286,32 @@ static int emu_op_fft(const void* x, void* y, void* work, long long batch, int H
     return 0;
 }
 
+template <class T>
+static int emu_filter_ifft(const void* x, const void* f, void* y, long long batch, long long N, int k, int mode) {
+    if (k < 1 || N % k) { g_emu_error = "subsampling factor must divide the size"; return KYM_ERR_RUNTIME; }
+    const int n = (int)(N / k);
+    LenTable<T> lt;
+    const int l = lt.add(n);
+    Blob b;
+    size_t lens_off = 0;
+    std::string err;
+    if (!lt.emit(b, &lens_off, &err)) { g_emu_error = err; return KYM_ERR_UNSUPPORTED; }
+    b.relocate((uintptr_t)b.bytes.data());
+    const FftLen<T>* lens = reinterpret_cast<const FftLen<T>*>(b.bytes.data() + lens_off);
+    int nt = 256, m = 1, split = 1;
+    const long long elems = filter_ifft_shape(n, k, batch, &nt, &m, &split);
+    FilterIfftArgs o = {x, f, y, batch, (int)N, k, n, mode, m, split};
+    std::vector<cpx<T>> lds((size_t)elems + 1);
+    for (long long blk = 0; blk < (batch + m - 1) / m; ++blk) filter_ifft_body<T>(o, lens[l], blk, lds.data(), 0, 1);
+    return 0;
+}
+
+extern "C" int kym_emu_filter_ifft(const void* x, const void* f, void* y, long long batch, long long N, int k,
+                                   int mode, int dtype) {
+    if (dtype == 0) return emu_filter_ifft<float>(x, f, y, batch, N, k, mode);
+    return emu_filter_ifft<double>(x, f, y, batch, N, k, mode);
+}
+
 extern "C" int kym_emu_op_fft(const void* x, void* y, void* work, long long batch, int H, int W, int kind, int dtype,
                               long long lds_cap) {
     if (dtype == 0) return emu_op_fft<float>(x, y, work, batch, H, W, kind, lds_cap);

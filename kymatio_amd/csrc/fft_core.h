@@ -25,7 +25,8 @@ struct FftLen {
     int nrad;
     int rad[KYM_MAX_RADICES];  // DIT order (r_1 first)
     const cpx<T>* tw;          // n entries
-    const int* rev;            // n entries
+    const int* rev;            // n entries: storage position -> natural index
+    const int* irev;           // n entries: natural index -> storage position (inverse of rev)
 };
 
 // ---------------------------------------------------------------- butterflies

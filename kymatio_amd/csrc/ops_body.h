@@ -42,13 +42,32 @@ KYM_HD void fftop_lds_body(const FftOpArgs& o, const FftLen<T>* lens, int lH, in
     const FftLen<T>& fW = lens[lW];
     const int H = o.H, W = o.W, n = H * W;
     const bool inv = o.kind == 1 || o.kind == 3;
-    if (inv) load_2d<T, true>(o, fH, fW, b, lds, tid, nth);
-    else load_2d<T, false>(o, fH, fW, b, lds, tid, nth);
+    if (W == 1 && !inv) {
+        // 1D forward: coalesced reads, each value placed at its digit-reversed slot
+        for (int p = tid; p < n; p += nth) {
+            const long long src = b * n + p;
+            lds[fH.irev[p]] = o.kind == 2 ? cpx<T>{static_cast<const T*>(o.x)[src], T(0)}
+                                          : static_cast<const cpx<T>*>(o.x)[src];
+        }
+    } else if (inv) {
+        load_2d<T, true>(o, fH, fW, b, lds, tid, nth);
+    } else {
+        load_2d<T, false>(o, fH, fW, b, lds, tid, nth);
+    }
     wg_barrier();
     if (inv) {
         if (W > 1) fft_dif<true, 31>(lds, fW, Batch{H, 1, W, false}, tid, nth);
         fft_dif<true, 31>(lds, fH, Batch{W, W, 1, true}, tid, nth);
         const T inv_n = T(1) / T(n);
+        if (W == 1) {
+            // 1D inverse: coalesced writes, each read from its digit-reversed slot
+            for (int p = tid; p < n; p += nth) {
+                const cpx<T> v = lds[fH.irev[p]];
+                if (o.kind == 3) static_cast<T*>(o.y)[b * n + p] = v.re * inv_n;
+                else static_cast<cpx<T>*>(o.y)[b * n + p] = v * inv_n;
+            }
+            return;
+        }
         for (int p = tid; p < n; p += nth) {
             const int py = p / W, px = p - (p / W) * W;
             const long long dst = b * n + (long long)fH.rev[py] * W + fW.rev[px];
@@ -85,12 +104,15 @@ KYM_HD void fftop_multi_body(const FftOpArgs& o, const FftLen<T>* lens, int lH, 
     const int m = (int)((o.batch - b0 < o.nrow) ? o.batch - b0 : o.nrow);
     const bool inv = o.kind == 1 || o.kind == 3;
     const int total = m * n;
+    // reads and writes walk the rows in natural order (coalesced); the
+    // forward (DIT) stores each value at its digit-reversed slot, the inverse
+    // (DIF) reads its results from there
     for (int q = tid; q < total; q += nth) {
         const int t = q / n, p = q - t * n;
-        // forward (DIT) wants digit-reversed storage; inverse (DIF) natural
-        const long long src = (b0 + t) * n + (inv ? p : f.rev[p]);
-        if (o.kind == 2) lds[q] = {static_cast<const T*>(o.x)[src], T(0)};
-        else lds[q] = static_cast<const cpx<T>*>(o.x)[src];
+        const long long src = (b0 + t) * n + p;
+        const int slot = t * n + (inv ? p : f.irev[p]);
+        if (o.kind == 2) lds[slot] = {static_cast<const T*>(o.x)[src], T(0)};
+        else lds[slot] = static_cast<const cpx<T>*>(o.x)[src];
     }
     wg_barrier();
     if (inv) {
@@ -98,14 +120,100 @@ KYM_HD void fftop_multi_body(const FftOpArgs& o, const FftLen<T>* lens, int lH, 
         const T inv_n = T(1) / T(n);
         for (int q = tid; q < total; q += nth) {
             const int t = q / n, p = q - t * n;
-            const long long dst = (b0 + t) * n + f.rev[p];
-            if (o.kind == 3) static_cast<T*>(o.y)[dst] = lds[q].re * inv_n;
-            else static_cast<cpx<T>*>(o.y)[dst] = lds[q] * inv_n;
+            const cpx<T> v = lds[t * n + f.irev[p]];
+            if (o.kind == 3) static_cast<T*>(o.y)[b0 * n + q] = v.re * inv_n;
+            else static_cast<cpx<T>*>(o.y)[b0 * n + q] = v * inv_n;
         }
     } else {
         fft_dit<false, 31>(lds, f, Batch{m, 1, n, false}, tid, nth);
         for (int q = tid; q < total; q += nth) static_cast<cpx<T>*>(o.y)[b0 * n + q] = lds[q];
     }
+}
+
+// Filter, periodise and inverse-transform in one pass (1D):
+//   y[b] = ifft_n(P_k(x[b] * f)),  n = N / k,  P_k(v)[i] = (1/k) sum_a v[i + a n]
+// for rows b of x (batch, N) complex and a real filter f (N); the result is
+// kept complex (mode 0), its real part (mode 1) or its modulus (mode 2).  The
+// reference's ifft(subsample_fourier(cdgmm(x, f), k)) (and irfft / modulus of
+// it) as three ops and two intermediate tensors; here the periodised product
+// is formed while loading (natural order, as the inverse DIF wants it) and
+// o.nrow rows share a workgroup (the last one may hold fewer).
+struct FilterIfftArgs {
+    const void* x;
+    const void* f;
+    void* y;
+    long long batch;
+    int N, k, n, mode;
+    int nrow;    // rows per workgroup
+    int split;   // alias subsets summed by separate threads (a power of two dividing k)
+};
+
+// LDS: the m x n transform buffer, then (split > 1) split x m x n partial sums.
+// Thread q < split m n sums the aliases a = s, s + split, ... (s = q / (m n))
+// of point q mod (m n); the partials are added in the order s = 0, 1, ...
+template <class T>
+KYM_HD void filter_ifft_body(const FilterIfftArgs& o, const FftLen<T>& fl, long long blk, cpx<T>* lds, int tid,
+                             int nth) {
+    const int n = o.n, N = o.N, k = o.k, S = o.split;
+    const long long b0 = blk * o.nrow;
+    const int m = (int)((o.batch - b0 < o.nrow) ? o.batch - b0 : o.nrow);
+    const cpx<T>* x = static_cast<const cpx<T>*>(o.x);
+    const T* f = static_cast<const T*>(o.f);
+    const T inv_k = T(1) / T(k);
+    const int total = m * n;
+    cpx<T>* part = lds + (long long)o.nrow * n;
+    for (int q = tid; q < S * total; q += nth) {
+        const int s = q / total, e = q - s * total;
+        const int t = e / n, p = e - t * n;
+        const cpx<T>* xr = x + (b0 + t) * N + p;
+        cpx<T> acc = {T(0), T(0)};
+        for (int a = s; a < k; a += S) {
+            const T w = f[p + a * n];
+            const cpx<T> v = xr[a * n];
+            acc.re += v.re * w;
+            acc.im += v.im * w;
+        }
+        if (S == 1) lds[e] = acc * inv_k;
+        else part[q] = acc;
+    }
+    if (S > 1) {
+        wg_barrier();
+        for (int e = tid; e < total; e += nth) {
+            cpx<T> acc = part[e];
+            for (int s = 1; s < S; ++s) acc = acc + part[s * total + e];
+            lds[e] = acc * inv_k;
+        }
+    }
+    wg_barrier();
+    fft_dif<true, 31>(lds, fl, Batch{m, 1, n, false}, tid, nth);
+    const T inv_n = T(1) / T(n);
+    for (int q = tid; q < total; q += nth) {  // coalesced writes from the digit-reversed slots
+        const int t = q / n, p = q - t * n;
+        const long long dst = b0 * n + q;
+        const cpx<T> v = lds[t * n + fl.irev[p]];
+        if (o.mode == 0) static_cast<cpx<T>*>(o.y)[dst] = v * inv_n;
+        else if (o.mode == 1) static_cast<T*>(o.y)[dst] = v.re * inv_n;
+        else static_cast<T*>(o.y)[dst] = kym_sqrt(v.re * v.re + v.im * v.im) * inv_n;
+    }
+}
+
+// Launch shape of filter_ifft for output length n over `batch` rows: threads
+// (1024 for n >= 2048, else 256), rows per workgroup (about 4096 points, but
+// fewer when that leaves under 512 workgroups) and the alias split (threads
+// beyond the points sum alias subsets).  Returns the complex elements of LDS.
+inline long long filter_ifft_shape(int n, int k, long long batch, int* threads, int* rows, int* split) {
+    const int nt = n >= 2048 ? 1024 : 256;
+    long long m = n >= 4096 ? 1 : 4096 / n;
+    const long long want = (batch + 511) / 512;
+    if (m > want) m = want;
+    if (m > batch) m = batch;
+    if (m < 1) m = 1;
+    int S = 1;
+    while (S * 2 <= k && (long long)S * 2 * m * n <= nt) S *= 2;
+    *threads = nt;
+    *rows = (int)m;
+    *split = S;
+    return m * n * (S > 1 ? 1 + S : 1);
 }
 
 // Pass kernels for transforms that do not fit: axis 1 = rows (along W),

@@ -102,7 +102,7 @@ struct LenTable {
     // Writes FftLen array + tables into the blob; returns offset of the array.
     bool emit(Blob& b, size_t* lens_off, std::string* err) const {
         std::vector<FftLen<T>> fl(ns.size());
-        std::vector<size_t> tw_off(ns.size()), rev_off(ns.size());
+        std::vector<size_t> tw_off(ns.size()), rev_off(ns.size()), irev_off(ns.size());
         for (size_t i = 0; i < ns.size(); ++i) {
             const int n = ns[i];
             FftLen<T>& f = fl[i];
@@ -113,21 +113,27 @@ struct LenTable {
             }
             tw_off[i] = b.add(sizeof(cpx<T>) * n);
             rev_off[i] = b.add(sizeof(int) * n);
+            irev_off[i] = b.add(sizeof(int) * n);
             std::vector<double> re(n), im(n);
             kym_twiddles(n, re.data(), im.data());
             cpx<T>* tw = b.at<cpx<T>>(tw_off[i]);
             for (int t = 0; t < n; ++t) tw[t] = {(T)re[t], (T)im[t]};
             kym_rev_map(n, f.rad, f.nrad, b.at<int>(rev_off[i]));
+            const int* rv = b.at<int>(rev_off[i]);
+            int* iv = b.at<int>(irev_off[i]);
+            for (int t = 0; t < n; ++t) iv[rv[t]] = t;
         }
         *lens_off = b.add(sizeof(FftLen<T>) * ns.size());
         FftLen<T>* dst = b.at<FftLen<T>>(*lens_off);
         for (size_t i = 0; i < ns.size(); ++i) {
             fl[i].tw = reinterpret_cast<const cpx<T>*>(tw_off[i]);
             fl[i].rev = reinterpret_cast<const int*>(rev_off[i]);
+            fl[i].irev = reinterpret_cast<const int*>(irev_off[i]);
             dst[i] = fl[i];
             const size_t rec = *lens_off + i * sizeof(FftLen<T>);
             b.reloc.push_back(rec + offsetof(FftLen<T>, tw));
             b.reloc.push_back(rec + offsetof(FftLen<T>, rev));
+            b.reloc.push_back(rec + offsetof(FftLen<T>, irev));
         }
         return true;
     }

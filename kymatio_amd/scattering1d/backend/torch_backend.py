@@ -99,6 +99,28 @@ class TorchBackend1D(TorchBackend):
         return out
 
 
+    @classmethod
+    def filter_ifft(cls, x, filt, k, mode='complex'):
+        """ifft(subsample_fourier(cdgmm(x, filt), k)) in one HIP kernel
+        (kym_op_filter_ifft): x (..., N, 2) complex, filt (N, 1) real ->
+        (..., N / k, 2) complex, or its real part / modulus as (..., N / k, 1)
+        with mode 'real' / 'modulus'."""
+        cls.complex_check(x)
+        cls.contiguous_check(x)
+        _gpu(x)
+        N = x.shape[-2]
+        if N % k:
+            raise RuntimeError('subsampling factor must divide the size')
+        code = {'complex': 0, 'real': 1, 'modulus': 2}[mode]
+        f = filt.contiguous()
+        if f.dtype != x.dtype or f.numel() != N:
+            raise TypeError('Input and filter must be of the same dtype and length.')
+        y = torch.empty(x.shape[:-2] + (N // k, 2 if code == 0 else 1), dtype=x.dtype, device=x.device)
+        with torch.cuda.device(x.device):
+            _lib.check(_lib.lib().kym_op_filter_ifft(_ptr(x), _ptr(f), _ptr(y), x.numel() // (2 * N), N, k, code,
+                                                      _dt(x), _lib.stream_of(x.device)))
+        return y
+
     # ---- frequency-axis layout ops of time-frequency scattering
     # (kymatio/scattering1d/backend/torch_backend.py:150-223); the axis n1
     # sits before time, tensors end in a real/imag (or singleton) axis.

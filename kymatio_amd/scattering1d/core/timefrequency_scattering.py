@@ -20,13 +20,27 @@ def _time_filter(backend, x_hat, filt, level, k):
     return backend.subsample_fourier(backend.cdgmm(x_hat, filt['levels'][level]), 2 ** k)
 
 
+def _filtered(backend, x_hat, filt, level, k, mode):
+    """ifft (mode 'complex'), irfft ('real') or modulus of ifft ('modulus') of
+    _time_filter(x_hat, filt, level, k): one fused op where the backend has
+    it (the HIP backend's filter_ifft), else the reference's op sequence."""
+    fused = getattr(backend, 'filter_ifft', None)
+    if fused is not None:
+        return fused(x_hat, filt['levels'][level], 2 ** k, mode)
+    y = _time_filter(backend, x_hat, filt, level, k)
+    if mode == 'real':
+        return backend.irfft(y)
+    y = backend.ifft(y)
+    return backend.modulus(y) if mode == 'modulus' else y
+
+
 def time_scattering_widthfirst(U_0, backend, filters, log2_stride, average_local):
     """Zeroth order, then S_1 stacked over n1 (batch, n1, time), then for each
     n2 the complex Y_2 stacked over the n1 with j1 < j2 (width first)."""
     phi, psi1, psi2 = filters
     U_0_hat = backend.rfft(U_0)
     if average_local:
-        yield {'coef': backend.irfft(_time_filter(backend, U_0_hat, phi, 0, log2_stride)), 'j': (), 'n': ()}
+        yield {'coef': _filtered(backend, U_0_hat, phi, 0, log2_stride, 'real'), 'j': (), 'n': ()}
     else:
         yield {'coef': U_0, 'j': (), 'n': ()}
 
@@ -34,10 +48,10 @@ def time_scattering_widthfirst(U_0, backend, filters, log2_stride, average_local
     for n1, p1 in enumerate(psi1):
         j1 = p1['j']
         k1 = min(j1, log2_stride) if average_local else j1
-        U_1 = backend.modulus(backend.ifft(_time_filter(backend, U_0_hat, p1, 0, k1)))
+        U_1 = _filtered(backend, U_0_hat, p1, 0, k1, 'modulus')
         U_1_hat = backend.rfft(U_1)
         U_1_hats.append((j1, k1, U_1_hat))
-        S_1.append(backend.irfft(_time_filter(backend, U_1_hat, phi, k1, log2_stride - k1)))
+        S_1.append(_filtered(backend, U_1_hat, phi, k1, log2_stride - k1, 'real'))
     yield {'coef': backend.stack(S_1, 2), 'j': (-1,), 'n': (-1,), 'n1_max': len(S_1)}
 
     for n2, p2 in enumerate(psi2):
@@ -46,7 +60,7 @@ def time_scattering_widthfirst(U_0, backend, filters, log2_stride, average_local
         for j1, k1, U_1_hat in U_1_hats:
             if j2 > j1:
                 k2 = min(j2 - k1, log2_stride) if average_local else j2 - k1
-                Y_2.append(backend.ifft(_time_filter(backend, U_1_hat, p2, k1, k2)))
+                Y_2.append(_filtered(backend, U_1_hat, p2, k1, k2, 'complex'))
         if Y_2:
             yield {'coef': backend.stack(Y_2, 2), 'j': (-1, j2), 'n': (-1, n2), 'n1_max': len(Y_2)}
 
@@ -64,7 +78,7 @@ def frequency_scattering(X, backend, filters_fr, log2_stride_fr, average_local_f
     for n_fr, psi in enumerate(psis):
         j_fr = psi['j']
         k_fr = min(j_fr, log2_stride_fr) if average_local_fr else j_fr
-        Y = backend.swap_time_frequency(backend.ifft(_time_filter(backend, X_hat, psi, 0, k_fr)))
+        Y = backend.swap_time_frequency(_filtered(backend, X_hat, psi, 0, k_fr, 'complex'))
         yield {**X, 'coef': Y, 'n': X['n'][1:] + (n_fr,), 'j_fr': (j_fr,), 'n_fr': (n_fr,),
                'n1_stride': 2 ** j_fr, 'spin': np.sign(psi['xi'])}
 
@@ -84,7 +98,7 @@ def joint_timefrequency_scattering(U_0, backend, filters, log2_stride, average_l
 def time_averaging(U_2, backend, phi_f, log2_stride):
     k_in = U_2['j'][-1]
     U_hat = backend.rfft(U_2['coef'])
-    return {**U_2, 'coef': backend.irfft(_time_filter(backend, U_hat, phi_f, k_in, log2_stride - k_in))}
+    return {**U_2, 'coef': _filtered(backend, U_hat, phi_f, k_in, log2_stride - k_in, 'real')}
 
 
 def frequency_averaging(U_2, backend, phi_fr_f, log2_stride_fr, average_fr):
@@ -96,7 +110,7 @@ def frequency_averaging(U_2, backend, phi_fr_f, log2_stride_fr, average_fr):
         n1_stride = U_2['n1_max']
     else:  # 'local'
         k_in = min(U_2['j_fr'][-1], phi_fr_f['j'])
-        S_T = backend.irfft(_time_filter(backend, backend.rfft(U_T), phi_fr_f, k_in, log2_stride_fr - k_in))
+        S_T = _filtered(backend, backend.rfft(U_T), phi_fr_f, k_in, log2_stride_fr - k_in, 'real')
         n1_stride = 2 ** log2_stride_fr
     return {**U_2, 'coef': backend.swap_time_frequency(S_T), 'n1_stride': n1_stride}
 

@@ -88,3 +88,32 @@ def test_real_inverse_two_pass_needs_work(emu):
     x = np.ones((1, 68, 72)) + 0j
     with pytest.raises(RuntimeError, match='work buffer'):
         _run(emu, x, 68, 72, 3, 8192, with_work=False)
+
+
+@pytest.mark.parametrize('N,k,B', [(88, 1, 100), (8192, 4, 3), (4096, 32, 5), (64, 2, 700), (136, 8, 9),
+                                   (4096, 32, 64), (8192, 1, 2), (88, 4, 3000)])
+@pytest.mark.parametrize('mode', [0, 1, 2])
+def test_filter_ifft(emu, N, k, B, mode):
+    """kym_op_filter_ifft's body on the CPU harness: ifft(P_k(x f)) against
+    numpy FP64 of the reference's op sequence (cdgmm, subsample_fourier's
+    mean over the k aliases, ifft), kept complex, real or as the modulus."""
+    f = emu.lib.kym_emu_filter_ifft
+    f.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_longlong] * 2 + [ctypes.c_int] * 3
+    f.restype = ctypes.c_int
+    rng = np.random.default_rng(N + k + B + mode)
+    x = rng.standard_normal((B, N)) + 1j * rng.standard_normal((B, N))
+    filt = rng.standard_normal(N)
+    n = N // k
+    xin = np.stack([x.real, x.imag], -1).reshape(-1).copy()
+    out_len = B * n * (2 if mode == 0 else 1)
+    y = _guarded(out_len, np.float64)
+    assert f(xin.ctypes.data, filt.ctypes.data, y.ctypes.data, B, N, k, mode, 1) == 0
+    assert np.all(y[out_len:] == 12345.0), 'write past the output'
+    ref = np.fft.ifft((x * filt).reshape(B, k, n).mean(axis=1), axis=1)
+    if mode == 0:
+        got = y[:out_len].reshape(B, n, 2)
+        got = got[..., 0] + 1j * got[..., 1]
+    else:
+        got = y[:out_len].reshape(B, n)
+        ref = ref.real if mode == 1 else np.abs(ref)
+    assert np.abs(got - ref).max() <= 1e-12 * max(np.abs(ref).max(), 1)
