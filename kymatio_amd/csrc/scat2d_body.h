@@ -291,7 +291,9 @@ KYM_HD KYM_INL void herm_gather(cpx<T>* d, cpx<T>* m, const cpx<T>* U, const T* 
 template <int K, class T>
 KYM_HD KYM_INL void gather_herm_k(cpx<T>* dst, const cpx<T>* U, const T* f, int H, int W, T scale, int tid,
                                   int nth) {
-    constexpr int E = K >= 4 ? 1 : 2;  // pairs per thread and round
+    // pairs per thread and round: one from K = 2 on (A/B on MI355X at 256x256
+    // J=3: two pairs per round 266 us per 136x136 launch, one 257 us, four 287 us)
+    constexpr int E = K >= 2 ? 1 : 2;
     const int np = herm_pairs(H, W), Wp = W * K;
     const T* fs[1] = {f};
     for (int p0 = tid; p0 < np; p0 += E * nth) {
