@@ -21,18 +21,44 @@
 // dL/dU at E points (ys[e], xs[e]) of a node of H x W: the leaf term (GY:
 // HJ x WJ, already scaled by 1 / (n_J kl^2)) plus the children's dL/dX
 // (scaled here by 1 / kc^2), summed in that order.  Each child's loads for
-// all E points are issued together, four children at a time.
+// all E points are issued together, four children at a time.  A child of
+// resolution r_p + l reads its dL/dX at (y mod H / 2^l, x mod W / 2^l); the
+// plan lists children by ascending resolution, so these residues are kept
+// running (one conditional subtraction per level) instead of one integer
+// division per child and point; a child out of that order takes the division.
 template <int E, class T>
 KYM_HD KYM_INL void grad_u_elems(const Scat2DArgs<T>& a, const Node2D& nd, const cpx<T>* wsi, const cpx<T>* GY, int W,
                                  int HJ, int WJ, const int* ys, const int* xs, cpx<T>* acc) {
+    const int H = a.H[nd.res];
     long long idx[E];
+    int yr[E], xr[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         idx[e] = (long long)ys[e] * W + xs[e];
+        yr[e] = ys[e];
+        xr[e] = xs[e];
         const T ph = kym_ldg(a.filt + a.filt_off[nd.leaf_filt] + idx[e]);
         const cpx<T> gy = GY[(ys[e] % HJ) * WJ + (xs[e] % WJ)];
         acc[e] = {gy.re * ph, gy.im * ph};
     }
+    int lv = 0;  // level of the running residues yr, xr
+    auto child_at = [&](const Node2D& cn, int e, int Hc, int Wc) -> long long {
+        const int l = cn.res - nd.res;
+        if (l < lv) return (long long)(ys[e] % Hc) * Wc + (xs[e] % Wc);
+        return (long long)yr[e] * Wc + xr[e];
+    };
+    auto advance = [&](const Node2D& cn) {
+        const int l = cn.res - nd.res;
+        while (lv < l) {
+            ++lv;
+            const int hh = H >> lv, hw = W >> lv;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                yr[e] -= yr[e] >= hh ? hh : 0;
+                xr[e] -= xr[e] >= hw ? hw : 0;
+            }
+        }
+    };
     int c = nd.child_begin;
     for (; c + 4 <= nd.child_end; c += 4) {  // four children x E elements of loads in flight
         T fv[4][E];
@@ -40,13 +66,14 @@ KYM_HD KYM_INL void grad_u_elems(const Scat2DArgs<T>& a, const Node2D& nd, const
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const Node2D& cn = a.nodes[c + u];
+            advance(cn);
             const int Hc = a.H[cn.res], Wc = a.W[cn.res];
             const T* f = a.filt + a.filt_off[cn.filt];
             const T inv = T(1) / T(cn.k * cn.k);
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 fv[u][e] = kym_ldg(f + idx[e]) * inv;
-                gx[u][e] = kym_ldg(wsi + cn.gsave + (long long)(ys[e] % Hc) * Wc + (xs[e] % Wc));
+                gx[u][e] = kym_ldg(wsi + cn.gsave + child_at(cn, e, Hc, Wc));
             }
         }
 #pragma unroll
@@ -59,11 +86,12 @@ KYM_HD KYM_INL void grad_u_elems(const Scat2DArgs<T>& a, const Node2D& nd, const
     }
     for (; c < nd.child_end; ++c) {
         const Node2D& cn = a.nodes[c];
+        advance(cn);
         const int Hc = a.H[cn.res], Wc = a.W[cn.res];
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const T fv = kym_ldg(a.filt + a.filt_off[cn.filt] + idx[e]) / T(cn.k * cn.k);
-            const cpx<T> gx = kym_ldg(wsi + cn.gsave + (long long)(ys[e] % Hc) * Wc + (xs[e] % Wc));
+            const cpx<T> gx = kym_ldg(wsi + cn.gsave + child_at(cn, e, Hc, Wc));
             acc[e].re += gx.re * fv;
             acc[e].im += gx.im * fv;
         }
