@@ -272,6 +272,26 @@ int kym_op_fft(const void* x, void* y, void* work, long long batch, int H, int W
  * Status 2 if k does not divide N. */
 int kym_op_filter_ifft(const void* x, const void* f, void* y, long long batch, long long N, int k, int mode, int dtype,
                        void* stream);
+/* kym_op_filter_ifft with the output transposed: the batch rows are (b, t),
+ * t < tlen, and y is laid out (batch / tlen, n, tlen) -- the ifft and
+ * swap_time_frequency of kymatio/scattering1d/core/
+ * timefrequency_scattering.py:266-267 (backend/torch_backend.py:172-186)
+ * in one kernel; with mode 2 also the modulus that
+ * timefrequency_scattering.py:409 applies next.  tlen = 0 is
+ * kym_op_filter_ifft.  Status 3 if tlen does not divide batch. */
+int kym_op_filter_ifft_t(const void* x, const void* f, void* y, long long batch, long long N, int k, int mode,
+                         long long tlen, int dtype, void* stream);
+/* Low-pass of real lines in one kernel: y = Re ifft_n(subsample_fourier(
+ * cdgmm(fft_N(x), f), k)), n = N / k -- rfft, cdgmm, subsample_fourier and
+ * irfft of kymatio/scattering1d/backend/torch_backend.py:118-136 with
+ * kymatio/backend/torch_backend.py:148 as time_averaging and
+ * frequency_averaging apply them (kymatio/scattering1d/core/
+ * timefrequency_scattering.py:278-373).  x is
+ * (outer, N, S) real, y (outer, n, S) real: the lines run along the middle
+ * axis, so frequency_averaging's two swap_time_frequency are not needed
+ * (S = 1: contiguous lines).  Status 2 if k does not divide N. */
+int kym_op_lowpass(const void* x, const void* f, void* y, long long outer, long long N, long long S, int k, int dtype,
+                   void* stream);
 /* modulus_rotation (kymatio/scattering3d/backend/torch_backend.py:97-124):
  * y = sqrt(prev^2 + |x|^2) elementwise over n complex x; prev may be null
  * (the first m of a solid-harmonic order l). */

@@ -287,7 +287,8 @@ static int emu_op_fft(const void* x, void* y, void* work, long long batch, int H
 }
 
 template <class T>
-static int emu_filter_ifft(const void* x, const void* f, void* y, long long batch, long long N, int k, int mode) {
+static int emu_filter_ifft(const void* x, const void* f, void* y, long long batch, long long N, int k, int mode,
+                           int tlen = 0) {
     if (k < 1 || N % k) { g_emu_error = "subsampling factor must divide the size"; return KYM_ERR_RUNTIME; }
     const int n = (int)(N / k);
     LenTable<T> lt;
@@ -299,8 +300,8 @@ static int emu_filter_ifft(const void* x, const void* f, void* y, long long batc
     b.relocate((uintptr_t)b.bytes.data());
     const FftLen<T>* lens = reinterpret_cast<const FftLen<T>*>(b.bytes.data() + lens_off);
     int nt = 256, m = 1, split = 1;
-    const long long elems = filter_ifft_shape(n, k, batch, &nt, &m, &split);
-    FilterIfftArgs o = {x, f, y, batch, (int)N, k, n, mode, m, split};
+    const long long elems = filter_ifft_shape(n, k, batch, &nt, &m, &split, tlen);
+    FilterIfftArgs o = {x, f, y, batch, (int)N, k, n, mode, m, split, tlen};
     std::vector<cpx<T>> lds((size_t)elems + 1);
     for (long long blk = 0; blk < (batch + m - 1) / m; ++blk) filter_ifft_body<T>(o, lens[l], blk, lds.data(), 0, 1);
     return 0;
@@ -310,6 +311,42 @@ extern "C" int kym_emu_filter_ifft(const void* x, const void* f, void* y, long l
                                    int mode, int dtype) {
     if (dtype == 0) return emu_filter_ifft<float>(x, f, y, batch, N, k, mode);
     return emu_filter_ifft<double>(x, f, y, batch, N, k, mode);
+}
+
+// kym_op_filter_ifft_t on the CPU (one thread per workgroup).
+extern "C" int kym_emu_filter_ifft_t(const void* x, const void* f, void* y, long long batch, long long N, int k,
+                                     int mode, long long tlen, int dtype) {
+    if (tlen < 0 || (tlen > 0 && batch % tlen)) { g_emu_error = "the rows must be a whole number of lines"; return 3; }
+    if (dtype == 0) return emu_filter_ifft<float>(x, f, y, batch, N, k, mode, (int)tlen);
+    return emu_filter_ifft<double>(x, f, y, batch, N, k, mode, (int)tlen);
+}
+
+template <class T>
+static int emu_lowpass(const void* x, const void* f, void* y, long long outer, int N, int S, int k) {
+    if (k < 1 || N % k) { g_emu_error = "subsampling factor must divide the size"; return KYM_ERR_RUNTIME; }
+    const int n = N / k;
+    LenTable<T> lt;
+    const int lN = lt.add(N), ln = lt.add(n);
+    Blob b;
+    size_t lens_off = 0;
+    std::string err;
+    if (!lt.emit(b, &lens_off, &err)) { g_emu_error = err; return KYM_ERR_UNSUPPORTED; }
+    b.relocate((uintptr_t)b.bytes.data());
+    const FftLen<T>* lens = reinterpret_cast<const FftLen<T>*>(b.bytes.data() + lens_off);
+    int m = 1, nsb = 1;
+    long long nblk = 0;
+    const long long elems = lowpass_shape(N, n, S, outer, &m, &nsb, &nblk);
+    LowpassArgs o = {x, f, y, outer, N, S, k, n, m, nsb};
+    std::vector<cpx<T>> lds((size_t)elems + 1);
+    for (long long blk = 0; blk < nblk; ++blk) lowpass_body<T>(o, lens[lN], lens[ln], blk, lds.data(), 0, 1);
+    return 0;
+}
+
+// kym_op_lowpass on the CPU (one thread per workgroup).
+extern "C" int kym_emu_lowpass(const void* x, const void* f, void* y, long long outer, long long N, long long S,
+                               int k, int dtype) {
+    if (dtype == 0) return emu_lowpass<float>(x, f, y, outer, (int)N, (int)S, k);
+    return emu_lowpass<double>(x, f, y, outer, (int)N, (int)S, k);
 }
 
 extern "C" int kym_emu_op_fft(const void* x, void* y, void* work, long long batch, int H, int W, int kind, int dtype,

@@ -195,6 +195,12 @@ __global__ void __launch_bounds__(NT) k_filter_ifft(FilterIfftArgs o, const FftL
 }
 
 template <class T>
+__global__ void __launch_bounds__(256) k_lowpass(LowpassArgs o, const FftLen<T>* lens, int lN, int ln) {
+    extern __shared__ __attribute__((aligned(16))) char kym_smem_lp[];
+    lowpass_body<T>(o, lens[lN], lens[ln], blockIdx.x, reinterpret_cast<cpx<T>*>(kym_smem_lp), threadIdx.x, 256);
+}
+
+template <class T>
 __global__ void __launch_bounds__(256) k_fft2_pass(FftOpArgs o, FftPassIO io, const FftLen<T>* lens, int lH, int lW,
                                                    int axis) {
     extern __shared__ __attribute__((aligned(16))) char kym_smem_op2[];
@@ -296,24 +302,44 @@ static int fft_op(const void* x, void* y, void* work, long long batch, int H, in
 
 template <class T>
 static int filter_ifft_op(const void* x, const void* f, void* y, long long batch, long long N, int k, int mode,
-                          hipStream_t s) {
+                          int tlen, hipStream_t s) {
     const int n = (int)(N / k);
     OpTables t;
     int rc = op_tables<T>(n, 1, &t);
     if (rc != KYM_OK) return rc;
     const FftLen<T>* lens = reinterpret_cast<const FftLen<T>*>((char*)t.dev + t.lens_off);
     int nt = 256, m = 1, split = 1;
-    const size_t lds = (size_t)filter_ifft_shape(n, k, batch, &nt, &m, &split) * sizeof(cpx<T>);
+    const size_t lds = (size_t)filter_ifft_shape(n, k, batch, &nt, &m, &split, tlen) * sizeof(cpx<T>);
     if (lds > op_lds_limit()) return kym_ops_fail(KYM_ERR_UNSUPPORTED, "transform length too large for LDS");
     const void* kern = nt == 1024 ? (const void*)k_filter_ifft<T, 1024> : (const void*)k_filter_ifft<T, 256>;
     if (lds > 65536) OPS_HIP_TRY(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const long long nblk = (batch + m - 1) / m;
     if (nblk > 0x7fffffffll) return kym_ops_fail(KYM_ERR_VALUE, "batch too large");
-    FilterIfftArgs o = {x, f, y, batch, (int)N, k, n, mode, m, split};
+    FilterIfftArgs o = {x, f, y, batch, (int)N, k, n, mode, m, split, tlen};
     if (nt == 1024)
         hipLaunchKernelGGL((k_filter_ifft<T, 1024>), dim3((unsigned)nblk), dim3(1024), lds, s, o, lens, t.lH);
     else
         hipLaunchKernelGGL((k_filter_ifft<T, 256>), dim3((unsigned)nblk), dim3(256), lds, s, o, lens, t.lH);
+    OPS_HIP_TRY(hipGetLastError());
+    return KYM_OK;
+}
+
+template <class T>
+static int lowpass_op(const void* x, const void* f, void* y, long long outer, int N, int S, int k, hipStream_t s) {
+    const int n = N / k;
+    OpTables t;
+    int rc = op_tables<T>(N, n, &t);
+    if (rc != KYM_OK) return rc;
+    const FftLen<T>* lens = reinterpret_cast<const FftLen<T>*>((char*)t.dev + t.lens_off);
+    int m = 1, nsb = 1;
+    long long nblk = 0;
+    const size_t lds = (size_t)lowpass_shape(N, n, S, outer, &m, &nsb, &nblk) * sizeof(cpx<T>);
+    if (lds > op_lds_limit()) return kym_ops_fail(KYM_ERR_UNSUPPORTED, "transform length too large for LDS");
+    if (lds > 65536)
+        OPS_HIP_TRY(hipFuncSetAttribute((const void*)k_lowpass<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (nblk > 0x7fffffffll) return kym_ops_fail(KYM_ERR_VALUE, "batch too large");
+    LowpassArgs o = {x, f, y, outer, N, S, k, n, m, nsb};
+    hipLaunchKernelGGL(k_lowpass<T>, dim3((unsigned)nblk), dim3(256), lds, s, o, lens, t.lH, t.lW);
     OPS_HIP_TRY(hipGetLastError());
     return KYM_OK;
 }
@@ -509,16 +535,35 @@ extern "C" int kym_op_unpad2d(const void* x, void* y, long long batch, int H, in
     return bad_dtype();
 }
 
-extern "C" int kym_op_filter_ifft(const void* x, const void* f, void* y, long long batch, long long N, int k,
-                                  int mode, int dtype, void* stream) {
+extern "C" int kym_op_filter_ifft_t(const void* x, const void* f, void* y, long long batch, long long N, int k,
+                                    int mode, long long tlen, int dtype, void* stream) {
     if (k < 1 || N < 1 || N % k) return kym_ops_fail(KYM_ERR_RUNTIME, "subsampling factor must divide the size");
     if (N / k > (1 << 20)) return kym_ops_fail(KYM_ERR_UNSUPPORTED, "transform length too large for LDS");
     if (mode < 0 || mode > 2) return kym_ops_fail(KYM_ERR_VALUE, "mode must be 0 (complex), 1 (real) or 2 (modulus)");
+    if (tlen < 0 || tlen > 0x7fffffffll || (tlen > 0 && batch % tlen))
+        return kym_ops_fail(KYM_ERR_VALUE, "the rows must be a whole number of lines of tlen");
     if (batch <= 0) return KYM_OK;
     if (!x || !f || !y) return kym_ops_fail(KYM_ERR_VALUE, "null device pointer");
     hipStream_t s = (hipStream_t)stream;
-    if (dtype == KYM_F32) return filter_ifft_op<float>(x, f, y, batch, N, k, mode, s);
-    if (dtype == KYM_F64) return filter_ifft_op<double>(x, f, y, batch, N, k, mode, s);
+    if (dtype == KYM_F32) return filter_ifft_op<float>(x, f, y, batch, N, k, mode, (int)tlen, s);
+    if (dtype == KYM_F64) return filter_ifft_op<double>(x, f, y, batch, N, k, mode, (int)tlen, s);
+    return bad_dtype();
+}
+
+extern "C" int kym_op_filter_ifft(const void* x, const void* f, void* y, long long batch, long long N, int k,
+                                  int mode, int dtype, void* stream) {
+    return kym_op_filter_ifft_t(x, f, y, batch, N, k, mode, 0, dtype, stream);
+}
+
+extern "C" int kym_op_lowpass(const void* x, const void* f, void* y, long long outer, long long N, long long S,
+                              int k, int dtype, void* stream) {
+    if (k < 1 || N < 1 || N % k) return kym_ops_fail(KYM_ERR_RUNTIME, "subsampling factor must divide the size");
+    if (N > (1 << 20) || S < 1 || S > 0x7fffffffll) return kym_ops_fail(KYM_ERR_VALUE, "invalid line length or stride");
+    if (outer <= 0) return KYM_OK;
+    if (!x || !f || !y) return kym_ops_fail(KYM_ERR_VALUE, "null device pointer");
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == KYM_F32) return lowpass_op<float>(x, f, y, outer, (int)N, (int)S, k, s);
+    if (dtype == KYM_F64) return lowpass_op<double>(x, f, y, outer, (int)N, (int)S, k, s);
     return bad_dtype();
 }
 

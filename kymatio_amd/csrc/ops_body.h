@@ -146,6 +146,10 @@ struct FilterIfftArgs {
     int N, k, n, mode;
     int nrow;    // rows per workgroup
     int split;   // alias subsets summed by separate threads (a power of two dividing k)
+    // 0: y is (batch, n); tlen > 0: the rows are (b, t) with t < tlen and y is
+    // laid out (batch / tlen, n, tlen), the transform axis before t (the
+    // workgroup's rows then lie within one b: nrow divides tlen)
+    int tlen;
 };
 
 // LDS: the m x n transform buffer, then (split > 1) split x m x n partial sums.
@@ -167,9 +171,26 @@ KYM_HD void filter_ifft_body(const FilterIfftArgs& o, const FftLen<T>& fl, long 
         const int t = e / n, p = e - t * n;
         const cpx<T>* xr = x + (b0 + t) * N + p;
         cpx<T> acc = {T(0), T(0)};
-        for (int a = s; a < k; a += S) {
+        int a = s;
+        // eight aliases per round: their loads are issued before the first
+        // use (the sum keeps the order a = s, s + S, ...)
+        for (; a + 7 * S < k; a += 8 * S) {
+            T w[8];
+            cpx<T> v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                w[u] = kym_ldg(f + p + (a + u * S) * n);
+                v[u] = kym_ldg(xr + (long long)(a + u * S) * n);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                acc.re += v[u].re * w[u];
+                acc.im += v[u].im * w[u];
+            }
+        }
+        for (; a < k; a += S) {
             const T w = f[p + a * n];
-            const cpx<T> v = xr[a * n];
+            const cpx<T> v = xr[(long long)a * n];
             acc.re += v.re * w;
             acc.im += v.im * w;
         }
@@ -188,8 +209,18 @@ KYM_HD void filter_ifft_body(const FilterIfftArgs& o, const FftLen<T>& fl, long 
     fft_dif<true, 31>(lds, fl, Batch{m, 1, n, false}, tid, nth);
     const T inv_n = T(1) / T(n);
     for (int q = tid; q < total; q += nth) {  // coalesced writes from the digit-reversed slots
-        const int t = q / n, p = q - t * n;
-        const long long dst = b0 * n + q;
+        int t, p;
+        long long dst;
+        if (o.tlen > 0) {  // row t of this block fastest: (b, p, t0 + t)
+            p = q / m;
+            t = q - p * m;
+            const long long r0 = b0 / o.tlen, t0 = b0 - r0 * o.tlen;
+            dst = (r0 * n + p) * o.tlen + t0 + t;
+        } else {
+            t = q / n;
+            p = q - t * n;
+            dst = b0 * n + q;
+        }
         const cpx<T> v = lds[t * n + fl.irev[p]];
         if (o.mode == 0) static_cast<cpx<T>*>(o.y)[dst] = v * inv_n;
         else if (o.mode == 1) static_cast<T*>(o.y)[dst] = v.re * inv_n;
@@ -201,19 +232,136 @@ KYM_HD void filter_ifft_body(const FilterIfftArgs& o, const FftLen<T>& fl, long 
 // (1024 for n >= 2048, else 256), rows per workgroup (about 4096 points, but
 // fewer when that leaves under 512 workgroups) and the alias split (threads
 // beyond the points sum alias subsets).  Returns the complex elements of LDS.
-inline long long filter_ifft_shape(int n, int k, long long batch, int* threads, int* rows, int* split) {
+// With tlen > 0 the rows per workgroup divide tlen (FilterIfftArgs).
+inline long long filter_ifft_shape(int n, int k, long long batch, int* threads, int* rows, int* split, int tlen = 0) {
     const int nt = n >= 2048 ? 1024 : 256;
     long long m = n >= 4096 ? 1 : 4096 / n;
     const long long want = (batch + 511) / 512;
     if (m > want) m = want;
     if (m > batch) m = batch;
     if (m < 1) m = 1;
+    if (tlen > 0)
+        while (tlen % m) --m;
     int S = 1;
     while (S * 2 <= k && (long long)S * 2 * m * n <= nt) S *= 2;
     *threads = nt;
     *rows = (int)m;
     *split = S;
     return m * n * (S > 1 ? 1 + S : 1);
+}
+
+// Low-pass of real lines: y = Re ifft_n(P_k(fft_N(x) f)), the irfft of the
+// periodised product of the line's spectrum with the real filter f (what
+// rfft, cdgmm, subsample_fourier and irfft compute one after the other,
+// kymatio/scattering1d/backend/torch_backend.py:118-136), n = N / k.  x is
+// (outer, N, S) and y (outer, n, S): the lines run along the middle axis
+// (S = 1: contiguous lines).  A workgroup takes nrow lines: consecutive
+// outer indices (S = 1) or consecutive s of one outer index (S > 1).
+struct LowpassArgs {
+    const void* x;
+    const void* f;
+    void* y;
+    long long outer;
+    int N, S, k, n;
+    int nrow;  // lines per workgroup
+    int nsb;   // workgroups per outer index (S > 1), else 1
+};
+
+// LDS: nrow x N spectrum, then nrow x n periodised spectrum.
+template <class T>
+KYM_HD void lowpass_body(const LowpassArgs& o, const FftLen<T>& fN, const FftLen<T>& fn, long long blk, cpx<T>* lds,
+                         int tid, int nth) {
+    const int N = o.N, n = o.n, k = o.k, S = o.S;
+    long long ob;
+    int s0, m;
+    if (S == 1) {
+        ob = blk * o.nrow;
+        s0 = 0;
+        m = (int)((o.outer - ob < o.nrow) ? o.outer - ob : o.nrow);
+    } else {
+        ob = blk / o.nsb;
+        s0 = (int)(blk - ob * o.nsb) * o.nrow;
+        m = (S - s0 < o.nrow) ? S - s0 : o.nrow;
+    }
+    const T* x = static_cast<const T*>(o.x);
+    const T* f = static_cast<const T*>(o.f);
+    T* y = static_cast<T*>(o.y);
+    // line t, sample e: x[(ob + t) N + e] (S = 1) or x[(ob N + e) S + s0 + t];
+    // stored at its digit-reversed slot for the forward DIT
+    for (int i = tid; i < m * N; i += nth) {
+        int t, e;
+        long long src;
+        if (S == 1) {
+            t = i / N;
+            e = i - t * N;
+            src = (ob + t) * N + e;
+        } else {
+            e = i / m;
+            t = i - e * m;
+            src = (ob * N + e) * S + s0 + t;
+        }
+        lds[t * N + fN.irev[e]] = {kym_ldg(x + src), T(0)};
+    }
+    wg_barrier();
+    fft_dit<false, 31>(lds, fN, Batch{m, 1, N, false}, tid, nth);
+    wg_barrier();
+    cpx<T>* P = lds + (long long)o.nrow * N;
+    const T inv_k = T(1) / T(k);
+    for (int q = tid; q < m * n; q += nth) {
+        const int t = q / n, p = q - t * n;
+        const cpx<T>* X = lds + t * N + p;
+        cpx<T> acc = {T(0), T(0)};
+        for (int a = 0; a < k; ++a) {
+            const T w = kym_ldg(f + p + a * n);
+            const cpx<T> v = X[a * n];
+            acc.re += v.re * w;
+            acc.im += v.im * w;
+        }
+        P[q] = acc * inv_k;
+    }
+    wg_barrier();
+    fft_dif<true, 31>(P, fn, Batch{m, 1, n, false}, tid, nth);
+    wg_barrier();
+    const T inv_n = T(1) / T(n);
+    for (int q = tid; q < m * n; q += nth) {
+        int t, p;
+        long long dst;
+        if (S == 1) {
+            t = q / n;
+            p = q - t * n;
+            dst = (ob + t) * n + p;
+        } else {
+            p = q / m;
+            t = q - p * m;
+            dst = (ob * n + p) * S + s0 + t;
+        }
+        y[dst] = P[t * n + fn.irev[p]].re * inv_n;
+    }
+}
+
+// Launch shape of lowpass: lines per workgroup (about 4096 points, at most
+// S, a power of two for S > 1, fewer when that leaves under 512 workgroups)
+// and workgroups per outer index.  Returns the complex elements of LDS.
+inline long long lowpass_shape(int N, int n, int S, long long outer, int* rows, int* nsb, long long* nblk) {
+    long long m = N >= 4096 ? 1 : 4096 / N;
+    const long long lines = outer * S;
+    const long long want = (lines + 511) / 512;
+    if (m > want) m = want;
+    if (m < 1) m = 1;
+    if (S > 1) {
+        if (m > S) m = S;
+        long long p2 = 1;
+        while (p2 * 2 <= m) p2 *= 2;
+        m = p2;
+        *nsb = (int)((S + m - 1) / m);
+        *nblk = outer * *nsb;
+    } else {
+        if (m > outer) m = outer;
+        *nsb = 1;
+        *nblk = (outer + m - 1) / m;
+    }
+    *rows = (int)m;
+    return m * (N + n);
 }
 
 // Pass kernels for transforms that do not fit: axis 1 = rows (along W),

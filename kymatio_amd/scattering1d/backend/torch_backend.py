@@ -121,6 +121,57 @@ class TorchBackend1D(TorchBackend):
                                                       _dt(x), _lib.stream_of(x.device)))
         return y
 
+    @classmethod
+    def filter_ifft_swap(cls, x, filt, k, mode='modulus'):
+        """filter_ifft along the frequency axis of x (batch, T, N, 2), then
+        the time and frequency axes swapped: (batch, N / k, T, 2 | 1) -- the
+        ifft, swap_time_frequency and (mode 'modulus') modulus of
+        frequency scattering in one HIP kernel (kym_op_filter_ifft_t)."""
+        cls.complex_check(x)
+        cls.contiguous_check(x)
+        _gpu(x)
+        T, N = x.shape[-3], x.shape[-2]
+        if N % k:
+            raise RuntimeError('subsampling factor must divide the size')
+        code = {'complex': 0, 'real': 1, 'modulus': 2}[mode]
+        f = filt.contiguous()
+        if f.dtype != x.dtype or f.numel() != N:
+            raise TypeError('Input and filter must be of the same dtype and length.')
+        y = torch.empty(x.shape[:-3] + (N // k, T, 2 if code == 0 else 1), dtype=x.dtype, device=x.device)
+        with torch.cuda.device(x.device):
+            _lib.check(_lib.lib().kym_op_filter_ifft_t(_ptr(x), _ptr(f), _ptr(y), x.numel() // (2 * N), N, k, code,
+                                                        T, _dt(x), _lib.stream_of(x.device)))
+        return y
+
+    @classmethod
+    def lowpass(cls, x, filt, k, axis=-2):
+        """irfft(subsample_fourier(cdgmm(rfft(.), filt), k)) along the time
+        axis (-2) or the frequency axis (-3) of a real x (..., A, T, 1), in
+        one HIP kernel (kym_op_lowpass); that axis shrinks by k.  Along the
+        frequency axis it is what swap_time_frequency, rfft, cdgmm,
+        subsample_fourier, irfft and swap_time_frequency compute."""
+        cls.real_check(x)
+        cls.contiguous_check(x)
+        _gpu(x)
+        if axis == -2:
+            N, S = x.shape[-2], 1
+            shape = x.shape[:-2] + (N // k, 1)
+        elif axis == -3:
+            N, S = x.shape[-3], x.shape[-2]
+            shape = x.shape[:-3] + (N // k, S, 1)
+        else:
+            raise ValueError('axis must be -2 (time) or -3 (frequency)')
+        if N % k:
+            raise RuntimeError('subsampling factor must divide the size')
+        f = filt.contiguous()
+        if f.dtype != x.dtype or f.numel() != N:
+            raise TypeError('Input and filter must be of the same dtype and length.')
+        y = torch.empty(shape, dtype=x.dtype, device=x.device)
+        with torch.cuda.device(x.device):
+            _lib.check(_lib.lib().kym_op_lowpass(_ptr(x), _ptr(f), _ptr(y), x.numel() // (N * S), N, S, k, _dt(x),
+                                                  _lib.stream_of(x.device)))
+        return y
+
     # ---- frequency-axis layout ops of time-frequency scattering
     # (kymatio/scattering1d/backend/torch_backend.py:150-223); the axis n1
     # sits before time, tensors end in a real/imag (or singleton) axis.

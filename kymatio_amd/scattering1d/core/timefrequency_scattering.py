@@ -65,9 +65,12 @@ def time_scattering_widthfirst(U_0, backend, filters, log2_stride, average_local
             yield {'coef': backend.stack(Y_2, 2), 'j': (-1, j2), 'n': (-1, n2), 'n1_max': len(Y_2)}
 
 
-def frequency_scattering(X, backend, filters_fr, log2_stride_fr, average_local_fr, spinned):
+def frequency_scattering(X, backend, filters_fr, log2_stride_fr, average_local_fr, spinned, modulus=False):
     """Wavelets along n1 (zero-padded to the frequency filters' length N):
-    all n_fr for complex input (spinned), only spin >= 0 for real input."""
+    all n_fr for complex input (spinned), only spin >= 0 for real input.
+    modulus: yield |Y| (the modulus jtfs_average_and_format takes next),
+    with the filter, inverse, swap and modulus in one op where the backend
+    has it (filter_ifft_swap)."""
     phi, psis = filters_fr
     X_T = backend.pad_frequency(backend.swap_time_frequency(X['coef']), phi['N'] - X['n1_max'])
     if spinned:
@@ -78,25 +81,39 @@ def frequency_scattering(X, backend, filters_fr, log2_stride_fr, average_local_f
     for n_fr, psi in enumerate(psis):
         j_fr = psi['j']
         k_fr = min(j_fr, log2_stride_fr) if average_local_fr else j_fr
-        Y = backend.swap_time_frequency(_filtered(backend, X_hat, psi, 0, k_fr, 'complex'))
+        fused = getattr(backend, 'filter_ifft_swap', None) if modulus else None
+        if fused is not None:
+            Y = fused(X_hat, psi['levels'][0], 2 ** k_fr, 'modulus')
+        else:
+            Y = backend.swap_time_frequency(_filtered(backend, X_hat, psi, 0, k_fr, 'complex'))
+            if modulus:
+                Y = backend.modulus(Y)
         yield {**X, 'coef': Y, 'n': X['n'][1:] + (n_fr,), 'j_fr': (j_fr,), 'n_fr': (n_fr,),
                'n1_stride': 2 ** j_fr, 'spin': np.sign(psi['xi'])}
 
 
 def joint_timefrequency_scattering(U_0, backend, filters, log2_stride, average_local, filters_fr, log2_stride_fr,
-                                   average_local_fr):
+                                   average_local_fr, modulus=False):
     """Zeroth order; frequency scattering of S_1 (real, spin >= 0); then for
-    each n2 frequency scattering of Y_2 (complex, both spins)."""
+    each n2 frequency scattering of Y_2 (complex, both spins).  modulus:
+    paths after the first come as their modulus (frequency_scattering)."""
     time_gen = time_scattering_widthfirst(U_0, backend, filters, log2_stride, average_local)
     yield next(time_gen)
     S_1 = next(time_gen)
-    yield from frequency_scattering(S_1, backend, filters_fr, log2_stride_fr, average_local_fr, spinned=False)
+    yield from frequency_scattering(S_1, backend, filters_fr, log2_stride_fr, average_local_fr, spinned=False,
+                                    modulus=modulus)
     for Y_2 in time_gen:
-        yield from frequency_scattering(Y_2, backend, filters_fr, log2_stride_fr, average_local_fr, spinned=True)
+        yield from frequency_scattering(Y_2, backend, filters_fr, log2_stride_fr, average_local_fr, spinned=True,
+                                        modulus=modulus)
 
 
 def time_averaging(U_2, backend, phi_f, log2_stride):
+    """Low-pass phi_f along time, subsampled to 2^log2_stride: rfft then the
+    real filtered inverse, or one op where the backend has it (lowpass)."""
     k_in = U_2['j'][-1]
+    lowpass = getattr(backend, 'lowpass', None)
+    if lowpass is not None:
+        return {**U_2, 'coef': lowpass(U_2['coef'], phi_f['levels'][k_in], 2 ** (log2_stride - k_in), -2)}
     U_hat = backend.rfft(U_2['coef'])
     return {**U_2, 'coef': _filtered(backend, U_hat, phi_f, k_in, log2_stride - k_in, 'real')}
 
@@ -104,14 +121,18 @@ def time_averaging(U_2, backend, phi_f, log2_stride):
 def frequency_averaging(U_2, backend, phi_fr_f, log2_stride_fr, average_fr):
     if not average_fr:
         return {**U_2, 'n1_stride': 2 ** max(U_2['j_fr'][-1], 0)}
-    U_T = backend.swap_time_frequency(U_2['coef'])
     if average_fr == 'global':
-        S_T = backend.average_global(U_T)
+        S_T = backend.average_global(backend.swap_time_frequency(U_2['coef']))
         n1_stride = U_2['n1_max']
     else:  # 'local'
         k_in = min(U_2['j_fr'][-1], phi_fr_f['j'])
-        S_T = _filtered(backend, backend.rfft(U_T), phi_fr_f, k_in, log2_stride_fr - k_in, 'real')
         n1_stride = 2 ** log2_stride_fr
+        lowpass = getattr(backend, 'lowpass', None)
+        if lowpass is not None:  # along n1 in place of the two swaps
+            S = lowpass(U_2['coef'], phi_fr_f['levels'][k_in], 2 ** (log2_stride_fr - k_in), -3)
+            return {**U_2, 'coef': S, 'n1_stride': n1_stride}
+        U_T = backend.swap_time_frequency(U_2['coef'])
+        S_T = _filtered(backend, backend.rfft(U_T), phi_fr_f, k_in, log2_stride_fr - k_in, 'real')
     return {**U_2, 'coef': backend.swap_time_frequency(S_T), 'n1_stride': n1_stride}
 
 
@@ -127,13 +148,16 @@ def time_formatting(path, backend):
 
 
 def jtfs_average_and_format(U_gen, backend, phi_f, log2_stride, average, phi_fr_f, log2_stride_fr, average_fr,
-                            out_type, format):
+                            out_type, format, modulus_done=False):
+    """modulus_done: U_gen's paths after the first are already moduli
+    (joint_timefrequency_scattering with modulus=True)."""
     path = next(U_gen)
     if average == 'global':
         path['coef'] = backend.average_global(path['coef'])
     yield {**path, 'order': 0}
     for path in U_gen:
-        path['coef'] = backend.modulus(path['coef'])
+        if not modulus_done:
+            path['coef'] = backend.modulus(path['coef'])
         if average == 'global':
             path['coef'] = backend.average_global(path['coef'])
         elif average == 'local' and len(path['n']) > 1:

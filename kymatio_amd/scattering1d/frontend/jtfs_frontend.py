@@ -183,10 +183,13 @@ class TimeFrequencyScatteringTorch1D(ScatteringTorch1D):
         x = B.reshape_input(x.contiguous(), signal_shape)
         U_0 = B.pad(x, pad_left=self.pad_left, pad_right=self.pad_right)
         filters, filters_fr = self._device_filters(x.device, x.dtype)
+        # the modulus of every joint path rides on its frequential inverse
         U_gen = joint_timefrequency_scattering(U_0, B, filters, self.log2_stride, self.average == 'local',
-                                               filters_fr, self.log2_stride_fr, self.average_fr == 'local')
+                                               filters_fr, self.log2_stride_fr, self.average_fr == 'local',
+                                               modulus=True)
         S_gen = jtfs_average_and_format(U_gen, B, filters[0], self.log2_stride, self.average, filters_fr[0],
-                                        self.log2_stride_fr, self.average_fr, self.out_type, self.format)
+                                        self.log2_stride_fr, self.average_fr, self.out_type, self.format,
+                                        modulus_done=True)
         path = next(S_gen)
         if self.average != 'global':
             res = self.log2_stride if self.average else 0

@@ -179,3 +179,58 @@ def test_per_op_core_loop_matches_fused():
     assert torch.allclose(Sx, torch.from_numpy(d['Sx']).to(DEV))
     fused = Scattering2D(4, (32, 32)).to(DEV)
     assert torch.allclose(Sx, fused(x), rtol=1e-5, atol=1e-7)
+
+
+@pytest.mark.parametrize('B,T,N,k', [(3, 64, 88, 4), (2, 128, 88, 1), (4, 16, 22, 1)])
+@pytest.mark.parametrize('mode', ['modulus', 'complex'])
+def test_filter_ifft_swap_equals_op_sequence(B, T, N, k, mode):
+    """filter_ifft_swap (kym_op_filter_ifft_t) equals the ops it fuses --
+    filter_ifft, swap_time_frequency and modulus -- to FP32 rounding, and
+    numpy FP64 of the reference's sequence."""
+    rng = np.random.default_rng(B * T + N + k)
+    x = rng.standard_normal((B, T, N)) + 1j * rng.standard_normal((B, T, N))
+    f = rng.standard_normal(N)
+    xt = torch.view_as_real(torch.from_numpy(x.astype(np.complex64))).contiguous().to(DEV)
+    ft = torch.from_numpy(f.astype(np.float32))[:, None].to(DEV)
+    fused = B1.filter_ifft_swap(xt, ft, k, mode)
+    seq = B1.swap_time_frequency(B1.filter_ifft(xt, ft, k, 'complex'))
+    if mode == 'modulus':
+        seq = B1.modulus(seq)
+    assert fused.shape == seq.shape
+    scale = seq.abs().max().item()
+    assert (fused - seq).abs().max().item() <= 1e-6 * scale
+    ref = np.fft.ifft((x * f).reshape(B, T, k, N // k).mean(axis=2), axis=2).transpose(0, 2, 1)
+    got = fused.cpu().numpy().astype(np.float64)
+    if mode == 'modulus':
+        assert np.abs(got[..., 0] - np.abs(ref)).max() <= 1e-5 * np.abs(ref).max()
+    else:
+        assert np.abs(got[..., 0] + 1j * got[..., 1] - ref).max() <= 1e-5 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize('shape,axis,k', [((3, 22, 4096, 1), -2, 32), ((2, 88, 2048, 1), -2, 16),
+                                          ((5, 88, 128, 1), -3, 4), ((3, 22, 128, 1), -3, 1), ((2, 7, 256, 1), -2, 2)])
+def test_lowpass_equals_op_sequence(shape, axis, k):
+    """lowpass (kym_op_lowpass) equals rfft then filter_ifft's real mode
+    along time, and swap, rfft, filter_ifft and swap along frequency -- the
+    reference's time_averaging and frequency_averaging -- to FP32 rounding,
+    and numpy FP64."""
+    rng = np.random.default_rng(sum(shape) + k)
+    x = rng.standard_normal(shape)
+    N = shape[axis]
+    f = rng.standard_normal(N)
+    xt = torch.from_numpy(x.astype(np.float32)).to(DEV)
+    ft = torch.from_numpy(f.astype(np.float32))[:, None].to(DEV)
+    fused = B1.lowpass(xt, ft, k, axis)
+    if axis == -2:
+        seq = B1.filter_ifft(B1.rfft(xt), ft, k, 'real')
+    else:
+        seq = B1.swap_time_frequency(B1.filter_ifft(B1.rfft(B1.swap_time_frequency(xt)), ft, k, 'real'))
+    assert fused.shape == seq.shape
+    scale = seq.abs().max().item()
+    assert (fused - seq).abs().max().item() <= 1e-6 * scale
+    ax = x.ndim + axis
+    X = np.fft.fft(x, axis=ax) * f.reshape([-1 if i == ax else 1 for i in range(x.ndim)])
+    Xs = np.moveaxis(X, ax, -1)
+    ref = np.fft.ifft(Xs.reshape(Xs.shape[:-1] + (k, N // k)).mean(axis=-2), axis=-1).real
+    ref = np.moveaxis(ref, -1, ax)
+    assert np.abs(fused.cpu().numpy() - ref).max() <= 1e-5 * np.abs(ref).max()

@@ -117,3 +117,55 @@ def test_filter_ifft(emu, N, k, B, mode):
         got = y[:out_len].reshape(B, n)
         ref = ref.real if mode == 1 else np.abs(ref)
     assert np.abs(got - ref).max() <= 1e-12 * max(np.abs(ref).max(), 1)
+
+
+@pytest.mark.parametrize('N,k,T,B', [(88, 4, 64, 3), (88, 1, 128, 2), (22, 1, 16, 5), (64, 2, 8, 7)])
+@pytest.mark.parametrize('mode', [0, 2])
+def test_filter_ifft_transposed(emu, N, k, T, B, mode):
+    """kym_op_filter_ifft_t: the rows are (b, t) and the output comes out
+    as (b, n, t) -- ifft(P_k(x f)) followed by the swap of the time and
+    frequency axes, and the modulus for mode 2 -- against numpy FP64."""
+    f = emu.lib.kym_emu_filter_ifft_t
+    f.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_longlong] * 2 + [ctypes.c_int] * 2 + [ctypes.c_longlong,
+                                                                                          ctypes.c_int]
+    f.restype = ctypes.c_int
+    rng = np.random.default_rng(N + k + T + B + mode)
+    x = rng.standard_normal((B, T, N)) + 1j * rng.standard_normal((B, T, N))
+    filt = rng.standard_normal(N)
+    n = N // k
+    xin = np.stack([x.real, x.imag], -1).reshape(-1).copy()
+    out_len = B * T * n * (2 if mode == 0 else 1)
+    y = _guarded(out_len, np.float64)
+    assert f(xin.ctypes.data, filt.ctypes.data, y.ctypes.data, B * T, N, k, mode, T, 1) == 0
+    assert np.all(y[out_len:] == 12345.0), 'write past the output'
+    ref = np.fft.ifft((x * filt).reshape(B, T, k, n).mean(axis=2), axis=2).transpose(0, 2, 1)
+    if mode == 0:
+        got = y[:out_len].reshape(B, n, T, 2)
+        got = got[..., 0] + 1j * got[..., 1]
+    else:
+        got = y[:out_len].reshape(B, n, T)
+        ref = np.abs(ref)
+    assert np.abs(got - ref).max() <= 1e-12 * max(np.abs(ref).max(), 1)
+    assert f(xin.ctypes.data, filt.ctypes.data, y.ctypes.data, B * T + 1, N, k, mode, T, 1) != 0
+
+
+@pytest.mark.parametrize('N,k,S,outer', [(4096, 32, 1, 3), (2048, 16, 1, 5), (256, 2, 1, 40), (88, 4, 128, 3),
+                                         (22, 1, 128, 2), (88, 4, 5, 7), (64, 64, 1, 9)])
+def test_lowpass(emu, N, k, S, outer):
+    """kym_op_lowpass: irfft(P_k(rfft(x) f)) along the middle axis of
+    (outer, N, S) real lines -- the reference's rfft, cdgmm,
+    subsample_fourier and irfft -- against numpy FP64."""
+    f = emu.lib.kym_emu_lowpass
+    f.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_longlong] * 3 + [ctypes.c_int] * 2
+    f.restype = ctypes.c_int
+    rng = np.random.default_rng(N + k + S + outer)
+    x = rng.standard_normal((outer, N, S))
+    filt = rng.standard_normal(N)
+    n = N // k
+    y = _guarded(outer * n * S, np.float64)
+    assert f(x.ctypes.data, filt.ctypes.data, y.ctypes.data, outer, N, S, k, 1) == 0
+    assert np.all(y[outer * n * S:] == 12345.0), 'write past the output'
+    X = np.fft.fft(x, axis=1)
+    ref = np.fft.ifft((X * filt[None, :, None]).reshape(outer, k, n, S).mean(axis=1), axis=1).real
+    got = y[:outer * n * S].reshape(outer, n, S)
+    assert np.abs(got - ref).max() <= 1e-12 * max(np.abs(ref).max(), 1)
