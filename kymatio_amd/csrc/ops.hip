@@ -175,37 +175,45 @@ __global__ void k_unpad2d(const T* x, T* y, long long batch, int H, int W) {
 
 #include "ops_body.h"
 
-template <class T>
+template <class T, int MAXP>
 __global__ void __launch_bounds__(256) k_fft2_lds(FftOpArgs o, const FftLen<T>* lens, int lH, int lW) {
     extern __shared__ __attribute__((aligned(16))) char kym_smem_op[];
     if ((long long)blockIdx.x >= o.batch) return;
-    fftop_lds_body<T>(o, lens, lH, lW, blockIdx.x, reinterpret_cast<cpx<T>*>(kym_smem_op), threadIdx.x, blockDim.x);
+    fftop_lds_body<T, MAXP>(o, lens, lH, lW, blockIdx.x, reinterpret_cast<cpx<T>*>(kym_smem_op), threadIdx.x,
+                            blockDim.x);
 }
 
-template <class T>
+template <class T, int MAXP>
 __global__ void __launch_bounds__(256) k_fft1_multi(FftOpArgs o, const FftLen<T>* lens, int lH) {
     extern __shared__ __attribute__((aligned(16))) char kym_smem_op1[];
-    fftop_multi_body<T>(o, lens, lH, blockIdx.x, reinterpret_cast<cpx<T>*>(kym_smem_op1), threadIdx.x, blockDim.x);
+    fftop_multi_body<T, MAXP>(o, lens, lH, blockIdx.x, reinterpret_cast<cpx<T>*>(kym_smem_op1), threadIdx.x,
+                              blockDim.x);
 }
 
-template <class T, int NT>
+template <class T, int NT, int MAXP>
 __global__ void __launch_bounds__(NT) k_filter_ifft(FilterIfftArgs o, const FftLen<T>* lens, int l) {
     extern __shared__ __attribute__((aligned(16))) char kym_smem_fi[];
-    filter_ifft_body<T>(o, lens[l], blockIdx.x, reinterpret_cast<cpx<T>*>(kym_smem_fi), threadIdx.x, NT);
+    filter_ifft_body<T, MAXP>(o, lens[l], blockIdx.x, reinterpret_cast<cpx<T>*>(kym_smem_fi), threadIdx.x, NT);
 }
 
-template <class T>
+template <class T, int MAXP>
 __global__ void __launch_bounds__(256) k_lowpass(LowpassArgs o, const FftLen<T>* lens, int lN, int ln) {
     extern __shared__ __attribute__((aligned(16))) char kym_smem_lp[];
-    lowpass_body<T>(o, lens[lN], lens[ln], blockIdx.x, reinterpret_cast<cpx<T>*>(kym_smem_lp), threadIdx.x, 256);
+    lowpass_body<T, MAXP>(o, lens[lN], lens[ln], blockIdx.x, reinterpret_cast<cpx<T>*>(kym_smem_lp), threadIdx.x, 256);
 }
 
-template <class T>
+template <class T, int MAXP>
 __global__ void __launch_bounds__(256) k_fft2_pass(FftOpArgs o, FftPassIO io, const FftLen<T>* lens, int lH, int lW,
                                                    int axis) {
     extern __shared__ __attribute__((aligned(16))) char kym_smem_op2[];
-    fftop_pass_body<T>(o, io, lens, lH, lW, axis, blockIdx.x, reinterpret_cast<cpx<T>*>(kym_smem_op2), threadIdx.x,
-                       blockDim.x);
+    fftop_pass_body<T, MAXP>(o, io, lens, lH, lW, axis, blockIdx.x, reinterpret_cast<cpx<T>*>(kym_smem_op2),
+                             threadIdx.x, blockDim.x);
+}
+
+// Prime class (7, 17 or 31) of a set of FFT lengths.
+static int op_pclass(int a, int b = 1) {
+    const int mp = kym_max_odd_prime(a) > kym_max_odd_prime(b) ? kym_max_odd_prime(a) : kym_max_odd_prime(b);
+    return kym_prime_class(mp);
 }
 
 // FFT length tables, built once per (length set, dtype) and kept for the
@@ -256,9 +264,9 @@ static size_t op_lds_limit() {
     return 65536;
 }
 
-template <class T>
-static int fft_op(const void* x, void* y, void* work, long long batch, int H, int W, int kind, hipStream_t s,
-                  long long lds_cap) {
+template <class T, int MAXP>
+static int fft_op_p(const void* x, void* y, void* work, long long batch, int H, int W, int kind, hipStream_t s,
+                    long long lds_cap) {
     OpTables t;
     int rc = op_tables<T>(H, W, &t);
     if (rc != KYM_OK) return rc;
@@ -270,7 +278,7 @@ static int fft_op(const void* x, void* y, void* work, long long batch, int H, in
         const long long nblk = (batch + m - 1) / m;
         if (nblk > 0x7fffffffll) return kym_ops_fail(KYM_ERR_VALUE, "batch too large");
         FftOpArgs o = {x, y, batch, H, W, kind, 0, (int)m, (int)nblk};
-        hipLaunchKernelGGL(k_fft1_multi<T>, dim3((unsigned)nblk), dim3(256), (size_t)(m * H) * sizeof(cpx<T>), s, o,
+        hipLaunchKernelGGL((k_fft1_multi<T, MAXP>), dim3((unsigned)nblk), dim3(256), (size_t)(m * H) * sizeof(cpx<T>), s, o,
                            lens, t.lH);
         OPS_HIP_TRY(hipGetLastError());
         return KYM_OK;
@@ -285,14 +293,14 @@ static int fft_op(const void* x, void* y, void* work, long long batch, int H, in
         FftOpArgs o = {x, y, batch, H, W, kind, 0, st.nrow, st.nblk};
         if (st.route == 0) {
             if (st.lds > 65536)
-                OPS_HIP_TRY(hipFuncSetAttribute((const void*)k_fft2_lds<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                OPS_HIP_TRY(hipFuncSetAttribute((const void*)k_fft2_lds<T, MAXP>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                                 (int)st.lds));
-            hipLaunchKernelGGL(k_fft2_lds<T>, dim3((unsigned)batch), dim3(256), st.lds, s, o, lens, t.lH, t.lW);
+            hipLaunchKernelGGL((k_fft2_lds<T, MAXP>), dim3((unsigned)batch), dim3(256), st.lds, s, o, lens, t.lH, t.lW);
         } else {
             if (st.lds > 65536)
-                OPS_HIP_TRY(hipFuncSetAttribute((const void*)k_fft2_pass<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                OPS_HIP_TRY(hipFuncSetAttribute((const void*)k_fft2_pass<T, MAXP>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                                 (int)st.lds));
-            hipLaunchKernelGGL(k_fft2_pass<T>, dim3((unsigned)(batch * st.nblk)), dim3(256), st.lds, s, o, st.io, lens,
+            hipLaunchKernelGGL((k_fft2_pass<T, MAXP>), dim3((unsigned)(batch * st.nblk)), dim3(256), st.lds, s, o, st.io, lens,
                                t.lH, t.lW, st.axis);
         }
         OPS_HIP_TRY(hipGetLastError());
@@ -301,8 +309,18 @@ static int fft_op(const void* x, void* y, void* work, long long batch, int H, in
 }
 
 template <class T>
-static int filter_ifft_op(const void* x, const void* f, void* y, long long batch, long long N, int k, int mode,
-                          int tlen, hipStream_t s) {
+static int fft_op(const void* x, void* y, void* work, long long batch, int H, int W, int kind, hipStream_t s,
+                  long long lds_cap) {
+    switch (op_pclass(H, W)) {
+        case 7: return fft_op_p<T, 7>(x, y, work, batch, H, W, kind, s, lds_cap);
+        case 17: return fft_op_p<T, 17>(x, y, work, batch, H, W, kind, s, lds_cap);
+        default: return fft_op_p<T, 31>(x, y, work, batch, H, W, kind, s, lds_cap);
+    }
+}
+
+template <class T, int MAXP>
+static int filter_ifft_op_p(const void* x, const void* f, void* y, long long batch, long long N, int k, int mode,
+                            int tlen, hipStream_t s) {
     const int n = (int)(N / k);
     OpTables t;
     int rc = op_tables<T>(n, 1, &t);
@@ -311,21 +329,32 @@ static int filter_ifft_op(const void* x, const void* f, void* y, long long batch
     int nt = 256, m = 1, split = 1;
     const size_t lds = (size_t)filter_ifft_shape(n, k, batch, &nt, &m, &split, tlen) * sizeof(cpx<T>);
     if (lds > op_lds_limit()) return kym_ops_fail(KYM_ERR_UNSUPPORTED, "transform length too large for LDS");
-    const void* kern = nt == 1024 ? (const void*)k_filter_ifft<T, 1024> : (const void*)k_filter_ifft<T, 256>;
+    const void* kern =
+        nt == 1024 ? (const void*)k_filter_ifft<T, 1024, MAXP> : (const void*)k_filter_ifft<T, 256, MAXP>;
     if (lds > 65536) OPS_HIP_TRY(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const long long nblk = (batch + m - 1) / m;
     if (nblk > 0x7fffffffll) return kym_ops_fail(KYM_ERR_VALUE, "batch too large");
     FilterIfftArgs o = {x, f, y, batch, (int)N, k, n, mode, m, split, tlen};
     if (nt == 1024)
-        hipLaunchKernelGGL((k_filter_ifft<T, 1024>), dim3((unsigned)nblk), dim3(1024), lds, s, o, lens, t.lH);
+        hipLaunchKernelGGL((k_filter_ifft<T, 1024, MAXP>), dim3((unsigned)nblk), dim3(1024), lds, s, o, lens, t.lH);
     else
-        hipLaunchKernelGGL((k_filter_ifft<T, 256>), dim3((unsigned)nblk), dim3(256), lds, s, o, lens, t.lH);
+        hipLaunchKernelGGL((k_filter_ifft<T, 256, MAXP>), dim3((unsigned)nblk), dim3(256), lds, s, o, lens, t.lH);
     OPS_HIP_TRY(hipGetLastError());
     return KYM_OK;
 }
 
 template <class T>
-static int lowpass_op(const void* x, const void* f, void* y, long long outer, int N, int S, int k, hipStream_t s) {
+static int filter_ifft_op(const void* x, const void* f, void* y, long long batch, long long N, int k, int mode,
+                          int tlen, hipStream_t s) {
+    switch (op_pclass((int)(N / k))) {
+        case 7: return filter_ifft_op_p<T, 7>(x, f, y, batch, N, k, mode, tlen, s);
+        case 17: return filter_ifft_op_p<T, 17>(x, f, y, batch, N, k, mode, tlen, s);
+        default: return filter_ifft_op_p<T, 31>(x, f, y, batch, N, k, mode, tlen, s);
+    }
+}
+
+template <class T, int MAXP>
+static int lowpass_op_p(const void* x, const void* f, void* y, long long outer, int N, int S, int k, hipStream_t s) {
     const int n = N / k;
     OpTables t;
     int rc = op_tables<T>(N, n, &t);
@@ -336,12 +365,21 @@ static int lowpass_op(const void* x, const void* f, void* y, long long outer, in
     const size_t lds = (size_t)lowpass_shape(N, n, S, outer, &m, &nsb, &nblk) * sizeof(cpx<T>);
     if (lds > op_lds_limit()) return kym_ops_fail(KYM_ERR_UNSUPPORTED, "transform length too large for LDS");
     if (lds > 65536)
-        OPS_HIP_TRY(hipFuncSetAttribute((const void*)k_lowpass<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        OPS_HIP_TRY(hipFuncSetAttribute((const void*)k_lowpass<T, MAXP>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     if (nblk > 0x7fffffffll) return kym_ops_fail(KYM_ERR_VALUE, "batch too large");
     LowpassArgs o = {x, f, y, outer, N, S, k, n, m, nsb};
-    hipLaunchKernelGGL(k_lowpass<T>, dim3((unsigned)nblk), dim3(256), lds, s, o, lens, t.lH, t.lW);
+    hipLaunchKernelGGL((k_lowpass<T, MAXP>), dim3((unsigned)nblk), dim3(256), lds, s, o, lens, t.lH, t.lW);
     OPS_HIP_TRY(hipGetLastError());
     return KYM_OK;
+}
+
+template <class T>
+static int lowpass_op(const void* x, const void* f, void* y, long long outer, int N, int S, int k, hipStream_t s) {
+    switch (op_pclass(N, N / k)) {
+        case 7: return lowpass_op_p<T, 7>(x, f, y, outer, N, S, k, s);
+        case 17: return lowpass_op_p<T, 17>(x, f, y, outer, N, S, k, s);
+        default: return lowpass_op_p<T, 31>(x, f, y, outer, N, S, k, s);
+    }
 }
 
 // ------------------------------------------------------------ C ABI

@@ -1,6 +1,8 @@
 // Bodies of the per-op FFT kernels (ops.hip), written as workgroup
 // functions of (tid, nthreads) like the scattering bodies so the CPU test
-// harness can run them.
+// harness can run them.  MAXP: the largest odd prime radix their run-time
+// FFTs handle (7, 17 or 31, kym_prime_class of the lengths): the butterfly
+// registers, and with them occupancy and spills, follow it.
 #pragma once
 #include "fft_core.h"
 
@@ -35,7 +37,7 @@ KYM_HD void load_2d(const FftOpArgs& o, const FftLen<T>& fH, const FftLen<T>& fW
 }
 
 // One transform per workgroup, whole in LDS.
-template <class T>
+template <class T, int MAXP = 31>
 KYM_HD void fftop_lds_body(const FftOpArgs& o, const FftLen<T>* lens, int lH, int lW, long long b, cpx<T>* lds,
                            int tid, int nth) {
     const FftLen<T>& fH = lens[lH];
@@ -56,8 +58,8 @@ KYM_HD void fftop_lds_body(const FftOpArgs& o, const FftLen<T>* lens, int lH, in
     }
     wg_barrier();
     if (inv) {
-        if (W > 1) fft_dif<true, 31>(lds, fW, Batch{H, 1, W, false}, tid, nth);
-        fft_dif<true, 31>(lds, fH, Batch{W, W, 1, true}, tid, nth);
+        if (W > 1) fft_dif<true, MAXP>(lds, fW, Batch{H, 1, W, false}, tid, nth);
+        fft_dif<true, MAXP>(lds, fH, Batch{W, W, 1, true}, tid, nth);
         const T inv_n = T(1) / T(n);
         if (W == 1) {
             // 1D inverse: coalesced writes, each read from its digit-reversed slot
@@ -75,8 +77,8 @@ KYM_HD void fftop_lds_body(const FftOpArgs& o, const FftLen<T>* lens, int lH, in
             else static_cast<cpx<T>*>(o.y)[dst] = lds[p] * inv_n;
         }
     } else {
-        if (W > 1) fft_dit<false, 31>(lds, fW, Batch{H, 1, W, false}, tid, nth);
-        fft_dit<false, 31>(lds, fH, Batch{W, W, 1, true}, tid, nth);
+        if (W > 1) fft_dit<false, MAXP>(lds, fW, Batch{H, 1, W, false}, tid, nth);
+        fft_dit<false, MAXP>(lds, fH, Batch{W, W, 1, true}, tid, nth);
         for (int p = tid; p < n; p += nth) static_cast<cpx<T>*>(o.y)[b * n + p] = lds[p];
     }
 }
@@ -95,7 +97,7 @@ inline long long fftop_multi_count(int H, int W, long long batch) {
 // the other in LDS and run as one batch of m rows, so a workgroup keeps its
 // threads busy on lengths far below its thread count (the frequency axis of
 // time-frequency scattering: tens of points, batches of 10^5).
-template <class T>
+template <class T, int MAXP = 31>
 KYM_HD void fftop_multi_body(const FftOpArgs& o, const FftLen<T>* lens, int lH, long long blk, cpx<T>* lds, int tid,
                              int nth) {
     const FftLen<T>& f = lens[lH];
@@ -116,7 +118,7 @@ KYM_HD void fftop_multi_body(const FftOpArgs& o, const FftLen<T>* lens, int lH, 
     }
     wg_barrier();
     if (inv) {
-        fft_dif<true, 31>(lds, f, Batch{m, 1, n, false}, tid, nth);
+        fft_dif<true, MAXP>(lds, f, Batch{m, 1, n, false}, tid, nth);
         const T inv_n = T(1) / T(n);
         for (int q = tid; q < total; q += nth) {
             const int t = q / n, p = q - t * n;
@@ -125,7 +127,7 @@ KYM_HD void fftop_multi_body(const FftOpArgs& o, const FftLen<T>* lens, int lH, 
             else static_cast<cpx<T>*>(o.y)[b0 * n + q] = v * inv_n;
         }
     } else {
-        fft_dit<false, 31>(lds, f, Batch{m, 1, n, false}, tid, nth);
+        fft_dit<false, MAXP>(lds, f, Batch{m, 1, n, false}, tid, nth);
         for (int q = tid; q < total; q += nth) static_cast<cpx<T>*>(o.y)[b0 * n + q] = lds[q];
     }
 }
@@ -155,7 +157,7 @@ struct FilterIfftArgs {
 // LDS: the m x n transform buffer, then (split > 1) split x m x n partial sums.
 // Thread q < split m n sums the aliases a = s, s + split, ... (s = q / (m n))
 // of point q mod (m n); the partials are added in the order s = 0, 1, ...
-template <class T>
+template <class T, int MAXP = 31>
 KYM_HD void filter_ifft_body(const FilterIfftArgs& o, const FftLen<T>& fl, long long blk, cpx<T>* lds, int tid,
                              int nth) {
     const int n = o.n, N = o.N, k = o.k, S = o.split;
@@ -206,7 +208,7 @@ KYM_HD void filter_ifft_body(const FilterIfftArgs& o, const FftLen<T>& fl, long 
         }
     }
     wg_barrier();
-    fft_dif<true, 31>(lds, fl, Batch{m, 1, n, false}, tid, nth);
+    fft_dif<true, MAXP>(lds, fl, Batch{m, 1, n, false}, tid, nth);
     const T inv_n = T(1) / T(n);
     for (int q = tid; q < total; q += nth) {  // coalesced writes from the digit-reversed slots
         int t, p;
@@ -268,7 +270,7 @@ struct LowpassArgs {
 };
 
 // LDS: nrow x N spectrum, then nrow x n periodised spectrum.
-template <class T>
+template <class T, int MAXP = 31>
 KYM_HD void lowpass_body(const LowpassArgs& o, const FftLen<T>& fN, const FftLen<T>& fn, long long blk, cpx<T>* lds,
                          int tid, int nth) {
     const int N = o.N, n = o.n, k = o.k, S = o.S;
@@ -303,7 +305,7 @@ KYM_HD void lowpass_body(const LowpassArgs& o, const FftLen<T>& fN, const FftLen
         lds[t * N + fN.irev[e]] = {kym_ldg(x + src), T(0)};
     }
     wg_barrier();
-    fft_dit<false, 31>(lds, fN, Batch{m, 1, N, false}, tid, nth);
+    fft_dit<false, MAXP>(lds, fN, Batch{m, 1, N, false}, tid, nth);
     wg_barrier();
     cpx<T>* P = lds + (long long)o.nrow * N;
     const T inv_k = T(1) / T(k);
@@ -311,7 +313,19 @@ KYM_HD void lowpass_body(const LowpassArgs& o, const FftLen<T>& fN, const FftLen
         const int t = q / n, p = q - t * n;
         const cpx<T>* X = lds + t * N + p;
         cpx<T> acc = {T(0), T(0)};
-        for (int a = 0; a < k; ++a) {
+        int a = 0;
+        for (; a + 7 < k; a += 8) {  // the filter loads of a round issued together
+            T w[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) w[u] = kym_ldg(f + p + (a + u) * n);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const cpx<T> v = X[(a + u) * n];
+                acc.re += v.re * w[u];
+                acc.im += v.im * w[u];
+            }
+        }
+        for (; a < k; ++a) {
             const T w = kym_ldg(f + p + a * n);
             const cpx<T> v = X[a * n];
             acc.re += v.re * w;
@@ -320,7 +334,7 @@ KYM_HD void lowpass_body(const LowpassArgs& o, const FftLen<T>& fN, const FftLen
         P[q] = acc * inv_k;
     }
     wg_barrier();
-    fft_dif<true, 31>(P, fn, Batch{m, 1, n, false}, tid, nth);
+    fft_dif<true, MAXP>(P, fn, Batch{m, 1, n, false}, tid, nth);
     wg_barrier();
     const T inv_n = T(1) / T(n);
     for (int q = tid; q < m * n; q += nth) {
@@ -376,7 +390,7 @@ struct FftPassIO {
     int scale_last;  // multiply by 1/(H W) on this pass
 };
 
-template <class T>
+template <class T, int MAXP = 31>
 KYM_HD void fftop_pass_body(const FftOpArgs& o, const FftPassIO& io, const FftLen<T>* lens, int lH, int lW, int axis,
                             long long task, cpx<T>* lds, int tid, int nth) {
     const long long b = task / o.nblk;
@@ -398,8 +412,8 @@ KYM_HD void fftop_pass_body(const FftOpArgs& o, const FftPassIO& io, const FftLe
         else lds[i] = static_cast<const cpx<T>*>(io.src)[b * n + idx];
     }
     wg_barrier();
-    if (inv) fft_dif<true, 31>(lds, f, Batch{nt, 1, len, false}, tid, nth);
-    else fft_dit<false, 31>(lds, f, Batch{nt, 1, len, false}, tid, nth);
+    if (inv) fft_dif<true, MAXP>(lds, f, Batch{nt, 1, len, false}, tid, nth);
+    else fft_dit<false, MAXP>(lds, f, Batch{nt, 1, len, false}, tid, nth);
     const T sc = io.scale_last ? T(1) / T(n) : T(1);
     for (int i = tid; i < nt * len; i += nth) {
         const int t = i / len, e = i - (i / len) * len;
