@@ -281,6 +281,16 @@ int kym_op_filter_ifft(const void* x, const void* f, void* y, long long batch, l
  * kym_op_filter_ifft.  Status 3 if tlen does not divide batch. */
 int kym_op_filter_ifft_t(const void* x, const void* f, void* y, long long batch, long long N, int k, int mode,
                          long long tlen, int dtype, void* stream);
+/* kym_op_filter_ifft over grouped rows: nf > 1 filters the x rows with a
+ * bank of nf filters f (nf, N) -- row r is x row r / nf with filter r mod nf,
+ * y (rows, n) -- the per-n1 filters of the first order of
+ * kymatio/scattering1d/core/timefrequency_scattering.py:133-153 for all n1 of
+ * one j1 at once; grp > 0 writes row r to output row (r / grp) gtot + goff +
+ * r mod grp, so a group's (B, grp) rows land in the (B, gtot) stack that
+ * timefrequency_scattering.py:157 and :184 build with backend.stack.
+ * Status 3 if nf does not divide rows or the row map does not fit. */
+int kym_op_filter_ifft_g(const void* x, const void* f, void* y, long long rows, long long N, int k, int mode, int nf,
+                         int grp, int gtot, int goff, int dtype, void* stream);
 /* Low-pass of real lines in one kernel: y = Re ifft_n(subsample_fourier(
  * cdgmm(fft_N(x), f), k)), n = N / k -- rfft, cdgmm, subsample_fourier and
  * irfft of kymatio/scattering1d/backend/torch_backend.py:118-136 with

@@ -288,7 +288,7 @@ static int emu_op_fft(const void* x, void* y, void* work, long long batch, int H
 
 template <class T>
 static int emu_filter_ifft(const void* x, const void* f, void* y, long long batch, long long N, int k, int mode,
-                           int tlen = 0) {
+                           int tlen = 0, int nf = 1, int grp = 0, int gtot = 0, int goff = 0) {
     if (k < 1 || N % k) { g_emu_error = "subsampling factor must divide the size"; return KYM_ERR_RUNTIME; }
     const int n = (int)(N / k);
     LenTable<T> lt;
@@ -301,7 +301,7 @@ static int emu_filter_ifft(const void* x, const void* f, void* y, long long batc
     const FftLen<T>* lens = reinterpret_cast<const FftLen<T>*>(b.bytes.data() + lens_off);
     int nt = 256, m = 1, split = 1;
     const long long elems = filter_ifft_shape(n, k, batch, &nt, &m, &split, tlen);
-    FilterIfftArgs o = {x, f, y, batch, (int)N, k, n, mode, m, split, tlen};
+    FilterIfftArgs o = {x, f, y, batch, (int)N, k, n, mode, m, split, tlen, nf, grp, gtot, goff};
     std::vector<cpx<T>> lds((size_t)elems + 1);
     for (long long blk = 0; blk < (batch + m - 1) / m; ++blk) filter_ifft_body<T>(o, lens[l], blk, lds.data(), 0, 1);
     return 0;
@@ -319,6 +319,17 @@ extern "C" int kym_emu_filter_ifft_t(const void* x, const void* f, void* y, long
     if (tlen < 0 || (tlen > 0 && batch % tlen)) { g_emu_error = "the rows must be a whole number of lines"; return 3; }
     if (dtype == 0) return emu_filter_ifft<float>(x, f, y, batch, N, k, mode, (int)tlen);
     return emu_filter_ifft<double>(x, f, y, batch, N, k, mode, (int)tlen);
+}
+
+// kym_op_filter_ifft_g on the CPU (one thread per workgroup).
+extern "C" int kym_emu_filter_ifft_g(const void* x, const void* f, void* y, long long rows, long long N, int k,
+                                     int mode, int nf, int grp, int gtot, int goff, int dtype) {
+    if (nf < 1 || rows % nf || grp < 0 || (grp > 0 && (rows % grp || gtot < goff + grp))) {
+        g_emu_error = "invalid rows";
+        return 3;
+    }
+    if (dtype == 0) return emu_filter_ifft<float>(x, f, y, rows, N, k, mode, 0, nf, grp, gtot, goff);
+    return emu_filter_ifft<double>(x, f, y, rows, N, k, mode, 0, nf, grp, gtot, goff);
 }
 
 template <class T>

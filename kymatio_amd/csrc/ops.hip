@@ -318,9 +318,16 @@ static int fft_op(const void* x, void* y, void* work, long long batch, int H, in
     }
 }
 
+// Row layout of a filter_ifft launch (FilterIfftArgs): transposed rows,
+// filter bank, output row map.
+struct FilterIfftRows {
+    int tlen, nf, grp, gtot, goff;
+};
+
 template <class T, int MAXP>
 static int filter_ifft_op_p(const void* x, const void* f, void* y, long long batch, long long N, int k, int mode,
-                            int tlen, hipStream_t s) {
+                            FilterIfftRows rl, hipStream_t s) {
+    const int tlen = rl.tlen;
     const int n = (int)(N / k);
     OpTables t;
     int rc = op_tables<T>(n, 1, &t);
@@ -334,7 +341,7 @@ static int filter_ifft_op_p(const void* x, const void* f, void* y, long long bat
     if (lds > 65536) OPS_HIP_TRY(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const long long nblk = (batch + m - 1) / m;
     if (nblk > 0x7fffffffll) return kym_ops_fail(KYM_ERR_VALUE, "batch too large");
-    FilterIfftArgs o = {x, f, y, batch, (int)N, k, n, mode, m, split, tlen};
+    FilterIfftArgs o = {x, f, y, batch, (int)N, k, n, mode, m, split, tlen, rl.nf, rl.grp, rl.gtot, rl.goff};
     if (nt == 1024)
         hipLaunchKernelGGL((k_filter_ifft<T, 1024, MAXP>), dim3((unsigned)nblk), dim3(1024), lds, s, o, lens, t.lH);
     else
@@ -345,11 +352,11 @@ static int filter_ifft_op_p(const void* x, const void* f, void* y, long long bat
 
 template <class T>
 static int filter_ifft_op(const void* x, const void* f, void* y, long long batch, long long N, int k, int mode,
-                          int tlen, hipStream_t s) {
+                          FilterIfftRows rl, hipStream_t s) {
     switch (op_pclass((int)(N / k))) {
-        case 7: return filter_ifft_op_p<T, 7>(x, f, y, batch, N, k, mode, tlen, s);
-        case 17: return filter_ifft_op_p<T, 17>(x, f, y, batch, N, k, mode, tlen, s);
-        default: return filter_ifft_op_p<T, 31>(x, f, y, batch, N, k, mode, tlen, s);
+        case 7: return filter_ifft_op_p<T, 7>(x, f, y, batch, N, k, mode, rl, s);
+        case 17: return filter_ifft_op_p<T, 17>(x, f, y, batch, N, k, mode, rl, s);
+        default: return filter_ifft_op_p<T, 31>(x, f, y, batch, N, k, mode, rl, s);
     }
 }
 
@@ -583,8 +590,26 @@ extern "C" int kym_op_filter_ifft_t(const void* x, const void* f, void* y, long 
     if (batch <= 0) return KYM_OK;
     if (!x || !f || !y) return kym_ops_fail(KYM_ERR_VALUE, "null device pointer");
     hipStream_t s = (hipStream_t)stream;
-    if (dtype == KYM_F32) return filter_ifft_op<float>(x, f, y, batch, N, k, mode, (int)tlen, s);
-    if (dtype == KYM_F64) return filter_ifft_op<double>(x, f, y, batch, N, k, mode, (int)tlen, s);
+    const FilterIfftRows rl = {(int)tlen, 1, 0, 0, 0};
+    if (dtype == KYM_F32) return filter_ifft_op<float>(x, f, y, batch, N, k, mode, rl, s);
+    if (dtype == KYM_F64) return filter_ifft_op<double>(x, f, y, batch, N, k, mode, rl, s);
+    return bad_dtype();
+}
+
+extern "C" int kym_op_filter_ifft_g(const void* x, const void* f, void* y, long long rows, long long N, int k,
+                                    int mode, int nf, int grp, int gtot, int goff, int dtype, void* stream) {
+    if (k < 1 || N < 1 || N % k) return kym_ops_fail(KYM_ERR_RUNTIME, "subsampling factor must divide the size");
+    if (N / k > (1 << 20)) return kym_ops_fail(KYM_ERR_UNSUPPORTED, "transform length too large for LDS");
+    if (mode < 0 || mode > 2) return kym_ops_fail(KYM_ERR_VALUE, "mode must be 0 (complex), 1 (real) or 2 (modulus)");
+    if (nf < 1 || rows % nf) return kym_ops_fail(KYM_ERR_VALUE, "rows must be a whole number of filter banks");
+    if (grp < 0 || (grp > 0 && (rows % grp || goff < 0 || gtot < goff + grp)))
+        return kym_ops_fail(KYM_ERR_VALUE, "invalid output row map");
+    if (rows <= 0) return KYM_OK;
+    if (!x || !f || !y) return kym_ops_fail(KYM_ERR_VALUE, "null device pointer");
+    hipStream_t s = (hipStream_t)stream;
+    const FilterIfftRows rl = {0, nf, grp, gtot, goff};
+    if (dtype == KYM_F32) return filter_ifft_op<float>(x, f, y, rows, N, k, mode, rl, s);
+    if (dtype == KYM_F64) return filter_ifft_op<double>(x, f, y, rows, N, k, mode, rl, s);
     return bad_dtype();
 }
 

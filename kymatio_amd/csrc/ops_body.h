@@ -152,7 +152,18 @@ struct FilterIfftArgs {
     // laid out (batch / tlen, n, tlen), the transform axis before t (the
     // workgroup's rows then lie within one b: nrow divides tlen)
     int tlen;
+    // nf > 1: a bank of nf filters (nf, N); row r filters x row r / nf with
+    // filter r mod nf.  grp > 0 (tlen = 0): row r is written to output row
+    // (r / grp) gtot + goff + r mod grp (rows of a (B, grp) block placed in a
+    // (B, gtot) output at goff).
+    int nf, grp, gtot, goff;
 };
+
+KYM_HD KYM_INL long long filter_ifft_out_row(const FilterIfftArgs& o, long long r) {
+    if (o.grp <= 0) return r;
+    const long long b = r / o.grp;
+    return b * o.gtot + o.goff + (r - b * o.grp);
+}
 
 // LDS: the m x n transform buffer, then (split > 1) split x m x n partial sums.
 // Thread q < split m n sums the aliases a = s, s + split, ... (s = q / (m n))
@@ -171,7 +182,9 @@ KYM_HD void filter_ifft_body(const FilterIfftArgs& o, const FftLen<T>& fl, long 
     for (int q = tid; q < S * total; q += nth) {
         const int s = q / total, e = q - s * total;
         const int t = e / n, p = e - t * n;
-        const cpx<T>* xr = x + (b0 + t) * N + p;
+        const long long row = b0 + t;
+        const cpx<T>* xr = x + (o.nf > 1 ? row / o.nf : row) * N + p;
+        const T* fr = o.nf > 1 ? f + (row % o.nf) * N : f;
         cpx<T> acc = {T(0), T(0)};
         int a = s;
         // eight aliases per round: their loads are issued before the first
@@ -181,7 +194,7 @@ KYM_HD void filter_ifft_body(const FilterIfftArgs& o, const FftLen<T>& fl, long 
             cpx<T> v[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
-                w[u] = kym_ldg(f + p + (a + u * S) * n);
+                w[u] = kym_ldg(fr + p + (a + u * S) * n);
                 v[u] = kym_ldg(xr + (long long)(a + u * S) * n);
             }
 #pragma unroll
@@ -191,7 +204,7 @@ KYM_HD void filter_ifft_body(const FilterIfftArgs& o, const FftLen<T>& fl, long 
             }
         }
         for (; a < k; a += S) {
-            const T w = f[p + a * n];
+            const T w = fr[p + a * n];
             const cpx<T> v = xr[(long long)a * n];
             acc.re += v.re * w;
             acc.im += v.im * w;
@@ -221,7 +234,7 @@ KYM_HD void filter_ifft_body(const FilterIfftArgs& o, const FftLen<T>& fl, long 
         } else {
             t = q / n;
             p = q - t * n;
-            dst = b0 * n + q;
+            dst = filter_ifft_out_row(o, b0 + t) * n + p;
         }
         const cpx<T> v = lds[t * n + fl.irev[p]];
         if (o.mode == 0) static_cast<cpx<T>*>(o.y)[dst] = v * inv_n;

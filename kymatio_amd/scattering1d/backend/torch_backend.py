@@ -122,6 +122,53 @@ class TorchBackend1D(TorchBackend):
         return y
 
     @classmethod
+    def filter_ifft_bank(cls, x, bank, k, mode='complex'):
+        """filter_ifft of x (..., N, 2) with every filter of bank (G, N):
+        (..., G, N / k, 2 | 1), one HIP kernel (kym_op_filter_ifft_g) for
+        what the first order does filter by filter."""
+        cls.complex_check(x)
+        cls.contiguous_check(x)
+        _gpu(x)
+        N, G = x.shape[-2], bank.shape[0]
+        if N % k:
+            raise RuntimeError('subsampling factor must divide the size')
+        code = {'complex': 0, 'real': 1, 'modulus': 2}[mode]
+        f = bank.contiguous()
+        if f.dtype != x.dtype or f.numel() != G * N:
+            raise TypeError('Input and filters must be of the same dtype and length.')
+        y = torch.empty(x.shape[:-2] + (G, N // k, 2 if code == 0 else 1), dtype=x.dtype, device=x.device)
+        with torch.cuda.device(x.device):
+            _lib.check(_lib.lib().kym_op_filter_ifft_g(_ptr(x), _ptr(f), _ptr(y), (x.numel() // (2 * N)) * G, N, k,
+                                                        code, G, 0, 0, 0, _dt(x), _lib.stream_of(x.device)))
+        return y
+
+    @classmethod
+    def filter_ifft_into(cls, x, filt, k, mode, out, goff, gtot):
+        """filter_ifft of x (..., G, N, 2) written into out (..., gtot, N / k,
+        2 | 1) at [..., goff:goff + G, :, :] (kym_op_filter_ifft_g's row
+        map): groups of n1 stacked as backend.stack would, without the copy.
+        out None: a new output."""
+        cls.complex_check(x)
+        cls.contiguous_check(x)
+        _gpu(x)
+        G, N = x.shape[-3], x.shape[-2]
+        if N % k:
+            raise RuntimeError('subsampling factor must divide the size')
+        code = {'complex': 0, 'real': 1, 'modulus': 2}[mode]
+        shape = x.shape[:-3] + (gtot, N // k, 2 if code == 0 else 1)
+        if out is None:
+            out = torch.empty(shape, dtype=x.dtype, device=x.device)
+        if tuple(out.shape) != tuple(shape) or not out.is_contiguous() or goff < 0 or goff + G > gtot:
+            raise RuntimeError('output does not fit the rows')
+        f = filt.contiguous()
+        if f.dtype != x.dtype or f.numel() != N:
+            raise TypeError('Input and filter must be of the same dtype and length.')
+        with torch.cuda.device(x.device):
+            _lib.check(_lib.lib().kym_op_filter_ifft_g(_ptr(x), _ptr(f), _ptr(out), x.numel() // (2 * N), N, k,
+                                                        code, 1, G, gtot, goff, _dt(x), _lib.stream_of(x.device)))
+        return out
+
+    @classmethod
     def filter_ifft_swap(cls, x, filt, k, mode='modulus'):
         """filter_ifft along the frequency axis of x (batch, T, N, 2), then
         the time and frequency axes swapped: (batch, N / k, T, 2 | 1) -- the

@@ -34,15 +34,34 @@ def _filtered(backend, x_hat, filt, level, k, mode):
     return backend.modulus(y) if mode == 'modulus' else y
 
 
-def time_scattering_widthfirst(U_0, backend, filters, log2_stride, average_local):
+def psi1_runs(psi1, log2_stride, average_local):
+    """Runs [a, b) of consecutive n1 with one j1 (so one k1): (a, b, j1, k1)."""
+    runs = []
+    for n1, p1 in enumerate(psi1):
+        j1 = p1['j']
+        k1 = min(j1, log2_stride) if average_local else j1
+        if runs and runs[-1][2] == j1:
+            runs[-1] = (runs[-1][0], n1 + 1, j1, k1)
+        else:
+            runs.append((n1, n1 + 1, j1, k1))
+    return runs
+
+
+def time_scattering_widthfirst(U_0, backend, filters, log2_stride, average_local, banks=None):
     """Zeroth order, then S_1 stacked over n1 (batch, n1, time), then for each
-    n2 the complex Y_2 stacked over the n1 with j1 < j2 (width first)."""
+    n2 the complex Y_2 stacked over the n1 with j1 < j2 (width first).
+    banks ({a: the psi1 filters of run a, stacked}, psi1_runs) with a
+    backend that has filter_ifft_bank: every op runs once per run of n1 with
+    one j1 instead of once per n1, the same values in the same layout."""
     phi, psi1, psi2 = filters
     U_0_hat = backend.rfft(U_0)
     if average_local:
         yield {'coef': _filtered(backend, U_0_hat, phi, 0, log2_stride, 'real'), 'j': (), 'n': ()}
     else:
         yield {'coef': U_0, 'j': (), 'n': ()}
+    if banks is not None and getattr(backend, 'filter_ifft_bank', None) is not None:
+        yield from _widthfirst_by_runs(U_0_hat, backend, filters, log2_stride, average_local, banks)
+        return
 
     U_1_hats, S_1 = [], []
     for n1, p1 in enumerate(psi1):
@@ -63,6 +82,32 @@ def time_scattering_widthfirst(U_0, backend, filters, log2_stride, average_local
                 Y_2.append(_filtered(backend, U_1_hat, p2, k1, k2, 'complex'))
         if Y_2:
             yield {'coef': backend.stack(Y_2, 2), 'j': (-1, j2), 'n': (-1, n2), 'n1_max': len(Y_2)}
+
+
+def _widthfirst_by_runs(U_0_hat, backend, filters, log2_stride, average_local, banks):
+    phi, psi1, psi2 = filters
+    runs = psi1_runs(psi1, log2_stride, average_local)
+    n1_max = len(psi1)
+    U_1_hats, S_1 = [], None
+    for a, b, j1, k1 in runs:
+        U_1 = backend.filter_ifft_bank(U_0_hat, banks[a], 2 ** k1, 'modulus')
+        U_1_hat = backend.rfft(U_1)
+        U_1_hats.append((a, b, j1, k1, U_1_hat))
+        S_1 = backend.filter_ifft_into(U_1_hat, phi['levels'][k1], 2 ** (log2_stride - k1), 'real', S_1, a, n1_max)
+    yield {'coef': S_1, 'j': (-1,), 'n': (-1,), 'n1_max': n1_max}
+
+    for n2, p2 in enumerate(psi2):
+        j2 = p2['j']
+        members = [r for r in U_1_hats if j2 > r[2]]
+        count = sum(r[1] - r[0] for r in members)
+        if not count:
+            continue
+        Y_2, off = None, 0
+        for a, b, j1, k1, U_1_hat in members:
+            k2 = min(j2 - k1, log2_stride) if average_local else j2 - k1
+            Y_2 = backend.filter_ifft_into(U_1_hat, p2['levels'][k1], 2 ** k2, 'complex', Y_2, off, count)
+            off += b - a
+        yield {'coef': Y_2, 'j': (-1, j2), 'n': (-1, n2), 'n1_max': count}
 
 
 def frequency_scattering(X, backend, filters_fr, log2_stride_fr, average_local_fr, spinned, modulus=False):
@@ -93,11 +138,12 @@ def frequency_scattering(X, backend, filters_fr, log2_stride_fr, average_local_f
 
 
 def joint_timefrequency_scattering(U_0, backend, filters, log2_stride, average_local, filters_fr, log2_stride_fr,
-                                   average_local_fr, modulus=False):
+                                   average_local_fr, modulus=False, banks=None):
     """Zeroth order; frequency scattering of S_1 (real, spin >= 0); then for
     each n2 frequency scattering of Y_2 (complex, both spins).  modulus:
-    paths after the first come as their modulus (frequency_scattering)."""
-    time_gen = time_scattering_widthfirst(U_0, backend, filters, log2_stride, average_local)
+    paths after the first come as their modulus (frequency_scattering);
+    banks: see time_scattering_widthfirst."""
+    time_gen = time_scattering_widthfirst(U_0, backend, filters, log2_stride, average_local, banks)
     yield next(time_gen)
     S_1 = next(time_gen)
     yield from frequency_scattering(S_1, backend, filters_fr, log2_stride_fr, average_local_fr, spinned=False,

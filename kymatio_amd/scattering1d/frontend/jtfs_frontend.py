@@ -20,7 +20,7 @@ import torch
 
 from ...frontend.base_frontend import ScatteringBase
 from ...graphs import GraphCache
-from ..core.timefrequency_scattering import jtfs_average_and_format, joint_timefrequency_scattering
+from ..core.timefrequency_scattering import jtfs_average_and_format, joint_timefrequency_scattering, psi1_runs
 from ..filter_bank import anden_generator, scattering_filter_factory, spin
 from ..utils import parse_T
 from .torch_frontend import ScatteringTorch1D
@@ -138,7 +138,8 @@ class TimeFrequencyScatteringTorch1D(ScatteringTorch1D):
     # ------------------------------------------------------------ transform
     def _device_filters(self, device, dtype):
         """The filter dicts with their levels as device tensors (from the
-        module's buffers, so .to() / .double() carry over)."""
+        module's buffers, so .to() / .double() carry over), and the psi1
+        filters of each run of one j1 stacked into a bank (psi1_runs)."""
         bufs = self._filter_buffers()
         key = (str(device), dtype)
         stamp = tuple((b.data_ptr(), b._version) for b in bufs)
@@ -152,7 +153,9 @@ class TimeFrequencyScatteringTorch1D(ScatteringTorch1D):
 
         phi, psi1, psi2 = take(self.phi_f), [take(p) for p in self.psi1_f], [take(p) for p in self.psi2_f]
         fr = (take(self.filters_fr[0]), [take(p) for p in self.filters_fr[1]])
-        out = ([phi, psi1, psi2], fr)
+        runs = psi1_runs(psi1, self.log2_stride, self.average == 'local')
+        banks = {a: torch.stack([psi1[n]['levels'][0].reshape(-1) for n in range(a, b)]) for a, b, _, _ in runs}
+        out = ([phi, psi1, psi2], fr, banks)
         self._fr_dev[key] = (stamp, out)
         return out
 
@@ -182,11 +185,12 @@ class TimeFrequencyScatteringTorch1D(ScatteringTorch1D):
         batch_shape, signal_shape = x.shape[:-1], x.shape[-1:]
         x = B.reshape_input(x.contiguous(), signal_shape)
         U_0 = B.pad(x, pad_left=self.pad_left, pad_right=self.pad_right)
-        filters, filters_fr = self._device_filters(x.device, x.dtype)
-        # the modulus of every joint path rides on its frequential inverse
+        filters, filters_fr, banks = self._device_filters(x.device, x.dtype)
+        # the modulus of every joint path rides on its frequential inverse;
+        # the first and second orders run by runs of n1 with one j1
         U_gen = joint_timefrequency_scattering(U_0, B, filters, self.log2_stride, self.average == 'local',
                                                filters_fr, self.log2_stride_fr, self.average_fr == 'local',
-                                               modulus=True)
+                                               modulus=True, banks=banks)
         S_gen = jtfs_average_and_format(U_gen, B, filters[0], self.log2_stride, self.average, filters_fr[0],
                                         self.log2_stride_fr, self.average_fr, self.out_type, self.format,
                                         modulus_done=True)

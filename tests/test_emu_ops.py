@@ -169,3 +169,56 @@ def test_lowpass(emu, N, k, S, outer):
     ref = np.fft.ifft((X * filt[None, :, None]).reshape(outer, k, n, S).mean(axis=1), axis=1).real
     got = y[:outer * n * S].reshape(outer, n, S)
     assert np.abs(got - ref).max() <= 1e-12 * max(np.abs(ref).max(), 1)
+
+
+@pytest.mark.parametrize('N,k,B,nf,mode', [(256, 4, 3, 5, 2), (128, 1, 4, 3, 0), (64, 8, 2, 1, 1)])
+def test_filter_ifft_bank(emu, N, k, B, nf, mode):
+    """kym_op_filter_ifft_g with a bank of nf filters: row r = x row r / nf
+    with filter r mod nf, against numpy FP64 of ifft(P_k(x f))."""
+    f = emu.lib.kym_emu_filter_ifft_g
+    f.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_longlong] * 2 + [ctypes.c_int] * 7
+    f.restype = ctypes.c_int
+    rng = np.random.default_rng(N + k + B + nf)
+    x = rng.standard_normal((B, N)) + 1j * rng.standard_normal((B, N))
+    bank = rng.standard_normal((nf, N))
+    n = N // k
+    xin = np.stack([x.real, x.imag], -1).reshape(-1).copy()
+    c = 2 if mode == 0 else 1
+    out_len = B * nf * n * c
+    y = _guarded(out_len, np.float64)
+    assert f(xin.ctypes.data, bank.ctypes.data, y.ctypes.data, B * nf, N, k, mode, nf, 0, 0, 0, 1) == 0
+    assert np.all(y[out_len:] == 12345.0), 'write past the output'
+    ref = np.fft.ifft((x[:, None, :] * bank[None]).reshape(B, nf, k, n).mean(axis=2), axis=-1)
+    if mode == 0:
+        got = y[:out_len].reshape(B, nf, n, 2)
+        got = got[..., 0] + 1j * got[..., 1]
+    else:
+        got = y[:out_len].reshape(B, nf, n)
+        ref = ref.real if mode == 1 else np.abs(ref)
+    assert np.abs(got - ref).max() <= 1e-12 * max(np.abs(ref).max(), 1)
+
+
+def test_filter_ifft_row_map(emu):
+    """kym_op_filter_ifft_g's output row map: the rows of a (B, G) block land
+    at [:, goff:goff + G] of a (B, gtot) output and nothing else is written."""
+    f = emu.lib.kym_emu_filter_ifft_g
+    f.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_longlong] * 2 + [ctypes.c_int] * 7
+    f.restype = ctypes.c_int
+    B, G, gtot, goff, N, k = 3, 4, 9, 2, 64, 2
+    n = N // k
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal((B, G, N)) + 1j * rng.standard_normal((B, G, N))
+    filt = rng.standard_normal(N)
+    xin = np.stack([x.real, x.imag], -1).reshape(-1).copy()
+    y = _guarded(B * gtot * n * 2, np.float64)
+    y[:B * gtot * n * 2] = -7.0
+    assert f(xin.ctypes.data, filt.ctypes.data, y.ctypes.data, B * G, N, k, 0, 1, G, gtot, goff, 1) == 0
+    out = y[:B * gtot * n * 2].reshape(B, gtot, n, 2)
+    ref = np.fft.ifft((x * filt).reshape(B, G, k, n).mean(axis=2), axis=-1)
+    got = out[:, goff:goff + G, :, 0] + 1j * out[:, goff:goff + G, :, 1]
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    untouched = np.ones(gtot, bool)
+    untouched[goff:goff + G] = False
+    assert np.all(out[:, untouched] == -7.0)
+    assert np.all(y[B * gtot * n * 2:] == 12345.0)
+    assert f(xin.ctypes.data, filt.ctypes.data, y.ctypes.data, B * G, N, k, 0, 1, G, G + 1, 2, 1) != 0
