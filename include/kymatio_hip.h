@@ -58,9 +58,11 @@ int kym_scat2d_create(int M, int N, int J, int L, int max_order, int pre_pad, in
  * 2..4, default 2: the workspace is split in S parts and chunks rotate over
  * the caller's stream and S - 1 streams of the plan, joined back to the
  * caller's stream before kym_scat2d_forward's work ends), pair (1:
- * one LDS node per workgroup; 2, the default: sibling nodes of one parent
- * share a workgroup and one read of the parent's spectrum, where a kernel
- * for the size is compiled)}; ntune <= 7; 0 = built-in heuristic.  Results
+ * one LDS node per workgroup; 2: sibling nodes of one parent share a
+ * workgroup and one read of the parent's spectrum wherever a pair kernel
+ * for the size is compiled; 0, the default: where that is the default
+ * kernel for the size, or the launch's deepest periodisation is 16 or
+ * more)}; ntune <= 7; 0 = built-in heuristic.  Results
  * are the same for every choice up to FP32 rounding. */
 int kym_scat2d_create_tuned(int M, int N, int J, int L, int max_order, int pre_pad, int dtype,
                             long long lds_limit, const int* tune, int ntune, void** plan);

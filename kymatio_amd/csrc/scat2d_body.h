@@ -930,18 +930,25 @@ KYM_HD KYM_INL void gather_pair_s(cpx<T>* da, cpx<T>* db, const cpx<T>* U, const
             for (int s = 0; s < 2; ++s) {
                 const int yy = s ? ym : y, xx = s ? xm : x;
                 cpx<T> va = {T(0), T(0)}, vb = {T(0), T(0)};
+                auto term = [&](int al, int b) {
+                    const int idx = (yy + al * H) * Wp + xx + b * W;
+                    const cpx<T> uv = kym_ldg(U + idx);
+                    const T wa = kym_ldg(fa + idx), wb = kym_ldg(fb + idx);
+                    va.re += uv.re * wa;
+                    va.im += uv.im * wa;
+                    vb.re += uv.re * wb;
+                    vb.im += uv.im * wb;
+                };
+                if constexpr (K <= 4) {
 #pragma unroll
-                for (int al = 0; al < K; ++al)
+                    for (int al = 0; al < K; ++al)
 #pragma unroll
-                    for (int b = 0; b < K; ++b) {
-                        const int idx = (yy + al * H) * Wp + xx + b * W;
-                        const cpx<T> uv = kym_ldg(U + idx);
-                        const T wa = kym_ldg(fa + idx), wb = kym_ldg(fb + idx);
-                        va.re += uv.re * wa;
-                        va.im += uv.im * wa;
-                        vb.re += uv.re * wb;
-                        vb.im += uv.im * wb;
-                    }
+                        for (int b = 0; b < K; ++b) term(al, b);
+                } else {  // K^2 terms: rows rolled, four columns per step
+                    for (int al = 0; al < K; ++al)
+#pragma unroll 4
+                        for (int b = 0; b < K; ++b) term(al, b);
+                }
                 if (s) { m[0] = va; m[1] = vb; } else { d[0] = va; d[1] = vb; }
             }
         }
@@ -986,10 +993,13 @@ KYM_HD KYM_INL void body_lds_pair_s(const Scat2DArgs<T>& a, int task, cpx<T>* ld
     const T* fa = a.filt + a.filt_off[na.filt];
     const T* fb = a.filt + a.filt_off[nb.filt];
     const T scale = T(1) / T(na.k * na.k);
-    switch (na.k) {
+    switch (na.k) {  // the plan pairs k = 2^i <= 32
         case 1: gather_pair_s<1, H, W, NT>(buf, stash, U, fa, fb, scale, tid); break;
         case 2: gather_pair_s<2, H, W, NT>(buf, stash, U, fa, fb, scale, tid); break;
-        default: gather_pair_s<4, H, W, NT>(buf, stash, U, fa, fb, scale, tid); break;  // the plan pairs k <= 4
+        case 4: gather_pair_s<4, H, W, NT>(buf, stash, U, fa, fb, scale, tid); break;
+        case 8: gather_pair_s<8, H, W, NT>(buf, stash, U, fa, fb, scale, tid); break;
+        case 16: gather_pair_s<16, H, W, NT>(buf, stash, U, fa, fb, scale, tid); break;
+        default: gather_pair_s<32, H, W, NT>(buf, stash, U, fa, fb, scale, tid); break;
     }
     wg_barrier();
     node_after_gather(a, g, na, il, buf, tid, NT);
@@ -1308,7 +1318,9 @@ KYM_HD void body_fwd_cols(const P& fp, const Scat2DArgs<T>& a, int task, cpx<T>*
 // node kernel (FP32).  A launch uses one when its sizes match; every other
 // size runs the run-time-sized bodies.  pair = 1: sibling pairs per
 // workgroup (body_lds_pair_s; sizes whose LDS holds the second node's
-// spectrum beside the first node).
+// spectrum beside the first node); pair = 2: the same, used by default only
+// where a launch's deepest periodisation is k >= 16, or when the plan asks
+// for every pair kernel (tuning field pair = 2).
 #define KYM_NODE_SPECS(X)              \
     X(0, 136, 136, 34, 34, 1024, 0)    \
     X(1, 68, 68, 34, 34, 512, 0)       \
@@ -1319,7 +1331,10 @@ KYM_HD void body_fwd_cols(const P& fp, const Scat2DArgs<T>& a, int task, cpx<T>*
     X(6, 80, 80, 10, 10, 256, 0)       \
     X(7, 136, 136, 34, 34, 512, 0)     \
     X(8, 68, 68, 34, 34, 512, 1)       \
-    X(9, 68, 68, 34, 34, 256, 1)
+    X(9, 68, 68, 34, 34, 256, 1)       \
+    X(10, 20, 20, 10, 10, 256, 2)      \
+    X(11, 40, 40, 10, 10, 256, 2)      \
+    X(12, 80, 80, 10, 10, 512, 2)
 
 // want_threads = 0: the first (default) specialisation of these sizes.
 inline int node_spec_id(int H, int W, int HJ, int WJ, int want_threads, int* threads, int want_pair = 0) {

@@ -30,7 +30,8 @@ struct Scat2DTune {
     int row_block = 0;      // rows per workgroup of the big-node row passes
     int pass_threads = 0;   // threads of the big-node pass workgroups (compiled choices only)
     int streams = 0;        // inference chunks in flight: 1 to 4 (0 = default 2; see kym_scat2d_forward)
-    int pair = 0;           // LDS nodes: 1 one node per workgroup, 2 sibling pairs where compiled (0 = heuristic: 2)
+    int pair = 0;           // LDS nodes: 1 one node per workgroup, 2 sibling pairs wherever a pair kernel is
+                            // compiled (0 = heuristic: the default pair kernels, the others from k = 16)
 };
 
 template <class T>
@@ -387,7 +388,7 @@ struct Scat2DPlan {
             if (!n0.big) {
                 const size_t lds = fir ? node_lds(n0) : (elems + (n0.leaf_mode == 0 ? leaf_elems : 0)) * cs;
                 // sibling pairs: the nodes sorted by parent, every parent with an even
-                // number of children here, periodisation k <= 4, a compiled pair kernel
+                // number of children here, periodisation k <= 32, a compiled pair kernel
                 if (tune.pair != 1 && specs && sizeof(T) == 4 && n0.fir == 1) {
                     std::vector<int> sib = ids;
                     std::stable_sort(sib.begin(), sib.end(),
@@ -396,10 +397,16 @@ struct Scat2DPlan {
                     for (size_t i = 0; ok && i < sib.size(); i += 2) {
                         const Node2D& u = nodes[sib[i]];
                         const Node2D& v = nodes[sib[i + 1]];
-                        ok = u.order >= 1 && u.parent == v.parent && u.k == v.k && u.k <= 4 && v.fir == 1;
+                        ok = u.order >= 1 && u.parent == v.parent && u.k == v.k && u.k <= 32 && v.fir == 1;
                     }
-                    int pnt = 0;
-                    const int psid = ok ? node_spec_id(H[r], W[r], H[J], W[J], tune.node_threads, &pnt, 1) : -1;
+                    int pnt = 0, kmax = 1;
+                    for (int id : sib) kmax = std::max(kmax, nodes[id].k);
+                    // the other pair kernels by default where the deepest periodisation
+                    // of the launch is 16 or more (measured: they gain at 256² J=5 and
+                    // 512² J=6, and lose at 128² J=4, k <= 8)
+                    int psid = ok ? node_spec_id(H[r], W[r], H[J], W[J], tune.node_threads, &pnt, 1) : -1;
+                    if (ok && psid < 0 && (tune.pair == 2 || kmax >= 16))
+                        psid = node_spec_id(H[r], W[r], H[J], W[J], tune.node_threads, &pnt, 2);
                     // twiddles, the node region, the second node's stash
                     const size_t need = cs * kym_tw_elems(H[r], W[r]) + (lds + cs - 1) / cs * cs + elems * cs;
                     if (psid >= 0 && need <= lds_limit) {

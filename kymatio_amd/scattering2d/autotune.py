@@ -17,7 +17,7 @@ import torch
 DEFAULT_CACHE = os.path.join(os.path.expanduser('~'), '.cache', 'kymatio_amd', 'autotune2d.json')
 
 # (max_pow2, node_threads, col_block, row_block, pass_threads, lds_limit,
-#  images_per_chunk[, streams[, pair]]) - 0 = heuristic (streams: 2, pair: 2)
+#  images_per_chunk[, streams[, pair]]) - 0 = heuristic (streams: 2, pair: the default pair kernels)
 FIELDS = ('max_pow2_radix', 'node_threads', 'col_block', 'row_block', 'pass_threads', 'lds_limit', 'images_per_chunk',
           'streams', 'pair')
 CANDIDATES = [
@@ -43,6 +43,8 @@ CANDIDATES = [
     (16, 0, 8, 8, 0, 0, 0, 4),
     (16, 0, 8, 8, 0, 0, 0, 0, 1),
     (16, 0, 8, 8, 192, 0, 0, 0, 1),
+    (16, 0, 0, 0, 0, 0, 0, 0, 2),
+    (16, 0, 8, 8, 0, 0, 0, 0, 2),
 ]
 
 _TABLE = {}

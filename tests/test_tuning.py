@@ -126,3 +126,31 @@ def test_compiled_passes_at_other_sizes(emu, side):
     out, info = emu.scat2d(x, J, (side, side), f, tune=[16, 0, 8, 8])
     assert info['spec_launches'] >= 10  # the big-node passes included
     assert channel_rel_err(out, G['s%d_Sx' % side][:1]) < RTOL_CHANNEL
+
+
+def test_opt_in_pair_kernels(emu):
+    """pair = 2 also runs the 80 x 80, 40 x 40 and 20 x 20 nodes as sibling
+    pairs (periodisations up to k = 8 at 128x128 J=4, up to 16 at 256x256
+    J=5): outputs against the reference's (cfg4) and the oracle's
+    (golden_mixed2d.npz).  The default plan keeps one node per workgroup at
+    128x128 J=4 and pairs the 20 x 20 nodes of 256x256 J=5 (k up to 16)."""
+    tag, x, J, shape, kw, Sx = _case('cfg4')
+    Mp, Np = compute_padding(shape[0], shape[1], J)
+    f = pack_filters_2d(filter_bank(Mp, Np, J, 8))
+    out, info = emu.scat2d(x[:2], J, shape, f, tune=[16, 0, 0, 0, 0, 0, 2], chunk=2)
+    assert info['pair_launches'] == 3
+    assert channel_rel_err(out, Sx[:2]) < RTOL_CHANNEL
+    _, info0 = emu.scat2d(x[:1], J, shape, f)
+    assert info0['pair_launches'] == 0
+    from tests.helpers import golden
+    G = golden('golden_mixed2d.npz')
+    J = int(G['s256_J'])
+    x = np.random.default_rng(int(G['s256_seed'])).random((1, 256, 256), dtype=np.float32)
+    Mp, Np = compute_padding(256, 256, J)
+    f = pack_filters_2d(filter_bank(Mp, Np, J, 8))
+    out, info = emu.scat2d(x, J, (256, 256), f, tune=[16, 0, 8, 8, 0, 0, 2])
+    assert info['pair_launches'] == 3
+    assert channel_rel_err(out, G['s256_Sx'][:1]) < RTOL_CHANNEL
+    out, info = emu.scat2d(x, J, (256, 256), f, tune=[16, 0, 8, 8])
+    assert info['pair_launches'] == 1
+    assert channel_rel_err(out, G['s256_Sx'][:1]) < RTOL_CHANNEL
