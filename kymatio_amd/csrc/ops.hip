@@ -202,6 +202,13 @@ __global__ void __launch_bounds__(256) k_lowpass(LowpassArgs o, const FftLen<T>*
     lowpass_body<T, MAXP>(o, lens[lN], lens[ln], blockIdx.x, reinterpret_cast<cpx<T>*>(kym_smem_lp), threadIdx.x, 256);
 }
 
+template <class T, int N, int NN>
+__global__ void __launch_bounds__(256) k_lowpass_s(LowpassArgs o, const FftLen<T>* lens, int lN, int ln) {
+    extern __shared__ __attribute__((aligned(16))) char kym_smem_lps[];
+    lowpass_body_s<T, N, NN, 256>(o, lens[lN], lens[ln], blockIdx.x, reinterpret_cast<cpx<T>*>(kym_smem_lps),
+                                  threadIdx.x);
+}
+
 template <class T, int MAXP>
 __global__ void __launch_bounds__(256) k_fft2_pass(FftOpArgs o, FftPassIO io, const FftLen<T>* lens, int lH, int lW,
                                                    int axis) {
@@ -380,8 +387,36 @@ static int lowpass_op_p(const void* x, const void* f, void* y, long long outer, 
     return KYM_OK;
 }
 
+// The compile-time low-pass for (N, n), or nullptr.
+template <class T>
+static const void* lowpass_spec_kernel(int id) {
+    if constexpr (sizeof(T) != 4) return nullptr;  // FP32 only
+    else switch (id) {
+#define KYM_LP_K(id, nn, n2) \
+    case id: return (const void*)k_lowpass_s<T, nn, n2>;
+        KYM_LOWPASS_SPECS(KYM_LP_K)
+#undef KYM_LP_K
+        default: return nullptr;
+    }
+}
+
 template <class T>
 static int lowpass_op(const void* x, const void* f, void* y, long long outer, int N, int S, int k, hipStream_t s) {
+    if (const void* kern = lowpass_spec_kernel<T>(lowpass_spec_id(N, N / k, S))) {
+        const int n = N / k, M = N >= 4096 ? 1 : 4096 / N;
+        OpTables t;
+        int rc = op_tables<T>(N, n, &t);
+        if (rc != KYM_OK) return rc;
+        const FftLen<T>* lens = reinterpret_cast<const FftLen<T>*>((char*)t.dev + t.lens_off);
+        const size_t lds = (size_t)M * (N + n) * sizeof(cpx<T>);
+        if (lds > 65536) OPS_HIP_TRY(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        const long long nblk = (outer + M - 1) / M;
+        if (nblk > 0x7fffffffll) return kym_ops_fail(KYM_ERR_VALUE, "batch too large");
+        LowpassArgs o = {x, f, y, outer, N, S, k, n, M, 1};
+        void* args[] = {&o, (void*)&lens, &t.lH, &t.lW};
+        OPS_HIP_TRY(hipLaunchKernel(kern, dim3((unsigned)nblk), dim3(256), args, lds, s));
+        return KYM_OK;
+    }
     switch (op_pclass(N, N / k)) {
         case 7: return lowpass_op_p<T, 7>(x, f, y, outer, N, S, k, s);
         case 17: return lowpass_op_p<T, 17>(x, f, y, outer, N, S, k, s);

@@ -5,6 +5,7 @@
 // registers, and with them occupancy and spills, follow it.
 #pragma once
 #include "fft_core.h"
+#include "fft_static.h"
 
 // ------------------------------------------------------------ FFT op
 // kind: 0 c2c forward, 1 c2c inverse (1/n), 2 r2c forward (real input),
@@ -364,6 +365,79 @@ KYM_HD void lowpass_body(const LowpassArgs& o, const FftLen<T>& fN, const FftLen
         }
         y[dst] = P[t * n + fn.irev[p]].re * inv_n;
     }
+}
+
+// lowpass_body for contiguous lines (S = 1) with the lengths fixed at
+// compile time: M = 4096 / N lines of N per workgroup, the forward and the
+// inverse transforms by the static passes (constant strides and trip
+// counts, twiddles from the length tables).  Lines beyond `outer` load
+// zeros and are not stored.  Same arithmetic as lowpass_body.
+template <class T, int N, int NN, int NT>
+KYM_HD void lowpass_body_s(const LowpassArgs& o, const FftLen<T>& fN, const FftLen<T>& fn, long long blk,
+                           cpx<T>* lds, int tid) {
+    constexpr int M = N >= 4096 ? 1 : 4096 / N, K = N / NN;
+    const long long ob = blk * M;
+    const int m = (int)((o.outer - ob < M) ? o.outer - ob : M);
+    const T* x = static_cast<const T*>(o.x);
+    const T* f = static_cast<const T*>(o.f);
+    T* y = static_cast<T*>(o.y);
+    for (int i = tid; i < M * N; i += NT) {
+        const int t = i / N, e = i - t * N;
+        lds[t * N + kym_ldg(fN.irev + e)] = {t < m ? kym_ldg(x + (ob + t) * N + e) : T(0), T(0)};
+    }
+    wg_barrier();
+    fft_dit_s<false, N, M, 1, N, false, NT>(lds, fN.tw, tid);
+    cpx<T>* P = lds + M * N;
+    const T inv_k = T(1) / T(K);
+    for (int q = tid; q < M * NN; q += NT) {
+        const int t = q / NN, p = q - t * NN;
+        const cpx<T>* X = lds + t * N + p;
+        cpx<T> acc = {T(0), T(0)};
+        int a = 0;
+        for (; a + 7 < K; a += 8) {
+            T w[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) w[u] = kym_ldg(f + p + (a + u) * NN);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const cpx<T> v = X[(a + u) * NN];
+                acc.re += v.re * w[u];
+                acc.im += v.im * w[u];
+            }
+        }
+        for (; a < K; ++a) {
+            const T w = kym_ldg(f + p + a * NN);
+            const cpx<T> v = X[a * NN];
+            acc.re += v.re * w;
+            acc.im += v.im * w;
+        }
+        P[q] = acc * inv_k;
+    }
+    wg_barrier();
+    fft_dif_s<true, NN, M, 1, NN, false, NT>(P, fn.tw, tid);
+    const T inv_n = T(1) / T(NN);
+    for (int q = tid; q < m * NN; q += NT) {
+        const int t = q / NN, p = q - t * NN;
+        y[(ob + t) * NN + p] = P[t * NN + kym_ldg(fn.irev + p)].re * inv_n;
+    }
+}
+
+// Compile-time low-pass shapes X(id, N, n): time averaging of the
+// time-frequency transform at the default averaging (outputs of 128).
+#define KYM_LOWPASS_SPECS(X) \
+    X(0, 4096, 128)          \
+    X(1, 2048, 128)          \
+    X(2, 1024, 128)          \
+    X(3, 512, 128)           \
+    X(4, 256, 128)
+
+inline int lowpass_spec_id(int N, int n, int S) {
+    if (S != 1) return -1;
+#define KYM_LP_MATCH(id, nn, n2) \
+    if (N == nn && n == n2) return id;
+    KYM_LOWPASS_SPECS(KYM_LP_MATCH)
+#undef KYM_LP_MATCH
+    return -1;
 }
 
 // Launch shape of lowpass: lines per workgroup (about 4096 points, at most

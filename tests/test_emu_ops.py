@@ -222,3 +222,24 @@ def test_filter_ifft_row_map(emu):
     assert np.all(out[:, untouched] == -7.0)
     assert np.all(y[B * gtot * n * 2:] == 12345.0)
     assert f(xin.ctypes.data, filt.ctypes.data, y.ctypes.data, B * G, N, k, 0, 1, G, G + 1, 2, 1) != 0
+
+
+@pytest.mark.parametrize('N,k,outer', [(4096, 32, 3), (2048, 16, 5), (1024, 8, 6), (512, 4, 9), (256, 2, 20)])
+def test_lowpass_compiled_lengths(emu, N, k, outer):
+    """The compile-time low-pass (FP32, lines of 256 to 4096 down to 128
+    outputs, several lines per workgroup with a partial last one) against
+    numpy FP64 to FP32 rounding."""
+    f = emu.lib.kym_emu_lowpass
+    f.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_longlong] * 3 + [ctypes.c_int] * 2
+    f.restype = ctypes.c_int
+    rng = np.random.default_rng(N + k + outer)
+    x = rng.standard_normal((outer, N)).astype(np.float32)
+    filt = rng.standard_normal(N).astype(np.float32)
+    n = N // k
+    y = _guarded(outer * n, np.float32)
+    assert f(x.ctypes.data, filt.ctypes.data, y.ctypes.data, outer, N, 1, k, 0) == 0
+    assert np.all(y[outer * n:] == 12345.0), 'write past the output'
+    X = np.fft.fft(x.astype(np.float64), axis=1) * filt.astype(np.float64)
+    ref = np.fft.ifft(X.reshape(outer, k, n).mean(axis=1), axis=1).real
+    got = y[:outer * n].reshape(outer, n)
+    assert np.abs(got - ref).max() <= 2e-6 * np.abs(ref).max()
