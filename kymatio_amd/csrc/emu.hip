@@ -299,6 +299,25 @@ static int emu_filter_ifft(const void* x, const void* f, void* y, long long batc
     if (!lt.emit(b, &lens_off, &err)) { g_emu_error = err; return KYM_ERR_UNSUPPORTED; }
     b.relocate((uintptr_t)b.bytes.data());
     const FftLen<T>* lens = reinterpret_cast<const FftLen<T>*>(b.bytes.data() + lens_off);
+    {  // the compile-time body, as the GPU entry selects it
+        FilterIfftArgs o = {x, f, y, batch, (int)N, k, n, mode, 0, 1, tlen, nf, grp, gtot, goff};
+        int M = 0;
+        const int sid = sizeof(T) == 4 ? filter_ifft_spec_id(n, batch, o, &M) : -1;
+        if (sid >= 0) {
+            o.nrow = M;
+            std::vector<cpx<T>> lds((size_t)M * n + 1);
+            for (long long blk = 0; blk < (batch + M - 1) / M; ++blk) {
+                switch (sid) {
+#define KYM_FI_E(id, nn, mm) \
+    case id: filter_ifft_body_s<T, nn, mm, 1>(o, lens[l], blk, lds.data(), 0); break;
+                    KYM_FILTER_IFFT_SPECS(KYM_FI_E)
+#undef KYM_FI_E
+                    default: break;
+                }
+            }
+            return 0;
+        }
+    }
     int nt = 256, m = 1, split = 1;
     const long long elems = filter_ifft_shape(n, k, batch, &nt, &m, &split, tlen);
     FilterIfftArgs o = {x, f, y, batch, (int)N, k, n, mode, m, split, tlen, nf, grp, gtot, goff};

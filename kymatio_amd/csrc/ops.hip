@@ -202,6 +202,12 @@ __global__ void __launch_bounds__(256) k_lowpass(LowpassArgs o, const FftLen<T>*
     lowpass_body<T, MAXP>(o, lens[lN], lens[ln], blockIdx.x, reinterpret_cast<cpx<T>*>(kym_smem_lp), threadIdx.x, 256);
 }
 
+template <class T, int NN, int M>
+__global__ void __launch_bounds__(256) k_filter_ifft_s(FilterIfftArgs o, const FftLen<T>* lens, int l) {
+    extern __shared__ __attribute__((aligned(16))) char kym_smem_fis[];
+    filter_ifft_body_s<T, NN, M, 256>(o, lens[l], blockIdx.x, reinterpret_cast<cpx<T>*>(kym_smem_fis), threadIdx.x);
+}
+
 template <class T, int N, int NN>
 __global__ void __launch_bounds__(256) k_lowpass_s(LowpassArgs o, const FftLen<T>* lens, int lN, int ln) {
     extern __shared__ __attribute__((aligned(16))) char kym_smem_lps[];
@@ -358,8 +364,39 @@ static int filter_ifft_op_p(const void* x, const void* f, void* y, long long bat
 }
 
 template <class T>
+static const void* filter_ifft_spec_kernel(int id) {
+    if constexpr (sizeof(T) != 4) return nullptr;  // FP32 only
+    else switch (id) {
+#define KYM_FI_K(id, nn, mm) \
+    case id: return (const void*)k_filter_ifft_s<T, nn, mm>;
+        KYM_FILTER_IFFT_SPECS(KYM_FI_K)
+#undef KYM_FI_K
+        default: return nullptr;
+    }
+}
+
+template <class T>
 static int filter_ifft_op(const void* x, const void* f, void* y, long long batch, long long N, int k, int mode,
                           FilterIfftRows rl, hipStream_t s) {
+    {
+        const int n = (int)(N / k);
+        FilterIfftArgs o = {x, f, y, batch, (int)N, k, n, mode, 0, 1, rl.tlen, rl.nf, rl.grp, rl.gtot, rl.goff};
+        int M = 0;
+        if (const void* kern = filter_ifft_spec_kernel<T>(filter_ifft_spec_id(n, batch, o, &M))) {
+            OpTables t;
+            int rc = op_tables<T>(n, 1, &t);
+            if (rc != KYM_OK) return rc;
+            const FftLen<T>* lens = reinterpret_cast<const FftLen<T>*>((char*)t.dev + t.lens_off);
+            const size_t lds = (size_t)M * n * sizeof(cpx<T>);
+            if (lds > 65536) OPS_HIP_TRY(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            const long long nblk = (batch + M - 1) / M;
+            if (nblk > 0x7fffffffll) return kym_ops_fail(KYM_ERR_VALUE, "batch too large");
+            o.nrow = M;
+            void* args[] = {&o, (void*)&lens, &t.lH};
+            OPS_HIP_TRY(hipLaunchKernel(kern, dim3((unsigned)nblk), dim3(256), args, lds, s));
+            return KYM_OK;
+        }
+    }
     switch (op_pclass((int)(N / k))) {
         case 7: return filter_ifft_op_p<T, 7>(x, f, y, batch, N, k, mode, rl, s);
         case 17: return filter_ifft_op_p<T, 17>(x, f, y, batch, N, k, mode, rl, s);

@@ -244,6 +244,92 @@ KYM_HD void filter_ifft_body(const FilterIfftArgs& o, const FftLen<T>& fl, long 
     }
 }
 
+// filter_ifft_body with the transform length n = NN and the rows per
+// workgroup M fixed at compile time (no alias split, one filter, natural or
+// transposed rows as FilterIfftArgs says; with tlen > 0, M divides tlen).
+// Rows beyond the batch gather zeros and are not stored.  Same arithmetic
+// as filter_ifft_body without a split.
+template <class T, int NN, int M, int NT>
+KYM_HD void filter_ifft_body_s(const FilterIfftArgs& o, const FftLen<T>& fl, long long blk, cpx<T>* lds, int tid) {
+    const int N = o.N, k = o.k;
+    const long long b0 = blk * M;
+    const int m = (int)((o.batch - b0 < M) ? o.batch - b0 : M);
+    const cpx<T>* x = static_cast<const cpx<T>*>(o.x);
+    const T* f = static_cast<const T*>(o.f);
+    const T inv_k = T(1) / T(k);
+    for (int q = tid; q < M * NN; q += NT) {
+        const int t = q / NN, p = q - t * NN;
+        cpx<T> acc = {T(0), T(0)};
+        if (t < m) {
+            const cpx<T>* xr = x + (b0 + t) * N + p;
+            int a = 0;
+            for (; a + 7 < k; a += 8) {
+                T w[8];
+                cpx<T> v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    w[u] = kym_ldg(f + p + (a + u) * NN);
+                    v[u] = kym_ldg(xr + (long long)(a + u) * NN);
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    acc.re += v[u].re * w[u];
+                    acc.im += v[u].im * w[u];
+                }
+            }
+            for (; a < k; ++a) {
+                const T w = kym_ldg(f + p + a * NN);
+                const cpx<T> v = kym_ldg(xr + (long long)a * NN);
+                acc.re += v.re * w;
+                acc.im += v.im * w;
+            }
+        }
+        lds[q] = acc * inv_k;
+    }
+    wg_barrier();
+    fft_dif_s<true, NN, M, 1, NN, false, NT>(lds, fl.tw, tid);
+    const T inv_n = T(1) / T(NN);
+    for (int q = tid; q < m * NN; q += NT) {
+        int t, p;
+        long long dst;
+        if (o.tlen > 0) {
+            p = q / m;
+            t = q - p * m;
+            const long long r0 = b0 / o.tlen, t0 = b0 - r0 * o.tlen;
+            dst = (r0 * NN + p) * o.tlen + t0 + t;
+        } else {
+            t = q / NN;
+            p = q - t * NN;
+            dst = b0 * NN + q;
+        }
+        const cpx<T> v = lds[t * NN + kym_ldg(fl.irev + p)];
+        if (o.mode == 0) static_cast<cpx<T>*>(o.y)[dst] = v * inv_n;
+        else if (o.mode == 1) static_cast<T*>(o.y)[dst] = v.re * inv_n;
+        else static_cast<T*>(o.y)[dst] = kym_sqrt(v.re * v.re + v.im * v.im) * inv_n;
+    }
+}
+
+// Compile-time filter_ifft shapes X(id, n, rows per workgroup): the
+// frequency axis of the time-frequency transform (padded to 88, and 22
+// after a subsampling by 4).
+#define KYM_FILTER_IFFT_SPECS(X) \
+    X(0, 88, 32)                 \
+    X(1, 22, 128)
+
+// The compile-time shape for output length n over `batch` rows (one filter,
+// no row map; tlen a multiple of its rows), or -1.
+inline int filter_ifft_spec_id(int n, long long batch, const FilterIfftArgs& o, int* rows) {
+    if (o.nf > 1 || o.grp > 0) return -1;
+#define KYM_FI_MATCH(id, nn, mm)                                                   \
+    if (n == nn && batch >= mm && (o.tlen == 0 || o.tlen % mm == 0)) {             \
+        *rows = mm;                                                                \
+        return id;                                                                 \
+    }
+    KYM_FILTER_IFFT_SPECS(KYM_FI_MATCH)
+#undef KYM_FI_MATCH
+    return -1;
+}
+
 // Launch shape of filter_ifft for output length n over `batch` rows: threads
 // (1024 for n >= 2048, else 256), rows per workgroup (about 4096 points, but
 // fewer when that leaves under 512 workgroups) and the alias split (threads

@@ -243,3 +243,35 @@ def test_lowpass_compiled_lengths(emu, N, k, outer):
     ref = np.fft.ifft(X.reshape(outer, k, n).mean(axis=1), axis=1).real
     got = y[:outer * n].reshape(outer, n)
     assert np.abs(got - ref).max() <= 2e-6 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize('N,k,T,B,mode', [(88, 1, 64, 3, 2), (88, 4, 128, 2, 2), (88, 1, 32, 2, 0), (22, 1, 128, 3, 2),
+                                          (88, 1, 0, 70, 1)])
+def test_filter_ifft_compiled_lengths(emu, N, k, T, B, mode):
+    """The compile-time filter_ifft (FP32, outputs of 88 or 22, transposed
+    rows or natural ones with a partial last workgroup) against numpy FP64 to
+    FP32 rounding."""
+    f = emu.lib.kym_emu_filter_ifft_t
+    f.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_longlong] * 2 + [ctypes.c_int] * 2 + [ctypes.c_longlong,
+                                                                                          ctypes.c_int]
+    f.restype = ctypes.c_int
+    rng = np.random.default_rng(N + k + T + B + mode)
+    rows = B * T if T else B
+    x = (rng.standard_normal((rows, N)) + 1j * rng.standard_normal((rows, N))).astype(np.complex64)
+    filt = rng.standard_normal(N).astype(np.float32)
+    n = N // k
+    xin = np.stack([x.real, x.imag], -1).reshape(-1).copy()
+    c = 2 if mode == 0 else 1
+    y = _guarded(rows * n * c, np.float32)
+    assert f(xin.ctypes.data, filt.ctypes.data, y.ctypes.data, rows, N, k, mode, T, 0) == 0
+    assert np.all(y[rows * n * c:] == 12345.0), 'write past the output'
+    ref = np.fft.ifft((x.astype(np.complex128) * filt).reshape(rows, k, n).mean(axis=1), axis=1)
+    if T:
+        ref = ref.reshape(B, T, n).transpose(0, 2, 1)
+    if mode == 0:
+        got = y[:rows * n * 2].reshape(ref.shape + (2,))
+        got = got[..., 0] + 1j * got[..., 1]
+    else:
+        got = y[:rows * n].reshape(ref.shape)
+        ref = ref.real if mode == 1 else np.abs(ref)
+    assert np.abs(got - ref).max() <= 2e-6 * np.abs(ref).max()
