@@ -366,15 +366,16 @@ static int emu_lowpass(const void* x, const void* f, void* y, long long outer, i
     int m = 1, nsb = 1;
     long long nblk = 0;
     const long long elems = lowpass_shape(N, n, S, outer, &m, &nsb, &nblk);
-    const int sid = sizeof(T) == 4 ? lowpass_spec_id(N, n, S) : -1;
+    int M = 0;
+    const int sid = sizeof(T) == 4 ? lowpass_spec_id(N, n, S, &M) : -1;
     if (sid >= 0) {  // the compile-time body, as the GPU entry selects it
-        const int M = N >= 4096 ? 1 : 4096 / N;
-        LowpassArgs o = {x, f, y, outer, N, S, k, n, M, 1};
+        LowpassArgs o = {x, f, y, outer, N, S, k, n, M, S == 1 ? 1 : S / M};
+        const long long nb = S == 1 ? (outer + M - 1) / M : outer * (S / M);
         std::vector<cpx<T>> lds((size_t)M * (N + n) + 1);
-        for (long long blk = 0; blk < (outer + M - 1) / M; ++blk) {
+        for (long long blk = 0; blk < nb; ++blk) {
             switch (sid) {
-#define KYM_LP_E(id, nn, n2) \
-    case id: lowpass_body_s<T, nn, n2, 1>(o, lens[lN], lens[ln], blk, lds.data(), 0); break;
+#define KYM_LP_E(id, nn, n2, mm, sv) \
+    case id: lowpass_spec_body<T, id, true>(o, lens[lN], lens[ln], blk, lds.data(), 0); break;
                 KYM_LOWPASS_SPECS(KYM_LP_E)
 #undef KYM_LP_E
                 default: break;

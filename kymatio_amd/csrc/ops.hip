@@ -208,11 +208,11 @@ __global__ void __launch_bounds__(256) k_filter_ifft_s(FilterIfftArgs o, const F
     filter_ifft_body_s<T, NN, M, 256>(o, lens[l], blockIdx.x, reinterpret_cast<cpx<T>*>(kym_smem_fis), threadIdx.x);
 }
 
-template <class T, int N, int NN>
+template <class T, int SPEC>
 __global__ void __launch_bounds__(256) k_lowpass_s(LowpassArgs o, const FftLen<T>* lens, int lN, int ln) {
     extern __shared__ __attribute__((aligned(16))) char kym_smem_lps[];
-    lowpass_body_s<T, N, NN, 256>(o, lens[lN], lens[ln], blockIdx.x, reinterpret_cast<cpx<T>*>(kym_smem_lps),
-                                  threadIdx.x);
+    lowpass_spec_body<T, SPEC, false>(o, lens[lN], lens[ln], blockIdx.x, reinterpret_cast<cpx<T>*>(kym_smem_lps),
+                                      threadIdx.x);
 }
 
 template <class T, int MAXP>
@@ -429,8 +429,8 @@ template <class T>
 static const void* lowpass_spec_kernel(int id) {
     if constexpr (sizeof(T) != 4) return nullptr;  // FP32 only
     else switch (id) {
-#define KYM_LP_K(id, nn, n2) \
-    case id: return (const void*)k_lowpass_s<T, nn, n2>;
+#define KYM_LP_K(id, nn, n2, mm, sv) \
+    case id: return (const void*)k_lowpass_s<T, id>;
         KYM_LOWPASS_SPECS(KYM_LP_K)
 #undef KYM_LP_K
         default: return nullptr;
@@ -439,17 +439,18 @@ static const void* lowpass_spec_kernel(int id) {
 
 template <class T>
 static int lowpass_op(const void* x, const void* f, void* y, long long outer, int N, int S, int k, hipStream_t s) {
-    if (const void* kern = lowpass_spec_kernel<T>(lowpass_spec_id(N, N / k, S))) {
-        const int n = N / k, M = N >= 4096 ? 1 : 4096 / N;
+    int M = 0;
+    if (const void* kern = lowpass_spec_kernel<T>(lowpass_spec_id(N, N / k, S, &M))) {
+        const int n = N / k;
         OpTables t;
         int rc = op_tables<T>(N, n, &t);
         if (rc != KYM_OK) return rc;
         const FftLen<T>* lens = reinterpret_cast<const FftLen<T>*>((char*)t.dev + t.lens_off);
         const size_t lds = (size_t)M * (N + n) * sizeof(cpx<T>);
         if (lds > 65536) OPS_HIP_TRY(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        const long long nblk = (outer + M - 1) / M;
+        const long long nblk = S == 1 ? (outer + M - 1) / M : outer * (S / M);
         if (nblk > 0x7fffffffll) return kym_ops_fail(KYM_ERR_VALUE, "batch too large");
-        LowpassArgs o = {x, f, y, outer, N, S, k, n, M, 1};
+        LowpassArgs o = {x, f, y, outer, N, S, k, n, M, S == 1 ? 1 : S / M};
         void* args[] = {&o, (void*)&lens, &t.lH, &t.lW};
         OPS_HIP_TRY(hipLaunchKernel(kern, dim3((unsigned)nblk), dim3(256), args, lds, s));
         return KYM_OK;

@@ -508,22 +508,84 @@ KYM_HD void lowpass_body_s(const LowpassArgs& o, const FftLen<T>& fN, const FftL
     }
 }
 
-// Compile-time low-pass shapes X(id, N, n): time averaging of the
-// time-frequency transform at the default averaging (outputs of 128).
-#define KYM_LOWPASS_SPECS(X) \
-    X(0, 4096, 128)          \
-    X(1, 2048, 128)          \
-    X(2, 1024, 128)          \
-    X(3, 512, 128)           \
-    X(4, 256, 128)
+// lowpass_body for strided lines (S > 1) with the lengths and the lines per
+// workgroup M fixed at compile time: M consecutive s of one outer index (S a
+// multiple of M).  Same arithmetic as lowpass_body.
+template <class T, int N, int NN, int M, int NT>
+KYM_HD void lowpass_body_sv(const LowpassArgs& o, const FftLen<T>& fN, const FftLen<T>& fn, long long blk,
+                            cpx<T>* lds, int tid) {
+    constexpr int K = N / NN;
+    const int S = o.S, nsb = S / M;
+    const long long ob = blk / nsb;
+    const int s0 = (int)(blk - ob * nsb) * M;
+    const T* x = static_cast<const T*>(o.x);
+    const T* f = static_cast<const T*>(o.f);
+    T* y = static_cast<T*>(o.y);
+    for (int i = tid; i < M * N; i += NT) {
+        const int e = i / M, t = i - e * M;
+        lds[t * N + kym_ldg(fN.irev + e)] = {kym_ldg(x + (ob * N + e) * S + s0 + t), T(0)};
+    }
+    wg_barrier();
+    fft_dit_s<false, N, M, 1, N, false, NT>(lds, fN.tw, tid);
+    cpx<T>* P = lds + M * N;
+    const T inv_k = T(1) / T(K);
+    for (int q = tid; q < M * NN; q += NT) {
+        const int t = q / NN, p = q - t * NN;
+        const cpx<T>* X = lds + t * N + p;
+        cpx<T> acc = {T(0), T(0)};
+#pragma unroll
+        for (int a = 0; a < K; ++a) {
+            const T w = kym_ldg(f + p + a * NN);
+            const cpx<T> v = X[a * NN];
+            acc.re += v.re * w;
+            acc.im += v.im * w;
+        }
+        P[q] = acc * inv_k;
+    }
+    wg_barrier();
+    fft_dif_s<true, NN, M, 1, NN, false, NT>(P, fn.tw, tid);
+    const T inv_n = T(1) / T(NN);
+    for (int q = tid; q < M * NN; q += NT) {
+        const int p = q / M, t = q - p * M;
+        y[(ob * NN + p) * S + s0 + t] = P[t * NN + kym_ldg(fn.irev + p)].re * inv_n;
+    }
+}
 
-inline int lowpass_spec_id(int N, int n, int S) {
-    if (S != 1) return -1;
-#define KYM_LP_MATCH(id, nn, n2) \
-    if (N == nn && n == n2) return id;
+// Compile-time low-pass shapes X(id, N, n, lines per workgroup M, strided):
+// time averaging of the time-frequency transform at the default averaging
+// (outputs of 128; M = 4096 / N contiguous lines) and frequency averaging
+// along the padded n1 axis (88 or 22 points, M lines of stride S, S a
+// multiple of M).
+#define KYM_LOWPASS_SPECS(X)  \
+    X(0, 4096, 128, 1, 0)     \
+    X(1, 2048, 128, 2, 0)     \
+    X(2, 1024, 128, 4, 0)     \
+    X(3, 512, 128, 8, 0)      \
+    X(4, 256, 128, 16, 0)     \
+    X(5, 88, 22, 32, 1)       \
+    X(6, 22, 22, 64, 1)
+
+inline int lowpass_spec_id(int N, int n, int S, int* rows) {
+#define KYM_LP_MATCH(id, nn, n2, mm, sv)                                       \
+    if (N == nn && n == n2 && (sv ? (S > 1 && S % mm == 0) : S == 1)) {        \
+        *rows = mm;                                                            \
+        return id;                                                             \
+    }
     KYM_LOWPASS_SPECS(KYM_LP_MATCH)
 #undef KYM_LP_MATCH
     return -1;
+}
+
+template <class T, int SPEC, bool EMU>
+KYM_HD KYM_INL void lowpass_spec_body(const LowpassArgs& o, const FftLen<T>& fN, const FftLen<T>& fn, long long blk,
+                                      cpx<T>* lds, int tid) {
+#define KYM_LP_BODY(id, nn, n2, mm, sv)                                                          \
+    if constexpr (SPEC == id) {                                                                  \
+        if constexpr (sv) lowpass_body_sv<T, nn, n2, mm, (EMU ? 1 : 256)>(o, fN, fn, blk, lds, tid); \
+        else lowpass_body_s<T, nn, n2, (EMU ? 1 : 256)>(o, fN, fn, blk, lds, tid);               \
+    }
+    KYM_LOWPASS_SPECS(KYM_LP_BODY)
+#undef KYM_LP_BODY
 }
 
 // Launch shape of lowpass: lines per workgroup (about 4096 points, at most

@@ -275,3 +275,23 @@ def test_filter_ifft_compiled_lengths(emu, N, k, T, B, mode):
         got = y[:rows * n].reshape(ref.shape)
         ref = ref.real if mode == 1 else np.abs(ref)
     assert np.abs(got - ref).max() <= 2e-6 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize('N,k,S,outer', [(88, 4, 128, 3), (22, 1, 128, 2), (88, 4, 64, 2)])
+def test_lowpass_compiled_strided(emu, N, k, S, outer):
+    """The compile-time low-pass along a strided axis (FP32, the padded
+    frequency axis of 88 or 22 points) against numpy FP64 to FP32 rounding."""
+    f = emu.lib.kym_emu_lowpass
+    f.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_longlong] * 3 + [ctypes.c_int] * 2
+    f.restype = ctypes.c_int
+    rng = np.random.default_rng(N + k + S + outer)
+    x = rng.standard_normal((outer, N, S)).astype(np.float32)
+    filt = rng.standard_normal(N).astype(np.float32)
+    n = N // k
+    y = _guarded(outer * n * S, np.float32)
+    assert f(x.ctypes.data, filt.ctypes.data, y.ctypes.data, outer, N, S, k, 0) == 0
+    assert np.all(y[outer * n * S:] == 12345.0), 'write past the output'
+    X = np.fft.fft(x.astype(np.float64), axis=1) * filt.astype(np.float64)[None, :, None]
+    ref = np.fft.ifft(X.reshape(outer, k, n, S).mean(axis=1), axis=1).real
+    got = y[:outer * n * S].reshape(outer, n, S)
+    assert np.abs(got - ref).max() <= 2e-6 * np.abs(ref).max()
