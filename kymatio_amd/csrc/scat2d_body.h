@@ -1098,6 +1098,18 @@ KYM_HD void fir_rows_to_slot(const Scat2DArgs<T>& a, const Node2D& nd, const T* 
                  tid, nth);
 }
 
+// The row spectra of a real field are Hermitian along x (row[W - x] =
+// conj(row[x])), so the row passes store columns x <= W / 2 only and the
+// forward column pass (body_fwd_cols) writes the other half by symmetry.
+template <class T>
+KYM_HD KYM_INL void store_half_rows(cpx<T>* dst, const cpx<T>* src, int nr, int W, int tid, int nth) {
+    const int Wh = W / 2 + 1;
+    for (int i = tid; i < nr * Wh; i += nth) {
+        const int rr = i / Wh, x = i - rr * Wh;
+        dst[(long long)rr * W + x] = src[rr * W + x];
+    }
+}
+
 // Root row pass: rows of the padded image (row storage positions
 // digit-reversed), forward DIT along x.  task -> (image, row block).  With a
 // FIR leaf the block's rows are first loaded in natural order as reals (after
@@ -1143,8 +1155,7 @@ KYM_HD void body_root_rows(const P& fp, const Scat2DArgs<T>& a, int task, cpx<T>
     }
     wg_barrier();
     fp.row_fwd(lds, fW, nr, tid, nth);
-    cpx<T>* dst = a.ws + il * a.ws_img + nd.slot + (long long)r0 * W;
-    for (int i = tid; i < nr * W; i += nth) dst[i] = lds[i];
+    store_half_rows(a.ws + il * a.ws_img + nd.slot + (long long)r0 * W, lds, nr, W, tid, nth);
 }
 
 // Gather + inverse column DIF.  Shared by big nodes (filter psi, slot) and
@@ -1214,7 +1225,7 @@ KYM_HD void body_rows_mod(const P& fp, const Scat2DArgs<T>& a, int task, cpx<T>*
         fir_rows_to_slot(a, nd, R, rs, nr, r0, a.lens[a.lenH[r]].rev, wsi, R + nr * rs, tid, nth);
         if (!nd.fwd) return;
         fp.row_fwd_real(lds, R, rs, tid);
-        for (int i = tid; i < nr * W; i += nth) buf[i] = lds[i];
+        store_half_rows(buf, lds, nr, W, tid, nth);
         return;
     }
     fp.row_inv(lds, fW, nr, tid, nth);
@@ -1232,7 +1243,7 @@ KYM_HD void body_rows_mod(const P& fp, const Scat2DArgs<T>& a, int task, cpx<T>*
     if constexpr (kym_fir<T>) fir_rows_to_slot(a, nd, R, rs, nr, r0, a.lens[a.lenH[r]].rev, wsi, R + nr * rs, tid, nth);
     if (!nd.fwd) return;
     fp.row_fwd(lds, fW, nr, tid, nth);
-    for (int i = tid; i < nr * W; i += nth) buf[i] = lds[i];
+    store_half_rows(buf, lds, nr, W, tid, nth);
 }
 
 // Column pass of a big node's FIR leaf: tslot -> LDS (row stride
@@ -1298,7 +1309,8 @@ KYM_HD void body_fwd_cols(const P& fp, const Scat2DArgs<T>& a, int task, cpx<T>*
     const int r = nd.res, H = a.H[r], W = a.W[r];
     const FftLen<T>& fH = a.lens[a.lenH[r]];
     const int c0 = cb * a.blk, nc = (W - c0 < a.blk) ? W - c0 : a.blk;
-    cpx<T>* buf = a.ws + il * a.ws_img + nd.slot + c0;
+    cpx<T>* spec = a.ws + il * a.ws_img + nd.slot;
+    const cpx<T>* buf = spec + c0;
     copy_batched<8>(
         H * nc,
         [&](int i) {
@@ -1308,9 +1320,16 @@ KYM_HD void body_fwd_cols(const P& fp, const Scat2DArgs<T>& a, int task, cpx<T>*
         [&](int i, cpx<T> v) { lds[i] = v; }, tid, nth);
     wg_barrier();
     fp.col_fwd(lds, fH, nc, tid, nth);
+    // the launch covers columns 0 .. W / 2 (store_half_rows; columns of its
+    // last block beyond that are transformed, not stored); column W - x of the
+    // spectrum is X[v][W - x] = conj(X[(H - v) mod H][x])
+    const int Wh = W / 2;
     for (int i = tid; i < H * nc; i += nth) {
-        const int y = i / nc, c = i - y * nc;
-        buf[(long long)y * W + c] = lds[i];
+        const int v = i / nc, x = c0 + i - v * nc;
+        if (x > Wh) continue;
+        const cpx<T> X = lds[i];
+        spec[(long long)v * W + x] = X;
+        if (x > 0 && W - x != x) spec[(long long)(v == 0 ? 0 : H - v) * W + (W - x)] = {X.re, -X.im};
     }
 }
 

@@ -460,7 +460,10 @@ struct Scat2DPlan {
                     pass(K2_ICOL, W[r] / cb + (W[r] % cb != 0), cb, (size_t)H[r] * cb * cs);
                     pass(K2_ROWS_MOD, H[r] / rb + (H[r] % rb != 0), rb, row_lds(r, rb));
                 }
-                if (n0.fwd) pass(K2_FWD_COLS, W[r] / cb + (W[r] % cb != 0), cb, (size_t)H[r] * cb * cs);
+                // the row passes keep columns 0 .. W / 2 of the spectra of real
+                // fields; the forward column pass covers them and writes the rest
+                // by Hermitian symmetry (body_fwd_cols)
+                if (n0.fwd) pass(K2_FWD_COLS, (W[r] / 2 + 1 + cb - 1) / cb, cb, (size_t)H[r] * cb * cs);
             }
             emit_leaves(ids);
         };
