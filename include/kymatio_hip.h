@@ -293,6 +293,17 @@ int kym_op_filter_ifft_t(const void* x, const void* f, void* y, long long batch,
  * Status 3 if nf does not divide rows or the row map does not fit. */
 int kym_op_filter_ifft_g(const void* x, const void* f, void* y, long long rows, long long N, int k, int mode, int nf,
                          int grp, int gtot, int goff, int dtype, void* stream);
+/* Frequency scattering of one input in one kernel: x (B, A, T) real (c = 1)
+ * or complex (c = 2) along the log-frequency axis a < A, zero-padded to N;
+ * for each of nf <= 16 wavelets f[i] (N reals, host array of device
+ * pointers) y[i] (B, N / ks[i], T) real = |ifft(subsample_fourier(cdgmm(
+ * fft(x_padded), f[i]), ks[i]))| along that axis -- the swap_time_frequency,
+ * pad_frequency, cfft / rfft, and per wavelet ifft and swap_time_frequency
+ * of kymatio/scattering1d/core/timefrequency_scattering.py:244-267, with the
+ * modulus that timefrequency_scattering.py:409 applies next, reading x once.
+ * Status 2 if a ks[i] does not divide N. */
+int kym_op_freq_scatter(const void* x, void* const* ys, const void* const* fs, const int* ks, int nf, long long B,
+                        int A, int T, int c, int N, int dtype, void* stream);
 /* Low-pass of real lines in one kernel: y = Re ifft_n(subsample_fourier(
  * cdgmm(fft_N(x), f), k)), n = N / k -- rfft, cdgmm, subsample_fourier and
  * irfft of kymatio/scattering1d/backend/torch_backend.py:118-136 with

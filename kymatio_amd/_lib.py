@@ -64,6 +64,8 @@ _SIGNATURES = {
     'kym_op_filter_ifft': [_c_vp, _c_vp, _c_vp, _c_ll, _c_ll, _c_i, _c_i, _c_i, _c_vp],
     'kym_op_filter_ifft_t': [_c_vp, _c_vp, _c_vp, _c_ll, _c_ll, _c_i, _c_i, _c_ll, _c_i, _c_vp],
     'kym_op_filter_ifft_g': [_c_vp, _c_vp, _c_vp, _c_ll, _c_ll, _c_i, _c_i, _c_i, _c_i, _c_i, _c_i, _c_i, _c_vp],
+    'kym_op_freq_scatter': [_c_vp, ctypes.POINTER(_c_vp), ctypes.POINTER(_c_vp), ctypes.POINTER(_c_i), _c_i, _c_ll,
+                            _c_i, _c_i, _c_i, _c_i, _c_i, _c_vp],
     'kym_op_lowpass': [_c_vp, _c_vp, _c_vp, _c_ll, _c_ll, _c_ll, _c_i, _c_i, _c_vp],
     'kym_op_integrals': [_c_vp, _c_vp, _c_ll, _c_ll, _c_vp, _c_i, _c_i, _c_vp],
     'kym_op_harmonic_products': [_c_vp, _c_vp, _c_vp, _c_ll, _c_i, _c_ll, _c_i, _c_vp],

@@ -352,6 +352,42 @@ extern "C" int kym_emu_filter_ifft_g(const void* x, const void* f, void* y, long
 }
 
 template <class T>
+static int emu_freq_scatter(const void* x, void* const* ys, const void* const* fs, const int* ks, int nf, long long B,
+                            int A, int TT, int c, int N) {
+    LenTable<T> lt;
+    const int lN = lt.add(N);
+    int ln[KYM_FS_MAX];
+    for (int i = 0; i < nf; ++i) ln[i] = lt.add(N / ks[i]);
+    Blob b;
+    size_t lens_off = 0;
+    std::string err;
+    if (!lt.emit(b, &lens_off, &err)) { g_emu_error = err; return KYM_ERR_UNSUPPORTED; }
+    b.relocate((uintptr_t)b.bytes.data());
+    const FftLen<T>* lens = reinterpret_cast<const FftLen<T>*>(b.bytes.data() + lens_off);
+    FreqScatterArgs o = {};
+    o.x = x; o.B = B; o.A = A; o.T = TT; o.c = c; o.N = N; o.nf = nf;
+    o.m = freq_scatter_rows(N, TT);
+    o.nt_blk = (TT + o.m - 1) / o.m;
+    o.lenN = lens + lN;
+    for (int i = 0; i < nf; ++i) {
+        o.f[i] = fs[i]; o.y[i] = ys[i]; o.k[i] = ks[i]; o.lenn[i] = lens + ln[i];
+    }
+    std::vector<cpx<T>> lds((size_t)2 * o.m * N + 1);
+    for (long long blk = 0; blk < B * o.nt_blk; ++blk) freq_scatter_body<T, 31>(o, blk, lds.data(), 0, 1);
+    return 0;
+}
+
+// kym_op_freq_scatter on the CPU (one thread per workgroup).
+extern "C" int kym_emu_freq_scatter(const void* x, void* const* ys, const void* const* fs, const int* ks, int nf,
+                                    long long B, int A, int T, int c, int N, int dtype) {
+    if (nf < 1 || nf > KYM_FS_MAX || A < 1 || A > N) { g_emu_error = "invalid arguments"; return 3; }
+    for (int i = 0; i < nf; ++i)
+        if (ks[i] < 1 || N % ks[i]) { g_emu_error = "subsampling factor must divide the size"; return 2; }
+    if (dtype == 0) return emu_freq_scatter<float>(x, ys, fs, ks, nf, B, A, T, c, N);
+    return emu_freq_scatter<double>(x, ys, fs, ks, nf, B, A, T, c, N);
+}
+
+template <class T>
 static int emu_lowpass(const void* x, const void* f, void* y, long long outer, int N, int S, int k) {
     if (k < 1 || N % k) { g_emu_error = "subsampling factor must divide the size"; return KYM_ERR_RUNTIME; }
     const int n = N / k;

@@ -202,6 +202,12 @@ __global__ void __launch_bounds__(256) k_lowpass(LowpassArgs o, const FftLen<T>*
     lowpass_body<T, MAXP>(o, lens[lN], lens[ln], blockIdx.x, reinterpret_cast<cpx<T>*>(kym_smem_lp), threadIdx.x, 256);
 }
 
+template <class T, int MAXP>
+__global__ void __launch_bounds__(256) k_freq_scatter(FreqScatterArgs o) {
+    extern __shared__ __attribute__((aligned(16))) char kym_smem_fs[];
+    freq_scatter_body<T, MAXP>(o, blockIdx.x, reinterpret_cast<cpx<T>*>(kym_smem_fs), threadIdx.x, 256);
+}
+
 template <class T, int NN, int M>
 __global__ void __launch_bounds__(256) k_filter_ifft_s(FilterIfftArgs o, const FftLen<T>* lens, int l) {
     extern __shared__ __attribute__((aligned(16))) char kym_smem_fis[];
@@ -462,6 +468,45 @@ static int lowpass_op(const void* x, const void* f, void* y, long long outer, in
     }
 }
 
+template <class T>
+static int freq_scatter_op(const void* x, void* const* ys, const void* const* fs, const int* ks, int nf, long long B,
+                           int A, int T_, int c, int N, hipStream_t s) {
+    FreqScatterArgs o = {};
+    o.x = x;
+    o.B = B;
+    o.A = A;
+    o.T = T_;
+    o.c = c;
+    o.N = N;
+    o.nf = nf;
+    o.m = freq_scatter_rows(N, T_);
+    o.nt_blk = (T_ + o.m - 1) / o.m;
+    int mp = kym_max_odd_prime(N);
+    for (int i = 0; i < nf; ++i) {
+        OpTables t;
+        int rc = op_tables<T>(N, N / ks[i], &t);
+        if (rc != KYM_OK) return rc;
+        const FftLen<T>* lens = reinterpret_cast<const FftLen<T>*>((char*)t.dev + t.lens_off);
+        o.lenN = lens + t.lH;
+        o.lenn[i] = lens + t.lW;
+        o.f[i] = fs[i];
+        o.y[i] = ys[i];
+        o.k[i] = ks[i];
+        mp = std::max(mp, kym_max_odd_prime(N / ks[i]));
+    }
+    const size_t lds = (size_t)2 * o.m * N * sizeof(cpx<T>);
+    if (lds > op_lds_limit()) return kym_ops_fail(KYM_ERR_UNSUPPORTED, "transform length too large for LDS");
+    const long long nblk = B * o.nt_blk;
+    if (nblk > 0x7fffffffll) return kym_ops_fail(KYM_ERR_VALUE, "batch too large");
+    const void* kern = kym_prime_class(mp) == 7    ? (const void*)k_freq_scatter<T, 7>
+                       : kym_prime_class(mp) == 17 ? (const void*)k_freq_scatter<T, 17>
+                                                   : (const void*)k_freq_scatter<T, 31>;
+    if (lds > 65536) OPS_HIP_TRY(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    void* args[] = {&o};
+    OPS_HIP_TRY(hipLaunchKernel(kern, dim3((unsigned)nblk), dim3(256), args, lds, s));
+    return KYM_OK;
+}
+
 // ------------------------------------------------------------ C ABI
 template <class T>
 static int launch_pad2d(const void* x, void* y, long long batch, int M, int N, int top, int left, int P, int Q,
@@ -689,6 +734,24 @@ extern "C" int kym_op_filter_ifft_g(const void* x, const void* f, void* y, long 
 extern "C" int kym_op_filter_ifft(const void* x, const void* f, void* y, long long batch, long long N, int k,
                                   int mode, int dtype, void* stream) {
     return kym_op_filter_ifft_t(x, f, y, batch, N, k, mode, 0, dtype, stream);
+}
+
+extern "C" int kym_op_freq_scatter(const void* x, void* const* ys, const void* const* fs, const int* ks, int nf,
+                                   long long B, int A, int T, int c, int N, int dtype, void* stream) {
+    if (nf < 1 || nf > KYM_FS_MAX) return kym_ops_fail(KYM_ERR_VALUE, "between 1 and 16 wavelets per call");
+    if (c != 1 && c != 2) return kym_ops_fail(KYM_ERR_VALUE, "input must be real (c = 1) or complex (c = 2)");
+    if (A < 1 || A > N || T < 1 || N < 1) return kym_ops_fail(KYM_ERR_VALUE, "invalid sizes");
+    if (!ys || !fs || !ks) return kym_ops_fail(KYM_ERR_VALUE, "null pointer array");
+    for (int i = 0; i < nf; ++i) {
+        if (ks[i] < 1 || N % ks[i]) return kym_ops_fail(KYM_ERR_RUNTIME, "subsampling factor must divide the size");
+        if (!ys[i] || !fs[i]) return kym_ops_fail(KYM_ERR_VALUE, "null device pointer");
+    }
+    if (B <= 0) return KYM_OK;
+    if (!x) return kym_ops_fail(KYM_ERR_VALUE, "null device pointer");
+    hipStream_t s = (hipStream_t)stream;
+    if (dtype == KYM_F32) return freq_scatter_op<float>(x, ys, fs, ks, nf, B, A, T, c, N, s);
+    if (dtype == KYM_F64) return freq_scatter_op<double>(x, ys, fs, ks, nf, B, A, T, c, N, s);
+    return bad_dtype();
 }
 
 extern "C" int kym_op_lowpass(const void* x, const void* f, void* y, long long outer, long long N, long long S,

@@ -330,6 +330,88 @@ inline int filter_ifft_spec_id(int n, long long batch, const FilterIfftArgs& o, 
     return -1;
 }
 
+// Frequency scattering of one (b, t-block) in one workgroup: the rows x[b][a][t]
+// (a < A along the log-frequency axis, t < T; real (c = 1) or complex
+// (c = 2)), zero-padded to N along a, are transformed along a; then for every
+// wavelet i: Y = |ifft_n(P_k(X f_i))|, n = N / k_i, stored as y_i[b][p][t].
+// That is the swap, pad_frequency, cfft / rfft, and per wavelet filter_ifft,
+// swap and modulus of kymatio/scattering1d/core/timefrequency_scattering.py:
+// 244-267 with the modulus of :409, in one pass over x.
+#define KYM_FS_MAX 16
+struct FreqScatterArgs {
+    const void* x;
+    long long B;
+    int A, T, c, N, nf, m, nt_blk;  // m t-values per workgroup, nt_blk workgroups per b
+    const void* f[KYM_FS_MAX];
+    void* y[KYM_FS_MAX];
+    int k[KYM_FS_MAX];
+    const void* lenN;               // FftLen<T> of N
+    const void* lenn[KYM_FS_MAX];   // FftLen<T> of N / k[i]
+};
+
+// LDS: m x N spectrum, then m x N work rows.
+template <class T, int MAXP>
+KYM_HD void freq_scatter_body(const FreqScatterArgs& o, long long blk, cpx<T>* lds, int tid, int nth) {
+    const FftLen<T>& fN = *static_cast<const FftLen<T>*>(o.lenN);
+    const int N = o.N, A = o.A, TT = o.T, m = o.m;
+    const long long b = blk / o.nt_blk;
+    const int t0 = (int)(blk - b * o.nt_blk) * m;
+    const int mm = (TT - t0 < m) ? TT - t0 : m;
+    cpx<T>* X = lds;
+    cpx<T>* P = lds + (long long)m * N;
+    // rows t of the block along a, each value at its digit-reversed slot
+    for (int i = tid; i < m * N; i += nth) {
+        const int a = i / m, t = i - a * m;
+        cpx<T> v = {T(0), T(0)};
+        if (a < A && t < mm) {
+            const long long src = (b * A + a) * (long long)TT + t0 + t;
+            if (o.c == 2) v = kym_ldg(static_cast<const cpx<T>*>(o.x) + src);
+            else v = {kym_ldg(static_cast<const T*>(o.x) + src), T(0)};
+        }
+        X[t * N + kym_ldg(fN.irev + a)] = v;
+    }
+    wg_barrier();
+    fft_dit<false, MAXP>(X, fN, Batch{m, 1, N, false}, tid, nth);
+    wg_barrier();
+    for (int w = 0; w < o.nf; ++w) {
+        const int k = o.k[w], n = N / k;
+        const T* f = static_cast<const T*>(o.f[w]);
+        const T inv_k = T(1) / T(k);
+        for (int q = tid; q < m * n; q += nth) {
+            const int t = q / n, p = q - t * n;
+            const cpx<T>* xr = X + t * N + p;
+            cpx<T> acc = {T(0), T(0)};
+            for (int a = 0; a < k; ++a) {
+                const T fv = kym_ldg(f + p + a * n);
+                const cpx<T> v = xr[a * n];
+                acc.re += v.re * fv;
+                acc.im += v.im * fv;
+            }
+            P[q] = acc * inv_k;
+        }
+        wg_barrier();
+        const FftLen<T>& fl = *static_cast<const FftLen<T>*>(o.lenn[w]);
+        fft_dif<true, MAXP>(P, fl, Batch{m, 1, n, false}, tid, nth);
+        wg_barrier();
+        const T inv_n = T(1) / T(n);
+        T* y = static_cast<T*>(o.y[w]);
+        for (int q = tid; q < mm * n; q += nth) {
+            const int p = q / mm, t = q - p * mm;
+            const cpx<T> v = P[t * n + kym_ldg(fl.irev + p)];
+            y[(b * n + p) * (long long)TT + t0 + t] = kym_sqrt(v.re * v.re + v.im * v.im) * inv_n;
+        }
+        wg_barrier();
+    }
+}
+
+// t-values per workgroup of freq_scatter: about 4096 points, a power of two,
+// at most T.
+inline int freq_scatter_rows(int N, int T) {
+    int m = 1;
+    while (m * 2 * N <= 4096 && m * 2 <= T) m *= 2;
+    return m;
+}
+
 // Launch shape of filter_ifft for output length n over `batch` rows: threads
 // (1024 for n >= 2048, else 256), rows per workgroup (about 4096 points, but
 // fewer when that leaves under 512 workgroups) and the alias split (threads

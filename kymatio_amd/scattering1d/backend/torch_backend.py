@@ -191,6 +191,41 @@ class TorchBackend1D(TorchBackend):
         return y
 
     @classmethod
+    def freq_scatter(cls, x, filts, ks, N):
+        """Frequency scattering of x (..., A, T, 1 | 2) along A (zero-padded to
+        N) by every wavelet in filts (each (N, 1)), subsampled by ks, with
+        the modulus: [(..., N / k, T, 1)], one HIP kernel per 16 wavelets
+        (kym_op_freq_scatter) reading x once."""
+        cls.contiguous_check(x)
+        _gpu(x)
+        c = x.shape[-1]
+        if c not in (1, 2):
+            raise TypeError('The input should be real or complex.')
+        A, T = x.shape[-3], x.shape[-2]
+        if A > N:
+            raise RuntimeError('the frequency axis is longer than its padded length')
+        B = x.numel() // (A * T * c)
+        fs = []
+        for f in filts:
+            f = f.contiguous()
+            if f.dtype != x.dtype or f.numel() != N:
+                raise TypeError('Input and filter must be of the same dtype and length.')
+            fs.append(f)
+        for k in ks:
+            if N % k:
+                raise RuntimeError('subsampling factor must divide the size')
+        ys = [torch.empty(x.shape[:-3] + (N // k, T, 1), dtype=x.dtype, device=x.device) for k in ks]
+        with torch.cuda.device(x.device):
+            for i0 in range(0, len(fs), 16):
+                n = min(16, len(fs) - i0)
+                yp = (ctypes.c_void_p * n)(*[y.data_ptr() for y in ys[i0:i0 + n]])
+                fp = (ctypes.c_void_p * n)(*[f.data_ptr() for f in fs[i0:i0 + n]])
+                kp = (ctypes.c_int * n)(*ks[i0:i0 + n])
+                _lib.check(_lib.lib().kym_op_freq_scatter(_ptr(x), yp, fp, kp, n, B, A, T, c, N, _dt(x),
+                                                           _lib.stream_of(x.device)))
+        return ys
+
+    @classmethod
     def lowpass(cls, x, filt, k, axis=-2):
         """irfft(subsample_fourier(cdgmm(rfft(.), filt), k)) along the time
         axis (-2) or the frequency axis (-3) of a real x (..., A, T, 1), in

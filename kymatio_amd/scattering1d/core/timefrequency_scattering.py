@@ -113,10 +113,22 @@ def _widthfirst_by_runs(U_0_hat, backend, filters, log2_stride, average_local, b
 def frequency_scattering(X, backend, filters_fr, log2_stride_fr, average_local_fr, spinned, modulus=False):
     """Wavelets along n1 (zero-padded to the frequency filters' length N):
     all n_fr for complex input (spinned), only spin >= 0 for real input.
-    modulus: yield |Y| (the modulus jtfs_average_and_format takes next),
-    with the filter, inverse, swap and modulus in one op where the backend
-    has it (filter_ifft_swap)."""
+    modulus: yield |Y| (the modulus jtfs_average_and_format takes next):
+    every wavelet from one read of X where the backend has freq_scatter,
+    else filter, inverse, swap and modulus as one op per wavelet where it
+    has filter_ifft_swap."""
     phi, psis = filters_fr
+    fused = getattr(backend, 'freq_scatter', None) if modulus else None
+    if fused is not None:  # every wavelet in one pass over X (freq_scatter)
+        if not spinned:
+            psis = [p for p in psis if p['xi'] >= 0]
+        ks = [2 ** (min(p['j'], log2_stride_fr) if average_local_fr else p['j']) for p in psis]
+        Ys = fused(X['coef'], [p['levels'][0] for p in psis], ks, phi['N'])
+        for n_fr, (psi, Y) in enumerate(zip(psis, Ys)):
+            j_fr = psi['j']
+            yield {**X, 'coef': Y, 'n': X['n'][1:] + (n_fr,), 'j_fr': (j_fr,), 'n_fr': (n_fr,),
+                   'n1_stride': 2 ** j_fr, 'spin': np.sign(psi['xi'])}
+        return
     X_T = backend.pad_frequency(backend.swap_time_frequency(X['coef']), phi['N'] - X['n1_max'])
     if spinned:
         X_hat = backend.cfft(X_T)

@@ -234,3 +234,21 @@ def test_lowpass_equals_op_sequence(shape, axis, k):
     ref = np.fft.ifft(Xs.reshape(Xs.shape[:-1] + (k, N // k)).mean(axis=-2), axis=-1).real
     ref = np.moveaxis(ref, -1, ax)
     assert np.abs(fused.cpu().numpy() - ref).max() <= 1e-5 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize('c', [1, 2])
+def test_freq_scatter_equals_op_sequence(c):
+    """freq_scatter (kym_op_freq_scatter) equals the ops it fuses -- swap,
+    pad_frequency, cfft / rfft, then per wavelet filter_ifft, swap and
+    modulus -- to FP32 rounding."""
+    B, A, T, N, ks = 3, 10, 64, 88, [1, 2, 4, 4, 1]
+    rng = np.random.default_rng(c)
+    x = torch.from_numpy(rng.standard_normal((B, 1, A, T, c)).astype(np.float32)).to(DEV)
+    fs = [torch.from_numpy(rng.standard_normal((N, 1)).astype(np.float32)).to(DEV) for _ in ks]
+    fused = B1.freq_scatter(x, fs, ks, N)
+    X_T = B1.pad_frequency(B1.swap_time_frequency(x), N - A)
+    X_hat = B1.cfft(X_T) if c == 2 else B1.rfft(X_T)
+    for y, f, k in zip(fused, fs, ks):
+        seq = B1.modulus(B1.swap_time_frequency(B1.filter_ifft(X_hat, f, k, 'complex')))
+        assert y.shape == seq.shape
+        assert (y - seq).abs().max().item() <= 1e-6 * seq.abs().max().item()

@@ -295,3 +295,34 @@ def test_lowpass_compiled_strided(emu, N, k, S, outer):
     ref = np.fft.ifft(X.reshape(outer, k, n, S).mean(axis=1), axis=1).real
     got = y[:outer * n * S].reshape(outer, n, S)
     assert np.abs(got - ref).max() <= 2e-6 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize('B,A,T,c,N,ks', [(2, 10, 64, 2, 88, [1, 2, 4, 4, 2, 1, 4]), (3, 38, 16, 1, 88, [4, 1, 2]),
+                                          (2, 7, 40, 2, 24, [1, 3, 8])])
+def test_freq_scatter(emu, B, A, T, c, N, ks):
+    """kym_op_freq_scatter: x (B, A, T) zero-padded to N along A, transformed
+    along A, then per wavelet |ifft(P_k(X f))| laid out (B, N / k, T) --
+    against numpy FP64 of the reference's swap, pad, fft, cdgmm,
+    subsample_fourier, ifft, swap and modulus (T not a multiple of the
+    workgroup's t-block in the last case)."""
+    f = emu.lib.kym_emu_freq_scatter
+    f.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_int, ctypes.c_longlong] + [ctypes.c_int] * 5
+    f.restype = ctypes.c_int
+    rng = np.random.default_rng(B + A + T + c + N)
+    x = rng.standard_normal((B, A, T)) + (1j * rng.standard_normal((B, A, T)) if c == 2 else 0)
+    xin = (np.stack([x.real, x.imag], -1) if c == 2 else x.real).reshape(-1).copy()
+    filts = [rng.standard_normal(N) for _ in ks]
+    ys = [_guarded(B * (N // k) * T, np.float64) for k in ks]
+    yp = (ctypes.c_void_p * len(ks))(*[y.ctypes.data for y in ys])
+    fp = (ctypes.c_void_p * len(ks))(*[g.ctypes.data for g in filts])
+    kp = (ctypes.c_int * len(ks))(*ks)
+    assert f(xin.ctypes.data, yp, fp, kp, len(ks), B, A, T, c, N, 1) == 0
+    xp = np.zeros((B, N, T), complex)
+    xp[:, :A] = x
+    X = np.fft.fft(xp, axis=1)
+    for y, g, k in zip(ys, filts, ks):
+        n = N // k
+        assert np.all(y[B * n * T:] == 12345.0), 'write past the output'
+        ref = np.abs(np.fft.ifft((X * g[None, :, None]).reshape(B, k, n, T).mean(axis=1), axis=1))
+        got = y[:B * n * T].reshape(B, n, T)
+        assert np.abs(got - ref).max() <= 1e-12 * max(np.abs(ref).max(), 1)
