@@ -373,7 +373,11 @@ static int emu_freq_scatter(const void* x, void* const* ys, const void* const* f
         o.f[i] = fs[i]; o.y[i] = ys[i]; o.k[i] = ks[i]; o.lenn[i] = lens + ln[i];
     }
     std::vector<cpx<T>> lds((size_t)2 * o.m * N + 1);
-    for (long long blk = 0; blk < B * o.nt_blk; ++blk) freq_scatter_body<T, 31>(o, blk, lds.data(), 0, 1);
+    const bool spec = sizeof(T) == 4 && freq_scatter_spec_ok(o);  // as the GPU entry selects it
+    for (long long blk = 0; blk < B * o.nt_blk; ++blk) {
+        if (spec) freq_scatter_body_s<T, 88, 32, 1>(o, blk, lds.data(), 0);
+        else freq_scatter_body<T, 31>(o, blk, lds.data(), 0, 1);
+    }
     return 0;
 }
 

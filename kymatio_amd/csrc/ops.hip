@@ -208,6 +208,12 @@ __global__ void __launch_bounds__(256) k_freq_scatter(FreqScatterArgs o) {
     freq_scatter_body<T, MAXP>(o, blockIdx.x, reinterpret_cast<cpx<T>*>(kym_smem_fs), threadIdx.x, 256);
 }
 
+template <class T>
+__global__ void __launch_bounds__(256) k_freq_scatter_s88(FreqScatterArgs o) {
+    extern __shared__ __attribute__((aligned(16))) char kym_smem_fss[];
+    freq_scatter_body_s<T, 88, 32, 256>(o, blockIdx.x, reinterpret_cast<cpx<T>*>(kym_smem_fss), threadIdx.x);
+}
+
 template <class T, int NN, int M>
 __global__ void __launch_bounds__(256) k_filter_ifft_s(FilterIfftArgs o, const FftLen<T>* lens, int l) {
     extern __shared__ __attribute__((aligned(16))) char kym_smem_fis[];
@@ -501,6 +507,7 @@ static int freq_scatter_op(const void* x, void* const* ys, const void* const* fs
     const void* kern = kym_prime_class(mp) == 7    ? (const void*)k_freq_scatter<T, 7>
                        : kym_prime_class(mp) == 17 ? (const void*)k_freq_scatter<T, 17>
                                                    : (const void*)k_freq_scatter<T, 31>;
+    if (sizeof(T) == 4 && freq_scatter_spec_ok(o)) kern = (const void*)k_freq_scatter_s88<T>;
     if (lds > 65536) OPS_HIP_TRY(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     void* args[] = {&o};
     OPS_HIP_TRY(hipLaunchKernel(kern, dim3((unsigned)nblk), dim3(256), args, lds, s));

@@ -404,6 +404,73 @@ KYM_HD void freq_scatter_body(const FreqScatterArgs& o, long long blk, cpx<T>* l
     }
 }
 
+// freq_scatter_body with N and the t-values per workgroup M fixed at compile
+// time (static passes) and the output lengths N / k among 88, 44 and 22 as
+// compiled (freq_scatter_spec_ok).  Same arithmetic.
+template <class T, int N, int M, int NT>
+KYM_HD void freq_scatter_body_s(const FreqScatterArgs& o, long long blk, cpx<T>* lds, int tid) {
+    const FftLen<T>& fN = *static_cast<const FftLen<T>*>(o.lenN);
+    const int A = o.A, TT = o.T;
+    const long long b = blk / o.nt_blk;
+    const int t0 = (int)(blk - b * o.nt_blk) * M;
+    const int mm = (TT - t0 < M) ? TT - t0 : M;
+    cpx<T>* X = lds;
+    cpx<T>* P = lds + M * N;
+    for (int i = tid; i < M * N; i += NT) {
+        const int a = i / M, t = i - a * M;
+        cpx<T> v = {T(0), T(0)};
+        if (a < A && t < mm) {
+            const long long src = (b * A + a) * (long long)TT + t0 + t;
+            if (o.c == 2) v = kym_ldg(static_cast<const cpx<T>*>(o.x) + src);
+            else v = {kym_ldg(static_cast<const T*>(o.x) + src), T(0)};
+        }
+        X[t * N + kym_ldg(fN.irev + a)] = v;
+    }
+    wg_barrier();
+    fft_dit_s<false, N, M, 1, N, false, NT>(X, fN.tw, tid);
+    for (int w = 0; w < o.nf; ++w) {
+        const int k = o.k[w], n = N / k;
+        const T* f = static_cast<const T*>(o.f[w]);
+        const T inv_k = T(1) / T(k);
+        for (int q = tid; q < M * n; q += NT) {
+            const int t = q / n, p = q - t * n;
+            const cpx<T>* xr = X + t * N + p;
+            cpx<T> acc = {T(0), T(0)};
+            for (int a = 0; a < k; ++a) {
+                const T fv = kym_ldg(f + p + a * n);
+                const cpx<T> v = xr[a * n];
+                acc.re += v.re * fv;
+                acc.im += v.im * fv;
+            }
+            P[q] = acc * inv_k;
+        }
+        wg_barrier();
+        const FftLen<T>& fl = *static_cast<const FftLen<T>*>(o.lenn[w]);
+        if constexpr (N == 88) {
+            if (n == 88) fft_dif_s<true, 88, M, 1, 88, false, NT>(P, fl.tw, tid);
+            else if (n == 44) fft_dif_s<true, 44, M, 1, 44, false, NT>(P, fl.tw, tid);
+            else fft_dif_s<true, 22, M, 1, 22, false, NT>(P, fl.tw, tid);
+        }
+        const T inv_n = T(1) / T(n);
+        T* y = static_cast<T*>(o.y[w]);
+        for (int q = tid; q < mm * n; q += NT) {
+            const int p = q / mm, t = q - p * mm;
+            const cpx<T> v = P[t * n + kym_ldg(fl.irev + p)];
+            y[(b * n + p) * (long long)TT + t0 + t] = kym_sqrt(v.re * v.re + v.im * v.im) * inv_n;
+        }
+        wg_barrier();
+    }
+}
+
+// The compile-time freq_scatter applies: N = 88, 32 t-values per workgroup,
+// every output length 88, 44 or 22.
+inline bool freq_scatter_spec_ok(const FreqScatterArgs& o) {
+    if (o.N != 88 || o.m != 32) return false;
+    for (int i = 0; i < o.nf; ++i)
+        if (o.k[i] != 1 && o.k[i] != 2 && o.k[i] != 4) return false;
+    return true;
+}
+
 // t-values per workgroup of freq_scatter: about 4096 points, a power of two,
 // at most T.
 inline int freq_scatter_rows(int N, int T) {

@@ -326,3 +326,32 @@ def test_freq_scatter(emu, B, A, T, c, N, ks):
         ref = np.abs(np.fft.ifft((X * g[None, :, None]).reshape(B, k, n, T).mean(axis=1), axis=1))
         got = y[:B * n * T].reshape(B, n, T)
         assert np.abs(got - ref).max() <= 1e-12 * max(np.abs(ref).max(), 1)
+
+
+@pytest.mark.parametrize('T,c', [(64, 2), (128, 1), (40, 2)])
+def test_freq_scatter_compiled(emu, T, c):
+    """The compile-time freq_scatter (FP32, N = 88, outputs of 88, 44 and 22,
+    32 t-values per workgroup with a partial last one at T = 40) against
+    numpy FP64 to FP32 rounding."""
+    f = emu.lib.kym_emu_freq_scatter
+    f.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_int, ctypes.c_longlong] + [ctypes.c_int] * 5
+    f.restype = ctypes.c_int
+    B, A, N, ks = 2, 10, 88, [1, 2, 4, 4, 1]
+    rng = np.random.default_rng(T + c)
+    x = rng.standard_normal((B, A, T)) + (1j * rng.standard_normal((B, A, T)) if c == 2 else 0)
+    xin = (np.stack([x.real, x.imag], -1) if c == 2 else x.real).reshape(-1).astype(np.float32)
+    filts = [rng.standard_normal(N).astype(np.float32) for _ in ks]
+    ys = [_guarded(B * (N // k) * T, np.float32) for k in ks]
+    yp = (ctypes.c_void_p * len(ks))(*[y.ctypes.data for y in ys])
+    fp = (ctypes.c_void_p * len(ks))(*[g.ctypes.data for g in filts])
+    kp = (ctypes.c_int * len(ks))(*ks)
+    assert f(xin.ctypes.data, yp, fp, kp, len(ks), B, A, T, c, N, 0) == 0
+    xp = np.zeros((B, N, T), complex)
+    xp[:, :A] = x.astype(np.complex64)
+    X = np.fft.fft(xp, axis=1)
+    for y, g, k in zip(ys, filts, ks):
+        n = N // k
+        assert np.all(y[B * n * T:] == 12345.0), 'write past the output'
+        ref = np.abs(np.fft.ifft((X * g.astype(np.float64)[None, :, None]).reshape(B, k, n, T).mean(axis=1), axis=1))
+        got = y[:B * n * T].reshape(B, n, T)
+        assert np.abs(got - ref).max() <= 2e-6 * np.abs(ref).max()
