@@ -411,7 +411,7 @@ static int emu_lowpass(const void* x, const void* f, void* y, long long outer, i
     if (sid >= 0) {  // the compile-time body, as the GPU entry selects it
         LowpassArgs o = {x, f, y, outer, N, S, k, n, M, S == 1 ? 1 : S / M};
         const long long nb = S == 1 ? (outer + M - 1) / M : outer * (S / M);
-        std::vector<cpx<T>> lds((size_t)M * (N + n) + 1);
+        std::vector<cpx<T>> lds((size_t)lowpass_spec_lds(N, n, S, M) + 1);
         for (long long blk = 0; blk < nb; ++blk) {
             switch (sid) {
 #define KYM_LP_E(id, nn, n2, mm, sv) \

@@ -458,7 +458,7 @@ static int lowpass_op(const void* x, const void* f, void* y, long long outer, in
         int rc = op_tables<T>(N, n, &t);
         if (rc != KYM_OK) return rc;
         const FftLen<T>* lens = reinterpret_cast<const FftLen<T>*>((char*)t.dev + t.lens_off);
-        const size_t lds = (size_t)M * (N + n) * sizeof(cpx<T>);
+        const size_t lds = (size_t)lowpass_spec_lds(N, n, S, M) * sizeof(cpx<T>);
         if (lds > 65536) OPS_HIP_TRY(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         const long long nblk = S == 1 ? (outer + M - 1) / M : outer * (S / M);
         if (nblk > 0x7fffffffll) return kym_ops_fail(KYM_ERR_VALUE, "batch too large");

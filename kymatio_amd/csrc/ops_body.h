@@ -603,50 +603,48 @@ KYM_HD void lowpass_body(const LowpassArgs& o, const FftLen<T>& fN, const FftLen
 }
 
 // lowpass_body for contiguous lines (S = 1) with the lengths fixed at
-// compile time: M = 4096 / N lines of N per workgroup, the forward and the
-// inverse transforms by the static passes (constant strides and trip
-// counts, twiddles from the length tables).  Lines beyond `outer` load
-// zeros and are not stored.  Same arithmetic as lowpass_body.
+// compile time: M = 8192 / N real lines of N per workgroup, two per complex
+// transform (line u and line u + M / 2 as z = a + i b; their spectra are
+// A[q] = (Z[q] + conj(Z[-q])) / 2 and B[q] = (Z[q] - conj(Z[-q])) / (2 i),
+// formed inside the alias sums), by the static passes.  Lines beyond
+// `outer` load zeros and are not stored.  The same sums as lowpass_body, the
+// line spectra to rounding.
 template <class T, int N, int NN, int NT>
 KYM_HD void lowpass_body_s(const LowpassArgs& o, const FftLen<T>& fN, const FftLen<T>& fn, long long blk,
                            cpx<T>* lds, int tid) {
-    constexpr int M = N >= 4096 ? 1 : 4096 / N, K = N / NN;
+    constexpr int M = 8192 / N, M2 = M / 2, K = N / NN;
     const long long ob = blk * M;
     const int m = (int)((o.outer - ob < M) ? o.outer - ob : M);
     const T* x = static_cast<const T*>(o.x);
     const T* f = static_cast<const T*>(o.f);
     T* y = static_cast<T*>(o.y);
-    for (int i = tid; i < M * N; i += NT) {
-        const int t = i / N, e = i - t * N;
-        lds[t * N + kym_ldg(fN.irev + e)] = {t < m ? kym_ldg(x + (ob + t) * N + e) : T(0), T(0)};
+    for (int i = tid; i < M2 * N; i += NT) {
+        const int u = i / N, e = i - u * N;
+        const T va = u < m ? kym_ldg(x + (ob + u) * N + e) : T(0);
+        const T vb = u + M2 < m ? kym_ldg(x + (ob + u + M2) * N + e) : T(0);
+        lds[u * N + kym_ldg(fN.irev + e)] = {va, vb};
     }
     wg_barrier();
-    fft_dit_s<false, N, M, 1, N, false, NT>(lds, fN.tw, tid);
-    cpx<T>* P = lds + M * N;
-    const T inv_k = T(1) / T(K);
+    fft_dit_s<false, N, M2, 1, N, false, NT>(lds, fN.tw, tid);
+    cpx<T>* P = lds + M2 * N;
+    const T inv_2k = T(1) / T(2 * K);
     for (int q = tid; q < M * NN; q += NT) {
         const int t = q / NN, p = q - t * NN;
-        const cpx<T>* X = lds + t * N + p;
+        const bool second = t >= M2;
+        const cpx<T>* Z = lds + (second ? t - M2 : t) * N;
         cpx<T> acc = {T(0), T(0)};
-        int a = 0;
-        for (; a + 7 < K; a += 8) {
-            T w[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) w[u] = kym_ldg(f + p + (a + u) * NN);
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const cpx<T> v = X[(a + u) * NN];
-                acc.re += v.re * w[u];
-                acc.im += v.im * w[u];
-            }
+#pragma unroll 8
+        for (int a = 0; a < K; ++a) {
+            const int i = p + a * NN;
+            const T w = kym_ldg(f + i);
+            const cpx<T> z = Z[i], zm = Z[i == 0 ? 0 : N - i];
+            // 2 A = (z.re + zm.re, z.im - zm.im); 2 B = (z.im + zm.im, zm.re - z.re)
+            const T re = second ? z.im + zm.im : z.re + zm.re;
+            const T im = second ? zm.re - z.re : z.im - zm.im;
+            acc.re += re * w;
+            acc.im += im * w;
         }
-        for (; a < K; ++a) {
-            const T w = kym_ldg(f + p + a * NN);
-            const cpx<T> v = X[a * NN];
-            acc.re += v.re * w;
-            acc.im += v.im * w;
-        }
-        P[q] = acc * inv_k;
+        P[q] = acc * inv_2k;
     }
     wg_barrier();
     fft_dif_s<true, NN, M, 1, NN, false, NT>(P, fn.tw, tid);
@@ -702,15 +700,15 @@ KYM_HD void lowpass_body_sv(const LowpassArgs& o, const FftLen<T>& fN, const Fft
 
 // Compile-time low-pass shapes X(id, N, n, lines per workgroup M, strided):
 // time averaging of the time-frequency transform at the default averaging
-// (outputs of 128; M = 4096 / N contiguous lines) and frequency averaging
-// along the padded n1 axis (88 or 22 points, M lines of stride S, S a
-// multiple of M).
+// (outputs of 128; M = 8192 / N contiguous lines, two per transform) and
+// frequency averaging along the padded n1 axis (88 or 22 points, M lines of
+// stride S, S a multiple of M).
 #define KYM_LOWPASS_SPECS(X)  \
-    X(0, 4096, 128, 1, 0)     \
-    X(1, 2048, 128, 2, 0)     \
-    X(2, 1024, 128, 4, 0)     \
-    X(3, 512, 128, 8, 0)      \
-    X(4, 256, 128, 16, 0)     \
+    X(0, 4096, 128, 2, 0)     \
+    X(1, 2048, 128, 4, 0)     \
+    X(2, 1024, 128, 8, 0)     \
+    X(3, 512, 128, 16, 0)     \
+    X(4, 256, 128, 32, 0)     \
     X(5, 88, 22, 32, 1)       \
     X(6, 22, 22, 64, 1)
 
@@ -723,6 +721,12 @@ inline int lowpass_spec_id(int N, int n, int S, int* rows) {
     KYM_LOWPASS_SPECS(KYM_LP_MATCH)
 #undef KYM_LP_MATCH
     return -1;
+}
+
+// Complex elements of LDS of a compile-time low-pass: two lines per
+// transform for contiguous lines (S = 1), one per line along a stride.
+inline long long lowpass_spec_lds(int N, int n, int S, int M) {
+    return S == 1 ? (long long)(M / 2) * N + (long long)M * n : (long long)M * (N + n);
 }
 
 template <class T, int SPEC, bool EMU>
