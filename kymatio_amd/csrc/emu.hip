@@ -86,8 +86,13 @@ static int emu_forward2d(int M, int N, int J, int L, int max_order, int pre_pad,
 
 template <class T>
 static int emu_train2d(int M, int N, int J, int L, int max_order, int pre_pad, long long lds_limit, const void* x,
-                       const void* filt, void* out, const void* dS, void* gx, long long batch, int nbwd) {
+                       const void* filt, void* out, const void* dS, void* gx, long long batch, int nbwd,
+                       const int* tune = nullptr, int ntune = 0) {
     Scat2DPlan<T> p;
+    int* fields[7] = {&p.tune.max_pow2, &p.tune.node_threads, &p.tune.col_block, &p.tune.row_block,
+                      &p.tune.pass_threads, &p.tune.streams, &p.tune.pair};
+    for (int i = 0; i < ntune && i < 7; ++i)
+        if (tune[i] != 0) *fields[i] = tune[i];
     KymError e = {0, ""};
     if (!p.build(M, N, J, L, max_order, pre_pad, (size_t)lds_limit, &e)) {
         g_emu_error = e.msg;
@@ -139,6 +144,18 @@ extern "C" int kym_emu_scat2d_train(int M, int N, int J, int L, int max_order, i
                                     void* gx, long long batch, int nbwd) {
     if (dtype == 0) return emu_train2d<float>(M, N, J, L, max_order, pre_pad, lds_limit, x, filt, out, dS, gx, batch, nbwd);
     return emu_train2d<double>(M, N, J, L, max_order, pre_pad, lds_limit, x, filt, out, dS, gx, batch, nbwd);
+}
+
+// As kym_emu_scat2d_train with plan tuning choices (kym_scat2d_create_tuned).
+extern "C" int kym_emu_scat2d_train_tuned(int M, int N, int J, int L, int max_order, int pre_pad, int dtype,
+                                          long long lds_limit, const void* x, const void* filt, void* out,
+                                          const void* dS, void* gx, long long batch, int nbwd, const int* tune,
+                                          int ntune) {
+    if (dtype == 0)
+        return emu_train2d<float>(M, N, J, L, max_order, pre_pad, lds_limit, x, filt, out, dS, gx, batch, nbwd, tune,
+                                  ntune);
+    return emu_train2d<double>(M, N, J, L, max_order, pre_pad, lds_limit, x, filt, out, dS, gx, batch, nbwd, tune,
+                               ntune);
 }
 
 // Leaf-tap descriptor of the plan (kym_scat2d_leaf_taps).

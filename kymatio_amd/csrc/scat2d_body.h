@@ -1396,8 +1396,11 @@ KYM_HD void dispatch_node_spec(int id, const Scat2DArgs<T>& a, int task, cpx<T>*
 
 // Compile-time specialisations X(id, H, W, col block, row block, threads) of
 // the big-node passes (FP32): root rows, inverse columns, rows + modulus,
-// forward columns.  272: 256x256 at J=3; 160 / 320 / 640: 128x128 at J=4,
-// 256x256 at J=5 and 512x512 at J=6 (the mixed-size batch).
+// forward columns, and the backward's passes.  272: 256x256 at J=3; 160 /
+// 320 / 640: 128x128 at J=4, 256x256 at J=5 and 512x512 at J=6 (the
+// mixed-size batch).  An untuned forward takes the first entry of its sizes
+// and rows, so each size lists its forward block first; 160 x 160 by 16 x 8
+// serves the backward (its gradient gathers read 16 consecutive columns).
 #define KYM_PASS_SPECS(X)         \
     X(0, 272, 272, 16, 8, 256)    \
     X(1, 272, 272, 16, 4, 256)    \
@@ -1409,7 +1412,18 @@ KYM_HD void dispatch_node_spec(int id, const Scat2DArgs<T>& a, int task, cpx<T>*
     X(7, 272, 272, 8, 8, 192)     \
     X(8, 160, 160, 8, 8, 256)     \
     X(9, 320, 320, 8, 8, 256)     \
-    X(10, 640, 640, 8, 8, 256)
+    X(10, 640, 640, 8, 8, 256)    \
+    X(11, 160, 160, 16, 8, 256)   \
+    X(12, 160, 160, 32, 8, 256)
+
+// Columns of the first specialisation of these sizes and rows (0: none).
+inline int pass_spec_first_cols(int H, int W, int rb) {
+#define KYM_PS_FIRST(id, h, w, c, r, nt) \
+    if (H == h && W == w && rb == r) return c;
+    KYM_PASS_SPECS(KYM_PS_FIRST)
+#undef KYM_PS_FIRST
+    return 0;
+}
 
 // want_threads = 0: the first specialisation of these sizes and blocks.
 inline int pass_spec_id(int H, int W, int cb, int rb, int want_threads, int* threads) {

@@ -432,12 +432,12 @@ struct Scat2DPlan {
             } else {
                 int cb = col_blk(H[r]);
                 const int rb = row_blk(r);
-                // the heuristic's column block yields to 8 columns where only that
-                // block has compile-time passes for these sizes (160, 320, 640)
-                if (!tune.col_block && specs && sizeof(T) == 4 && cb > 8 && W[r] % 8 == 0 && H[r] % rb == 0) {
-                    int t0 = 0;
-                    if (pass_spec_id(H[r], W[r], cb, rb, 0, &t0) < 0 && pass_spec_id(H[r], W[r], 8, rb, 0, &t0) >= 0)
-                        cb = 8;
+                // an untuned column block is the first compiled one for these sizes
+                // and rows (the spec table lists each size's forward choice first:
+                // 16 columns at 272, 8 at 160, 320 and 640)
+                if (!tune.col_block && specs && sizeof(T) == 4 && H[r] % rb == 0) {
+                    const int c0 = pass_spec_first_cols(H[r], W[r], rb);
+                    if (c0 > 0 && W[r] % c0 == 0) cb = c0;
                 }
                 // compile-time passes when the sizes match and every block is full
                 int pnt = 0;
@@ -524,7 +524,9 @@ struct Scat2DPlan {
                     max_lds = std::max(max_lds, launches.back().lds_bytes);
                 }
             } else {
-                int cb = col_blk(H[r]), rb = std::max(1, std::min(16, (int)(pass_budget / (cs * W[r]))));
+                int cb = col_blk(H[r]);
+                int rb = std::max(1, std::min(tune.row_block ? tune.row_block : 16,
+                                              (int)((tune.row_block ? lds_limit : pass_budget) / (cs * W[r]))));
                 // compile-time passes for these sizes and blocks; untuned blocks
                 // take 8 x 8 where only that shape is compiled
                 int pnt = 0, psid = -1;
@@ -534,6 +536,9 @@ struct Scat2DPlan {
                         return (H[r] % rr == 0 && W[r] % c == 0) ? pass_spec_id(H[r], W[r], c, rr, want, &pnt) : -1;
                     };
                     psid = spec_for(cb, rb);
+                    // untuned: the heuristic's columns by 8 rows (160: 16 x 8, 5.7% faster
+                    // than 8 x 8 on forward + backward at 128x128 J=4), else 8 x 8
+                    if (psid < 0 && !tune.col_block && !tune.row_block && (psid = spec_for(cb, 8)) >= 0) rb = 8;
                     if (psid < 0 && !tune.col_block && !tune.row_block && (psid = spec_for(8, 8)) >= 0) cb = rb = 8;
                 }
                 auto bpass = [&](int kind, int nblk, int blk, size_t lds) {

@@ -166,13 +166,17 @@ def emu_scat1d(emu, module, x, dtype=np.float32, dS=None, nbwd=1):
     return out, info[0], gx
 
 
-def emu_train2d(emu, x, J, shape, filters_packed, dS, L=8, max_order=2, pre_pad=0, lds_limit=160 * 1024, nbwd=1):
+def emu_train2d(emu, x, J, shape, filters_packed, dS, L=8, max_order=2, pre_pad=0, lds_limit=160 * 1024, nbwd=1,
+                tune=None):
     """Training forward + hand-written backward (nbwd times on the same saved
-    state) of the 2D bodies on the CPU harness; returns (S, dL/dx of the last
-    backward) for the upstream gradient dS."""
-    f = emu.lib.kym_emu_scat2d_train
-    f.argtypes = [ctypes.c_int] * 7 + [ctypes.c_longlong] + [ctypes.c_void_p] * 5 + [ctypes.c_longlong, ctypes.c_int]
+    state) of the 2D bodies on the CPU harness, with the plan tuning choices
+    `tune` if given; returns (S, dL/dx of the last backward) for the upstream
+    gradient dS."""
+    f = emu.lib.kym_emu_scat2d_train_tuned
+    f.argtypes = [ctypes.c_int] * 7 + [ctypes.c_longlong] + [ctypes.c_void_p] * 5 + [ctypes.c_longlong, ctypes.c_int,
+                                                                                      ctypes.c_void_p, ctypes.c_int]
     f.restype = ctypes.c_int
+    tv = (ctypes.c_int * 7)(*(list(tune or []) + [0] * 7)[:7])
     dt = filters_packed.dtype
     filters_packed = emu.with_taps(filters_packed, J, shape, L, max_order, pre_pad, lds_limit)
     xx = np.ascontiguousarray(x, dt)
@@ -180,7 +184,8 @@ def emu_train2d(emu, x, J, shape, filters_packed, dS, L=8, max_order=2, pre_pad=
     out = np.zeros(dS.shape, dt)
     gx = np.zeros(xx.shape, dt)
     rc = f(shape[0], shape[1], J, L, max_order, pre_pad, 0 if dt == np.float32 else 1, lds_limit, xx.ctypes.data,
-           filters_packed.ctypes.data, out.ctypes.data, dd.ctypes.data, gx.ctypes.data, x.shape[0], nbwd)
+           filters_packed.ctypes.data, out.ctypes.data, dd.ctypes.data, gx.ctypes.data, x.shape[0], nbwd, tv,
+           len(tune or []))
     if rc != 0:
         raise RuntimeError(emu.lib.kym_emu_last_error().decode())
     return out, gx

@@ -115,17 +115,19 @@ def test_backward_matches_autograd_through_oracle(emu, shape, J, kw, lds):
     assert np.abs(gx - g).max() <= 1e-12 * np.abs(g).max()
 
 
-@pytest.mark.parametrize('shape,J', [((32, 32), 2), ((64, 64), 3), ((128, 128), 4)])
-def test_backward_fp32_compiled_nodes(emu, shape, J):
+@pytest.mark.parametrize('shape,J,tune', [((32, 32), 2, None), ((64, 64), 3, None), ((128, 128), 4, None),
+                                          ((128, 128), 4, (16, 0, 16, 8)), ((128, 128), 4, (16, 0, 32, 8))])
+def test_backward_fp32_compiled_nodes(emu, shape, J, tune):
     """FP32 backward through the compile-time node kernels (40², 20² and 80²
     nodes with 10² leaves) and, at 128² J=4, the compile-time 160² backward
-    passes: equals torch autograd through the FP64 oracle to FP32 rounding,
-    on two backward passes of one forward."""
+    passes (the default blocks, and 16 or 32 columns by 8 rows): equals torch
+    autograd through the FP64 oracle to FP32 rounding, on two backward passes
+    of one forward."""
     fb64, _ = _filters(shape, J, 8, np.float64)
     _, f32 = _filters(shape, J, 8, np.float32)
     rng = np.random.default_rng(3)
     x = rng.standard_normal((2,) + shape)
-    _, info = emu.scat2d(x[:1].astype(np.float32), J, shape, f32)
+    _, info = emu.scat2d(x[:1].astype(np.float32), J, shape, f32, tune=list(tune) if tune else None)
     assert info['bwd_spec_launches'] >= (11 if J == 4 else 1)  # at J=4: 6 node launches + 5 of the 160² passes
     sc = scat2d.Scattering2DTorchCPU(J, shape, dtype=torch.float64, filters=fb64)
     xt = torch.from_numpy(x).requires_grad_()
@@ -133,7 +135,7 @@ def test_backward_fp32_compiled_nodes(emu, shape, J):
     w = rng.standard_normal(S.shape)
     (S * torch.from_numpy(w)).sum().backward()
     from tests.helpers import emu_train2d
-    out, gx = emu_train2d(emu, x.astype(np.float32), J, shape, f32, w.astype(np.float32), nbwd=2)
+    out, gx = emu_train2d(emu, x.astype(np.float32), J, shape, f32, w.astype(np.float32), nbwd=2, tune=tune)
     g = xt.grad.numpy()
     assert np.abs(out - S.detach().numpy()).max() <= 1e-5 * np.abs(S.detach().numpy()).max()
     assert np.abs(gx - g).max() <= 1e-5 * np.abs(g).max()
