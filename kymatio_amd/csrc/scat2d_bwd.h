@@ -98,6 +98,114 @@ KYM_HD KYM_INL void grad_u_elems(const Scat2DArgs<T>& a, const Node2D& nd, const
     }
 }
 
+// dL/dU at the A x B alias points (ys[e] + i H/2, xs[e] + j W/2), i < A,
+// j < B, of E base points (acc[e][i B + j]).  Every child of a node is at
+// half its resolution or coarser and its leaf at 2^-J of the padded size, so
+// H/2 and W/2 are multiples of every child's and the leaf's sizes (the
+// padded sizes are multiples of 2^J): all alias points of a base point read
+// the same child gradient and leaf value.  Each is loaded once per base
+// point; the filters are read per point.  Per point the terms are summed in
+// grad_u_elems' order, so results are identical to it.
+template <int A, int B, int E, class T>
+KYM_HD KYM_INL void grad_u_alias(const Scat2DArgs<T>& a, const Node2D& nd, const cpx<T>* wsi, const cpx<T>* GY, int W,
+                                 int HJ, int WJ, const int* ys, const int* xs, cpx<T> (*acc)[A * B]) {
+    constexpr int P = A * B;
+    const int H = a.H[nd.res], hh = H / 2, hw = W / 2;
+    int idx[E][P];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const cpx<T> gy = GY[(ys[e] % HJ) * WJ + (xs[e] % WJ)];
+        const T* ph = a.filt + a.filt_off[nd.leaf_filt];
+#pragma unroll
+        for (int i = 0; i < A; ++i)
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                const int q = i * B + j;
+                idx[e][q] = (ys[e] + i * hh) * W + xs[e] + j * hw;
+                const T f = kym_ldg(ph + idx[e][q]);
+                acc[e][q] = {gy.re * f, gy.im * f};
+            }
+    }
+    int yr[E], xr[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) { yr[e] = ys[e]; xr[e] = xs[e]; }
+    int lv = 0;  // level of the running residues yr, xr (as in grad_u_elems)
+    auto child_at = [&](const Node2D& cn, int e, int Hc, int Wc) -> long long {
+        const int l = cn.res - nd.res;
+        if (l < lv) return (long long)(ys[e] % Hc) * Wc + (xs[e] % Wc);
+        return (long long)yr[e] * Wc + xr[e];
+    };
+    auto advance = [&](const Node2D& cn) {
+        const int l = cn.res - nd.res;
+        while (lv < l) {
+            ++lv;
+            const int h2 = H >> lv, w2 = W >> lv;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                yr[e] -= yr[e] >= h2 ? h2 : 0;
+                xr[e] -= xr[e] >= w2 ? w2 : 0;
+            }
+        }
+    };
+    int c = nd.child_begin;
+    for (; c + 4 <= nd.child_end; c += 4) {
+        T fv[4][E][P];
+        cpx<T> gx[4][E];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const Node2D& cn = a.nodes[c + u];
+            advance(cn);
+            const int Hc = a.H[cn.res], Wc = a.W[cn.res];
+            const T* f = a.filt + a.filt_off[cn.filt];
+            const T inv = T(1) / T(cn.k * cn.k);
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                gx[u][e] = kym_ldg(wsi + cn.gsave + child_at(cn, e, Hc, Wc));
+#pragma unroll
+                for (int q = 0; q < P; ++q) fv[u][e][q] = kym_ldg(f + idx[e][q]) * inv;
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+#pragma unroll
+            for (int q = 0; q < P; ++q)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    acc[e][q].re += gx[u][e].re * fv[u][e][q];
+                    acc[e][q].im += gx[u][e].im * fv[u][e][q];
+                }
+    }
+    for (; c < nd.child_end; ++c) {
+        const Node2D& cn = a.nodes[c];
+        advance(cn);
+        const int Hc = a.H[cn.res], Wc = a.W[cn.res];
+        const T* f = a.filt + a.filt_off[cn.filt];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const cpx<T> gx = kym_ldg(wsi + cn.gsave + child_at(cn, e, Hc, Wc));
+#pragma unroll
+            for (int q = 0; q < P; ++q) {
+                const T fv = kym_ldg(f + idx[e][q]) / T(cn.k * cn.k);
+                acc[e][q].re += gx.re * fv;
+                acc[e][q].im += gx.im * fv;
+            }
+        }
+    }
+}
+
+// Whether a node's alias points share their child and leaf reads: sizes
+// even, the leaf's a divisor of the half sizes (false at J = 0), and every
+// child coarser than the node (false at the root, whose j1 = 0 children are
+// at its own resolution).
+template <class T>
+KYM_HD KYM_INL bool grad_alias_ok(const Scat2DArgs<T>& a, const Node2D& nd) {
+    const int H = a.H[nd.res], W = a.W[nd.res], HJ = a.H[a.J], WJ = a.W[a.J];
+    if (H % 2 != 0 || W % 2 != 0 || (H / 2) % HJ != 0 || (W / 2) % WJ != 0) return false;
+    for (int c = nd.child_begin; c < nd.child_end; ++c)
+        if (a.nodes[c].res <= nd.res) return false;
+    return true;
+}
+
 // Leaf gradient into B (HJ x WJ, natural), scaled by 1 / (n_J kl^2).
 template <class T, class G>
 KYM_HD void leaf_grad(const Scat2DArgs<T>& a, const G& g, const Node2D& nd, int img, cpx<T>* B, int tid, int nth) {
@@ -132,20 +240,46 @@ KYM_HD void bwd_node_body(const Scat2DArgs<T>& a, const G& g, const Node2D& nd, 
     } else {
         GY = wsi + nd.gslot;
     }
-    constexpr int E = 2;  // elements per thread and round
-    for (int i0 = tid; i0 < n; i0 += E * nth) {
-        int ys[E], xs[E];
+    constexpr int E = 2;  // elements (base points) per thread and round
+    if (grad_alias_ok(a, nd)) {
+        // the points (y, x), (y, x + W/2), (y + H/2, x), (y + H/2, x + W/2)
+        // share their child and leaf reads
+        const int hh = g.H / 2, hw = g.W / 2, nb = hh * hw;
+        for (int i0 = tid; i0 < nb; i0 += E * nth) {
+            int ys[E], xs[E];
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const int i = i0 + e * nth < n ? i0 + e * nth : i0;
-            ys[e] = i / g.W;
-            xs[e] = i - ys[e] * g.W;
+            for (int e = 0; e < E; ++e) {
+                const int i = i0 + e * nth < nb ? i0 + e * nth : i0;
+                ys[e] = i / hw;
+                xs[e] = i - ys[e] * hw;
+            }
+            cpx<T> gu[E][4];
+            grad_u_alias<2, 2, E>(a, nd, wsi, GY, g.W, g.HJ, g.WJ, ys, xs, gu);
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+                if (i0 + e * nth < nb) {
+                    const int p = ys[e] * g.W + xs[e];
+                    lds[p] = gu[e][0];
+                    lds[p + hw] = gu[e][1];
+                    lds[p + hh * g.W] = gu[e][2];
+                    lds[p + hh * g.W + hw] = gu[e][3];
+                }
         }
-        cpx<T> gu[E];
-        grad_u_elems<E>(a, nd, wsi, GY, g.W, g.HJ, g.WJ, ys, xs, gu);
+    } else {
+        for (int i0 = tid; i0 < n; i0 += E * nth) {
+            int ys[E], xs[E];
 #pragma unroll
-        for (int e = 0; e < E; ++e)
-            if (i0 + e * nth < n) lds[i0 + e * nth] = gu[e];
+            for (int e = 0; e < E; ++e) {
+                const int i = i0 + e * nth < n ? i0 + e * nth : i0;
+                ys[e] = i / g.W;
+                xs[e] = i - ys[e] * g.W;
+            }
+            cpx<T> gu[E];
+            grad_u_elems<E>(a, nd, wsi, GY, g.W, g.HJ, g.WJ, ys, xs, gu);
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+                if (i0 + e * nth < n) lds[i0 + e * nth] = gu[e];
+        }
     }
     wg_barrier();
     g.inv2d(lds, tid, nth);  // unnormalised: Re() is dL/dm (or dL/dpad(x) at the root)
@@ -240,20 +374,42 @@ KYM_HD void body_bwd_gcols(const P& fp, const Scat2DArgs<T>& a, int task, cpx<T>
     const int r = nd.res, H = a.H[r], W = a.W[r], J = a.J;
     const int c0 = cb * a.blk, nc = (W - c0 < a.blk) ? W - c0 : a.blk;
     cpx<T>* wsi = a.ws + il * a.ws_img;
-    constexpr int E = 2;  // elements per thread and round
-    for (int i0 = tid; i0 < H * nc; i0 += E * nth) {
-        int ys[E], xs[E];
+    constexpr int E = 2;  // elements (base points) per thread and round
+    if (grad_alias_ok(a, nd)) {
+        // rows y and y + H/2 of a column share their child and leaf reads
+        const int nb = (H / 2) * nc;
+        for (int i0 = tid; i0 < nb; i0 += E * nth) {
+            int ys[E], xs[E];
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const int i = i0 + e * nth < H * nc ? i0 + e * nth : i0;
-            ys[e] = i / nc;
-            xs[e] = c0 + i - ys[e] * nc;
+            for (int e = 0; e < E; ++e) {
+                const int i = i0 + e * nth < nb ? i0 + e * nth : i0;
+                ys[e] = i / nc;
+                xs[e] = c0 + i - ys[e] * nc;
+            }
+            cpx<T> g[E][2];
+            grad_u_alias<2, 1, E>(a, nd, wsi, wsi + nd.gslot, W, a.H[J], a.W[J], ys, xs, g);
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+                if (i0 + e * nth < nb) {
+                    lds[i0 + e * nth] = g[e][0];
+                    lds[i0 + e * nth + nb] = g[e][1];
+                }
         }
-        cpx<T> g[E];
-        grad_u_elems<E>(a, nd, wsi, wsi + nd.gslot, W, a.H[J], a.W[J], ys, xs, g);
+    } else {
+        for (int i0 = tid; i0 < H * nc; i0 += E * nth) {
+            int ys[E], xs[E];
 #pragma unroll
-        for (int e = 0; e < E; ++e)
-            if (i0 + e * nth < H * nc) lds[i0 + e * nth] = g[e];
+            for (int e = 0; e < E; ++e) {
+                const int i = i0 + e * nth < H * nc ? i0 + e * nth : i0;
+                ys[e] = i / nc;
+                xs[e] = c0 + i - ys[e] * nc;
+            }
+            cpx<T> g[E];
+            grad_u_elems<E>(a, nd, wsi, wsi + nd.gslot, W, a.H[J], a.W[J], ys, xs, g);
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+                if (i0 + e * nth < H * nc) lds[i0 + e * nth] = g[e];
+        }
     }
     wg_barrier();
     fp.col_inv(lds, a.lens[a.lenH[r]], nc, tid, nth);
