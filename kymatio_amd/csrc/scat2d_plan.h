@@ -535,7 +535,11 @@ struct Scat2DPlan {
                     auto spec_for = [&](int c, int rr) {
                         return (H[r] % rr == 0 && W[r] % c == 0) ? pass_spec_id(H[r], W[r], c, rr, want, &pnt) : -1;
                     };
-                    psid = spec_for(cb, rb);
+                    // untuned root: 8 x 8 blocks (one node per image: twice the
+                    // workgroups of 16 columns)
+                    if (n0.order == 0 && !tune.col_block && !tune.row_block && (psid = spec_for(8, 8)) >= 0)
+                        cb = rb = 8;
+                    if (psid < 0) psid = spec_for(cb, rb);
                     // untuned: the heuristic's columns by 8 rows (160: 16 x 8, 5.7% faster
                     // than 8 x 8 on forward + backward at 128x128 J=4), else 8 x 8
                     if (psid < 0 && !tune.col_block && !tune.row_block && (psid = spec_for(cb, 8)) >= 0) rb = 8;
