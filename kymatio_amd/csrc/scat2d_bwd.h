@@ -240,30 +240,21 @@ KYM_HD void bwd_node_body(const Scat2DArgs<T>& a, const G& g, const Node2D& nd, 
     } else {
         GY = wsi + nd.gslot;
     }
-    constexpr int E = 2;  // elements (base points) per thread and round
-    if (grad_alias_ok(a, nd)) {
+    constexpr int E = 2;  // elements per thread and round
+    if (nd.child_end > nd.child_begin && grad_alias_ok(a, nd)) {
         // the points (y, x), (y, x + W/2), (y + H/2, x), (y + H/2, x + W/2)
-        // share their child and leaf reads
+        // share their child and leaf reads (one base point per thread and
+        // round; nodes without children keep the plain loop)
         const int hh = g.H / 2, hw = g.W / 2, nb = hh * hw;
-        for (int i0 = tid; i0 < nb; i0 += E * nth) {
-            int ys[E], xs[E];
-#pragma unroll
-            for (int e = 0; e < E; ++e) {
-                const int i = i0 + e * nth < nb ? i0 + e * nth : i0;
-                ys[e] = i / hw;
-                xs[e] = i - ys[e] * hw;
-            }
-            cpx<T> gu[E][4];
-            grad_u_alias<2, 2, E>(a, nd, wsi, GY, g.W, g.HJ, g.WJ, ys, xs, gu);
-#pragma unroll
-            for (int e = 0; e < E; ++e)
-                if (i0 + e * nth < nb) {
-                    const int p = ys[e] * g.W + xs[e];
-                    lds[p] = gu[e][0];
-                    lds[p + hw] = gu[e][1];
-                    lds[p + hh * g.W] = gu[e][2];
-                    lds[p + hh * g.W + hw] = gu[e][3];
-                }
+        for (int i = tid; i < nb; i += nth) {
+            const int y = i / hw, x = i - y * hw;
+            cpx<T> gu[1][4];
+            grad_u_alias<2, 2, 1>(a, nd, wsi, GY, g.W, g.HJ, g.WJ, &y, &x, gu);
+            const int p = y * g.W + x;
+            lds[p] = gu[0][0];
+            lds[p + hw] = gu[0][1];
+            lds[p + hh * g.W] = gu[0][2];
+            lds[p + hh * g.W + hw] = gu[0][3];
         }
     } else {
         for (int i0 = tid; i0 < n; i0 += E * nth) {
@@ -375,7 +366,7 @@ KYM_HD void body_bwd_gcols(const P& fp, const Scat2DArgs<T>& a, int task, cpx<T>
     const int c0 = cb * a.blk, nc = (W - c0 < a.blk) ? W - c0 : a.blk;
     cpx<T>* wsi = a.ws + il * a.ws_img;
     constexpr int E = 2;  // elements (base points) per thread and round
-    if (grad_alias_ok(a, nd)) {
+    if (nd.child_end > nd.child_begin && grad_alias_ok(a, nd)) {
         // rows y and y + H/2 of a column share their child and leaf reads
         const int nb = (H / 2) * nc;
         for (int i0 = tid; i0 < nb; i0 += E * nth) {
