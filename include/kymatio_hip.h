@@ -57,7 +57,9 @@ int kym_scat2d_create(int M, int N, int J, int L, int max_order, int pre_pad, in
  * forward runs its chunks one after the other on the caller's stream; S in
  * 2..4, default 2: the workspace is split in S parts and chunks rotate over
  * the caller's stream and S - 1 streams of the plan, joined back to the
- * caller's stream before kym_scat2d_forward's work ends), pair (1:
+ * caller's stream before kym_scat2d_forward's work ends; forward_train and
+ * backward split the batch in S chunks the same way, each image's saved
+ * state at its own offset of the training workspace), pair (1:
  * one LDS node per workgroup; 2: sibling nodes of one parent share a
  * workgroup and one read of the parent's spectrum wherever a pair kernel
  * for the size is compiled; 0, the default: where that is the default

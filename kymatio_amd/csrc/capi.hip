@@ -97,7 +97,8 @@ struct ChunkStreams {
 // 85, 85, 85, 1).
 static int chunk_streams(const ChunkStreams* cs, int want, int mode, long long batch, long long* chunk) {
     int ns = want == 0 ? 2 : want;
-    if (mode != 0 || !cs || !cs->aux[0]) ns = 1;
+    if (!cs || !cs->aux[0]) ns = 1;
+    (void)mode;
     while (ns > 1 && (*chunk < ns || batch <= *chunk / ns * (ns - 1))) --ns;
     if (ns > 1) {
         const long long part = *chunk / ns;
@@ -317,11 +318,13 @@ static int chunk_of(const Scat2DPlan<T>& p, int mode, long long batch, long long
 
 // mode 0: inference forward (batch in chunks); 1: training forward (saves
 // u, whole batch in the workspace); 2: backward from the saved workspace.
-// With S streams (inference; tune.streams, 0 = 2; at least S images of
-// workspace) the workspace is split in S parts: chunk k runs on stream
-// k mod S in part k mod S, so chunk k + S reuses a part only after chunk k on
-// the same stream.  The extra streams wait for the caller's stream before
-// their first chunk, and the caller's stream waits for them at the end.
+// With S streams (tune.streams, 0 = 2; at least S images) chunk k runs on
+// stream k mod S.  Inference splits the workspace in S parts and chunk k
+// uses part k mod S, so chunk k + S reuses a part only after chunk k on the
+// same stream; training keeps image i's state at image offset i of the
+// workspace whatever the chunks, so the backward finds what the forward
+// saved.  The extra streams wait for the caller's stream before their first
+// chunk, and the caller's stream waits for them at the end.
 template <class T>
 static int run2d(const Scat2DPlan<T>& p, int mode, const void* x, const void* filt, void* out, const void* dS,
                  void* gx, long long batch, void* ws, long long ws_bytes, hipStream_t stream, uintptr_t base,
@@ -339,7 +342,7 @@ static int run2d(const Scat2DPlan<T>& p, int mode, const void* x, const void* fi
         const int nimg = (int)std::min(chunk, batch - i0);
         const int si = k % ns;
         const hipStream_t st = si ? cs->aux[si - 1] : stream;
-        void* wsk = static_cast<char*>(ws) + si * chunk * img_elems * (long long)sizeof(cpx<T>);
+        void* wsk = static_cast<char*>(ws) + (mode == 0 ? si * chunk : i0) * img_elems * (long long)sizeof(cpx<T>);
         for (const Launch2D& l : launches) {
             Launch2DCall<T> c;
             rc = launch_call(p, l, mode, x, filt, out, dS, gx, img_elems, wsk, i0, nimg, base, &c);
@@ -443,8 +446,11 @@ extern "C" int kym_scat2d_forward_train(void* plan, const void* x, const void* f
     KYM_HIP_TRY(hipGetDevice(&cur));
     if (cur != h->device) return fail(KYM_ERR_TYPE, "plan was created for another device");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    if (h->pf) return run2d(*h->pf, 1, x, filt, out, nullptr, nullptr, batch, ws, ws_bytes, s, (uintptr_t)h->dblob, &h->prof);
-    return run2d(*h->pd, 1, x, filt, out, nullptr, nullptr, batch, ws, ws_bytes, s, (uintptr_t)h->dblob, &h->prof);
+    if (h->pf)
+        return run2d(*h->pf, 1, x, filt, out, nullptr, nullptr, batch, ws, ws_bytes, s, (uintptr_t)h->dblob, &h->prof,
+                     &h->cs);
+    return run2d(*h->pd, 1, x, filt, out, nullptr, nullptr, batch, ws, ws_bytes, s, (uintptr_t)h->dblob, &h->prof,
+                 &h->cs);
 }
 
 extern "C" int kym_scat2d_backward(void* plan, const void* dS, const void* filt, void* gx, long long batch,
@@ -458,8 +464,11 @@ extern "C" int kym_scat2d_backward(void* plan, const void* dS, const void* filt,
     KYM_HIP_TRY(hipGetDevice(&cur));
     if (cur != h->device) return fail(KYM_ERR_TYPE, "plan was created for another device");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    if (h->pf) return run2d(*h->pf, 2, nullptr, filt, nullptr, dS, gx, batch, ws, ws_bytes, s, (uintptr_t)h->dblob, nullptr);
-    return run2d(*h->pd, 2, nullptr, filt, nullptr, dS, gx, batch, ws, ws_bytes, s, (uintptr_t)h->dblob, nullptr);
+    if (h->pf)
+        return run2d(*h->pf, 2, nullptr, filt, nullptr, dS, gx, batch, ws, ws_bytes, s, (uintptr_t)h->dblob, nullptr,
+                     &h->cs);
+    return run2d(*h->pd, 2, nullptr, filt, nullptr, dS, gx, batch, ws, ws_bytes, s, (uintptr_t)h->dblob, nullptr,
+                 &h->cs);
 }
 
 template <class T>

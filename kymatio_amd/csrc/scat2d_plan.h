@@ -29,7 +29,7 @@ struct Scat2DTune {
     int col_block = 0;      // columns per workgroup of the big-node column passes
     int row_block = 0;      // rows per workgroup of the big-node row passes
     int pass_threads = 0;   // threads of the big-node pass workgroups (compiled choices only)
-    int streams = 0;        // inference chunks in flight: 1 to 4 (0 = default 2; see kym_scat2d_forward)
+    int streams = 0;        // chunks in flight (inference and training): 1 to 4 (0 = default 2)
     int pair = 0;           // LDS nodes: 1 one node per workgroup, 2 sibling pairs wherever a pair kernel is
                             // compiled (0 = heuristic: the default pair kernels, the others from k = 16)
 };
