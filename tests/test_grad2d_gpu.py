@@ -125,3 +125,28 @@ def test_no_grad_path_unchanged():
     a = sc(x)
     b = sc(x.clone().requires_grad_())
     assert torch.allclose(a, b.detach(), rtol=1e-6, atol=1e-7)
+
+
+@pytest.mark.parametrize('B', [3, 8])
+def test_two_stream_training_matches_one_stream(B):
+    """The training forward and the backward split the batch over two
+    streams (each image's saved state at its own workspace offset): outputs
+    and gradients equal those of the one-stream plan bit for bit, also for an
+    odd batch (3 images: three one-image chunks alternating over the two
+    streams; 8 images: two chunks of 4), on two backward passes of one
+    forward."""
+    rng = np.random.default_rng(21)
+    x = torch.from_numpy(rng.standard_normal((B, 64, 64)).astype(np.float32)).to(DEV)
+    res = []
+    for streams in (1, 2):
+        sc = Scattering2D(3, (64, 64)).to(DEV)
+        sc.tune = [16, 0, 0, 0, 0, streams]
+        xt = x.clone().requires_grad_()
+        S = sc(xt)
+        w = torch.from_numpy(rng.standard_normal(tuple(S.shape)).astype(np.float32)).to(DEV) if not res else res[0][2]
+        grads = torch.autograd.grad((S * w).sum(), xt, retain_graph=True)[0], \
+            torch.autograd.grad((S * w).sum(), xt)[0]
+        res.append((S.detach(), grads, w))
+    (S1, (g1a, g1b), _), (S2, (g2a, g2b), _) = res
+    assert torch.equal(S1, S2)
+    assert torch.equal(g1a, g2a) and torch.equal(g1b, g2b) and torch.equal(g2a, g2b)
