@@ -616,6 +616,15 @@ KYM_HD KYM_INL long long fir_lds_reals(int H, int W, int ow, int kl, int cnt_y, 
 // Band length of a leaf pass: the samples one 16-output tile touches.
 KYM_HD KYM_INL int fir_kmax(int n, int kl, int cnt) { return (15 * kl + cnt < n) ? 15 * kl + cnt : n; }
 
+// Row passes copy a band table of up to KYM_FIR_BAND_LDS bytes to LDS and
+// read a larger one (J >= 5 at 320 and 640 points: 20 and 40 KB, the whole
+// row for every output) from global memory, where every workgroup of the
+// launch shares it through L2: the rows then fit two workgroups per CU.
+#define KYM_FIR_BAND_LDS (16 * 1024)
+KYM_HD KYM_INL constexpr bool fir_band_in_lds(int kmax, int elem_bytes) {
+    return 16ll * kmax * elem_bytes <= KYM_FIR_BAND_LDS;
+}
+
 // band[k][d] = c[(k - kl d) mod n] (zero beyond the taps), k < kmax, d < 16:
 // inside a tile's window the circulant factor depends only on (k, d).  The
 // frontend stores it next to the taps; passes copy it to LDS.
@@ -1085,17 +1094,21 @@ struct StatPass {
 
 // Row pass of a big node's FIR leaf: rows rr < nr of R (natural columns,
 // row stride rs) are the node's rows rev[r0 + rr]; T[y][x] goes to the
-// tslot.  Bx: band table space after R (fir_kmax(W, ..) x 16 reals).
+// tslot.  Bx: band table space after R (fir_kmax(W, ..) x 16 reals) where
+// the band is kept in LDS (fir_band_in_lds), else unused.
 template <class T>
 KYM_HD void fir_rows_to_slot(const Scat2DArgs<T>& a, const Node2D& nd, const T* R, int rs, int nr, int r0,
                              const int* rev, cpx<T>* wsi, T* Bx, int tid, int nth) {
     const int r = nd.res, W = a.W[r], ow = a.ow, kl = nd.leaf_k;
     const FirAxis fx = a.fir[2 * r + 1];
-    fir_band(Bx, a.filt, fx, tid, nth);
-    wg_barrier();
+    const bool in_lds = fir_band_in_lds(fx.kmax, (int)sizeof(T));
+    if (in_lds) fir_band(Bx, a.filt, fx, tid, nth);
+    wg_barrier();  // also: R complete
     T* Tg = reinterpret_cast<T*>(wsi + nd.tslot);
-    fir_rows_mma(R, rs, nr, W, kl, ow, fx, Bx, [&](int p, int x, float v) { Tg[(long long)rev[r0 + p] * ow + x] = v; },
-                 tid, nth);
+    auto out = [&](int p, int x, float v) { Tg[(long long)rev[r0 + p] * ow + x] = v; };
+    // two calls, so that each reads its band through a known address space
+    if (in_lds) fir_rows_mma(R, rs, nr, W, kl, ow, fx, Bx, out, tid, nth);
+    else fir_rows_mma(R, rs, nr, W, kl, ow, fx, a.filt + fx.band_off, out, tid, nth);
 }
 
 // The row spectra of a real field are Hermitian along x (row[W - x] =

@@ -348,7 +348,8 @@ struct Scat2DPlan {
         // and the leaf's band table along x
         auto row_bytes = [&](int Ww) { return cs * Ww + (fir ? sizeof(T) * (Ww + 1) : 0); };
         auto band_bytes = [&](int r) {
-            return fir ? sizeof(T) * 16 * (size_t)fir_axes[2 * r + 1].kmax : 0;
+            const int km = fir_axes[2 * r + 1].kmax;
+            return fir && fir_band_in_lds(km, (int)sizeof(T)) ? sizeof(T) * 16 * (size_t)km : 0;
         };
         // FIR row passes hold 8 rows at most, so that four of their workgroups share a CU
         auto row_blk = [&](int r) {
