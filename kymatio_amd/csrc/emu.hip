@@ -24,6 +24,7 @@ static void emu_spec(const Launch2D& l, const Scat2DArgs<T>& a, int t, cpx<T>* l
         case K2_LDS_NODE: dispatch_node_spec<T, true>(l.spec, a, t, lds, 0); break;
         case K2_ROOT_ROWS: dispatch_pass_spec<T, K2_ROOT_ROWS, true>(l.spec, a, t, lds, 0); break;
         case K2_ICOL: dispatch_pass_spec<T, K2_ICOL, true>(l.spec, a, t, lds, 0); break;
+        case K2_ICOL_PAIR: dispatch_pass_spec<T, K2_ICOL_PAIR, true>(l.spec, a, t, lds, 0); break;
         case K2_ROWS_MOD: dispatch_pass_spec<T, K2_ROWS_MOD, true>(l.spec, a, t, lds, 0); break;
         case K2_FWD_COLS: dispatch_pass_spec<T, K2_FWD_COLS, true>(l.spec, a, t, lds, 0); break;
         default: break;
@@ -48,11 +49,12 @@ static int emu_forward2d(int M, int N, int J, int L, int max_order, int pre_pad,
     const uintptr_t base = (uintptr_t)p.blob.bytes.data();
     if (info) {
         info[0] = p.C; info[1] = p.oh; info[2] = p.ow; info[3] = (long long)p.launches.size();
-        int nbig = 0, nspec = 0, npair = 0;
+        int nbig = 0, nspec = 0, npair = 0, ncpair = 0;
         for (auto& nd : p.nodes) nbig += nd.big;
         for (auto& l : p.launches) {
             nspec += l.spec >= 0;
-            npair += l.pair;
+            npair += l.pair && l.kind == K2_LDS_NODE;
+            ncpair += l.kind == K2_ICOL_PAIR;
         }
         info[4] = nbig;
         info[5] = nspec;
@@ -60,6 +62,7 @@ static int emu_forward2d(int M, int N, int J, int L, int max_order, int pre_pad,
         int nbspec = 0;
         for (auto& l : p.bwd_launches) nbspec += l.spec >= 0;
         info[7] = nbspec;
+        info[8] = ncpair;
     }
     if (!x) return 0;
     if (chunk <= 0 || chunk > batch) chunk = batch;

@@ -91,6 +91,7 @@ Scat2DKernel<T> pick_pass_spec(int kind, int id) {
     switch (kind) {
         case K2_ROOT_ROWS: return pick_pass_spec_k<T, K2_ROOT_ROWS>(id);
         case K2_ICOL: return pick_pass_spec_k<T, K2_ICOL>(id);
+        case K2_ICOL_PAIR: return pick_pass_spec_k<T, K2_ICOL_PAIR>(id);
         case K2_ROWS_MOD: return pick_pass_spec_k<T, K2_ROWS_MOD>(id);
         case K2_FWD_COLS: return pick_pass_spec_k<T, K2_FWD_COLS>(id);
         case K2B_GCOLS: return pick_pass_spec_k<T, K2B_GCOLS>(id);

@@ -343,6 +343,72 @@ KYM_HD KYM_INL void gather_block(cpx<T>* dst, int ncols, int c0, const cpx<T>* U
     }
 }
 
+// Two siblings' periodised products at (y, x): the same parent values
+// with the filters fa and fb (one load of each parent value for both).
+template <int K, class T>
+KYM_HD KYM_INL void gather_periodized_pair_k(const cpx<T>* U, const T* fa, const T* fb, int Wp, int H, int W, int y,
+                                             int x, cpx<T>* va, cpx<T>* vb) {
+    cpx<T> sa = {T(0), T(0)}, sb = {T(0), T(0)};
+#pragma unroll
+    for (int a = 0; a < K; ++a) {
+        const int row = (y + a * H) * Wp + x;
+#pragma unroll
+        for (int b = 0; b < K; ++b) {
+            const cpx<T> u = kym_ldg(U + row + b * W);
+            const T ga = kym_ldg(fa + row + b * W), gb = kym_ldg(fb + row + b * W);
+            sa.re += u.re * ga;
+            sa.im += u.im * ga;
+            sb.re += u.re * gb;
+            sb.im += u.im * gb;
+        }
+    }
+    *va = sa;
+    *vb = sb;
+}
+
+// gather_block for two siblings (same parent, same k) into dsta and dstb.
+template <int K, class T>
+KYM_HD KYM_INL void gather_block_pair_k(cpx<T>* dsta, cpx<T>* dstb, int ncols, int c0, const cpx<T>* U, const T* fa,
+                                        const T* fb, int H, int W, T scale, int tid, int nth) {
+    constexpr int E = K >= 4 ? 1 : 4 / K;
+    const int Wp = W * K;  // the parent's row length
+    const int n = H * ncols;
+    int i = tid;
+    for (; i + (E - 1) * nth < n; i += E * nth) {
+        cpx<T> va[E], vb[E];
+#pragma unroll
+        for (int u = 0; u < E; ++u) {
+            const int e = i + u * nth, y = e / ncols, c = e - y * ncols;
+            gather_periodized_pair_k<K>(U, fa, fb, Wp, H, W, y, c0 + c, &va[u], &vb[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < E; ++u) {
+            dsta[i + u * nth] = va[u] * scale;
+            dstb[i + u * nth] = vb[u] * scale;
+        }
+    }
+    for (; i < n; i += nth) {
+        const int y = i / ncols, c = i - y * ncols;
+        cpx<T> va, vb;
+        gather_periodized_pair_k<K>(U, fa, fb, Wp, H, W, y, c0 + c, &va, &vb);
+        dsta[i] = va * scale;
+        dstb[i] = vb * scale;
+    }
+}
+
+// Returns false for a k without a pair gather (the plan pairs k = 2 and 4).
+template <class T>
+KYM_HD KYM_INL bool gather_block_pair(cpx<T>* dsta, cpx<T>* dstb, int ncols, int c0, const cpx<T>* U, const T* fa,
+                                      const T* fb, int H, int W, int k, int tid, int nth) {
+    const T scale = T(1) / T(k * k);
+    switch (k) {
+        case 1: gather_block_pair_k<1>(dsta, dstb, ncols, c0, U, fa, fb, H, W, scale, tid, nth); return true;
+        case 2: gather_block_pair_k<2>(dsta, dstb, ncols, c0, U, fa, fb, H, W, scale, tid, nth); return true;
+        case 4: gather_block_pair_k<4>(dsta, dstb, ncols, c0, U, fa, fb, H, W, scale, tid, nth); return true;
+        default: return false;
+    }
+}
+
 // The whole H x W child spectrum (a node in LDS): by mirror pairs where the
 // sizes are even and K is a power of two up to 32, else gather_block.
 template <class T>
@@ -1199,6 +1265,39 @@ KYM_HD void body_icol(const P& fp, const Scat2DArgs<T>& a, int task, cpx<T>* lds
                 wsi + nd.slot, c0, nc, lds, tid, nth);
 }
 
+// Gather + inverse column DIF of two sibling big nodes (same parent and
+// periodisation; the plan lists them at group[2 gi] and group[2 gi + 1]):
+// one load of each parent value serves both filters.  Full column blocks
+// only (compile-time passes).  LDS: the two blocks, H x blk each.
+// task -> (image, pair, column block).
+template <class T, class P>
+KYM_HD void body_icol_pair(const P& fp, const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
+    const int per = a.ngroup * a.nblk;
+    const int il = task / per, rem = task - il * per;
+    const int gi = rem / a.nblk, cb = rem - gi * a.nblk;
+    const Node2D& na = a.nodes[a.group[2 * gi]];
+    const Node2D& nb = a.nodes[a.group[2 * gi + 1]];
+    const Node2D& pn = a.nodes[na.parent];
+    const int r = na.res, H = a.H[r], W = a.W[r];
+    const int c0 = cb * a.blk, nc = a.blk;
+    cpx<T>* wsi = a.ws + il * a.ws_img;
+    cpx<T>* la = lds;
+    cpx<T>* lb = lds + H * nc;
+    gather_block_pair(la, lb, nc, c0, wsi + pn.slot, a.filt + a.filt_off[na.filt], a.filt + a.filt_off[nb.filt], H, W,
+                      na.k, tid, nth);
+    wg_barrier();
+    const FftLen<T>& fH = a.lens[a.lenH[r]];
+    fp.col_inv(la, fH, nc, tid, nth);
+    fp.col_inv(lb, fH, nc, tid, nth);
+    cpx<T>* da = wsi + na.slot;
+    cpx<T>* db = wsi + nb.slot;
+    for (int i = tid; i < H * nc; i += nth) {
+        const int p = i / nc, c = i - (i / nc) * nc;
+        da[(long long)p * W + c0 + c] = la[i];
+        db[(long long)p * W + c0 + c] = lb[i];
+    }
+}
+
 template <class T, int MAXP>
 KYM_HD void body_leaf_icol(const Scat2DArgs<T>& a, int task, cpx<T>* lds, int tid, int nth) {
     const DynPass<T, MAXP> fp;
@@ -1451,7 +1550,8 @@ inline int pass_spec_id(int H, int W, int cb, int rb, int want_threads, int* thr
 }
 
 enum Kind2D {
-    K2_LDS_NODE = 0, K2_LEAF, K2_ROOT_ROWS, K2_ICOL, K2_ROWS_MOD, K2_FWD_COLS, K2_LEAF_ICOL, K2_LEAF_ROWS, K2_FIR_COLS
+    K2_LDS_NODE = 0, K2_LEAF, K2_ROOT_ROWS, K2_ICOL, K2_ROWS_MOD, K2_FWD_COLS, K2_LEAF_ICOL, K2_LEAF_ROWS, K2_FIR_COLS,
+    K2_ICOL_PAIR
 };
 
 template <class T, int KIND, class P>
@@ -1460,6 +1560,7 @@ KYM_HD KYM_INL void dispatch_pass(const P& fp, const Scat2DArgs<T>& a, int task,
     else if constexpr (KIND == K2_ICOL) body_icol<T>(fp, a, task, lds, tid, nth);
     else if constexpr (KIND == K2_ROWS_MOD) body_rows_mod<T>(fp, a, task, lds, tid, nth);
     else if constexpr (KIND == K2_FWD_COLS) body_fwd_cols<T>(fp, a, task, lds, tid, nth);
+    else if constexpr (KIND == K2_ICOL_PAIR) body_icol_pair<T>(fp, a, task, lds, tid, nth);
 }
 
 // The plan's FFT length record of length n (one of its H_r or W_r).

@@ -325,7 +325,8 @@ struct Scat2DPlan {
                                        kind == K2_LEAF || kind == K2_LEAF_ROWS || kind == K2_FIR_COLS;
                 const int npass = nd.order == 0 ? 2 : nd.fwd ? 3 : 2;  // big-node passes sharing the node's bytes
                 if (kind == K2_LDS_NODE) mb += node_model(id);
-                if (kind == K2_ROOT_ROWS || kind == K2_FWD_COLS || kind == K2_ICOL || kind == K2_ROWS_MOD)
+                if (kind == K2_ROOT_ROWS || kind == K2_FWD_COLS || kind == K2_ICOL || kind == K2_ICOL_PAIR ||
+                    kind == K2_ROWS_MOD)
                     mb += node_model(id) / npass;
                 if (leaf_here) mb += leaf_model;
             }
@@ -458,7 +459,37 @@ struct Scat2DPlan {
                 };
                 if (is_root) pass(K2_ROOT_ROWS, H[r] / rb + (H[r] % rb != 0), rb, row_lds(r, rb));
                 else {
-                    pass(K2_ICOL, W[r] / cb + (W[r] % cb != 0), cb, (size_t)H[r] * cb * cs);
+                    // sibling pairs in the gather + inverse column pass (one read of
+                    // the parent's columns for two filters): compiled passes, k = 2 or
+                    // 4 (the mixed batch's 320 and 160 nodes under 640 and 320
+                    // parents: 7.34k -> 7.73k images/s; with k = 1, the root's
+                    // children at 272, the headline lost 0.9%), the two blocks within
+                    // 64 KB; pair = 1 keeps one node per task
+                    const size_t two = 2 * (size_t)H[r] * cb * cs, tw = cs * kym_tw_elems(H[r], W[r]);
+                    bool paired = false;
+                    if (tune.pair != 1 && psid >= 0 && two + tw <= std::min(lds_limit, (size_t)64 * 1024)) {
+                        std::vector<int> sib = ids;
+                        std::stable_sort(sib.begin(), sib.end(),
+                                         [&](int u, int v) { return nodes[u].parent < nodes[v].parent; });
+                        bool ok = sib.size() % 2 == 0;
+                        for (size_t i = 0; ok && i < sib.size(); i += 2) {
+                            const Node2D& u = nodes[sib[i]];
+                            const Node2D& v = nodes[sib[i + 1]];
+                            ok = u.parent == v.parent && u.k == v.k && (u.k == 2 || u.k == 4);
+                        }
+                        if (ok) {
+                            push(K2_ICOL_PAIR, sib, W[r] / cb, cb, 256, two);
+                            Launch2D& lb = launches.back();
+                            lb.ngroup /= 2;
+                            lb.pair = 1;
+                            lb.spec = psid;
+                            lb.threads = pnt;
+                            lb.lds_bytes = two + tw;
+                            max_lds = std::max(max_lds, lb.lds_bytes);
+                            paired = true;
+                        }
+                    }
+                    if (!paired) pass(K2_ICOL, W[r] / cb + (W[r] % cb != 0), cb, (size_t)H[r] * cb * cs);
                     pass(K2_ROWS_MOD, H[r] / rb + (H[r] % rb != 0), rb, row_lds(r, rb));
                 }
                 // the row passes keep columns 0 .. W / 2 of the spectra of real

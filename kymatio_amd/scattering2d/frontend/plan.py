@@ -33,7 +33,7 @@ class Scat2DPlanHandle:
         self.dtype = dtype
 
     KIND_NAMES = ['lds_node', 'leaf', 'root_rows', 'icol', 'rows_mod', 'fwd_cols', 'leaf_icol', 'leaf_rows',
-                  'fir_cols']
+                  'fir_cols', 'icol_pair']
 
     def launches(self):
         """Per-launch records: kind, nodes, tasks per node, threads, LDS bytes,

@@ -99,7 +99,7 @@ class Emu:
         code = 0 if dt == np.float32 else 1
         M, N = shape
         B = x.shape[0]
-        info = (ctypes.c_longlong * 8)()
+        info = (ctypes.c_longlong * 9)()
         rc = self.lib.kym_emu_scat2d_forward(M, N, J, L, max_order, pre_pad, code, lds_limit, None, None, None,
                                              B, 0, info)
         if rc != 0:
@@ -120,7 +120,7 @@ class Emu:
         if rc != 0:
             raise RuntimeError(self.lib.kym_emu_last_error().decode())
         return out, {'launches': info[3], 'big_nodes': info[4], 'spec_launches': info[5], 'pair_launches': info[6],
-                     'bwd_spec_launches': info[7]}
+                     'bwd_spec_launches': info[7], 'col_pair_launches': info[8]}
 
 
 def emu_scat1d(emu, module, x, dtype=np.float32, dS=None, nbwd=1):

@@ -154,3 +154,30 @@ def test_opt_in_pair_kernels(emu):
     out, info = emu.scat2d(x, J, (256, 256), f, tune=[16, 0, 8, 8])
     assert info['pair_launches'] == 1
     assert channel_rel_err(out, G['s256_Sx'][:1]) < RTOL_CHANNEL
+
+
+@pytest.mark.parametrize('side,pairs', [(256, 2), (512, 4)])
+def test_column_pass_sibling_pairs(emu, side, pairs):
+    """The gather + inverse column passes of sibling big nodes with k = 2 or
+    4 (256x256 J=5, 512x512 J=6) run as pairs sharing the parent's columns:
+    outputs against the oracle's (golden_mixed2d.npz), and the same with
+    pairs off (pair = 1).  The headline's plan (k = 1 children at 272) and
+    128x128 J=4 keep one node per task."""
+    from tests.helpers import golden
+    G = golden('golden_mixed2d.npz')
+    J = int(G['s%d_J' % side])
+    x = np.random.default_rng(int(G['s%d_seed' % side])).random((1, side, side), dtype=np.float32)
+    Mp, Np = compute_padding(side, side, J)
+    f = pack_filters_2d(filter_bank(Mp, Np, J, 8))
+    out, info = emu.scat2d(x, J, (side, side), f, tune=[16, 0, 8, 8])
+    assert info['col_pair_launches'] == pairs
+    assert channel_rel_err(out, G['s%d_Sx' % side][:1]) < RTOL_CHANNEL
+    out1, info1 = emu.scat2d(x, J, (side, side), f, tune=[16, 0, 8, 8, 0, 0, 1])
+    assert info1['col_pair_launches'] == 0
+    assert channel_rel_err(out1, G['s%d_Sx' % side][:1]) < RTOL_CHANNEL
+    for tag in ('cfg2', 'cfg4'):
+        _, xc, Jc, shape, kw, _ = _case(tag)
+        Mc, Nc = compute_padding(shape[0], shape[1], Jc)
+        fc = pack_filters_2d(filter_bank(Mc, Nc, Jc, 8))
+        _, ic = emu.scat2d(xc[:1], Jc, shape, fc, tune=[16, 0, 8, 8], **kw)
+        assert ic['col_pair_launches'] == 0
