@@ -64,7 +64,9 @@ int kym_scat2d_create(int M, int N, int J, int L, int max_order, int pre_pad, in
  * workgroup and one read of the parent's spectrum wherever a pair kernel
  * for the size is compiled; 0, the default: where that is the default
  * kernel for the size, or the launch's deepest periodisation is 16 or
- * more)}; ntune <= 7; 0 = built-in heuristic.  Results
+ * more; 0 and 2 also pair siblings with k = 2 or 4 in the big nodes'
+ * gather + inverse column pass where it is compiled and two blocks fit
+ * 64 KB of LDS, 1 does not)}; ntune <= 7; 0 = built-in heuristic.  Results
  * are the same for every choice up to FP32 rounding. */
 int kym_scat2d_create_tuned(int M, int N, int J, int L, int max_order, int pre_pad, int dtype,
                             long long lds_limit, const int* tune, int ntune, void** plan);

@@ -31,7 +31,8 @@ struct Scat2DTune {
     int pass_threads = 0;   // threads of the big-node pass workgroups (compiled choices only)
     int streams = 0;        // chunks in flight (inference and training): 1 to 4 (0 = default 2)
     int pair = 0;           // LDS nodes: 1 one node per workgroup, 2 sibling pairs wherever a pair kernel is
-                            // compiled (0 = heuristic: the default pair kernels, the others from k = 16)
+                            // compiled (0 = heuristic: the default pair kernels, the others from k = 16);
+                            // 0 and 2 also pair the big-node column passes with k = 2, 4 (1 does not)
 };
 
 template <class T>
