@@ -691,6 +691,13 @@ KYM_HD KYM_INL constexpr bool fir_band_in_lds(int kmax, int elem_bytes) {
     return 16ll * kmax * elem_bytes <= KYM_FIR_BAND_LDS;
 }
 
+// Row passes with a band in global memory whose block (up to 16 rows) and
+// output width (up to 16) make one MFMA tile split its k-steps over four
+// waves, with 4 KB of LDS for their partial sums (KYM_FIR_SPLIT_LDS, where
+// the band would be); 8 rows of 640 samples take 160 k-steps (J = 6).
+#define KYM_FIR_SPLIT_LDS (4 * 64 * 4 * 4)
+KYM_HD KYM_INL constexpr bool fir_rows_split(int rows, int ow) { return rows <= 16 && ow <= 16; }
+
 // band[k][d] = c[(k - kl d) mod n] (zero beyond the taps), k < kmax, d < 16:
 // inside a tile's window the circulant factor depends only on (k, d).  The
 // frontend stores it next to the taps; passes copy it to LDS.
@@ -757,6 +764,82 @@ KYM_HD void fir_rows_mma(const T* R, int rs, int nrows, int W, int kl, int ow, c
             if (p < nrows && x < ow) out(p, x, val);
         });
     }
+}
+
+// fir_rows_mma for one tile (nrows <= 16, ow <= 16) whose band is read from
+// global memory: the tile's k-steps (up to 160, each waiting on L2) are split
+// over up to four waves, whose partial sums meet in `red` (KYM_FIR_SPLIT_LDS).
+template <class T, class OUT>
+KYM_HD void fir_rows_mma_split(const T* R, int rs, int nrows, int W, int kl, int ow, const FirAxis& fx, const T* Bx,
+                               OUT out, int tid, int nth, T* red) {
+    const int lane = lane_id(tid), wv = wave_id(tid), nw = wave_count(nth);
+    const int nmt = (nrows + 15) >> 4, nnt = (ow + 15) >> 4;
+    // k-steps [k0, k1) of tile t into c
+    auto tile_steps = [&](Mma16Acc& c, int t, int k0, int k1) {
+        const int p0 = (t / nnt) << 4, x0 = (t % nnt) << 4;
+        const int span = kl * ((ow - 1 - x0 < 15) ? ow - 1 - x0 : 15) + fx.cnt;
+        const int len = span < W ? span : W;
+        int qs = kl * (x0 + 1) + fx.lo;
+        if (qs < 0) qs += W;
+        const int nfull = len >> 2;                                // steps with every kk < len
+        const int s_a = (W - qs) >> 2 < nfull ? (W - qs) >> 2 : nfull;  // no lane wraps before s_a
+        const int s_b = (W - qs + 3) >> 2 < nfull ? (W - qs + 3) >> 2 : nfull;  // every lane wrapped from s_b
+        if (k1 < 0) k1 = nfull + ((len & 3) ? 1 : 0);
+        auto row = [&](int l) { const int p = p0 + (l & 15); return (p < nrows ? p : nrows - 1) * rs; };
+        auto general = [&](int s) {
+            mma16_step(
+                c, lane,
+                [&](int l) -> float {
+                    const int kk = 4 * s + (l >> 4);
+                    int q = qs + kk;
+                    q = q >= W ? q - W : q;
+                    q = q < W ? q : W - 1;
+                    const float v = R[row(l) + q];
+                    return kk < len ? v : 0.f;
+                },
+                [&](int l) -> float {
+                    const int kk = 4 * s + (l >> 4);
+                    const float v = Bx[(kk < len ? kk : len - 1) * 16 + (l & 15)];
+                    return kk < len ? v : 0.f;
+                });
+        };
+        const int a0 = k0, a1 = s_a < k1 ? s_a : k1;
+        if (a1 > a0)
+            mma16_strided<4, 64>(
+                c, lane, [&](int l) { return R + row(l) + qs + 4 * a0 + (l >> 4); },
+                [&](int l) { return Bx + (4 * a0 + (l >> 4)) * 16 + (l & 15); }, a1 - a0);
+        for (int st = (s_a > k0 ? s_a : k0); st < (s_b < k1 ? s_b : k1); ++st) general(st);
+        const int b0 = s_b > k0 ? s_b : k0, b1 = nfull < k1 ? nfull : k1;
+        if (b1 > b0)
+            mma16_strided<4, 64>(
+                c, lane, [&](int l) { return R + row(l) + qs - W + 4 * b0 + (l >> 4); },
+                [&](int l) { return Bx + (4 * b0 + (l >> 4)) * 16 + (l & 15); }, b1 - b0);
+        if ((len & 3) && nfull >= k0 && nfull < k1) general(nfull);
+    };
+    auto tile_out = [&](int t, int l, int v, float val) {
+        const int p = ((t / nnt) << 4) + 4 * (l >> 4) + v, x = ((t % nnt) << 4) + (l & 15);
+        if (p < nrows && x < ow) out(p, x, val);
+    };
+    // One tile (nrows <= 16, ow <= 16) and scratch `red` (4 waves x 64 lanes
+    // x 4 floats): up to four waves take a quarter of its k-steps each and
+    // sum through LDS.
+    const int split = nw < 4 ? nw : 4;
+    Mma16Acc c;
+    mma16_zero(c);
+    if (wv < split) {
+        const int span = kl * (ow - 1 < 15 ? ow - 1 : 15) + fx.cnt;  // tile 0: x0 = 0
+        const int len = span < W ? span : W;
+        const int nsteps = (len >> 2) + ((len & 3) ? 1 : 0);
+        tile_steps(c, 0, wv * nsteps / split, (wv + 1) * nsteps / split);
+        if (split > 1) mma16_store(c, lane, [&](int l, int v, float val) { red[(wv * 64 + l) * 4 + v] = val; });
+    }
+    if (split > 1) wg_barrier();
+    if (wv == 0)
+        mma16_store(c, lane, [&](int l, int v, float val) {
+            float sum = val;
+            for (int w = 1; w < split; ++w) sum += red[(w * 64 + l) * 4 + v];
+            tile_out(0, l, v, sum);
+        });
 }
 
 // Column pass on the matrix cores: S[y][x] = sum_p c[(p - kl (y+1) - lo) mod H] Tb[p][x]
@@ -1161,7 +1244,9 @@ struct StatPass {
 // Row pass of a big node's FIR leaf: rows rr < nr of R (natural columns,
 // row stride rs) are the node's rows rev[r0 + rr]; T[y][x] goes to the
 // tslot.  Bx: band table space after R (fir_kmax(W, ..) x 16 reals) where
-// the band is kept in LDS (fir_band_in_lds), else unused.
+// the band is kept in LDS (fir_band_in_lds); else, where the rows hold one
+// MFMA tile (fir_rows_split), the 4 KB of partial sums of the waves that
+// share the tile's k-steps.
 template <class T>
 KYM_HD void fir_rows_to_slot(const Scat2DArgs<T>& a, const Node2D& nd, const T* R, int rs, int nr, int r0,
                              const int* rev, cpx<T>* wsi, T* Bx, int tid, int nth) {
@@ -1169,11 +1254,13 @@ KYM_HD void fir_rows_to_slot(const Scat2DArgs<T>& a, const Node2D& nd, const T* 
     const FirAxis fx = a.fir[2 * r + 1];
     const bool in_lds = fir_band_in_lds(fx.kmax, (int)sizeof(T));
     if (in_lds) fir_band(Bx, a.filt, fx, tid, nth);
+    T* red = (!in_lds && fir_rows_split(a.blk, ow)) ? Bx : nullptr;
     wg_barrier();  // also: R complete
     T* Tg = reinterpret_cast<T*>(wsi + nd.tslot);
     auto out = [&](int p, int x, float v) { Tg[(long long)rev[r0 + p] * ow + x] = v; };
-    // two calls, so that each reads its band through a known address space
+    // separate calls, so that each reads its band through a known address space
     if (in_lds) fir_rows_mma(R, rs, nr, W, kl, ow, fx, Bx, out, tid, nth);
+    else if (red) fir_rows_mma_split(R, rs, nr, W, kl, ow, fx, a.filt + fx.band_off, out, tid, nth, red);
     else fir_rows_mma(R, rs, nr, W, kl, ow, fx, a.filt + fx.band_off, out, tid, nth);
 }
 

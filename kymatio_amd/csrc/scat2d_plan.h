@@ -362,7 +362,11 @@ struct Scat2DPlan {
             int rb = (int)((pass_budget - std::min(pass_budget / 2, band_bytes(r))) / row_bytes(W[r]));
             return std::max(1, std::min(fir ? 8 : 16, rb));
         };
-        auto row_lds = [&](int r, int rb) { return (size_t)rb * row_bytes(W[r]) + band_bytes(r); };
+        auto row_lds = [&](int r, int rb) {
+            const bool glob = fir && !fir_band_in_lds(fir_axes[2 * r + 1].kmax, (int)sizeof(T));
+            return (size_t)rb * row_bytes(W[r]) + band_bytes(r) +
+                   (glob && fir_rows_split(rb, ow) ? KYM_FIR_SPLIT_LDS : 0);
+        };
         // column pass of a big node's leaf: its row-filtered field plus the band table along y
         auto fir_cols_lds = [&](int r) {
             return sizeof(T) * ((size_t)H[r] * fir_tstride(W[r], ow) +
