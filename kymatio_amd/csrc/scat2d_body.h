@@ -773,7 +773,7 @@ template <class T, class OUT>
 KYM_HD void fir_rows_mma_split(const T* R, int rs, int nrows, int W, int kl, int ow, const FirAxis& fx, const T* Bx,
                                OUT out, int tid, int nth, T* red) {
     const int lane = lane_id(tid), wv = wave_id(tid), nw = wave_count(nth);
-    const int nmt = (nrows + 15) >> 4, nnt = (ow + 15) >> 4;
+    const int nnt = (ow + 15) >> 4;
     // k-steps [k0, k1) of tile t into c
     auto tile_steps = [&](Mma16Acc& c, int t, int k0, int k1) {
         const int p0 = (t / nnt) << 4, x0 = (t % nnt) << 4;
