@@ -99,13 +99,14 @@ KYM_HD KYM_INL void grad_u_elems(const Scat2DArgs<T>& a, const Node2D& nd, const
 }
 
 // dL/dU at the A x B alias points (ys[e] + i H/2, xs[e] + j W/2), i < A,
-// j < B, of E base points (acc[e][i B + j]).  Every child of a node is at
-// half its resolution or coarser and its leaf at 2^-J of the padded size, so
-// H/2 and W/2 are multiples of every child's and the leaf's sizes (the
-// padded sizes are multiples of 2^J): all alias points of a base point read
-// the same child gradient and leaf value.  Each is loaded once per base
-// point; the filters are read per point.  Per point the terms are summed in
-// grad_u_elems' order, so results are identical to it.
+// j < B, of E base points (acc[e][i B + j]).  A child at half the node's
+// resolution or coarser, and the leaf at 2^-J of the padded size, have sizes
+// dividing H/2 and W/2 (the padded sizes are multiples of 2^J): all alias
+// points of a base point read the same child gradient and leaf value, loaded
+// once per base point.  Children at the node's own resolution (the root's
+// j1 = 0 children, listed first) are read per point, and the filters are.
+// Per point the terms are summed in grad_u_elems' order, so results are
+// identical to it.
 template <int A, int B, int E, class T>
 KYM_HD KYM_INL void grad_u_alias(const Scat2DArgs<T>& a, const Node2D& nd, const cpx<T>* wsi, const cpx<T>* GY, int W,
                                  int HJ, int WJ, const int* ys, const int* xs, cpx<T> (*acc)[A * B]) {
@@ -148,6 +149,22 @@ KYM_HD KYM_INL void grad_u_alias(const Scat2DArgs<T>& a, const Node2D& nd, const
         }
     };
     int c = nd.child_begin;
+    // leading children at the node's own resolution (the root's j1 = 0
+    // children): each point reads its own gradient, at the filter's index
+    for (; c < nd.child_end && a.nodes[c].res == nd.res; ++c) {
+        const Node2D& cn = a.nodes[c];
+        const T* f = a.filt + a.filt_off[cn.filt];
+        const T inv = T(1) / T(cn.k * cn.k);
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+#pragma unroll
+            for (int q = 0; q < P; ++q) {
+                const T fv = kym_ldg(f + idx[e][q]) * inv;
+                const cpx<T> gx = kym_ldg(wsi + cn.gsave + idx[e][q]);
+                acc[e][q].re += gx.re * fv;
+                acc[e][q].im += gx.im * fv;
+            }
+    }
     for (; c + 4 <= nd.child_end; c += 4) {
         T fv[4][E][P];
         cpx<T> gx[4][E];
@@ -194,14 +211,16 @@ KYM_HD KYM_INL void grad_u_alias(const Scat2DArgs<T>& a, const Node2D& nd, const
 }
 
 // Whether a node's alias points share their child and leaf reads: sizes
-// even, the leaf's a divisor of the half sizes (false at J = 0), and every
-// child coarser than the node (false at the root, whose j1 = 0 children are
-// at its own resolution).
+// even, the leaf's a divisor of the half sizes (false at J = 0), and, after
+// a leading run of children at the node's own resolution (the root's j1 = 0
+// children, read per point), every child coarser than the node.
 template <class T>
 KYM_HD KYM_INL bool grad_alias_ok(const Scat2DArgs<T>& a, const Node2D& nd) {
     const int H = a.H[nd.res], W = a.W[nd.res], HJ = a.H[a.J], WJ = a.W[a.J];
     if (H % 2 != 0 || W % 2 != 0 || (H / 2) % HJ != 0 || (W / 2) % WJ != 0) return false;
-    for (int c = nd.child_begin; c < nd.child_end; ++c)
+    int c = nd.child_begin;
+    while (c < nd.child_end && a.nodes[c].res == nd.res) ++c;
+    for (; c < nd.child_end; ++c)
         if (a.nodes[c].res <= nd.res) return false;
     return true;
 }
